@@ -1,0 +1,133 @@
+/*
+ * sx_oracle.h -- CPU restatement of sentinel-x's src/hmmer Plan7 search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP kernels
+ * of the sentinel-x_amd package.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only to check or to time a
+ * CPU baseline; no product path calls it.
+ *
+ * Every function restates one reference function in the same operation
+ * order, in IEEE single precision where the reference uses f32, compiled with
+ * -ffp-contract=off (Rust never fuses a*b+c), and with glibc logf/log/expf/exp
+ * (what Rust's f32::ln / f64::ln / exp lower to on x86_64-linux-gnu).
+ *
+ * Layouts shared with the product library:
+ *   msc, mo : NCODE x (M+1) floats, residue-major, column 0 = -inf / 0.
+ *   tsc, tp : 8 x (M+1) floats in the order MM MI MD IM II DM DD BM.
+ *   xsc     : 16 floats: n_loop n_move j_loop j_move c_loop c_move e_move e_loop
+ *             then the same eight as probabilities (XscP).
+ *   sp      : 5 x (L+1) floats: N, B, E, J, C rows.
+ *
+ * Pinning (what checks each function independently of this restatement) is
+ * listed per function below; "parity unpinned" marks a value no golden vector
+ * of the reference fixes.
+ */
+#ifndef SX_ORACLE_H
+#define SX_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SXO_K 20
+#define SXO_NCODE 25
+#define SXO_CODE_STAR 24
+
+enum { SXO_TMM = 0, SXO_TMI, SXO_TMD, SXO_TIM, SXO_TII, SXO_TDM, SXO_TDD, SXO_TBM, SXO_NT };
+
+/* alphabet.rs:58-93 -- returns the number of codes written (whitespace dropped). */
+size_t sxo_encode(const uint8_t* seq, size_t n, uint8_t* out);
+/* alphabet.rs:36-39 */
+const float* sxo_background(void);
+
+/* model.rs:146-245 + occupancy model.rs:248-260.
+ * mat: (M+1) x 20 probabilities (row 0 unused), t: (M+1) x 7 transitions.
+ * Pinned by model.rs:364-393 (entry normalisation, flat model => 0 scores,
+ * '*' = -inf) and by the golden .hmm fixtures end to end. */
+void sxo_configure(int m, const float* mat, const float* t,
+                   float* msc, float* tsc, float* mo, float* tp, float* msv_tbm);
+
+/* model.rs:284-312 (Xsc::new) */
+void sxo_xsc(int64_t l, int multihit, float* xsc16);
+/* model.rs:334-337 */
+float sxo_null_score(int64_t l);
+
+/* dp.rs:92-145 (msv_kernel).  Parity unpinned by the reference; pinned by
+ * a generic max-product over the explicit MSV state graph (tests). */
+float sxo_msv(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_t l);
+
+/* dp.rs:152-189.  Parity unpinned by the reference; pinned by a generic
+ * max-product over the explicit Plan7 state graph (tests). */
+float sxo_viterbi(int m, const float* msc, const float* tsc, const float* xsc16,
+                  const uint8_t* dsq, int64_t l);
+
+/* dp.rs:229-298.  Returns lnf(xC*c_move) + lnscale in double; the reference's
+ * f32 score is (float) of it.  sp (5 x (L+1)) may be NULL. */
+double sxo_forward(int m, const float* mo, const float* tp, const float* xsc16,
+                   const uint8_t* dsq, int64_t l, float* sp);
+
+/* dp.rs:301-392.  sp (5 x (L+1)) must be given. */
+float sxo_backward(int m, const float* mo, const float* tp, const float* xsc16,
+                   const uint8_t* dsq, int64_t l, float* sp);
+
+/* FP64 twins of forward/backward: the same recurrences with every cell in
+ * double over the same f32 parameter tables (promoted), rescaled the same
+ * way.  They are the error reference for the f32 GPU Forward. */
+double sxo_forward64(int m, const float* mo, const float* tp, const float* xsc16,
+                     const uint8_t* dsq, int64_t l);
+double sxo_backward64(int m, const float* mo, const float* tp, const float* xsc16,
+                      const uint8_t* dsq, int64_t l);
+
+/* pipeline.rs:107-150 (bias_filter_score).  Parity unpinned by the
+ * reference; pinned by an independent log-space two-state forward (tests). */
+float sxo_bias_filter(int m, const float* compo, const uint8_t* dsq, int64_t l);
+
+/* pipeline.rs:87-95, 98-104 */
+double sxo_gumbel_surv(double x, double mu, double lambda);
+double sxo_exp_logsurv(double x, double tau, double lambda);
+/* dp.rs:32-40 */
+float sxo_logsum(float a, float b);
+
+/* Filter cascade of score_target, pipeline.rs:159-189.
+ * stats: msv mu, lambda, viterbi mu, lambda, forward tau, lambda.
+ * out: [0] msv, [1] bias filtersc, [2] viterbi, [3] forward (f32 value),
+ * returns the stage reached: 0 rejected by MSV, 1 by the bias-corrected MSV
+ * test, 2 by Viterbi, 3 by Forward, 4 passed every filter. */
+int sxo_cascade(int m, const float* msc, const float* tsc, const float* mo, const float* tp,
+                float msv_tbm, const float* compo, const double* stats6,
+                double f1, double f2, double f3,
+                const uint8_t* dsq, int64_t l, float* out4);
+
+/* Many-pair drivers over packed profiles and targets, used as the CPU
+ * baseline (the reference fans (profile, target) pairs out over rayon,
+ * search.rs:213-224).  Pairs are the flat index p*nt+t in [pair_begin,
+ * pair_end); workers take chunks dynamically.  out is indexed by the flat
+ * pair index minus pair_begin.
+ *   prof_m[p], prof_off[p]: offset (in floats) of profile p's (M+1) columns
+ *   inside the packed NCODE-row / 8-row tables (see sx_oracle.c). */
+typedef struct {
+    int np;
+    const int32_t* m;
+    const int64_t* off;      /* per-profile offset into msc/mo (NCODE*(M+1) each) */
+    const int64_t* toff;     /* per-profile offset into tsc/tp (8*(M+1) each) */
+    const float* msc;
+    const float* tsc;
+    const float* mo;
+    const float* tp;
+    const float* msv_tbm;
+    int nt;
+    const int64_t* seq_off;  /* nt+1 offsets into codes */
+    const uint8_t* codes;
+} sxo_batch;
+
+int sxo_batch_msv(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, float* out, int nthreads);
+int sxo_batch_viterbi(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, float* out, int nthreads);
+int sxo_batch_forward(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
+int sxo_batch_forward64(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
