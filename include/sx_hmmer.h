@@ -1,0 +1,136 @@
+/*
+ * sx_hmmer.h -- C ABI of the MI355X implementation of sentinel-x's src/hmmer Plan7
+ * search path (MSV -> bias -> Viterbi -> Forward/Backward filter cascade, plus the
+ * host-side domain definition and hmmsearch-style reporting).
+ *
+ * The reference is a Rust module with no FFI (src/hmmer/mod.rs:23-30).  Each entry
+ * point below names the reference interface it replaces; a Rust caller binds them with
+ * an `extern "C"` block.  Conventions: plain C types, caller-owned
+ * buffers, status codes (0 = SX_OK, negative = error, message via sx_last_error), no
+ * exceptions across the boundary.  A context owns one HIP device, one stream and the
+ * device-resident profile and target tables; it is used by one host thread at a time.
+ */
+#ifndef SX_HMMER_H
+#define SX_HMMER_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SX_OK 0
+#define SX_ERR_ARG (-1)
+#define SX_ERR_HIP (-2)
+#define SX_ERR_NOMEM (-3)
+#define SX_ERR_PARSE (-4)
+#define SX_ERR_STATE (-5)
+#define SX_ERR_CUTOFF (-6)
+#define SX_ERR_NODEVICE (-7)
+
+typedef struct sx_ctx sx_ctx;
+
+/* DP stages (dp.rs:69 msv, :152 viterbi_score, :229 forward, :301 backward). */
+enum { SX_STAGE_MSV = 0, SX_STAGE_VITERBI = 1, SX_STAGE_FORWARD = 2, SX_STAGE_BACKWARD = 3 };
+/* Launch forms: PLAIN = one launch per profile over blocks of targets in input order,
+ * one wavefront per pair; BATCHED = one persistent launch per stage and lane-group
+ * shape, length-bucketed tiles from a device-resident queue. */
+enum { SX_FORM_PLAIN = 0, SX_FORM_BATCHED = 1 };
+
+/* A parsed HMM in probability space, as parse.rs:107-236 produces it (model.rs:54-71). */
+typedef struct {
+    const char* name;
+    const char* acc;     /* NULL if absent */
+    const char* desc;    /* NULL if absent */
+    int32_t m;
+    const float* mat;    /* (m+1) x 20 match emission probabilities, row 0 unused */
+    const float* t;      /* (m+1) x 7 transitions MM MI MD IM II DM DD */
+    const float* compo;  /* 20 values, or NULL (then the background is used) */
+    int32_t has_ga, has_tc, has_nc;
+    float ga[2], tc[2], nc[2];
+    int32_t has_stats;   /* bit 0 MSV, bit 1 VITERBI, bit 2 FORWARD */
+    double stats[6];     /* msv mu, lambda; viterbi mu, lambda; forward tau, lambda */
+} sx_hmm;
+
+/* Create a context on HIP device `device`.  Fails with SX_ERR_NODEVICE when no HIP
+ * device is usable; there is no CPU fallback. */
+int sx_create(int device, sx_ctx** out);
+void sx_destroy(sx_ctx* ctx);
+const char* sx_last_error(const sx_ctx* ctx);
+
+/* Targets::new / Targets::from_pairs (search.rs:56-71) for already-digitised codes
+ * (alphabet.rs:91-93): target i is codes[offsets[i] .. offsets[i+1]).  The codes stay
+ * resident on the device until the next call. */
+int sx_set_targets(sx_ctx* ctx, const uint8_t* codes, const uint64_t* offsets, uint32_t n_targets);
+
+/* Hmm::configure (model.rs:146-245) for each model, then upload.  Replaces the
+ * per-batch `configure()` of search_hmm_file_models (search.rs:257-263). */
+int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n);
+/* Append the models of a HMMER3 file (parse.rs:42-54 open_hmm + HmmReader). */
+int sx_load_hmm_file(sx_ctx* ctx, const char* path, uint32_t* n_loaded);
+uint32_t sx_num_profiles(const sx_ctx* ctx);
+uint32_t sx_num_targets(const sx_ctx* ctx);
+/* Copy out the configured tables of profile p (for checking against the oracle):
+ * msc, mo: 25*(M+1); tsc, tp: 8*(M+1) in the order MM MI MD IM II DM DD BM. */
+int sx_profile_tables(const sx_ctx* ctx, uint32_t p, int32_t* m, float* msc, float* tsc, float* mo,
+                      float* tp, float* msv_tbm);
+
+/* One DP stage over every (profile, target) pair, the pair loop of search_profiles
+ * (search.rs:217-224) with dp.rs:69/152/229/301 per pair.  out: P*T values in
+ * (profile, target) order, float for MSV/VITERBI, double for FORWARD/BACKWARD.
+ * ms (may be NULL) receives the device time of the stage's launches. */
+int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double* ms);
+
+/* The same stage over an explicit pair list (prof[i], tgt[i]). */
+int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
+                   void* out, double* ms);
+
+/* Forward (dp.rs:229-298) or Backward (dp.rs:301-392) special-state rows for a pair
+ * list, as score_target consumes them (pipeline.rs:180-206).  sp receives 5*(L+1)
+ * floats per pair (rows N, B, E, J, C; pairs back to back in list order), score one
+ * double per pair.  stage is SX_STAGE_FORWARD or SX_STAGE_BACKWARD. */
+int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp,
+                double* score);
+
+/* Filter cascade of score_target (pipeline.rs:159-189) over every pair, on the device:
+ * MSV on all pairs, bias filter and Viterbi on survivors, Forward on Viterbi
+ * survivors.  Thresholds f1, f2, f3 as PipelineConfig (pipeline.rs:27-42).  Profiles
+ * must carry STATS (the reference skips the filters otherwise, pipeline.rs:164).
+ * Results: one record per pair that reached the bias filter (MSV survivors).
+ * stage = 1 rejected by the bias-corrected MSV test, 2 by Viterbi, 3 by Forward,
+ * 4 passed every filter. */
+typedef struct {
+    uint32_t profile, target;
+    float msv, bias, vit;
+    float fwd;       /* f32 score as the reference returns it */
+    double fwd64;    /* the same before the final f32 rounding */
+    int32_t stage;
+    int32_t pad;
+} sx_pair_scores;
+
+typedef struct {
+    double f1, f2, f3;
+    uint64_t n_pairs, n_msv_pass, n_bias_pass, n_vit_pass, n_fwd_pass;
+    double ms_msv, ms_bias, ms_vit, ms_fwd, ms_total;
+    uint64_t cells_msv, cells_vit, cells_fwd;
+} sx_cascade_stats;
+
+int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
+                   uint64_t* n_out);
+
+/* Choose the lane-group shape (Q positions per lane, G lanes per pair) for each model
+ * length from timings on the current profiles and targets; `budget_ms` bounds the
+ * time spent.  Without it a static table is used. */
+int sx_autotune(sx_ctx* ctx, int stage, double budget_ms);
+/* Install a shape table for the batched form of `stage`: rows (max_m[i], q[i], g[i])
+ * in increasing max_m; a model of length M uses the first row with M <= max_m.  n = 0
+ * restores the built-in choice.  Fails if a row names a shape with no kernel or one
+ * whose capacity q*g is below max_m. */
+int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q, const int32_t* g);
+/* Shape chosen for a model length: writes q, g. */
+int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,272 @@
+"""sentinel-x_amd: MI355X implementation of sentinel-x's src/hmmer Plan7 search path.
+
+The compute lives in libsxhmmer.so (HIP kernels for gfx950 behind the C ABI of
+include/sx_hmmer.h).  This module is a thin ctypes binding used by the tests, the
+benchmark and Python callers; it never falls back to a CPU implementation: if the
+library or a HIP device is missing, it raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsxhmmer.so")
+
+STAGE_MSV, STAGE_VITERBI, STAGE_FORWARD, STAGE_BACKWARD = 0, 1, 2, 3
+FORM_PLAIN, FORM_BATCHED = 0, 1
+STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward"}
+
+# Every entry point declared in include/sx_hmmer.h.
+EXPORTS = [
+    "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
+    "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
+    "sx_run_cascade", "sx_autotune", "sx_shape_for", "sx_set_shapes",
+]
+
+
+class SxError(RuntimeError):
+    pass
+
+
+def build(force: bool = True, jobs: int = 4) -> str:
+    """Compile libsxhmmer.so for gfx950 with the package Makefile."""
+    cmd = ["make", f"-j{jobs}"] + (["-B"] if force else [])
+    subprocess.run(cmd, cwd=HERE, check=True)
+    return LIB_PATH
+
+
+class sx_hmm(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char_p), ("acc", ctypes.c_char_p), ("desc", ctypes.c_char_p), ("m", ctypes.c_int32),
+        ("mat", ctypes.POINTER(ctypes.c_float)), ("t", ctypes.POINTER(ctypes.c_float)),
+        ("compo", ctypes.POINTER(ctypes.c_float)),
+        ("has_ga", ctypes.c_int32), ("has_tc", ctypes.c_int32), ("has_nc", ctypes.c_int32),
+        ("ga", ctypes.c_float * 2), ("tc", ctypes.c_float * 2), ("nc", ctypes.c_float * 2),
+        ("has_stats", ctypes.c_int32), ("stats", ctypes.c_double * 6),
+    ]
+
+
+class sx_pair_scores(ctypes.Structure):
+    _fields_ = [("profile", ctypes.c_uint32), ("target", ctypes.c_uint32), ("msv", ctypes.c_float),
+                ("bias", ctypes.c_float), ("vit", ctypes.c_float), ("fwd", ctypes.c_float),
+                ("fwd64", ctypes.c_double), ("stage", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class sx_cascade_stats(ctypes.Structure):
+    _fields_ = [("f1", ctypes.c_double), ("f2", ctypes.c_double), ("f3", ctypes.c_double),
+                ("n_pairs", ctypes.c_uint64), ("n_msv_pass", ctypes.c_uint64), ("n_bias_pass", ctypes.c_uint64),
+                ("n_vit_pass", ctypes.c_uint64), ("n_fwd_pass", ctypes.c_uint64),
+                ("ms_msv", ctypes.c_double), ("ms_bias", ctypes.c_double), ("ms_vit", ctypes.c_double),
+                ("ms_fwd", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("cells_msv", ctypes.c_uint64), ("cells_vit", ctypes.c_uint64), ("cells_fwd", ctypes.c_uint64)]
+
+
+PAIR_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("msv", "<f4"), ("bias", "<f4"), ("vit", "<f4"),
+                       ("fwd", "<f4"), ("fwd64", "<f8"), ("stage", "<i4"), ("pad", "<i4")])
+
+_lib = None
+
+
+def lib():
+    """Load libsxhmmer.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SxError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or `make` in {HERE}")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32
+        L.sx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.sx_destroy.argtypes = [vp]
+        L.sx_destroy.restype = None
+        L.sx_last_error.argtypes = [vp]
+        L.sx_last_error.restype = ctypes.c_char_p
+        L.sx_set_targets.argtypes = [vp, vp, vp, u32]
+        L.sx_set_profiles.argtypes = [vp, ctypes.POINTER(sx_hmm), u32]
+        L.sx_load_hmm_file.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(u32)]
+        L.sx_num_profiles.argtypes = [vp]
+        L.sx_num_profiles.restype = u32
+        L.sx_num_targets.argtypes = [vp]
+        L.sx_num_targets.restype = u32
+        L.sx_profile_tables.argtypes = [vp, u32, ctypes.POINTER(i32), vp, vp, vp, vp, vp]
+        L.sx_score_all.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
+        L.sx_score_pairs.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, u64, vp, ctypes.POINTER(ctypes.c_double)]
+        L.sx_specials.argtypes = [vp, ctypes.c_int, vp, vp, u64, vp, vp]
+        L.sx_run_cascade.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sx_cascade_stats), vp, u64,
+                                     ctypes.POINTER(u64)]
+        L.sx_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.sx_set_shapes.argtypes = [vp, ctypes.c_int, u32, vp, vp, vp]
+        for name in EXPORTS:
+            getattr(L, name)  # raises AttributeError if the library lacks a declared entry point
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+class Context:
+    """One HIP device with resident profiles and targets (see include/sx_hmmer.h)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.sx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise SxError(f"sx_create failed ({rc}): {self._L.sx_last_error(None).decode()}")
+        self._h = h
+        self._keep = []
+        self.lengths = np.zeros(0, np.int64)
+        self.n_profiles = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SxError(f"error {rc}: {self._L.sx_last_error(self._h).decode()}")
+
+    # -- data --
+    def set_targets(self, seqs: Sequence[np.ndarray]):
+        lens = np.array([len(s) for s in seqs], np.uint64)
+        off = np.zeros(len(seqs) + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs) else np.zeros(1, np.uint8)
+        if codes.size == 0:
+            codes = np.zeros(1, np.uint8)
+        self._chk(self._L.sx_set_targets(self._h, _ptr(codes), _ptr(off), len(seqs)))
+        self.lengths = lens.astype(np.int64)
+
+    def set_profiles(self, hmms):
+        """hmms: objects with name, m, mat (M+1,20), t (M+1,7), compo, ga/tc/nc, stats dict."""
+        arr = (sx_hmm * max(1, len(hmms)))()
+        keep = []
+        for i, h in enumerate(hmms):
+            mat = np.ascontiguousarray(h.mat, np.float32)
+            t = np.ascontiguousarray(h.t, np.float32)
+            keep += [mat, t]
+            e = arr[i]
+            nm = h.name.encode()
+            keep.append(nm)
+            e.name = nm
+            e.acc = getattr(h, "acc", None).encode() if getattr(h, "acc", None) else None
+            e.desc = getattr(h, "desc", None).encode() if getattr(h, "desc", None) else None
+            e.m = h.m
+            e.mat = mat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            e.t = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            if getattr(h, "compo", None) is not None:
+                c = np.ascontiguousarray(h.compo, np.float32)
+                keep.append(c)
+                e.compo = c.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            for nm2 in ("ga", "tc", "nc"):
+                v = getattr(h, nm2, None)
+                if v is not None:
+                    setattr(e, "has_" + nm2, 1)
+                    getattr(e, nm2)[0], getattr(e, nm2)[1] = v
+            st = getattr(h, "stats", {}) or {}
+            bits = 0
+            for b, key in enumerate(("MSV", "VITERBI", "FORWARD")):
+                if key in st:
+                    bits |= 1 << b
+                    e.stats[2 * b], e.stats[2 * b + 1] = st[key]
+            e.has_stats = bits
+        self._chk(self._L.sx_set_profiles(self._h, arr, len(hmms)))
+        self.n_profiles = len(hmms)
+
+    def load_hmm_file(self, path: str) -> int:
+        n = ctypes.c_uint32()
+        self._chk(self._L.sx_load_hmm_file(self._h, path.encode(), ctypes.byref(n)))
+        self.n_profiles = self._L.sx_num_profiles(self._h)
+        return n.value
+
+    def profile_tables(self, p: int):
+        m = ctypes.c_int32()
+        self._chk(self._L.sx_profile_tables(self._h, p, ctypes.byref(m), None, None, None, None, None))
+        w = m.value + 1
+        msc = np.empty((25, w), np.float32)
+        mo = np.empty((25, w), np.float32)
+        tsc = np.empty((8, w), np.float32)
+        tp = np.empty((8, w), np.float32)
+        tbm = np.zeros(1, np.float32)
+        self._chk(self._L.sx_profile_tables(self._h, p, ctypes.byref(m), _ptr(msc), _ptr(tsc), _ptr(mo), _ptr(tp),
+                                            _ptr(tbm)))
+        return msc, tsc, mo, tp, float(tbm[0])
+
+    # -- compute --
+    def score_all(self, stage: int, form: int = FORM_BATCHED, want: bool = True):
+        nt = len(self.lengths)
+        dt = np.float32 if stage < STAGE_FORWARD else np.float64
+        out = np.empty((self.n_profiles, nt), dt) if want else None
+        ms = ctypes.c_double()
+        self._chk(self._L.sx_score_all(self._h, stage, form, _ptr(out), ctypes.byref(ms)))
+        return out, ms.value
+
+    def score_pairs(self, stage: int, prof, tgt, form: int = FORM_BATCHED):
+        prof = np.ascontiguousarray(prof, np.uint32)
+        tgt = np.ascontiguousarray(tgt, np.uint32)
+        dt = np.float32 if stage < STAGE_FORWARD else np.float64
+        out = np.empty(len(prof), dt)
+        ms = ctypes.c_double()
+        self._chk(self._L.sx_score_pairs(self._h, stage, form, _ptr(prof), _ptr(tgt), len(prof), _ptr(out),
+                                         ctypes.byref(ms)))
+        return out, ms.value
+
+    def specials(self, stage: int, prof, tgt):
+        """Returns (list of (5, L+1) special-row arrays, scores)."""
+        prof = np.ascontiguousarray(prof, np.uint32)
+        tgt = np.ascontiguousarray(tgt, np.uint32)
+        L = self.lengths[tgt]
+        rows = int((L + 1).sum())
+        sp = np.empty(rows * 5, np.float32)
+        sc = np.empty(len(prof), np.float64)
+        self._chk(self._L.sx_specials(self._h, stage, _ptr(prof), _ptr(tgt), len(prof), _ptr(sp), _ptr(sc)))
+        out, o = [], 0
+        for l in L:
+            out.append(sp[o * 5:(o + l + 1) * 5].reshape(5, l + 1))
+            o += l + 1
+        return out, sc
+
+    def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, cap: Optional[int] = None):
+        st = sx_cascade_stats()
+        st.f1, st.f2, st.f3 = f1, f2, f3
+        if cap is None:
+            cap = max(1024, self.n_profiles * len(self.lengths) // 10)
+        out = np.empty(cap, PAIR_DTYPE)
+        n = ctypes.c_uint64()
+        self._chk(self._L.sx_run_cascade(self._h, form, ctypes.byref(st), _ptr(out), cap, ctypes.byref(n)))
+        return st, out[:n.value].copy()
+
+    def autotune(self, stage: int, budget_ms: float = 2000.0):
+        self._chk(self._L.sx_autotune(self._h, stage, budget_ms))
+
+    def set_shapes(self, stage: int, table):
+        """table: list of (max_m, q, g) rows in increasing max_m (the batched form's shape table)."""
+        mm = np.array([r[0] for r in table], np.int32)
+        q = np.array([r[1] for r in table], np.int32)
+        g = np.array([r[2] for r in table], np.int32)
+        self._chk(self._L.sx_set_shapes(self._h, stage, len(table), _ptr(mm), _ptr(q), _ptr(g)))
+
+    def shape_for(self, stage: int, m: int, form: int = FORM_BATCHED):
+        q, g = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self._L.sx_shape_for(self._h, stage, form, m, ctypes.byref(q), ctypes.byref(g)))
+        return q.value, g.value

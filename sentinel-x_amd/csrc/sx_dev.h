@@ -1,0 +1,81 @@
+// sx_dev.h -- device-side data layout shared by the HIP kernels and the host runtime.
+//
+// Profiles (configured as in sentinel-x src/hmmer/model.rs:146-245) live in HBM packed
+// back to back:
+//   msc, mo : NCODE x (M+1) floats each, residue-major (msc[x*(M+1)+k]), column 0 = -inf / 0
+//   tsc, tp : 8 x (M+1) floats each, rows MM MI MD IM II DM DD BM (t_xx[k] = edge out of node k)
+// Targets (digitised as alphabet.rs:58-93) are concatenated, each starting on a 16-byte
+// boundary so a group of lanes can fetch 16 residues with one 128-bit load; the buffer is
+// padded by 32 bytes past the last target.  Per target the host precomputes the
+// length-model scalars with glibc (model.rs:284-312, dp.rs:98-100), so device code never
+// evaluates a logarithm on a bit-exact path.
+#pragma once
+#include <stdint.h>
+
+#define SX_NCODE 25
+#define SX_CODE_STAR 24
+#define SX_NT 8
+enum { SX_TMM = 0, SX_TMI, SX_TMD, SX_TIM, SX_TII, SX_TDM, SX_TDD, SX_TBM };
+
+// Per-target length model: MSV tloop/tmove (f32 logf, dp.rs:99-100), Plan7 multihit
+// loop/move in nats (model.rs:286-289) and as probabilities (XscP).
+struct SxLen {
+    float msv_tloop, msv_tmove;
+    float lloop, lmove;
+    float ploop, pmove;
+    float pad0, pad1;
+};
+
+struct SxProfiles {
+    const float* msc;
+    const float* mo;
+    const float* tsc;
+    const float* tp;
+    const int64_t* off;    // per profile: offset of its NCODE x (M+1) block in msc / mo
+    const int64_t* toff;   // per profile: offset of its 8 x (M+1) block in tsc / tp
+    const int32_t* m;
+    const float* msv_tbm;
+};
+
+struct SxTargets {
+    const uint8_t* codes;
+    const int64_t* off;    // start of each target in codes (16-byte aligned)
+    const int32_t* len;
+    const SxLen* lm;
+};
+
+// A tile: one profile against `count` targets taken from tids[first .. first+count).
+// Result j goes to out[out + j].
+struct SxTile {
+    int32_t prof;
+    int32_t count;
+    int64_t first;
+    int64_t out;
+};
+
+// Work list for one launch.  Either explicit tiles, or the implicit all-pairs form:
+// tile t -> profile profs[t / nchunks], targets tids[(t % nchunks) * chunk ...],
+// output row profs[..] * out_stride.
+struct SxWork {
+    const SxTile* tiles;      // explicit tiles, or nullptr
+    const int32_t* profs;     // implicit form: profile ids
+    const int32_t* tids;      // target ids in processing order
+    int64_t ntiles;
+    int32_t nchunks;
+    int32_t chunk;
+    int32_t ntids;
+    int32_t pad;
+    int64_t out_stride;
+    unsigned long long* head; // persistent-queue head; nullptr -> tile = blockIdx.x (+ base)
+    int64_t tile_base;
+};
+
+// Raw scaled special states per row, written for survivors that need the host-side
+// domain definition (pipeline.rs:190-242): xN xB xE xJ xC as stored (scaled) f32
+// values plus the row's cumulative ln scale in f64.  The host applies lnf() + ls with
+// glibc exactly as dp.rs:284-292 / 372-377 do.
+struct SxSpecialsOut {
+    float* x5;        // 5 floats per row, rows 0..L of each pair, at offsets[pair]
+    double* ls;       // 1 double per row
+    const int64_t* row_off;  // per pair (pair = out index) first row offset
+};

@@ -1,0 +1,846 @@
+// sx_kernels.hip -- MI355X (gfx950) kernels for the Plan7 filter cascade of
+// sentinel-x src/hmmer: MSV (dp.rs:92-145), Viterbi score (dp.rs:152-189),
+// scaled-probability Forward (dp.rs:229-298) and Backward (dp.rs:301-392).
+//
+// Decomposition.  A (profile, target) pair is owned by a group of G lanes of one
+// wavefront (G = 1, 2, 4, ..., 64).  Lane j of the group keeps the model positions
+// k = j*Q+1 .. j*Q+Q of the current DP row in registers (Q per lane, fully unrolled), so
+// a group covers Q*G >= M positions; positions past M are padding that the tables make
+// inert (-inf scores, zero probabilities).  Rows (residues) are processed in order; the
+// row-global dependency xB(i) <- xE(i-1) is a group reduction per row (DPP butterflies
+// inside 16-lane rows, v_readlane across rows), the k-1 neighbour at a lane boundary is
+// one DPP shift per row, and the within-row D chain crossing lane boundaries is resolved
+// exactly by lazy-F propagation (Viterbi, max-plus) or by an affine DPP scan
+// (Forward/Backward, sum-product).  A workgroup stages one profile's emission and
+// transition tables in LDS and runs 256/G pairs of that profile at once, one target per
+// group.
+//
+// Exactness.  MSV and Viterbi use only f32 additions and maxima in the reference's
+// per-path order, so every value is bit-identical to the serial loops: rounding is
+// monotone, hence fl(max(a,b)+t) = max(fl(a+t), fl(b+t)), and a max may be taken in any
+// order.  v_max_f32 can differ from the reference's select (a > b ? a : b) only in the
+// sign of a zero result; no NaN or +inf can arise (all inputs are finite or -inf).
+// Forward/Backward reorder sums (FMA, tree reductions), so their contract is an error
+// bound against the FP64 oracle; the row log-scale accumulates in f64.
+#include <hip/hip_runtime.h>
+
+#include "sx_dev.h"
+
+#define SX_NEG_INF (-__builtin_inff())
+#define SX_LN2F 0.693147182464599609375f
+#define SX_RESCALE 1.0e4f
+
+namespace sxk {
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers (wave64, gfx9 DPP)
+// ---------------------------------------------------------------------------
+
+template <int CTRL, int RM = 0xF, int BM = 0xF, bool BC = false>
+__device__ __forceinline__ float dppf(float old, float src) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, RM, BM, BC));
+}
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63); }
+
+// Hides a value's loop invariance from the optimiser.  Used on the lane's LDS column
+// offset so that per-row transition reads stay LDS reads instead of being hoisted into
+// 8*Q registers (which is what we want only for small Q, see HoistT).
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Transition tables are kept in registers across rows when they are small enough.
+template <int Q>
+struct HoistT {
+    static constexpr bool value = Q <= 8;
+};
+
+struct OpMax {
+    __device__ __forceinline__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+struct OpAdd {
+    __device__ __forceinline__ float operator()(float a, float b) const { return a + b; }
+};
+
+// Reduction over the G lanes of a group; every lane of the group receives the same value
+// (each butterfly stage pairs lanes symmetrically and + / max are commutative).
+template <int G, class Op>
+__device__ __forceinline__ float grp_reduce(float v, Op op) {
+    if constexpr (G >= 2) v = op(v, dppf<0xB1>(v, v));   // quad_perm [1,0,3,2]: xor 1
+    if constexpr (G >= 4) v = op(v, dppf<0x4E>(v, v));   // quad_perm [2,3,0,1]: xor 2
+    if constexpr (G >= 8) v = op(v, dppf<0x141>(v, v));  // row_half_mirror
+    if constexpr (G >= 16) v = op(v, dppf<0x140>(v, v)); // row_mirror
+    if constexpr (G == 32) {
+        float a = rdlane(v, 0), b = rdlane(v, 16), c = rdlane(v, 32), d = rdlane(v, 48);
+        v = lane64() < 32 ? op(a, b) : op(c, d);
+    } else if constexpr (G == 64) {
+        float a = rdlane(v, 0), b = rdlane(v, 16), c = rdlane(v, 32), d = rdlane(v, 48);
+        v = op(op(a, b), op(c, d));
+    }
+    return v;
+}
+
+// Lane j of the group receives lane j-1's v; lane 0 receives b.
+template <int G>
+__device__ __forceinline__ float grp_from_prev(float v, float b) {
+    if constexpr (G == 1) {
+        return b;
+    } else if constexpr (G <= 16) {
+        float s = dppf<0x111>(b, v);  // row_shr:1, row lane 0 keeps b
+        if constexpr (G < 16) s = ((lane64() & (G - 1)) == 0) ? b : s;
+        return s;
+    } else {
+        float s = dppf<0x138>(b, v);  // wave_shr:1, lane 0 keeps b
+        if constexpr (G == 32) s = ((lane64() & 31) == 0) ? b : s;
+        return s;
+    }
+}
+
+// Lane j of the group receives lane j+1's v; lane G-1 receives b.
+template <int G>
+__device__ __forceinline__ float grp_from_next(float v, float b) {
+    if constexpr (G == 1) {
+        return b;
+    } else if constexpr (G <= 16) {
+        float s = dppf<0x101>(b, v);  // row_shl:1, row lane 15 keeps b
+        if constexpr (G < 16) s = ((lane64() & (G - 1)) == G - 1) ? b : s;
+        return s;
+    } else {
+        float s = dppf<0x130>(b, v);  // wave_shl:1, lane 63 keeps b
+        if constexpr (G == 32) s = ((lane64() & 31) == 31) ? b : s;
+        return s;
+    }
+}
+
+template <int S>
+struct RowShr;
+template <>
+struct RowShr<1> { static constexpr int c = 0x111; };
+template <>
+struct RowShr<2> { static constexpr int c = 0x112; };
+template <>
+struct RowShr<4> { static constexpr int c = 0x114; };
+template <>
+struct RowShr<8> { static constexpr int c = 0x118; };
+template <int S>
+struct RowShl;
+template <>
+struct RowShl<1> { static constexpr int c = 0x101; };
+template <>
+struct RowShl<2> { static constexpr int c = 0x102; };
+template <>
+struct RowShl<4> { static constexpr int c = 0x104; };
+template <>
+struct RowShl<8> { static constexpr int c = 0x108; };
+
+// One step of an inclusive (Hillis-Steele) scan of affine maps D -> A + B*D over the
+// lanes of a group; lanes with no partner S lanes away combine with the identity (0, 1).
+template <int G, int S, bool FROM_LEFT>
+__device__ __forceinline__ void scan_step(float& A, float& B) {
+    if constexpr (G > S && S < 16) {
+        float a2, b2;
+        if constexpr (FROM_LEFT) {
+            a2 = dppf<RowShr<S>::c>(0.0f, A);
+            b2 = dppf<RowShr<S>::c>(1.0f, B);
+        } else {
+            a2 = dppf<RowShl<S>::c>(0.0f, A);
+            b2 = dppf<RowShl<S>::c>(1.0f, B);
+        }
+        if constexpr (G < 16) {
+            const int jg = lane64() & (G - 1);
+            const bool ok = FROM_LEFT ? (jg >= S) : (jg + S < G);
+            a2 = ok ? a2 : 0.0f;
+            b2 = ok ? b2 : 1.0f;
+        }
+        A = fmaf(B, a2, A);
+        B = B * b2;
+    }
+}
+
+// Inclusive scan over the group of the affine maps (A, B).  FROM_LEFT: lane j ends with
+// the composition of lanes 0..j applied to 0; otherwise lanes j..G-1 applied to 0.  Only
+// A is meaningful afterwards.
+template <int G, bool FROM_LEFT>
+__device__ __forceinline__ void grp_affine_scan(float& A, float B) {
+    scan_step<G, 1, FROM_LEFT>(A, B);
+    scan_step<G, 2, FROM_LEFT>(A, B);
+    scan_step<G, 4, FROM_LEFT>(A, B);
+    scan_step<G, 8, FROM_LEFT>(A, B);
+    if constexpr (G >= 32) {
+        const int row = lane64() >> 4;
+        if constexpr (FROM_LEFT) {
+            // row totals sit in lane 15 of each 16-lane row
+            float a0 = rdlane(A, 15), a1 = rdlane(A, 31), b1 = rdlane(B, 31), a2 = rdlane(A, 47), b2 = rdlane(B, 47);
+            float e;
+            if constexpr (G == 64) {
+                float e1 = a0;
+                float e2 = fmaf(b1, e1, a1);
+                float e3 = fmaf(b2, e2, a2);
+                e = row == 0 ? 0.0f : row == 1 ? e1 : row == 2 ? e2 : e3;
+            } else {
+                e = row == 1 ? a0 : row == 3 ? a2 : 0.0f;
+            }
+            A = fmaf(B, e, A);
+        } else {
+            // suffix totals sit in lane 0 of each row
+            float a1 = rdlane(A, 16), b1 = rdlane(B, 16), a2 = rdlane(A, 32), b2 = rdlane(B, 32), a3 = rdlane(A, 48);
+            float e;
+            if constexpr (G == 64) {
+                float e2 = a3;
+                float e1 = fmaf(b2, e2, a2);
+                float e0 = fmaf(b1, e1, a1);
+                e = row == 0 ? e0 : row == 1 ? e1 : row == 2 ? e2 : 0.0f;
+            } else {
+                e = row == 0 ? a1 : row == 2 ? a3 : 0.0f;
+            }
+            A = fmaf(B, e, A);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// residue streams: one 128-bit load brings 16 residues; the next chunk is prefetched.
+// Targets start 16-byte aligned and the code buffer is padded by 64 bytes on both ends.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t pick_word(const uint4& c, int r) {
+    const int w = r >> 2;
+    return w == 0 ? c.x : w == 1 ? c.y : w == 2 ? c.z : c.w;
+}
+
+struct FwdStream {
+    const uint4* p;
+    uint4 cur, nxt;
+    int r;
+    __device__ __forceinline__ void init(const uint8_t* base) {
+        p = reinterpret_cast<const uint4*>(base);
+        cur = p[0];
+        nxt = p[1];
+        r = 0;
+    }
+    __device__ __forceinline__ int next() {
+        const int x = (int)((pick_word(cur, r) >> ((r & 3) * 8)) & 0xFFu);
+        if (++r == 16) {
+            r = 0;
+            cur = nxt;
+            ++p;
+            nxt = p[1];
+        }
+        return x;
+    }
+};
+
+// Reads positions L-1, L-2, ..., 0 of a target.
+struct RevStream {
+    const uint4* p;
+    uint4 cur, prv;
+    int r;
+    __device__ __forceinline__ void init(const uint8_t* base, int L) {
+        const int last = L - 1;
+        p = reinterpret_cast<const uint4*>(base) + (last >> 4);
+        cur = p[0];
+        prv = p[-1];
+        r = last & 15;
+    }
+    __device__ __forceinline__ int next() {
+        const int x = (int)((pick_word(cur, r) >> ((r & 3) * 8)) & 0xFFu);
+        if (--r < 0) {
+            r = 15;
+            cur = prv;
+            --p;
+            prv = p[-1];
+        }
+        return x;
+    }
+};
+
+// LDS row width for a (Q, G) configuration: columns 0 .. Q*G+1 (the extra column lets the
+// last lane read position k+1), rounded up to an odd count so that lanes reading the same
+// column for different residues hit different banks.
+template <int Q, int G>
+struct Geo {
+    static constexpr int CAP = Q * G;
+    static constexpr int WID = ((CAP + 2) & 1) ? (CAP + 2) : (CAP + 3);
+};
+
+// ---------------------------------------------------------------------------
+// MSV (dp.rs:92-145)
+// ---------------------------------------------------------------------------
+template <int Q, int G>
+__device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tbm, const uint8_t* seq, int L,
+                                          float tloop, float tmove) {
+    constexpr int WID = Geo<Q, G>::WID;
+    const int j = (int)(threadIdx.x & (G - 1));
+    const int kb = j * Q + 1;
+    float row[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) row[q] = SX_NEG_INF;
+    float xn = 0.0f, xb = tmove, xj = SX_NEG_INF, xc = SX_NEG_INF;
+    FwdStream rs;
+    rs.init(seq);
+    for (int i = 0; i < L; i++) {
+        const int x = rs.next();
+        const float* __restrict__ sc = em + x * WID + kb;
+        const float be = xb + tbm;
+        const float left = grp_from_prev<G>(row[Q - 1], SX_NEG_INF);
+        float e = SX_NEG_INF;
+#pragma unroll
+        for (int q = Q - 1; q >= 1; q--) {
+            const float v = sc[q] + fmaxf(row[q - 1], be);
+            row[q] = v;
+            e = fmaxf(e, v);
+        }
+        {
+            const float v = sc[0] + fmaxf(left, be);
+            row[0] = v;
+            e = fmaxf(e, v);
+        }
+        const float xe = grp_reduce<G>(e, OpMax());
+        xj = fmaxf(xj + tloop, xe - SX_LN2F);
+        xc = fmaxf(xc + tloop, xe - SX_LN2F);
+        xn = xn + tloop;
+        xb = fmaxf(xn + tmove, xj + tmove);
+    }
+    return xc + tmove;
+}
+
+// ---------------------------------------------------------------------------
+// Viterbi score (dp.rs:152-189)
+// ---------------------------------------------------------------------------
+template <int Q, int G>
+__device__ __forceinline__ float vit_pair(const float* __restrict__ em, const float* __restrict__ tr,
+                                          const uint8_t* seq, int L, float lloop, float lmove) {
+    constexpr int WID = Geo<Q, G>::WID;
+    const int j = (int)(threadIdx.x & (G - 1));
+    const int kb = j * Q + 1;
+    // tr rows: MM MI MD IM II DM DD BM; Tx(k) = tr[x*WID + k]
+    float mr[Q], ir[Q], dr[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        mr[q] = SX_NEG_INF;
+        ir[q] = SX_NEG_INF;
+        dr[q] = SX_NEG_INF;
+    }
+    float xn = 0.0f, xb = lmove, xj = SX_NEG_INF, xc = SX_NEG_INF;
+    FwdStream rs;
+    rs.init(seq);
+    for (int i = 0; i < L; i++) {
+        const int x = rs.next();
+        const bool ins_ok = x != SX_CODE_STAR;
+        const int kbo = HoistT<Q>::value ? kb : opaque(kb);
+        const float* __restrict__ tmm = tr + SX_TMM * WID + kbo;
+        const float* __restrict__ tmi = tr + SX_TMI * WID + kbo;
+        const float* __restrict__ tmd = tr + SX_TMD * WID + kbo;
+        const float* __restrict__ tim = tr + SX_TIM * WID + kbo;
+        const float* __restrict__ tii = tr + SX_TII * WID + kbo;
+        const float* __restrict__ tdm = tr + SX_TDM * WID + kbo;
+        const float* __restrict__ tdd = tr + SX_TDD * WID + kbo;
+        const float* __restrict__ tbm = tr + SX_TBM * WID + kbo;
+        const float* __restrict__ sc = em + x * WID + kb;
+        const float mL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
+        const float iL = grp_from_prev<G>(ir[Q - 1], SX_NEG_INF);
+        const float dL = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
+        // M and I, descending so that position q-1 still holds the previous row
+#pragma unroll
+        for (int q = Q - 1; q >= 0; q--) {
+            const float mp = q ? mr[q - 1] : mL;
+            const float ip = q ? ir[q - 1] : iL;
+            const float dp = q ? dr[q - 1] : dL;
+            const float s = fmaxf(fmaxf(mp + tmm[q - 1], ip + tim[q - 1]), fmaxf(xb + tbm[q], dp + tdm[q - 1]));
+            const float iv = fmaxf(mr[q] + tmi[q], ir[q] + tii[q]);
+            mr[q] = s + sc[q];
+            ir[q] = ins_ok ? iv : SX_NEG_INF;
+        }
+        // D chain, ascending; the value entering from the left lane is resolved below
+        const float mcL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
+        dr[0] = mcL + tmd[-1];
+#pragma unroll
+        for (int q = 1; q < Q; q++) dr[q] = fmaxf(mr[q - 1] + tmd[q - 1], dr[q - 1] + tdd[q - 1]);
+        if constexpr (G > 1) {
+            // lazy-F: re-run the chain from each lane's true incoming D until nothing changes
+            for (int it = 1; it < G; it++) {
+                const float din = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
+                const float c = din + tdd[-1];
+                if (!__any(c > dr[0])) break;
+                dr[0] = fmaxf(dr[0], c);
+#pragma unroll
+                for (int q = 1; q < Q; q++) dr[q] = fmaxf(dr[q], dr[q - 1] + tdd[q - 1]);
+            }
+        }
+        float e = SX_NEG_INF;
+#pragma unroll
+        for (int q = 0; q < Q; q++) e = fmaxf(e, fmaxf(mr[q], dr[q]));
+        const float xe = grp_reduce<G>(e, OpMax());
+        xj = fmaxf(xj + lloop, xe - SX_LN2F);
+        xc = fmaxf(xc + lloop, xe - SX_LN2F);
+        xn = xn + lloop;
+        xb = fmaxf(xn + lmove, xj + lmove);
+    }
+    return xc + lmove;
+}
+
+// ---------------------------------------------------------------------------
+// Forward (dp.rs:229-298), multihit probabilities e_move = e_loop = 1/2
+// ---------------------------------------------------------------------------
+template <int Q, int G, bool SPEC>
+__device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const float* __restrict__ tr,
+                                           const uint8_t* seq, int L, float ploop, float pmove, float* x5,
+                                           double* lsrow, bool writer) {
+    constexpr int WID = Geo<Q, G>::WID;
+    const int j = (int)(threadIdx.x & (G - 1));
+    const int kb = j * Q + 1;
+    const float* __restrict__ pmm = tr + SX_TMM * WID + kb;
+    const float* __restrict__ pmi = tr + SX_TMI * WID + kb;
+    const float* __restrict__ pmd = tr + SX_TMD * WID + kb;
+    const float* __restrict__ pim = tr + SX_TIM * WID + kb;
+    const float* __restrict__ pii = tr + SX_TII * WID + kb;
+    const float* __restrict__ pdm = tr + SX_TDM * WID + kb;
+    const float* __restrict__ pdd = tr + SX_TDD * WID + kb;
+    const float* __restrict__ pbm = tr + SX_TBM * WID + kb;
+    float blk = 1.0f;  // product of p_dd over the lane's block: the D-chain map's slope
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) blk *= pdd[q - 1];
+    }
+    float mr[Q], ir[Q], dr[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        mr[q] = 0.0f;
+        ir[q] = 0.0f;
+        dr[q] = 0.0f;
+    }
+    float xn = 1.0f, xb = pmove, xj = 0.0f, xc = 0.0f;
+    double lnscale = 0.0;
+    if constexpr (SPEC) {
+        if (writer) {
+            x5[0] = xn; x5[1] = xb; x5[2] = 0.0f; x5[3] = 0.0f; x5[4] = 0.0f;
+            lsrow[0] = 0.0;
+        }
+    }
+    FwdStream rs;
+    rs.init(seq);
+    for (int i = 0; i < L; i++) {
+        const int x = rs.next();
+        const float ins = x != SX_CODE_STAR ? 1.0f : 0.0f;
+        const int kbo = HoistT<Q>::value ? kb : opaque(kb);
+        const float* __restrict__ pmm = tr + SX_TMM * WID + kbo;
+        const float* __restrict__ pmi = tr + SX_TMI * WID + kbo;
+        const float* __restrict__ pmd = tr + SX_TMD * WID + kbo;
+        const float* __restrict__ pim = tr + SX_TIM * WID + kbo;
+        const float* __restrict__ pii = tr + SX_TII * WID + kbo;
+        const float* __restrict__ pdm = tr + SX_TDM * WID + kbo;
+        const float* __restrict__ pdd = tr + SX_TDD * WID + kbo;
+        const float* __restrict__ pbm = tr + SX_TBM * WID + kbo;
+        const float* __restrict__ eo = eot + x * WID + kb;
+        const float mL = grp_from_prev<G>(mr[Q - 1], 0.0f);
+        const float iL = grp_from_prev<G>(ir[Q - 1], 0.0f);
+        const float dL = grp_from_prev<G>(dr[Q - 1], 0.0f);
+#pragma unroll
+        for (int q = Q - 1; q >= 0; q--) {
+            const float mp = q ? mr[q - 1] : mL;
+            const float ip = q ? ir[q - 1] : iL;
+            const float dp = q ? dr[q - 1] : dL;
+            float s = xb * pbm[q];
+            s = fmaf(mp, pmm[q - 1], s);
+            s = fmaf(ip, pim[q - 1], s);
+            s = fmaf(dp, pdm[q - 1], s);
+            const float iv = fmaf(mr[q], pmi[q], ir[q] * pii[q]) * ins;
+            mr[q] = s * eo[q];
+            ir[q] = iv;
+        }
+        const float mcL = grp_from_prev<G>(mr[Q - 1], 0.0f);
+        dr[0] = mcL * pmd[-1];
+#pragma unroll
+        for (int q = 1; q < Q; q++) dr[q] = fmaf(dr[q - 1], pdd[q - 1], mr[q - 1] * pmd[q - 1]);
+        if constexpr (G > 1) {
+            float A = dr[Q - 1];
+            grp_affine_scan<G, true>(A, blk);
+            const float din = grp_from_prev<G>(A, 0.0f);
+            float P = pdd[-1];
+            dr[0] = fmaf(din, P, dr[0]);
+#pragma unroll
+            for (int q = 1; q < Q; q++) {
+                P *= pdd[q - 1];
+                dr[q] = fmaf(din, P, dr[q]);
+            }
+        }
+        float e = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; q++) e += mr[q] + dr[q];
+        float xe = grp_reduce<G>(e, OpAdd());
+        xj = fmaf(xj, ploop, xe * 0.5f);
+        xc = fmaf(xc, ploop, xe * 0.5f);
+        xn = xn * ploop;
+        xb = fmaf(xn, pmove, xj * pmove);
+        if (xe > SX_RESCALE) {
+            const float inv = 1.0f / xe;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                mr[q] *= inv;
+                ir[q] *= inv;
+                dr[q] *= inv;
+            }
+            xn *= inv;
+            xb *= inv;
+            xj *= inv;
+            xc *= inv;
+            lnscale += log((double)xe);
+            xe = 1.0f;
+        }
+        if constexpr (SPEC) {
+            if (writer) {
+                float* o = x5 + 5 * (i + 1);
+                o[0] = xn; o[1] = xb; o[2] = xe; o[3] = xj; o[4] = xc;
+                lsrow[i + 1] = lnscale;
+            }
+        }
+    }
+    const float fin = xc * pmove;
+    return (fin > 0.0f ? (double)logf(fin) : -(double)__builtin_inf()) + lnscale;
+}
+
+// ---------------------------------------------------------------------------
+// Backward (dp.rs:301-392)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double logsum_d(double a, double b) {
+    const double hi = a > b ? a : b, lo = a > b ? b : a;
+    const double d = hi - lo;
+    if (!(d < 30.0)) return hi;
+    return hi + log1p(exp(-d));
+}
+
+template <int Q, int G, bool SPEC>
+__device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const float* __restrict__ tr,
+                                           const uint8_t* seq, int L, float ploop, float pmove, float lloop,
+                                           float lmove, float* x5, double* lsrow, bool writer) {
+    constexpr int WID = Geo<Q, G>::WID;
+    const int j = (int)(threadIdx.x & (G - 1));
+    const int kb = j * Q + 1;
+    const float* __restrict__ pmm = tr + SX_TMM * WID + kb;
+    const float* __restrict__ pmi = tr + SX_TMI * WID + kb;
+    const float* __restrict__ pmd = tr + SX_TMD * WID + kb;
+    const float* __restrict__ pim = tr + SX_TIM * WID + kb;
+    const float* __restrict__ pii = tr + SX_TII * WID + kb;
+    const float* __restrict__ pdm = tr + SX_TDM * WID + kb;
+    const float* __restrict__ pdd = tr + SX_TDD * WID + kb;
+    const float* __restrict__ pbm = tr + SX_TBM * WID + kb;
+    float blk = 1.0f;
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) blk *= pdd[q];
+    }
+    float mr[Q], ir[Q], dr[Q];  // row i+1 values (mn, in, dn), overwritten by row i
+    float xc = pmove;
+    float xe = xc * 0.5f;
+    float xj = 0.0f, xn = 0.0f, xb = 0.0f;
+    // row L: dn[k] = xe + dn[k+1]*p_dd[k], mn[k] = xe + dn[k+1]*p_md[k]
+    {
+        dr[Q - 1] = xe;
+#pragma unroll
+        for (int q = Q - 2; q >= 0; q--) dr[q] = fmaf(dr[q + 1], pdd[q], xe);
+        if constexpr (G > 1) {
+            float A = dr[0];
+            grp_affine_scan<G, false>(A, blk);
+            const float din = grp_from_next<G>(A, 0.0f);
+            float P = pdd[Q - 1];
+            dr[Q - 1] = fmaf(din, P, dr[Q - 1]);
+#pragma unroll
+            for (int q = Q - 2; q >= 0; q--) {
+                P *= pdd[q];
+                dr[q] = fmaf(din, P, dr[q]);
+            }
+        }
+        const float dR = grp_from_next<G>(dr[0], 0.0f);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const float dn1 = q < Q - 1 ? dr[q + 1] : dR;
+            mr[q] = fmaf(dn1, pmd[q], xe);
+            ir[q] = 0.0f;
+        }
+    }
+    double lnscale = 0.0;
+    if constexpr (SPEC) {
+        if (writer) {
+            float* o = x5 + 5 * L;
+            o[0] = 0.0f; o[1] = 0.0f; o[2] = xe; o[3] = 0.0f; o[4] = xc;
+            lsrow[L] = 0.0;
+        }
+    }
+    RevStream rs;
+    rs.init(seq, L);
+    int xnext = rs.next();  // residue L (position L-1)
+    double n1 = -(double)__builtin_inf();
+    for (int i = L - 1; i >= 1; i--) {
+        const int kbo = HoistT<Q>::value ? kb : opaque(kb);
+        const float* __restrict__ pmm = tr + SX_TMM * WID + kbo;
+        const float* __restrict__ pmi = tr + SX_TMI * WID + kbo;
+        const float* __restrict__ pmd = tr + SX_TMD * WID + kbo;
+        const float* __restrict__ pim = tr + SX_TIM * WID + kbo;
+        const float* __restrict__ pii = tr + SX_TII * WID + kbo;
+        const float* __restrict__ pdm = tr + SX_TDM * WID + kbo;
+        const float* __restrict__ pdd = tr + SX_TDD * WID + kbo;
+        const float* __restrict__ pbm = tr + SX_TBM * WID + kbo;
+        const float* __restrict__ eo = eot + xnext * WID + kb;  // emissions of residue i+1
+        const float ins = xnext != SX_CODE_STAR ? 1.0f : 0.0f;
+        float bl = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; q++) bl = fmaf(mr[q] * pbm[q], eo[q], bl);
+        xb = grp_reduce<G>(bl, OpAdd());
+        xj = fmaf(xj, ploop, xb * pmove);
+        xc = xc * ploop;
+        xe = fmaf(xj, 0.5f, xc * 0.5f);
+        xn = fmaf(xn, ploop, xb * pmove);
+        float rowmax = fmaxf(fmaxf(fmaxf(fmaxf(xb, xe), xn), xj), xc);
+        const float mR = grp_from_next<G>(mr[0], 0.0f);  // mn at the next lane's first k
+        // D chain (descending): dc[k] = mnext*p_dm[k] + dc[k+1]*p_dd[k] + xe
+        {
+            const float mnx = mR * eo[Q];
+            dr[Q - 1] = fmaf(mnx, pdm[Q - 1], xe);
+        }
+#pragma unroll
+        for (int q = Q - 2; q >= 0; q--) {
+            const float mnx = mr[q + 1] * eo[q + 1];
+            dr[q] = fmaf(dr[q + 1], pdd[q], fmaf(mnx, pdm[q], xe));
+        }
+        if constexpr (G > 1) {
+            float A = dr[0];
+            grp_affine_scan<G, false>(A, blk);
+            const float din = grp_from_next<G>(A, 0.0f);
+            float P = pdd[Q - 1];
+            dr[Q - 1] = fmaf(din, P, dr[Q - 1]);
+#pragma unroll
+            for (int q = Q - 2; q >= 0; q--) {
+                P *= pdd[q];
+                dr[q] = fmaf(din, P, dr[q]);
+            }
+        }
+        const float dR = grp_from_next<G>(dr[0], 0.0f);
+        float lmax = SX_NEG_INF;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const float mnx = (q < Q - 1 ? mr[q + 1] : mR) * eo[q + 1];
+            const float iv = ir[q] * ins;
+            const float dn1 = q < Q - 1 ? dr[q + 1] : dR;
+            const float mv = fmaf(dn1, pmd[q], fmaf(mnx, pmm[q], iv * pmi[q]) + xe);
+            ir[q] = fmaf(mnx, pim[q], iv * pii[q]);
+            mr[q] = mv;
+            lmax = fmaxf(lmax, mv);
+        }
+        rowmax = fmaxf(rowmax, grp_reduce<G>(lmax, OpMax()));
+        if (rowmax > SX_RESCALE) {
+            const float inv = 1.0f / rowmax;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                mr[q] *= inv;
+                ir[q] *= inv;
+                dr[q] *= inv;
+            }
+            xb *= inv;
+            xe *= inv;
+            xn *= inv;
+            xj *= inv;
+            xc *= inv;
+            lnscale += log((double)rowmax);
+        }
+        if constexpr (SPEC) {
+            if (writer) {
+                float* o = x5 + 5 * i;
+                o[0] = xn; o[1] = xb; o[2] = xe; o[3] = xj; o[4] = xc;
+                lsrow[i] = lnscale;
+            }
+        }
+        if (i == 1) n1 = (xn > 0.0f ? log((double)xn) : -(double)__builtin_inf()) + lnscale;
+        xnext = rs.next();
+    }
+    // row 0: B(0) from residue 1
+    const float* __restrict__ eo = eot + xnext * WID + kb;
+    float bl = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; q++) bl = fmaf(mr[q] * pbm[q], eo[q], bl);
+    const float b0 = grp_reduce<G>(bl, OpAdd());
+    if constexpr (SPEC) {
+        if (writer) {
+            float* o = x5;
+            o[0] = 0.0f; o[1] = b0; o[2] = 0.0f; o[3] = 0.0f; o[4] = 0.0f;
+            lsrow[0] = lnscale;
+        }
+    }
+    const double lb0 = (b0 > 0.0f ? log((double)b0) : -(double)__builtin_inf()) + lnscale;
+    return logsum_d(n1 + (double)lloop, lb0 + (double)lmove);
+}
+
+// ---------------------------------------------------------------------------
+// tile loop: one workgroup = one profile staged in LDS, one target per lane group
+// ---------------------------------------------------------------------------
+enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3 };
+
+struct TileInfo {
+    int32_t prof, count;
+    int64_t first, out;
+    bool by_tid;  // implicit all-pairs form: result index = out + target id
+};
+
+__device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
+    TileInfo ti;
+    if (w.tiles) {
+        const SxTile tl = w.tiles[t];
+        ti.prof = tl.prof;
+        ti.count = tl.count;
+        ti.first = tl.first;
+        ti.out = tl.out;
+        ti.by_tid = false;
+    } else {
+        const int64_t pi = t / w.nchunks;
+        const int c = (int)(t - pi * w.nchunks);
+        ti.prof = w.profs[pi];
+        ti.first = (int64_t)c * w.chunk;
+        const int rem = w.ntids - (int)ti.first;
+        ti.count = rem < w.chunk ? rem : w.chunk;
+        ti.out = (int64_t)ti.prof * w.out_stride;
+        ti.by_tid = true;
+    }
+    return ti;
+}
+
+template <int Q, int G, int STAGE, bool SPEC>
+__global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out, SxSpecialsOut SP) {
+    constexpr int WID = Geo<Q, G>::WID;
+    constexpr int NTAB_E = SX_NCODE * WID;
+    extern __shared__ float lds[];
+    __shared__ long long s_tile;
+    const int gid = (int)(threadIdx.x / G);
+    const int jl = (int)(threadIdx.x & (G - 1));
+    int cur = -1;
+    int64_t t = (int64_t)blockIdx.x + W.tile_base;
+    for (;;) {
+        if (W.head) {
+            if (threadIdx.x == 0) s_tile = (long long)atomicAdd(W.head, 1ull);
+            __syncthreads();
+            t = s_tile;
+        }
+        if (t >= W.ntiles) break;
+        const TileInfo ti = tile_of(W, t);
+        if (ti.prof != cur) {
+            __syncthreads();
+            const int m = P.m[ti.prof];
+            const int w = m + 1;
+            if (STAGE == ST_MSV || STAGE == ST_VIT) {
+                const float* src = P.msc + P.off[ti.prof];
+                for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
+                    const int x = idx / WID, k = idx - x * WID;
+                    lds[idx] = k < w ? src[x * w + k] : SX_NEG_INF;
+                }
+            } else {
+                const float* src = P.mo + P.off[ti.prof];
+                for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
+                    const int x = idx / WID, k = idx - x * WID;
+                    lds[idx] = k < w ? src[x * w + k] : 0.0f;
+                }
+            }
+            if (STAGE != ST_MSV) {
+                const float* src = (STAGE == ST_VIT ? P.tsc : P.tp) + P.toff[ti.prof];
+                const float pad = STAGE == ST_VIT ? SX_NEG_INF : 0.0f;
+                for (int idx = threadIdx.x; idx < SX_NT * WID; idx += blockDim.x) {
+                    const int r = idx / WID, k = idx - r * WID;
+                    lds[NTAB_E + idx] = k < w ? src[r * w + k] : pad;
+                }
+            }
+            cur = ti.prof;
+            __syncthreads();
+        } else if (W.head) {
+            __syncthreads();  // s_tile is rewritten only after every lane has read it
+        }
+        if (gid < ti.count) {
+            const int tid = W.tids[ti.first + gid];
+            const int L = T.len[tid];
+            const uint8_t* seq = T.codes + T.off[tid];
+            const SxLen lm = T.lm[tid];
+            const int64_t o = ti.out + (ti.by_tid ? (int64_t)tid : (int64_t)gid);
+            if (STAGE == ST_MSV) {
+                float r = SX_NEG_INF;
+                if (L > 0) r = msv_pair<Q, G>(lds, P.msv_tbm[ti.prof], seq, L, lm.msv_tloop, lm.msv_tmove);
+                if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
+            } else if (STAGE == ST_VIT) {
+                float r = SX_NEG_INF;
+                if (L > 0) r = vit_pair<Q, G>(lds, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
+                if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
+            } else if (STAGE == ST_FWD) {
+                double r = -(double)__builtin_inf();
+                float* x5 = nullptr;
+                double* lsr = nullptr;
+                if constexpr (SPEC) {
+                    x5 = SP.x5 + 5 * SP.row_off[o];
+                    lsr = SP.ls + SP.row_off[o];
+                }
+                if (L > 0)
+                    r = fwd_pair<Q, G, SPEC>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, x5, lsr, jl == 0);
+                if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
+            } else {
+                double r = -(double)__builtin_inf();
+                float* x5 = nullptr;
+                double* lsr = nullptr;
+                if constexpr (SPEC) {
+                    x5 = SP.x5 + 5 * SP.row_off[o];
+                    lsr = SP.ls + SP.row_off[o];
+                }
+                if (L > 0)
+                    r = bwd_pair<Q, G, SPEC>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, lm.lloop, lm.lmove,
+                                             x5, lsr, jl == 0);
+                if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
+            }
+        }
+        if (!W.head) break;
+    }
+}
+
+}  // namespace sxk
+
+// ---------------------------------------------------------------------------
+// kernel registry
+// ---------------------------------------------------------------------------
+struct SxKernelEntry {
+    int stage;
+    int q;
+    int g;
+    int spec;
+    int wid;
+    const void* fn;
+};
+
+#define SXK_E(ST, Q, G, SP) \
+    {ST, Q, G, SP, sxk::Geo<Q, G>::WID, reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP>)}
+
+#define SXK_ALLG(ST, Q, SP)                                                                     \
+    SXK_E(ST, Q, 1, SP), SXK_E(ST, Q, 2, SP), SXK_E(ST, Q, 4, SP), SXK_E(ST, Q, 8, SP),         \
+        SXK_E(ST, Q, 16, SP), SXK_E(ST, Q, 32, SP), SXK_E(ST, Q, 64, SP)
+
+static const SxKernelEntry k_table[] = {
+    // MSV: one f32 row per lane
+    SXK_ALLG(0, 4, false), SXK_ALLG(0, 8, false), SXK_ALLG(0, 16, false), SXK_ALLG(0, 32, false),
+    SXK_ALLG(0, 64, false), SXK_E(0, 128, 1, false), SXK_E(0, 128, 2, false), SXK_E(0, 160, 1, false),
+    SXK_E(0, 192, 1, false), SXK_E(0, 256, 1, false),
+    // Viterbi, Forward, Backward: three rows per lane
+    SXK_ALLG(1, 4, false), SXK_ALLG(1, 8, false), SXK_ALLG(1, 16, false), SXK_ALLG(1, 32, false),
+    SXK_ALLG(1, 48, false), SXK_E(1, 64, 1, false), SXK_E(1, 64, 2, false), SXK_E(1, 64, 4, false),
+    SXK_ALLG(2, 4, false), SXK_ALLG(2, 8, false), SXK_ALLG(2, 16, false), SXK_ALLG(2, 32, false),
+    SXK_ALLG(2, 48, false), SXK_E(2, 64, 1, false), SXK_E(2, 64, 2, false), SXK_E(2, 64, 4, false),
+    SXK_ALLG(3, 4, false), SXK_ALLG(3, 8, false), SXK_ALLG(3, 16, false), SXK_ALLG(3, 32, false),
+    SXK_ALLG(3, 48, false), SXK_E(3, 64, 1, false), SXK_E(3, 64, 2, false), SXK_E(3, 64, 4, false),
+    // special-row writers for the host-side domain definition
+    SXK_ALLG(2, 4, true), SXK_ALLG(2, 8, true), SXK_ALLG(2, 16, true), SXK_ALLG(2, 32, true),
+    SXK_ALLG(2, 48, true),
+    SXK_ALLG(3, 4, true), SXK_ALLG(3, 8, true), SXK_ALLG(3, 16, true), SXK_ALLG(3, 32, true),
+    SXK_ALLG(3, 48, true),
+};
+
+extern "C" const SxKernelEntry* sx_kernel_table(int* n) {
+    *n = (int)(sizeof(k_table) / sizeof(k_table[0]));
+    return k_table;
+}

@@ -1,0 +1,743 @@
+// sx_runtime.cpp -- device context, data upload, launch planning (plain and batched
+// forms) and the C ABI declared in include/sx_hmmer.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sx_hmmer.h"
+#include "sx_dev.h"
+#include "sx_model.h"
+#include "sx_runtime.h"
+
+struct SxKernelEntry {
+    int stage, q, g, spec, wid;
+    const void* fn;
+};
+extern "C" const SxKernelEntry* sx_kernel_table(int* n);
+
+namespace sx {
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw SxError(SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+static const int BLOCK = 256;
+static const size_t LDS_MAX = 160 * 1024;
+
+template <class T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <class T>
+static T* dupload(const std::vector<T>& v) {
+    T* d = nullptr;
+    const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+    HIPCHK(hipMalloc((void**)&d, bytes));
+    if (!v.empty()) HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+DeviceBuf::~DeviceBuf() { release(); }
+void DeviceBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+void* DeviceBuf::ensure(size_t n) {
+    if (n <= bytes) return p;
+    release();
+    const size_t nb = std::max<size_t>(n, 256);
+    HIPCHK(hipMalloc(&p, nb));
+    bytes = nb;
+    return p;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0ull;
+}
+
+Ctx::Ctx(int dev) : device(dev) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw SxError(SX_ERR_NODEVICE, "no HIP device available");
+    if (dev < 0 || dev >= n) throw SxError(SX_ERR_NODEVICE, "HIP device index out of range");
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreate(&stream));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    n_cu = prop.multiProcessorCount;
+    HIPCHK(hipMalloc((void**)&d_heads, sizeof(unsigned long long) * 64));
+    int nk = 0;
+    const SxKernelEntry* tab = sx_kernel_table(&nk);
+    for (int i = 0; i < nk; i++) {
+        KernelInfo ki;
+        ki.stage = tab[i].stage;
+        ki.q = tab[i].q;
+        ki.g = tab[i].g;
+        ki.spec = tab[i].spec;
+        ki.wid = tab[i].wid;
+        ki.fn = tab[i].fn;
+        kernels[key(ki.stage, ki.q, ki.g, ki.spec)] = ki;
+    }
+}
+
+Ctx::~Ctx() {
+    (void)hipSetDevice(device);
+    free_profiles();
+    free_targets();
+    dfree(d_heads);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Ctx::free_profiles() {
+    dfree(d_msc); dfree(d_mo); dfree(d_tsc); dfree(d_tp);
+    dfree(d_off); dfree(d_toff); dfree(d_m); dfree(d_tbm);
+}
+
+void Ctx::free_targets() {
+    dfree(d_codes); dfree(d_soff); dfree(d_len); dfree(d_lm); dfree(d_sorted); dfree(d_identity);
+}
+
+const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec) const {
+    auto it = kernels.find(key(stage, q, g, spec ? 1 : 0));
+    if (it == kernels.end())
+        throw SxError(SX_ERR_STATE, "no kernel for stage " + std::to_string(stage) + " Q=" + std::to_string(q) +
+                                        " G=" + std::to_string(g));
+    return it->second;
+}
+
+size_t Ctx::lds_bytes(int stage, int wid) { return (size_t)(SX_NCODE + (stage == 0 ? 0 : SX_NT)) * wid * sizeof(float); }
+
+// ---------------------------------------------------------------------------
+// data upload
+// ---------------------------------------------------------------------------
+
+void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n) {
+    free_targets();
+    nt = n;
+    tlen.assign(n, 0);
+    std::vector<int64_t> off(n);
+    std::vector<SxLen> lm(n);
+    size_t pos = 64;  // front pad: the reverse residue stream may read one chunk before a target
+    for (uint32_t i = 0; i < n; i++) {
+        if (offsets[i + 1] < offsets[i]) throw SxError(SX_ERR_ARG, "target offsets must be non-decreasing");
+        const uint64_t L = offsets[i + 1] - offsets[i];
+        if (L > (uint64_t)INT32_MAX / 2) throw SxError(SX_ERR_ARG, "target too long");
+        tlen[i] = (int32_t)L;
+        off[i] = (int64_t)pos;
+        pos += (L + 15) & ~(size_t)15;
+        SxLen& s = lm[i];
+        msv_length(L, &s.msv_tloop, &s.msv_tmove);
+        const Xsc x = Xsc::make(L, true);
+        s.lloop = x.n_loop;
+        s.lmove = x.n_move;
+        s.ploop = x.pn_loop;
+        s.pmove = x.pn_move;
+        s.pad0 = s.pad1 = 0.0f;
+    }
+    pos += 64;  // back pad: prefetch of the chunk after a target's last one
+    std::vector<uint8_t> packed(pos, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        for (int32_t k = 0; k < tlen[i]; k++) {
+            const uint8_t c = codes[offsets[i] + k];
+            if (c >= SX_NCODE) throw SxError(SX_ERR_ARG, "residue code out of range (expected 0..24)");
+            packed[off[i] + k] = c;
+        }
+    }
+    h_toff = off;
+    d_codes = dupload(packed);
+    d_soff = dupload(off);
+    d_len = dupload(tlen);
+    d_lm = dupload(lm);
+    // length-sorted processing order (stable), used by the batched form
+    sorted.resize(n);
+    std::iota(sorted.begin(), sorted.end(), 0);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return tlen[a] < tlen[b]; });
+    d_sorted = dupload(sorted);
+    std::vector<int32_t> ident(n);
+    std::iota(ident.begin(), ident.end(), 0);
+    d_identity = dupload(ident);
+    total_len = 0;
+    for (auto L : tlen) total_len += L;
+}
+
+void Ctx::set_profiles(std::vector<Profile>&& ps) {
+    free_profiles();
+    profs = std::move(ps);
+    std::vector<float> msc, mo, tsc, tp, tbm;
+    std::vector<int64_t> off, toff;
+    std::vector<int32_t> ms;
+    for (const auto& p : profs) {
+        off.push_back((int64_t)msc.size());
+        toff.push_back((int64_t)tsc.size());
+        ms.push_back(p.m);
+        tbm.push_back(p.msv_tbm);
+        msc.insert(msc.end(), p.msc.begin(), p.msc.end());
+        mo.insert(mo.end(), p.mo.begin(), p.mo.end());
+        tsc.insert(tsc.end(), p.tsc.begin(), p.tsc.end());
+        tp.insert(tp.end(), p.tp.begin(), p.tp.end());
+    }
+    d_msc = dupload(msc);
+    d_mo = dupload(mo);
+    d_tsc = dupload(tsc);
+    d_tp = dupload(tp);
+    d_off = dupload(off);
+    d_toff = dupload(toff);
+    d_m = dupload(ms);
+    d_tbm = dupload(tbm);
+}
+
+SxProfiles Ctx::dprof() const {
+    SxProfiles P;
+    P.msc = d_msc; P.mo = d_mo; P.tsc = d_tsc; P.tp = d_tp;
+    P.off = d_off; P.toff = d_toff; P.m = d_m; P.msv_tbm = d_tbm;
+    return P;
+}
+
+SxTargets Ctx::dtarg() const {
+    SxTargets T;
+    T.codes = d_codes; T.off = d_soff; T.len = d_len; T.lm = d_lm;
+    return T;
+}
+
+// ---------------------------------------------------------------------------
+// shapes
+// ---------------------------------------------------------------------------
+
+// Default lane-group shape for a model length when no tuned table exists: the smallest
+// capacity Q*G >= M, preferring fewer lanes per pair (less cross-lane work per row) at
+// equal capacity, among shapes whose LDS tables fit.
+Shape Ctx::default_shape(int stage, int form, int m) const {
+    if (form == SX_FORM_PLAIN) {
+        for (int q : {4, 8, 16, 32, 48})
+            if (q * 64 >= m) return {q, 64};
+        throw SxError(SX_ERR_ARG, "model too long for the plain form (M > 3072)");
+    }
+    Shape best{0, 0};
+    long best_cap = 1L << 40;
+    for (const auto& kv : kernels) {
+        const KernelInfo& k = kv.second;
+        if (k.stage != stage || k.spec) continue;
+        const long cap = (long)k.q * k.g;
+        if (cap < m) continue;
+        if (lds_bytes(stage, k.wid) > LDS_MAX) continue;
+        // three DP rows per lane for Viterbi/Forward/Backward: keep Q modest
+        if (stage != 0 && k.g == 1 && k.q > 32) continue;
+        if (cap < best_cap || (cap == best_cap && k.g < best.g)) {
+            best_cap = cap;
+            best = {k.q, k.g};
+        }
+    }
+    if (!best.q) throw SxError(SX_ERR_ARG, "no kernel shape covers model length " + std::to_string(m));
+    return best;
+}
+
+Shape Ctx::shape_for(int stage, int form, int m) const {
+    if (form == SX_FORM_BATCHED) {
+        auto it = tuned.find(stage);
+        if (it != tuned.end()) {
+            for (const auto& e : it->second)
+                if (m <= e.first) return e.second;
+        }
+    }
+    return default_shape(stage, form, m);
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+
+int Ctx::persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, BLOCK, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    int64_t g = (int64_t)per_cu * n_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
+}
+
+void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
+                 int head_slot) {
+    const size_t lds = lds_bytes(stage, k.wid);
+    if (lds > LDS_MAX) throw SxError(SX_ERR_ARG, "profile tables exceed the LDS budget");
+    if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (persistent) {
+        w.head = d_heads + head_slot;
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, w.head, 1);
+        HIPCHK(hipGetLastError());
+    } else {
+        w.head = nullptr;
+    }
+    SxProfiles P = dprof();
+    SxTargets T = dtarg();
+    void* args[] = {&P, &T, &w, &out, &sp};
+    HIPCHK(hipLaunchKernel(k.fn, dim3(grid), dim3(BLOCK), args, lds, stream));
+}
+
+// All pairs of every profile against every target.  out is a device buffer of P*T
+// elements indexed profile*T + target.
+void Ctx::run_all(int stage, int form, void* dout) {
+    const int P = (int)profs.size();
+    SxSpecialsOut sp{};
+    if (form == SX_FORM_PLAIN) {
+        // one launch per profile, four pairs (one wavefront each) per workgroup, targets
+        // in input order
+        for (int p = 0; p < P; p++) {
+            const Shape s = shape_for(stage, form, profs[p].m);
+            const KernelInfo& k = kernel(stage, s.q, s.g, false);
+            SxWork w{};
+            w.tiles = nullptr;
+            w.profs = d_pidx + p;
+            w.tids = d_identity;
+            w.chunk = BLOCK / s.g;
+            w.ntids = (int32_t)nt;
+            w.nchunks = (int32_t)((nt + w.chunk - 1) / w.chunk);
+            w.ntiles = w.nchunks;
+            w.out_stride = nt;
+            w.tile_base = 0;
+            if (w.ntiles > 0) launch(k, stage, w, dout, sp, (int)w.ntiles, false, 0);
+        }
+        return;
+    }
+    // batched: profiles grouped by shape, one persistent launch per shape over
+    // length-sorted target chunks
+    std::map<std::pair<int, int>, std::vector<int32_t>> groups;
+    for (int p = 0; p < P; p++) {
+        const Shape s = shape_for(stage, form, profs[p].m);
+        groups[{s.q, s.g}].push_back(p);
+    }
+    int slot = 0;
+    pgroups_keep.clear();
+    for (auto& kv : groups) {
+        const int q = kv.first.first, g = kv.first.second;
+        const KernelInfo& k = kernel(stage, q, g, false);
+        pgroups_keep.emplace_back();
+        DeviceBuf& pb = pgroups_keep.back();
+        pb.ensure(kv.second.size() * sizeof(int32_t));
+        HIPCHK(hipMemcpyAsync(pb.p, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              stream));
+        SxWork w{};
+        w.profs = (const int32_t*)pb.p;
+        w.tids = d_sorted;
+        w.chunk = BLOCK / g;
+        w.ntids = (int32_t)nt;
+        w.nchunks = (int32_t)((nt + w.chunk - 1) / w.chunk);
+        w.ntiles = (int64_t)w.nchunks * (int64_t)kv.second.size();
+        w.out_stride = nt;
+        if (w.ntiles == 0) continue;
+        const int grid = persistent_grid(k, lds_bytes(stage, k.wid), w.ntiles);
+        launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
+    }
+}
+
+// Explicit pair list: pairs grouped by profile (and, for the batched form, ordered by
+// target length inside a profile) into tiles of one lane group per pair.
+void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* dout,
+                    bool spec, const std::vector<int64_t>* row_off, float* dx5, double* dls) {
+    std::vector<uint64_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    for (uint64_t i = 0; i < n; i++) {
+        if (prof[i] >= profs.size() || tgt[i] >= nt) throw SxError(SX_ERR_ARG, "pair index out of range");
+    }
+    if (form == SX_FORM_BATCHED)
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+            if (prof[a] != prof[b]) return prof[a] < prof[b];
+            return tlen[tgt[a]] < tlen[tgt[b]];
+        });
+    else
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return prof[a] < prof[b]; });
+    // tiles per shape
+    std::map<std::pair<int, int>, std::vector<SxTile>> tiles;
+    std::vector<int32_t> tids(n);
+    std::vector<int64_t> roff;
+    for (uint64_t i = 0; i < n; i++) tids[i] = (int32_t)tgt[order[i]];
+    uint64_t i = 0;
+    while (i < n) {
+        const uint32_t p = prof[order[i]];
+        uint64_t e = i;
+        while (e < n && prof[order[e]] == p) e++;
+        const Shape s = spec ? shape_for_spec(stage, profs[p].m) : shape_for(stage, form, profs[p].m);
+        const int ng = BLOCK / s.g;
+        for (uint64_t a = i; a < e; a += ng) {
+            SxTile t;
+            t.prof = (int32_t)p;
+            t.count = (int32_t)std::min<uint64_t>(ng, e - a);
+            t.first = (int64_t)a;
+            t.out = (int64_t)a;
+            tiles[{s.q, s.g}].push_back(t);
+        }
+        i = e;
+    }
+    scratch_tids.ensure(n * sizeof(int32_t));
+    HIPCHK(hipMemcpyAsync(scratch_tids.p, tids.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    SxSpecialsOut sp{};
+    if (spec) {
+        roff.resize(n);
+        for (uint64_t a = 0; a < n; a++) roff[a] = (*row_off)[order[a]];
+        scratch_roff.ensure(n * sizeof(int64_t));
+        HIPCHK(hipMemcpyAsync(scratch_roff.p, roff.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+        sp.x5 = dx5;
+        sp.ls = dls;
+        sp.row_off = (const int64_t*)scratch_roff.p;
+    }
+    // results come back in the sorted order; remember it for the caller
+    last_order = order;
+    size_t ttot = 0;
+    for (auto& kv : tiles) ttot += kv.second.size();
+    scratch_tiles.ensure(ttot * sizeof(SxTile));
+    size_t toff = 0;
+    int slot = 0;
+    for (auto& kv : tiles) {
+        const int q = kv.first.first, g = kv.first.second;
+        const KernelInfo& k = kernel(stage, q, g, spec);
+        SxTile* dt = (SxTile*)scratch_tiles.p + toff;
+        HIPCHK(hipMemcpyAsync(dt, kv.second.data(), kv.second.size() * sizeof(SxTile), hipMemcpyHostToDevice, stream));
+        toff += kv.second.size();
+        SxWork w{};
+        w.tiles = dt;
+        w.tids = (const int32_t*)scratch_tids.p;
+        w.ntiles = (int64_t)kv.second.size();
+        if (form == SX_FORM_BATCHED) {
+            const int grid = persistent_grid(k, lds_bytes(stage, k.wid), w.ntiles);
+            launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
+        } else {
+            launch(k, stage, w, dout, sp, (int)w.ntiles, false, 0);
+        }
+    }
+}
+
+Shape Ctx::shape_for_spec(int stage, int m) const {
+    // special-row writers exist for Q in {4..48} and every G
+    for (int g : {1, 2, 4, 8, 16, 32, 64})
+        for (int q : {4, 8, 16, 32, 48}) {
+            if ((long)q * g < m) continue;
+            if (g == 1 && q > 32) continue;
+            auto it = kernels.find(key(stage, q, g, 1));
+            if (it == kernels.end()) continue;
+            if (lds_bytes(stage, it->second.wid) > LDS_MAX) continue;
+            return {q, g};
+        }
+    throw SxError(SX_ERR_ARG, "no special-row kernel covers model length " + std::to_string(m));
+}
+
+void Ctx::ensure_pidx() {
+    if (d_pidx_n == profs.size() && d_pidx) return;
+    dfree(d_pidx);
+    std::vector<int32_t> v(profs.size());
+    std::iota(v.begin(), v.end(), 0);
+    d_pidx = dupload(v);
+    d_pidx_n = profs.size();
+}
+
+double Ctx::timed(const std::function<void()>& f) {
+    HIPCHK(hipEventRecord(ev0, stream));
+    f();
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    return ms;
+}
+
+}  // namespace sx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using sx::Ctx;
+using sx::SxError;
+
+struct sx_ctx {
+    Ctx* c = nullptr;
+    std::string err;
+};
+
+template <class F>
+static int guard(sx_ctx* ctx, F&& f) {
+    try {
+        if (ctx && ctx->c) (void)hipSetDevice(ctx->c->device);
+        f();
+        return SX_OK;
+    } catch (const SxError& e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->err = "out of host memory";
+        return SX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return SX_ERR_PARSE;
+    }
+}
+
+static std::string g_create_err;
+
+extern "C" int sx_create(int device, sx_ctx** out) {
+    if (!out) return SX_ERR_ARG;
+    *out = nullptr;
+    try {
+        auto* c = new sx_ctx;
+        try {
+            c->c = new Ctx(device);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+        return SX_OK;
+    } catch (const SxError& e) {
+        g_create_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return SX_ERR_HIP;
+    }
+}
+
+extern "C" void sx_destroy(sx_ctx* ctx) {
+    if (!ctx) return;
+    delete ctx->c;
+    delete ctx;
+}
+
+extern "C" const char* sx_last_error(const sx_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+extern "C" int sx_set_targets(sx_ctx* ctx, const uint8_t* codes, const uint64_t* offsets, uint32_t n) {
+    if (!ctx || (!offsets && n)) return SX_ERR_ARG;
+    return guard(ctx, [&] { ctx->c->set_targets(codes, offsets, n); });
+}
+
+static sx::Hmm hmm_from_c(const sx_hmm& h) {
+    sx::Hmm o;
+    if (h.m <= 0 || !h.mat || !h.t) throw SxError(SX_ERR_ARG, "model needs m > 0, mat and t");
+    o.name = h.name ? h.name : "";
+    if (h.acc) o.acc = std::string(h.acc);
+    if (h.desc) o.desc = std::string(h.desc);
+    o.m = h.m;
+    o.mat.resize((size_t)h.m + 1);
+    o.t.resize((size_t)h.m + 1);
+    for (int k = 0; k <= h.m; k++) {
+        for (int x = 0; x < sx::K; x++) o.mat[k][x] = h.mat[(size_t)k * sx::K + x];
+        for (int j = 0; j < 7; j++) o.t[k][j] = h.t[(size_t)k * 7 + j];
+    }
+    if (h.compo) {
+        std::array<float, sx::K> c;
+        for (int x = 0; x < sx::K; x++) c[x] = h.compo[x];
+        o.compo = c;
+    }
+    if (h.has_ga) o.ga = std::make_pair(h.ga[0], h.ga[1]);
+    if (h.has_tc) o.tc = std::make_pair(h.tc[0], h.tc[1]);
+    if (h.has_nc) o.nc = std::make_pair(h.nc[0], h.nc[1]);
+    o.stats.has_msv = h.has_stats & 1;
+    o.stats.has_vit = (h.has_stats >> 1) & 1;
+    o.stats.has_fwd = (h.has_stats >> 2) & 1;
+    o.stats.msv[0] = h.stats[0];
+    o.stats.msv[1] = h.stats[1];
+    o.stats.vit[0] = h.stats[2];
+    o.stats.vit[1] = h.stats[3];
+    o.stats.fwd[0] = h.stats[4];
+    o.stats.fwd[1] = h.stats[5];
+    return o;
+}
+
+extern "C" int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n) {
+    if (!ctx || (!hmms && n)) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        std::vector<sx::Profile> ps;
+        ps.reserve(n);
+        for (uint32_t i = 0; i < n; i++) ps.push_back(sx::configure(hmm_from_c(hmms[i])));
+        ctx->c->set_profiles(std::move(ps));
+    });
+}
+
+extern "C" int sx_load_hmm_file(sx_ctx* ctx, const char* path, uint32_t* n_loaded) {
+    if (!ctx || !path) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        std::vector<sx::Profile> ps = ctx->c->profs;
+        auto r = sx::HmmReader::open(path);
+        sx::Hmm h;
+        uint32_t k = 0;
+        while (r->next(h)) {
+            ps.push_back(sx::configure(h));
+            k++;
+        }
+        ctx->c->set_profiles(std::move(ps));
+        if (n_loaded) *n_loaded = k;
+    });
+}
+
+extern "C" uint32_t sx_num_profiles(const sx_ctx* ctx) { return ctx ? (uint32_t)ctx->c->profs.size() : 0; }
+extern "C" uint32_t sx_num_targets(const sx_ctx* ctx) { return ctx ? ctx->c->nt : 0; }
+
+extern "C" int sx_profile_tables(const sx_ctx* ctx, uint32_t p, int32_t* m, float* msc, float* tsc, float* mo,
+                                 float* tp, float* tbm) {
+    if (!ctx || p >= ctx->c->profs.size()) return SX_ERR_ARG;
+    const sx::Profile& pr = ctx->c->profs[p];
+    if (m) *m = pr.m;
+    if (msc) std::memcpy(msc, pr.msc.data(), pr.msc.size() * 4);
+    if (tsc) std::memcpy(tsc, pr.tsc.data(), pr.tsc.size() * 4);
+    if (mo) std::memcpy(mo, pr.mo.data(), pr.mo.size() * 4);
+    if (tp) std::memcpy(tp, pr.tp.data(), pr.tp.size() * 4);
+    if (tbm) *tbm = pr.msv_tbm;
+    return SX_OK;
+}
+
+static size_t elem_size(int stage) { return stage >= SX_STAGE_FORWARD ? 8 : 4; }
+
+extern "C" int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double* ms) {
+    if (!ctx || stage < 0 || stage > 3 || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED)) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = *ctx->c;
+        if (c.profs.empty() || c.nt == 0) {
+            if (ms) *ms = 0.0;
+            return;
+        }
+        c.ensure_pidx();
+        const size_t n = c.profs.size() * (size_t)c.nt;
+        void* d = c.scratch_out.ensure(n * elem_size(stage));
+        const double t = c.timed([&] { c.run_all(stage, form, d); });
+        if (ms) *ms = t;
+        if (out) HIPCHK(hipMemcpy(out, d, n * elem_size(stage), hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
+                              void* out, double* ms) {
+    if (!ctx || stage < 0 || stage > 3 || (n && (!prof || !tgt))) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = *ctx->c;
+        if (n == 0) {
+            if (ms) *ms = 0.0;
+            return;
+        }
+        const size_t es = elem_size(stage);
+        void* d = c.scratch_out.ensure(n * es);
+        const double t = c.timed([&] { c.run_pairs(stage, form, prof, tgt, n, d, false, nullptr, nullptr, nullptr); });
+        if (ms) *ms = t;
+        if (out) {
+            std::vector<uint8_t> tmp(n * es);
+            HIPCHK(hipMemcpy(tmp.data(), d, n * es, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < n; i++)
+                std::memcpy((uint8_t*)out + c.last_order[i] * es, tmp.data() + i * es, es);
+        }
+    });
+}
+
+extern "C" int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp,
+                           double* score) {
+    if (!ctx || (stage != SX_STAGE_FORWARD && stage != SX_STAGE_BACKWARD) || (n && (!prof || !tgt))) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = *ctx->c;
+        if (n == 0) return;
+        std::vector<int64_t> roff(n);
+        int64_t rows = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            if (tgt[i] >= c.nt || prof[i] >= c.profs.size()) throw SxError(SX_ERR_ARG, "pair index out of range");
+            roff[i] = rows;
+            rows += c.tlen[tgt[i]] + 1;
+        }
+        void* d = c.scratch_out.ensure(n * 8);
+        float* dx5 = (float*)c.scratch_x5.ensure((size_t)rows * 5 * sizeof(float));
+        double* dls = (double*)c.scratch_ls.ensure((size_t)rows * sizeof(double));
+        c.run_pairs(stage, SX_FORM_BATCHED, prof, tgt, n, d, true, &roff, dx5, dls);
+        HIPCHK(hipStreamSynchronize(c.stream));
+        std::vector<float> x5((size_t)rows * 5);
+        std::vector<double> ls((size_t)rows);
+        std::vector<double> dev(n);
+        HIPCHK(hipMemcpy(x5.data(), dx5, x5.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ls.data(), dls, ls.size() * 8, hipMemcpyDeviceToHost));
+        {
+            std::vector<double> tmp(n);
+            HIPCHK(hipMemcpy(tmp.data(), d, n * 8, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < n; i++) dev[c.last_order[i]] = tmp[i];
+        }
+        // ln of the stored values plus the row scale, with the host libm, as
+        // dp.rs:284-292 (Forward) and dp.rs:372-377, 383-390 (Backward) do
+        auto lnf = [](float x) { return x > 0.0f ? std::log(x) : sx::NEG_INF; };
+        for (uint64_t i = 0; i < n; i++) {
+            const int L = c.tlen[tgt[i]];
+            const float* r = x5.data() + (size_t)roff[i] * 5;
+            const double* lr = ls.data() + roff[i];
+            float* o = sp ? sp + (size_t)roff[i] * 5 : nullptr;
+            if (L == 0) {
+                if (o)
+                    for (int q = 0; q < 5; q++) o[q] = sx::NEG_INF;
+                if (score) score[i] = -INFINITY;
+                continue;
+            }
+            const sx::Xsc xs = sx::Xsc::make((size_t)L, true);
+            if (o) {
+                for (int row = 0; row <= L; row++) {
+                    const float l = (float)lr[row];
+                    for (int q = 0; q < 5; q++) o[(size_t)q * (L + 1) + row] = lnf(r[row * 5 + q]) + l;
+                }
+                if (stage == SX_STAGE_FORWARD) {
+                    o[0] = 0.0f;  // N(0) = ln 1
+                    for (int q = 2; q < 5; q++) o[(size_t)q * (L + 1)] = sx::NEG_INF;
+                } else {
+                    o[(size_t)2 * (L + 1)] = o[(size_t)3 * (L + 1)] = o[(size_t)4 * (L + 1)] = sx::NEG_INF;
+                    o[L] = sx::NEG_INF;                                   // N(L)
+                    o[(size_t)1 * (L + 1) + L] = sx::NEG_INF;             // B(L)
+                    o[(size_t)3 * (L + 1) + L] = sx::NEG_INF;             // J(L)
+                    const float n1 = L > 1 ? o[1] : sx::NEG_INF;
+                    o[0] = sx::logsum(n1 + xs.n_loop, o[(size_t)1 * (L + 1)] + xs.n_move);
+                }
+            }
+            if (score) {
+                if (stage == SX_STAGE_FORWARD) {
+                    const float fin = r[L * 5 + 4] * xs.pc_move;
+                    score[i] = (double)lnf(fin) + lr[L];
+                } else {
+                    score[i] = o ? (double)o[0] : dev[i];
+                }
+            }
+        }
+    });
+}
+
+extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g) {
+    if (!ctx || !q || !g) return SX_ERR_ARG;
+    try {
+        const sx::Shape s = ctx->c->shape_for(stage, form, m);
+        *q = s.q;
+        *g = s.g;
+        return SX_OK;
+    } catch (const SxError& e) {
+        return e.code;
+    }
+}
+
+extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q,
+                             const int32_t* g) {
+    if (!ctx || stage < 0 || stage > 3 || (n && (!max_m || !q || !g))) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = *ctx->c;
+        std::vector<std::pair<int, sx::Shape>> tab;
+        for (uint32_t i = 0; i < n; i++) {
+            const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false);
+            if ((long)q[i] * g[i] < max_m[i]) throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
+            if (Ctx::lds_bytes(stage, k.wid) > 160 * 1024) throw SxError(SX_ERR_ARG, "shape exceeds the LDS budget");
+            if (i && max_m[i] <= max_m[i - 1]) throw SxError(SX_ERR_ARG, "max_m must increase");
+            tab.push_back({max_m[i], sx::Shape{q[i], g[i]}});
+        }
+        if (n == 0)
+            c.tuned.erase(stage);
+        else
+            c.tuned[stage] = tab;
+    });
+}

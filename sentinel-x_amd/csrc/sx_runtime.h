@@ -1,0 +1,105 @@
+// sx_runtime.h -- host-side device context for the MI355X Plan7 search path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <deque>
+#include <functional>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sx_dev.h"
+#include "sx_model.h"
+
+namespace sx {
+
+struct SxError : std::runtime_error {
+    int code;
+    SxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct DeviceBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* ensure(size_t n);
+    void release();
+    DeviceBuf() = default;
+    DeviceBuf(const DeviceBuf&) = delete;
+    DeviceBuf& operator=(const DeviceBuf&) = delete;
+    DeviceBuf(DeviceBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    ~DeviceBuf();
+};
+
+struct KernelInfo {
+    int stage = 0, q = 0, g = 0, spec = 0, wid = 0;
+    const void* fn = nullptr;
+};
+
+struct Shape {
+    int q, g;
+};
+
+struct Ctx {
+    explicit Ctx(int dev);
+    ~Ctx();
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n_cu = 0;
+    std::map<long, KernelInfo> kernels;
+    static long key(int st, int q, int g, int sp) { return ((long)st << 32) | ((long)q << 16) | ((long)g << 1) | sp; }
+    const KernelInfo& kernel(int stage, int q, int g, bool spec) const;
+    static size_t lds_bytes(int stage, int wid);
+
+    // profiles
+    std::vector<Profile> profs;
+    float *d_msc = nullptr, *d_mo = nullptr, *d_tsc = nullptr, *d_tp = nullptr, *d_tbm = nullptr;
+    int64_t *d_off = nullptr, *d_toff = nullptr;
+    int32_t* d_m = nullptr;
+    int32_t* d_pidx = nullptr;
+    size_t d_pidx_n = 0;
+    void set_profiles(std::vector<Profile>&& ps);
+    void free_profiles();
+    void ensure_pidx();
+
+    // targets
+    uint32_t nt = 0;
+    std::vector<int32_t> tlen;
+    std::vector<int64_t> h_toff;
+    std::vector<int32_t> sorted;
+    int64_t total_len = 0;
+    uint8_t* d_codes = nullptr;
+    int64_t* d_soff = nullptr;
+    int32_t* d_len = nullptr;
+    SxLen* d_lm = nullptr;
+    int32_t* d_sorted = nullptr;
+    int32_t* d_identity = nullptr;
+    void set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n);
+    void free_targets();
+
+    SxProfiles dprof() const;
+    SxTargets dtarg() const;
+
+    // shapes: tuned[stage] = sorted (max M, shape) table
+    std::map<int, std::vector<std::pair<int, Shape>>> tuned;
+    Shape default_shape(int stage, int form, int m) const;
+    Shape shape_for(int stage, int form, int m) const;
+    Shape shape_for_spec(int stage, int m) const;
+
+    // launches
+    unsigned long long* d_heads = nullptr;
+    DeviceBuf scratch_out, scratch_tids, scratch_tiles, scratch_roff, scratch_x5, scratch_ls;
+    std::deque<DeviceBuf> pgroups_keep;
+    std::vector<uint64_t> last_order;
+    int persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const;
+    void launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
+                int head_slot);
+    void run_all(int stage, int form, void* dout);
+    void run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* dout, bool spec,
+                   const std::vector<int64_t>* row_off, float* dx5, double* dls);
+    double timed(const std::function<void()>& f);
+};
+
+}  // namespace sx

@@ -42,3 +42,29 @@ def pair_targets(orc):
             elif names:
                 seqs[-1] += line.strip().encode()
     return names, [orc.encode(s) for s in seqs]
+
+
+def load_package():
+    """Import the sentinel-x_amd package (its directory name is not a Python identifier)."""
+    import importlib.util
+    if "sentinel_x_amd" in sys.modules:
+        return sys.modules["sentinel_x_amd"]
+    pkg_dir = os.path.join(ROOT, "sentinel-x_amd")
+    spec = importlib.util.spec_from_file_location("sentinel_x_amd", os.path.join(pkg_dir, "__init__.py"),
+                                                  submodule_search_locations=[pkg_dir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["sentinel_x_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def sxa():
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def gctx(sxa):
+    ctx = sxa.Context(0)
+    yield ctx
+    ctx.close()
