@@ -206,32 +206,33 @@ __device__ __forceinline__ void grp_affine_scan(float& A, float B) {
 }
 
 // ---------------------------------------------------------------------------
-// residue streams: one 128-bit load brings 16 residues; the next chunk is prefetched.
-// Targets start 16-byte aligned and the code buffer is padded by 64 bytes on both ends.
+// residue streams: dword loads with a two-dword prefetch ring; a residue is one byte
+// taken from the current dword by a shift (no run-time selection among vector words:
+// hipcc for gfx950 miscompiled a four-way select over the components of a uint4 here,
+// returning the first component for the fourth).  Targets start 16-byte aligned and the
+// code buffer is padded by 64 bytes on both ends, so the prefetch may read past a
+// target's first or last dword.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t pick_word(const uint4& c, int r) {
-    const int w = r >> 2;
-    return w == 0 ? c.x : w == 1 ? c.y : w == 2 ? c.z : c.w;
-}
-
 struct FwdStream {
-    const uint4* p;
-    uint4 cur, nxt;
+    const uint32_t* p;
+    uint32_t cur, n1, n2;
     int r;
     __device__ __forceinline__ void init(const uint8_t* base) {
-        p = reinterpret_cast<const uint4*>(base);
+        p = reinterpret_cast<const uint32_t*>(base);
         cur = p[0];
-        nxt = p[1];
+        n1 = p[1];
+        n2 = p[2];
         r = 0;
     }
     __device__ __forceinline__ int next() {
-        const int x = (int)((pick_word(cur, r) >> ((r & 3) * 8)) & 0xFFu);
-        if (++r == 16) {
+        const int x = (int)((cur >> (r * 8)) & 0xFFu);
+        if (++r == 4) {
             r = 0;
-            cur = nxt;
+            cur = n1;
+            n1 = n2;
+            n2 = p[3];
             ++p;
-            nxt = p[1];
         }
         return x;
     }
@@ -239,23 +240,25 @@ struct FwdStream {
 
 // Reads positions L-1, L-2, ..., 0 of a target.
 struct RevStream {
-    const uint4* p;
-    uint4 cur, prv;
+    const uint32_t* p;
+    uint32_t cur, n1, n2;
     int r;
     __device__ __forceinline__ void init(const uint8_t* base, int L) {
         const int last = L - 1;
-        p = reinterpret_cast<const uint4*>(base) + (last >> 4);
+        p = reinterpret_cast<const uint32_t*>(base) + (last >> 2);
         cur = p[0];
-        prv = p[-1];
-        r = last & 15;
+        n1 = p[-1];
+        n2 = p[-2];
+        r = last & 3;
     }
     __device__ __forceinline__ int next() {
-        const int x = (int)((pick_word(cur, r) >> ((r & 3) * 8)) & 0xFFu);
+        const int x = (int)((cur >> (r * 8)) & 0xFFu);
         if (--r < 0) {
-            r = 15;
-            cur = prv;
+            r = 3;
+            cur = n1;
+            n1 = n2;
+            n2 = p[-3];
             --p;
-            prv = p[-1];
         }
         return x;
     }
