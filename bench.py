@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--plain-sample", type=float, default=0.05,
                     help="fraction of the profiles timed in the plain form")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-ms", type=float, default=4000.0, help="autotuner budget (0 = built-in shapes)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -89,6 +90,10 @@ def main():
     ctx.set_profiles(wl.hmms)
     ctx.set_targets(wl.seqs)
     stage = args.stage
+    if args.tune_ms > 0:
+        t0 = time.time()
+        ctx.autotune(stage, args.tune_ms)
+        log(f"[bench] autotuned shapes (max_m, Q, G, registers) in {time.time() - t0:.1f} s: {ctx.get_shapes(stage)}")
 
     for i in range(args.warmup):
         _, ms_w = ctx.score_all(stage, sxa.FORM_BATCHED, want=False)
@@ -106,6 +111,8 @@ def main():
     ctx_p = sxa.Context(0)
     ctx_p.set_profiles(wl.hmms[:n_plain])
     ctx_p.set_targets(wl.seqs)
+    if args.tune_ms > 0:
+        ctx_p.set_shapes(stage, ctx.get_shapes(stage))
     plain_cells = int(ms[:n_plain].sum()) * int(lens.sum())
     ctx_p.score_all(stage, sxa.FORM_PLAIN, want=False)
     _, ms_plain = ctx_p.score_all(stage, sxa.FORM_PLAIN, want=False)

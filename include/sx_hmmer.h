@@ -118,17 +118,26 @@ typedef struct {
 int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
                    uint64_t* n_out);
 
-/* Choose the lane-group shape (Q positions per lane, G lanes per pair) for each model
- * length from timings on the current profiles and targets; `budget_ms` bounds the
- * time spent.  Without it a static table is used. */
+/* Lane-group shapes.  A pair is scored by a group of G lanes of one wavefront, each lane
+ * holding Q consecutive model positions of the DP row in registers (capacity Q*G >= M);
+ * `hoist` = 1 keeps the profile's transition scores in registers across rows instead of
+ * reading them from LDS.  The batched form looks a model's shape up in a per-stage
+ * table of rows (max_m[i], q[i], g[i], hoist[i]) in increasing max_m; a model of length
+ * M uses the first row with M <= max_m.  These calls have no counterpart in the
+ * reference (a CPU loop has no launch shape).
+ *
+ * sx_autotune: build the table for `stage` by timing every covering shape on the
+ * current profiles and targets, per model-length bucket; budget_ms bounds the time. */
 int sx_autotune(sx_ctx* ctx, int stage, double budget_ms);
-/* Install a shape table for the batched form of `stage`: rows (max_m[i], q[i], g[i])
- * in increasing max_m; a model of length M uses the first row with M <= max_m.  n = 0
- * restores the built-in choice.  Fails if a row names a shape with no kernel or one
- * whose capacity q*g is below max_m. */
-int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q, const int32_t* g);
-/* Shape chosen for a model length: writes q, g. */
-int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g);
+/* Install a table (n = 0 restores the built-in choice).  Fails if a row names a shape
+ * with no kernel, whose capacity q*g is below max_m, or whose tables exceed LDS. */
+int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q, const int32_t* g,
+                  const int32_t* hoist);
+/* Read the installed table: up to cap rows; n receives the row count. */
+int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q, int32_t* g,
+                  int32_t* hoist);
+/* Shape used for a model length (form = SX_FORM_PLAIN or SX_FORM_BATCHED). */
+int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g, int32_t* hoist);
 
 #ifdef __cplusplus
 }

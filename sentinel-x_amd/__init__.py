@@ -25,7 +25,7 @@ STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward"}
 EXPORTS = [
     "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
-    "sx_run_cascade", "sx_autotune", "sx_shape_for", "sx_set_shapes",
+    "sx_run_cascade", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
 ]
 
 
@@ -99,8 +99,10 @@ def lib():
         L.sx_run_cascade.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sx_cascade_stats), vp, u64,
                                      ctypes.POINTER(u64)]
         L.sx_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_double]
-        L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
-        L.sx_set_shapes.argtypes = [vp, ctypes.c_int, u32, vp, vp, vp]
+        L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                   ctypes.POINTER(i32)]
+        L.sx_set_shapes.argtypes = [vp, ctypes.c_int, u32, vp, vp, vp, vp]
+        L.sx_get_shapes.argtypes = [vp, ctypes.c_int, u32, ctypes.POINTER(u32), vp, vp, vp, vp]
         for name in EXPORTS:
             getattr(L, name)  # raises AttributeError if the library lacks a declared entry point
         _lib = L
@@ -260,13 +262,22 @@ class Context:
         self._chk(self._L.sx_autotune(self._h, stage, budget_ms))
 
     def set_shapes(self, stage: int, table):
-        """table: list of (max_m, q, g) rows in increasing max_m (the batched form's shape table)."""
+        """table: rows (max_m, q, g[, hoist]) in increasing max_m (the batched form's shape table)."""
         mm = np.array([r[0] for r in table], np.int32)
         q = np.array([r[1] for r in table], np.int32)
         g = np.array([r[2] for r in table], np.int32)
-        self._chk(self._L.sx_set_shapes(self._h, stage, len(table), _ptr(mm), _ptr(q), _ptr(g)))
+        h = np.array([r[3] if len(r) > 3 else 0 for r in table], np.int32)
+        self._chk(self._L.sx_set_shapes(self._h, stage, len(table), _ptr(mm), _ptr(q), _ptr(g), _ptr(h)))
+
+    def get_shapes(self, stage: int):
+        n = ctypes.c_uint32()
+        cap = 64
+        mm, q, g, h = (np.zeros(cap, np.int32) for _ in range(4))
+        self._chk(self._L.sx_get_shapes(self._h, stage, cap, ctypes.byref(n), _ptr(mm), _ptr(q), _ptr(g), _ptr(h)))
+        k = min(n.value, cap)
+        return [(int(mm[i]), int(q[i]), int(g[i]), int(h[i])) for i in range(k)]
 
     def shape_for(self, stage: int, m: int, form: int = FORM_BATCHED):
-        q, g = ctypes.c_int32(), ctypes.c_int32()
-        self._chk(self._L.sx_shape_for(self._h, stage, form, m, ctypes.byref(q), ctypes.byref(g)))
-        return q.value, g.value
+        q, g, h = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self._L.sx_shape_for(self._h, stage, form, m, ctypes.byref(q), ctypes.byref(g), ctypes.byref(h)))
+        return q.value, g.value, h.value

@@ -50,17 +50,12 @@ __device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63); }
 
 // Hides a value's loop invariance from the optimiser.  Used on the lane's LDS column
 // offset so that per-row transition reads stay LDS reads instead of being hoisted into
-// 8*Q registers (which is what we want only for small Q, see HoistT).
+// 8*Q registers; kernels instantiated with HOIST keep the transitions in registers.
 __device__ __forceinline__ int opaque(int v) {
     asm volatile("" : "+v"(v));
     return v;
 }
 
-// Transition tables are kept in registers across rows when they are small enough.
-template <int Q>
-struct HoistT {
-    static constexpr bool value = Q <= 8;
-};
 
 struct OpMax {
     __device__ __forceinline__ float operator()(float a, float b) const { return fmaxf(a, b); }
@@ -317,7 +312,7 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tb
 // ---------------------------------------------------------------------------
 // Viterbi score (dp.rs:152-189)
 // ---------------------------------------------------------------------------
-template <int Q, int G>
+template <int Q, int G, bool HOIST>
 __device__ __forceinline__ float vit_pair(const float* __restrict__ em, const float* __restrict__ tr,
                                           const uint8_t* seq, int L, float lloop, float lmove) {
     constexpr int WID = Geo<Q, G>::WID;
@@ -337,7 +332,7 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, const fl
     for (int i = 0; i < L; i++) {
         const int x = rs.next();
         const bool ins_ok = x != SX_CODE_STAR;
-        const int kbo = HoistT<Q>::value ? kb : opaque(kb);
+        const int kbo = HOIST ? kb : opaque(kb);
         const float* __restrict__ tmm = tr + SX_TMM * WID + kbo;
         const float* __restrict__ tmi = tr + SX_TMI * WID + kbo;
         const float* __restrict__ tmd = tr + SX_TMD * WID + kbo;
@@ -392,25 +387,18 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, const fl
 // ---------------------------------------------------------------------------
 // Forward (dp.rs:229-298), multihit probabilities e_move = e_loop = 1/2
 // ---------------------------------------------------------------------------
-template <int Q, int G, bool SPEC>
+template <int Q, int G, bool SPEC, bool HOIST>
 __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const float* __restrict__ tr,
                                            const uint8_t* seq, int L, float ploop, float pmove, float* x5,
                                            double* lsrow, bool writer) {
     constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
     const int kb = j * Q + 1;
-    const float* __restrict__ pmm = tr + SX_TMM * WID + kb;
-    const float* __restrict__ pmi = tr + SX_TMI * WID + kb;
-    const float* __restrict__ pmd = tr + SX_TMD * WID + kb;
-    const float* __restrict__ pim = tr + SX_TIM * WID + kb;
-    const float* __restrict__ pii = tr + SX_TII * WID + kb;
-    const float* __restrict__ pdm = tr + SX_TDM * WID + kb;
-    const float* __restrict__ pdd = tr + SX_TDD * WID + kb;
-    const float* __restrict__ pbm = tr + SX_TBM * WID + kb;
     float blk = 1.0f;  // product of p_dd over the lane's block: the D-chain map's slope
     if constexpr (G > 1) {
+        const float* __restrict__ pdd0 = tr + SX_TDD * WID + kb;
 #pragma unroll
-        for (int q = 0; q < Q; q++) blk *= pdd[q - 1];
+        for (int q = 0; q < Q; q++) blk *= pdd0[q - 1];
     }
     float mr[Q], ir[Q], dr[Q];
 #pragma unroll
@@ -432,7 +420,7 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const 
     for (int i = 0; i < L; i++) {
         const int x = rs.next();
         const float ins = x != SX_CODE_STAR ? 1.0f : 0.0f;
-        const int kbo = HoistT<Q>::value ? kb : opaque(kb);
+        const int kbo = HOIST ? kb : opaque(kb);
         const float* __restrict__ pmm = tr + SX_TMM * WID + kbo;
         const float* __restrict__ pmi = tr + SX_TMI * WID + kbo;
         const float* __restrict__ pmd = tr + SX_TMD * WID + kbo;
@@ -513,25 +501,20 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const 
 // Backward (dp.rs:301-392)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double logsum_d(double a, double b) {
+    // dp.rs:32-40 in double; written without NaN compares (kernels build with -fno-honor-nans)
     const double hi = a > b ? a : b, lo = a > b ? b : a;
-    const double d = hi - lo;
-    if (!(d < 30.0)) return hi;
-    return hi + log1p(exp(-d));
+    if (lo == -(double)__builtin_inf() || hi - lo >= 30.0) return hi;
+    return hi + log1p(exp(lo - hi));
 }
 
-template <int Q, int G, bool SPEC>
+template <int Q, int G, bool SPEC, bool HOIST>
 __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const float* __restrict__ tr,
                                            const uint8_t* seq, int L, float ploop, float pmove, float lloop,
                                            float lmove, float* x5, double* lsrow, bool writer) {
     constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
     const int kb = j * Q + 1;
-    const float* __restrict__ pmm = tr + SX_TMM * WID + kb;
-    const float* __restrict__ pmi = tr + SX_TMI * WID + kb;
     const float* __restrict__ pmd = tr + SX_TMD * WID + kb;
-    const float* __restrict__ pim = tr + SX_TIM * WID + kb;
-    const float* __restrict__ pii = tr + SX_TII * WID + kb;
-    const float* __restrict__ pdm = tr + SX_TDM * WID + kb;
     const float* __restrict__ pdd = tr + SX_TDD * WID + kb;
     const float* __restrict__ pbm = tr + SX_TBM * WID + kb;
     float blk = 1.0f;
@@ -581,7 +564,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
     int xnext = rs.next();  // residue L (position L-1)
     double n1 = -(double)__builtin_inf();
     for (int i = L - 1; i >= 1; i--) {
-        const int kbo = HoistT<Q>::value ? kb : opaque(kb);
+        const int kbo = HOIST ? kb : opaque(kb);
         const float* __restrict__ pmm = tr + SX_TMM * WID + kbo;
         const float* __restrict__ pmi = tr + SX_TMI * WID + kbo;
         const float* __restrict__ pmd = tr + SX_TMD * WID + kbo;
@@ -712,7 +695,7 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
     return ti;
 }
 
-template <int Q, int G, int STAGE, bool SPEC>
+template <int Q, int G, int STAGE, bool SPEC, bool HOIST>
 __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out, SxSpecialsOut SP) {
     constexpr int WID = Geo<Q, G>::WID;
     constexpr int NTAB_E = SX_NCODE * WID;
@@ -772,7 +755,7 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_VIT) {
                 float r = SX_NEG_INF;
-                if (L > 0) r = vit_pair<Q, G>(lds, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
+                if (L > 0) r = vit_pair<Q, G, HOIST>(lds, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_FWD) {
                 double r = -(double)__builtin_inf();
@@ -783,7 +766,7 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
                     lsr = SP.ls + SP.row_off[o];
                 }
                 if (L > 0)
-                    r = fwd_pair<Q, G, SPEC>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, x5, lsr, jl == 0);
+                    r = fwd_pair<Q, G, SPEC, HOIST>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, x5, lsr, jl == 0);
                 if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
             } else {
                 double r = -(double)__builtin_inf();
@@ -794,7 +777,7 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
                     lsr = SP.ls + SP.row_off[o];
                 }
                 if (L > 0)
-                    r = bwd_pair<Q, G, SPEC>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, lm.lloop, lm.lmove,
+                    r = bwd_pair<Q, G, SPEC, HOIST>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, lm.lloop, lm.lmove,
                                              x5, lsr, jl == 0);
                 if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
             }
@@ -813,34 +796,40 @@ struct SxKernelEntry {
     int q;
     int g;
     int spec;
+    int hoist;
     int wid;
     const void* fn;
 };
 
-#define SXK_E(ST, Q, G, SP) \
-    {ST, Q, G, SP, sxk::Geo<Q, G>::WID, reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP>)}
+#define SXK_E(ST, Q, G, SP, H) \
+    {ST, Q, G, SP, H, sxk::Geo<Q, G>::WID, reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, H>)}
 
-#define SXK_ALLG(ST, Q, SP)                                                                     \
-    SXK_E(ST, Q, 1, SP), SXK_E(ST, Q, 2, SP), SXK_E(ST, Q, 4, SP), SXK_E(ST, Q, 8, SP),         \
-        SXK_E(ST, Q, 16, SP), SXK_E(ST, Q, 32, SP), SXK_E(ST, Q, 64, SP)
+#define SXK_ALLG(ST, Q, SP, H)                                                                          \
+    SXK_E(ST, Q, 1, SP, H), SXK_E(ST, Q, 2, SP, H), SXK_E(ST, Q, 4, SP, H), SXK_E(ST, Q, 8, SP, H),     \
+        SXK_E(ST, Q, 16, SP, H), SXK_E(ST, Q, 32, SP, H), SXK_E(ST, Q, 64, SP, H)
+
+// Viterbi / Forward / Backward shapes: transitions in registers (HOIST) for Q <= 16, in LDS
+// for Q >= 16; the autotuner picks among them per model length.
+#define SXK_DP3(ST)                                                                             \
+    SXK_ALLG(ST, 4, false, true), SXK_ALLG(ST, 8, false, true), SXK_ALLG(ST, 16, false, true),  \
+        SXK_ALLG(ST, 16, false, false), SXK_ALLG(ST, 32, false, false), SXK_ALLG(ST, 48, false, false), \
+        SXK_E(ST, 64, 1, false, false), SXK_E(ST, 64, 2, false, false), SXK_E(ST, 64, 4, false, false)
 
 static const SxKernelEntry k_table[] = {
     // MSV: one f32 row per lane
-    SXK_ALLG(0, 4, false), SXK_ALLG(0, 8, false), SXK_ALLG(0, 16, false), SXK_ALLG(0, 32, false),
-    SXK_ALLG(0, 64, false), SXK_E(0, 128, 1, false), SXK_E(0, 128, 2, false), SXK_E(0, 160, 1, false),
-    SXK_E(0, 192, 1, false), SXK_E(0, 256, 1, false),
+    SXK_ALLG(0, 4, false, false), SXK_ALLG(0, 8, false, false), SXK_ALLG(0, 16, false, false),
+    SXK_ALLG(0, 32, false, false), SXK_ALLG(0, 64, false, false),
+    SXK_E(0, 48, 1, false, false), SXK_E(0, 96, 1, false, false), SXK_E(0, 128, 1, false, false),
+    SXK_E(0, 160, 1, false, false), SXK_E(0, 192, 1, false, false), SXK_E(0, 256, 1, false, false),
+    SXK_E(0, 48, 2, false, false), SXK_E(0, 96, 2, false, false), SXK_E(0, 128, 2, false, false),
+    SXK_E(0, 48, 4, false, false),
     // Viterbi, Forward, Backward: three rows per lane
-    SXK_ALLG(1, 4, false), SXK_ALLG(1, 8, false), SXK_ALLG(1, 16, false), SXK_ALLG(1, 32, false),
-    SXK_ALLG(1, 48, false), SXK_E(1, 64, 1, false), SXK_E(1, 64, 2, false), SXK_E(1, 64, 4, false),
-    SXK_ALLG(2, 4, false), SXK_ALLG(2, 8, false), SXK_ALLG(2, 16, false), SXK_ALLG(2, 32, false),
-    SXK_ALLG(2, 48, false), SXK_E(2, 64, 1, false), SXK_E(2, 64, 2, false), SXK_E(2, 64, 4, false),
-    SXK_ALLG(3, 4, false), SXK_ALLG(3, 8, false), SXK_ALLG(3, 16, false), SXK_ALLG(3, 32, false),
-    SXK_ALLG(3, 48, false), SXK_E(3, 64, 1, false), SXK_E(3, 64, 2, false), SXK_E(3, 64, 4, false),
+    SXK_DP3(1), SXK_DP3(2), SXK_DP3(3),
     // special-row writers for the host-side domain definition
-    SXK_ALLG(2, 4, true), SXK_ALLG(2, 8, true), SXK_ALLG(2, 16, true), SXK_ALLG(2, 32, true),
-    SXK_ALLG(2, 48, true),
-    SXK_ALLG(3, 4, true), SXK_ALLG(3, 8, true), SXK_ALLG(3, 16, true), SXK_ALLG(3, 32, true),
-    SXK_ALLG(3, 48, true),
+    SXK_ALLG(2, 4, true, true), SXK_ALLG(2, 8, true, true), SXK_ALLG(2, 16, true, false),
+    SXK_ALLG(2, 32, true, false), SXK_ALLG(2, 48, true, false),
+    SXK_ALLG(3, 4, true, true), SXK_ALLG(3, 8, true, true), SXK_ALLG(3, 16, true, false),
+    SXK_ALLG(3, 32, true, false), SXK_ALLG(3, 48, true, false),
 };
 
 extern "C" const SxKernelEntry* sx_kernel_table(int* n) {

@@ -6,5 +6,3 @@
 extern "C" int sx_run_cascade(sx_ctx*, int, sx_cascade_stats*, sx_pair_scores*, uint64_t, uint64_t*) {
     return SX_ERR_STATE;
 }
-
-extern "C" int sx_autotune(sx_ctx*, int, double) { return SX_ERR_STATE; }

@@ -17,7 +17,7 @@
 #include "sx_runtime.h"
 
 struct SxKernelEntry {
-    int stage, q, g, spec, wid;
+    int stage, q, g, spec, hoist, wid;
     const void* fn;
 };
 extern "C" const SxKernelEntry* sx_kernel_table(int* n);
@@ -89,9 +89,10 @@ Ctx::Ctx(int dev) : device(dev) {
         ki.q = tab[i].q;
         ki.g = tab[i].g;
         ki.spec = tab[i].spec;
+        ki.hoist = tab[i].hoist;
         ki.wid = tab[i].wid;
         ki.fn = tab[i].fn;
-        kernels[key(ki.stage, ki.q, ki.g, ki.spec)] = ki;
+        kernels[key(ki.stage, ki.q, ki.g, ki.spec, ki.hoist)] = ki;
     }
 }
 
@@ -114,11 +115,11 @@ void Ctx::free_targets() {
     dfree(d_codes); dfree(d_soff); dfree(d_len); dfree(d_lm); dfree(d_sorted); dfree(d_identity);
 }
 
-const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec) const {
-    auto it = kernels.find(key(stage, q, g, spec ? 1 : 0));
+const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec, bool hoist) const {
+    auto it = kernels.find(key(stage, q, g, spec ? 1 : 0, hoist ? 1 : 0));
     if (it == kernels.end())
         throw SxError(SX_ERR_STATE, "no kernel for stage " + std::to_string(stage) + " Q=" + std::to_string(q) +
-                                        " G=" + std::to_string(g));
+                                        " G=" + std::to_string(g) + (hoist ? " (registers)" : " (LDS)"));
     return it->second;
 }
 
@@ -220,16 +221,17 @@ SxTargets Ctx::dtarg() const {
 // shapes
 // ---------------------------------------------------------------------------
 
-// Default lane-group shape for a model length when no tuned table exists: the smallest
-// capacity Q*G >= M, preferring fewer lanes per pair (less cross-lane work per row) at
-// equal capacity, among shapes whose LDS tables fit.
+// Built-in lane-group shape for a model length when no tuned table is installed: the
+// smallest capacity Q*G >= M among shapes whose LDS tables fit, preferring (at equal
+// capacity) register-resident transitions and then fewer lanes per pair.  Viterbi,
+// Forward and Backward avoid Q > 16 per lane, whose three register rows cut occupancy.
 Shape Ctx::default_shape(int stage, int form, int m) const {
     if (form == SX_FORM_PLAIN) {
         for (int q : {4, 8, 16, 32, 48})
-            if (q * 64 >= m) return {q, 64};
+            if (q * 64 >= m) return {q, 64, stage != 0 && q <= 8};
         throw SxError(SX_ERR_ARG, "model too long for the plain form (M > 3072)");
     }
-    Shape best{0, 0};
+    Shape best{0, 0, false};
     long best_cap = 1L << 40;
     for (const auto& kv : kernels) {
         const KernelInfo& k = kv.second;
@@ -237,11 +239,13 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
         const long cap = (long)k.q * k.g;
         if (cap < m) continue;
         if (lds_bytes(stage, k.wid) > LDS_MAX) continue;
-        // three DP rows per lane for Viterbi/Forward/Backward: keep Q modest
-        if (stage != 0 && k.g == 1 && k.q > 32) continue;
-        if (cap < best_cap || (cap == best_cap && k.g < best.g)) {
+        if (stage != 0 && k.q > 16 && cap < 1024) continue;
+        const bool better = cap < best_cap ||
+                            (cap == best_cap && (k.hoist > (int)best.hoist ||
+                                                 (k.hoist == (int)best.hoist && k.g < best.g)));
+        if (better) {
             best_cap = cap;
-            best = {k.q, k.g};
+            best = {k.q, k.g, k.hoist != 0};
         }
     }
     if (!best.q) throw SxError(SX_ERR_ARG, "no kernel shape covers model length " + std::to_string(m));
@@ -250,6 +254,7 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
 
 Shape Ctx::shape_for(int stage, int form, int m) const {
     if (form == SX_FORM_BATCHED) {
+        if (forced.count(stage)) return forced.at(stage);
         auto it = tuned.find(stage);
         if (it != tuned.end()) {
             for (const auto& e : it->second)
@@ -257,6 +262,20 @@ Shape Ctx::shape_for(int stage, int form, int m) const {
         }
     }
     return default_shape(stage, form, m);
+}
+
+Shape Ctx::shape_for_spec(int stage, int m) const {
+    // special-row writers: Q in {4, 8} with register-resident transitions, {16, 32, 48} in LDS
+    for (int g : {1, 2, 4, 8, 16, 32, 64})
+        for (int q : {4, 8, 16, 32, 48}) {
+            if ((long)q * g < m) continue;
+            if (g == 1 && q > 16) continue;
+            auto it = kernels.find(key(stage, q, g, 1, q <= 8));
+            if (it == kernels.end()) continue;
+            if (lds_bytes(stage, it->second.wid) > LDS_MAX) continue;
+            return {q, g, q <= 8};
+        }
+    throw SxError(SX_ERR_ARG, "no special-row kernel covers model length " + std::to_string(m));
 }
 
 // ---------------------------------------------------------------------------
@@ -289,43 +308,40 @@ void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecials
     HIPCHK(hipLaunchKernel(k.fn, dim3(grid), dim3(BLOCK), args, lds, stream));
 }
 
-// All pairs of every profile against every target.  out is a device buffer of P*T
-// elements indexed profile*T + target.
-void Ctx::run_all(int stage, int form, void* dout) {
-    const int P = (int)profs.size();
+// Implicit all-pairs launches for a set of profiles against a target-id list (device
+// array of ntids ids).  Results go to out[profile * out_stride + target].
+void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, const int32_t* dtids, int ntids,
+                       void* dout, int64_t out_stride) {
     SxSpecialsOut sp{};
     if (form == SX_FORM_PLAIN) {
-        // one launch per profile, four pairs (one wavefront each) per workgroup, targets
-        // in input order
-        for (int p = 0; p < P; p++) {
+        // one launch per profile; each workgroup takes 256/G pairs in input order
+        for (int32_t p : plist) {
             const Shape s = shape_for(stage, form, profs[p].m);
-            const KernelInfo& k = kernel(stage, s.q, s.g, false);
+            const KernelInfo& k = kernel(stage, s.q, s.g, false, s.hoist);
             SxWork w{};
-            w.tiles = nullptr;
             w.profs = d_pidx + p;
-            w.tids = d_identity;
+            w.tids = dtids;
             w.chunk = BLOCK / s.g;
-            w.ntids = (int32_t)nt;
-            w.nchunks = (int32_t)((nt + w.chunk - 1) / w.chunk);
+            w.ntids = ntids;
+            w.nchunks = (ntids + w.chunk - 1) / w.chunk;
             w.ntiles = w.nchunks;
-            w.out_stride = nt;
-            w.tile_base = 0;
+            w.out_stride = out_stride;
             if (w.ntiles > 0) launch(k, stage, w, dout, sp, (int)w.ntiles, false, 0);
         }
         return;
     }
     // batched: profiles grouped by shape, one persistent launch per shape over
     // length-sorted target chunks
-    std::map<std::pair<int, int>, std::vector<int32_t>> groups;
-    for (int p = 0; p < P; p++) {
+    std::map<std::tuple<int, int, int>, std::vector<int32_t>> groups;
+    for (int32_t p : plist) {
         const Shape s = shape_for(stage, form, profs[p].m);
-        groups[{s.q, s.g}].push_back(p);
+        groups[{s.q, s.g, s.hoist ? 1 : 0}].push_back(p);
     }
     int slot = 0;
-    pgroups_keep.clear();
     for (auto& kv : groups) {
-        const int q = kv.first.first, g = kv.first.second;
-        const KernelInfo& k = kernel(stage, q, g, false);
+        const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
+        const bool h = std::get<2>(kv.first) != 0;
+        const KernelInfo& k = kernel(stage, q, g, false, h);
         pgroups_keep.emplace_back();
         DeviceBuf& pb = pgroups_keep.back();
         pb.ensure(kv.second.size() * sizeof(int32_t));
@@ -333,16 +349,25 @@ void Ctx::run_all(int stage, int form, void* dout) {
                               stream));
         SxWork w{};
         w.profs = (const int32_t*)pb.p;
-        w.tids = d_sorted;
+        w.tids = dtids;
         w.chunk = BLOCK / g;
-        w.ntids = (int32_t)nt;
-        w.nchunks = (int32_t)((nt + w.chunk - 1) / w.chunk);
+        w.ntids = ntids;
+        w.nchunks = (ntids + w.chunk - 1) / w.chunk;
         w.ntiles = (int64_t)w.nchunks * (int64_t)kv.second.size();
-        w.out_stride = nt;
+        w.out_stride = out_stride;
         if (w.ntiles == 0) continue;
         const int grid = persistent_grid(k, lds_bytes(stage, k.wid), w.ntiles);
         launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
     }
+}
+
+// All pairs of every profile against every target.  out is a device buffer of P*T
+// elements indexed profile*T + target.
+void Ctx::run_all(int stage, int form, void* dout) {
+    std::vector<int32_t> plist(profs.size());
+    std::iota(plist.begin(), plist.end(), 0);
+    while (pgroups_keep.size() > 256) pgroups_keep.pop_front();
+    run_implicit(stage, form, plist, form == SX_FORM_PLAIN ? d_identity : d_sorted, (int)nt, dout, nt);
 }
 
 // Explicit pair list: pairs grouped by profile (and, for the batched form, ordered by
@@ -361,8 +386,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         });
     else
         std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return prof[a] < prof[b]; });
-    // tiles per shape
-    std::map<std::pair<int, int>, std::vector<SxTile>> tiles;
+    std::map<std::tuple<int, int, int>, std::vector<SxTile>> tiles;
     std::vector<int32_t> tids(n);
     std::vector<int64_t> roff;
     for (uint64_t i = 0; i < n; i++) tids[i] = (int32_t)tgt[order[i]];
@@ -379,7 +403,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
             t.count = (int32_t)std::min<uint64_t>(ng, e - a);
             t.first = (int64_t)a;
             t.out = (int64_t)a;
-            tiles[{s.q, s.g}].push_back(t);
+            tiles[{s.q, s.g, s.hoist ? 1 : 0}].push_back(t);
         }
         i = e;
     }
@@ -403,8 +427,9 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     size_t toff = 0;
     int slot = 0;
     for (auto& kv : tiles) {
-        const int q = kv.first.first, g = kv.first.second;
-        const KernelInfo& k = kernel(stage, q, g, spec);
+        const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
+        const bool h = std::get<2>(kv.first) != 0;
+        const KernelInfo& k = kernel(stage, q, g, spec, h);
         SxTile* dt = (SxTile*)scratch_tiles.p + toff;
         HIPCHK(hipMemcpyAsync(dt, kv.second.data(), kv.second.size() * sizeof(SxTile), hipMemcpyHostToDevice, stream));
         toff += kv.second.size();
@@ -421,18 +446,86 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     }
 }
 
-Shape Ctx::shape_for_spec(int stage, int m) const {
-    // special-row writers exist for Q in {4..48} and every G
-    for (int g : {1, 2, 4, 8, 16, 32, 64})
-        for (int q : {4, 8, 16, 32, 48}) {
-            if ((long)q * g < m) continue;
-            if (g == 1 && q > 32) continue;
-            auto it = kernels.find(key(stage, q, g, 1));
-            if (it == kernels.end()) continue;
-            if (lds_bytes(stage, it->second.wid) > LDS_MAX) continue;
-            return {q, g};
+// ---------------------------------------------------------------------------
+// autotuner: for each model-length bucket present in the profile set, time every
+// kernel shape that covers the bucket's longest model on a sample of that bucket's
+// profiles against a length-stratified sample of the targets, and keep the fastest.
+// ---------------------------------------------------------------------------
+std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
+    static const int edges[] = {8, 16, 24, 32, 48, 64, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768,
+                                1024, 1280, 1536, 2048, 2560, 3072, 4096};
+    std::vector<TuneRecord> log;
+    if (profs.empty() || nt == 0) return log;
+    ensure_pidx();
+    // target sample: every k-th id of the length-sorted order, at most 16384 ids
+    const int nts = (int)std::min<uint32_t>(nt, 16384);
+    std::vector<int32_t> ts(nts);
+    int64_t sum_l = 0;
+    for (int i = 0; i < nts; i++) {
+        ts[i] = sorted[(size_t)((double)i * nt / nts)];
+        sum_l += tlen[ts[i]];
+    }
+    DeviceBuf dts;
+    dts.ensure(ts.size() * sizeof(int32_t));
+    HIPCHK(hipMemcpy(dts.p, ts.data(), ts.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    DeviceBuf dout;
+    dout.ensure((size_t)nt * 8);
+    std::vector<std::pair<int, Shape>> table;
+    double spent = 0.0;
+    int lo = 0;
+    for (int hi : edges) {
+        std::vector<int32_t> inb;
+        int mmax = 0;
+        for (size_t p = 0; p < profs.size(); p++)
+            if (profs[p].m > lo && profs[p].m <= hi) {
+                inb.push_back((int32_t)p);
+                mmax = std::max(mmax, profs[p].m);
+            }
+        lo = hi;
+        if (inb.empty()) continue;
+        // up to 16 profiles, spread over the bucket, repeated if the bucket is small
+        std::vector<int32_t> sample;
+        const int want = (int)std::clamp<int64_t>((int64_t)(4e10 / ((double)mmax * (double)sum_l)), 1, 16);
+        for (int i = 0; i < want; i++) sample.push_back(inb[(size_t)((double)i * inb.size() / want)]);
+        int64_t cells = 0;
+        for (int32_t p : sample) cells += (int64_t)profs[p].m * sum_l;
+        std::vector<Shape> cands;
+        for (const auto& kv : kernels) {
+            const KernelInfo& k = kv.second;
+            if (k.stage != stage || k.spec) continue;
+            const long cap = (long)k.q * k.g;
+            if (cap < mmax || cap > 4L * std::max(mmax, 16)) continue;
+            if (lds_bytes(stage, k.wid) > LDS_MAX) continue;
+            cands.push_back({k.q, k.g, k.hoist != 0});
         }
-    throw SxError(SX_ERR_ARG, "no special-row kernel covers model length " + std::to_string(m));
+        Shape best = default_shape(stage, SX_FORM_BATCHED, mmax);
+        double best_ms = 1e300;
+        for (const Shape& s : cands) {
+            if (spent > budget_ms && best_ms < 1e300) break;
+            forced[stage] = s;
+            try {
+                run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);  // warm-up
+                const double ms = timed([&] {
+                    run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);
+                });
+                spent += 2 * ms;
+                log.push_back({hi, s, ms, cells / ms / 1e6});
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = s;
+                }
+            } catch (...) {
+                forced.erase(stage);
+                throw;
+            }
+            forced.erase(stage);
+        }
+        // a row never claims more than the shape covers; longer models fall through
+        table.push_back({std::min<int>(hi, best.q * best.g), best});
+    }
+    pgroups_keep.clear();
+    tuned[stage] = table;
+    return log;
 }
 
 void Ctx::ensure_pidx() {
@@ -710,12 +803,14 @@ extern "C" int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const u
     });
 }
 
-extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g) {
-    if (!ctx || !q || !g) return SX_ERR_ARG;
+extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g,
+                            int32_t* hoist) {
+    if (!ctx || !q || !g || stage < 0 || stage > 3) return SX_ERR_ARG;
     try {
         const sx::Shape s = ctx->c->shape_for(stage, form, m);
         *q = s.q;
         *g = s.g;
+        if (hoist) *hoist = s.hoist ? 1 : 0;
         return SX_OK;
     } catch (const SxError& e) {
         return e.code;
@@ -723,21 +818,44 @@ extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, i
 }
 
 extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q,
-                             const int32_t* g) {
+                             const int32_t* g, const int32_t* hoist) {
     if (!ctx || stage < 0 || stage > 3 || (n && (!max_m || !q || !g))) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         std::vector<std::pair<int, sx::Shape>> tab;
         for (uint32_t i = 0; i < n; i++) {
-            const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false);
+            const bool h = hoist ? hoist[i] != 0 : false;
+            const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false, h);
             if ((long)q[i] * g[i] < max_m[i]) throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
             if (Ctx::lds_bytes(stage, k.wid) > 160 * 1024) throw SxError(SX_ERR_ARG, "shape exceeds the LDS budget");
             if (i && max_m[i] <= max_m[i - 1]) throw SxError(SX_ERR_ARG, "max_m must increase");
-            tab.push_back({max_m[i], sx::Shape{q[i], g[i]}});
+            tab.push_back({max_m[i], sx::Shape{q[i], g[i], h}});
         }
         if (n == 0)
             c.tuned.erase(stage);
         else
             c.tuned[stage] = tab;
     });
+}
+
+extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q,
+                             int32_t* g, int32_t* hoist) {
+    if (!ctx || !n || stage < 0 || stage > 3) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = *ctx->c;
+        auto it = c.tuned.find(stage);
+        *n = it == c.tuned.end() ? 0 : (uint32_t)it->second.size();
+        if (it == c.tuned.end()) return;
+        for (uint32_t i = 0; i < *n && i < cap; i++) {
+            if (max_m) max_m[i] = it->second[i].first;
+            if (q) q[i] = it->second[i].second.q;
+            if (g) g[i] = it->second[i].second.g;
+            if (hoist) hoist[i] = it->second[i].second.hoist ? 1 : 0;
+        }
+    });
+}
+
+extern "C" int sx_autotune(sx_ctx* ctx, int stage, double budget_ms) {
+    if (!ctx || stage < 0 || stage > 3) return SX_ERR_ARG;
+    return guard(ctx, [&] { (void)ctx->c->autotune(stage, budget_ms); });
 }

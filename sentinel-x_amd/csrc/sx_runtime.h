@@ -5,6 +5,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -32,12 +33,20 @@ struct DeviceBuf {
 };
 
 struct KernelInfo {
-    int stage = 0, q = 0, g = 0, spec = 0, wid = 0;
+    int stage = 0, q = 0, g = 0, spec = 0, hoist = 0, wid = 0;
     const void* fn = nullptr;
 };
 
 struct Shape {
     int q, g;
+    bool hoist;  // transition tables kept in registers across rows (else read from LDS)
+};
+
+struct TuneRecord {
+    int max_m;
+    Shape shape;
+    double ms;
+    double gcups;
 };
 
 struct Ctx {
@@ -49,8 +58,10 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n_cu = 0;
     std::map<long, KernelInfo> kernels;
-    static long key(int st, int q, int g, int sp) { return ((long)st << 32) | ((long)q << 16) | ((long)g << 1) | sp; }
-    const KernelInfo& kernel(int stage, int q, int g, bool spec) const;
+    static long key(int st, int q, int g, int sp, int h) {
+        return ((long)st << 40) | ((long)q << 20) | ((long)g << 2) | ((long)sp << 1) | h;
+    }
+    const KernelInfo& kernel(int stage, int q, int g, bool spec, bool hoist) const;
     static size_t lds_bytes(int stage, int wid);
 
     // profiles
@@ -84,6 +95,8 @@ struct Ctx {
 
     // shapes: tuned[stage] = sorted (max M, shape) table
     std::map<int, std::vector<std::pair<int, Shape>>> tuned;
+    std::map<int, Shape> forced;  // autotuner: every model of the stage uses this shape
+    std::vector<TuneRecord> autotune(int stage, double budget_ms);
     Shape default_shape(int stage, int form, int m) const;
     Shape shape_for(int stage, int form, int m) const;
     Shape shape_for_spec(int stage, int m) const;
@@ -97,6 +110,8 @@ struct Ctx {
     void launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
                 int head_slot);
     void run_all(int stage, int form, void* dout);
+    void run_implicit(int stage, int form, const std::vector<int32_t>& plist, const int32_t* dtids, int ntids,
+                      void* dout, int64_t out_stride);
     void run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* dout, bool spec,
                    const std::vector<int64_t>* row_off, float* dx5, double* dls);
     double timed(const std::function<void()>& f);

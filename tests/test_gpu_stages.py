@@ -13,9 +13,10 @@ pytestmark = pytest.mark.gpu
 
 FWD_TOL = 1e-3  # nats, absolute, against the FP64 oracle
 
-MSV_SHAPES = [(q, g) for q in (4, 8, 16, 32, 64) for g in (1, 2, 4, 8, 16, 32, 64)] + \
+MSV_SHAPES = [(q, g) for q in (4, 8, 16, 32, 64) for g in (1, 2, 4, 8, 16, 32, 64)] + [(48, 1), (96, 1), (48, 2), (96, 2), (48, 4)] + \
     [(128, 1), (128, 2), (160, 1), (192, 1), (256, 1)]
-DP3_SHAPES = [(q, g) for q in (4, 8, 16, 32, 48) for g in (1, 2, 4, 8, 16, 32, 64)] + [(64, 1), (64, 2), (64, 4)]
+DP3_SHAPES = [(q, g, h) for q, h in ((4, 1), (8, 1), (16, 1), (16, 0), (32, 0), (48, 0))
+              for g in (1, 2, 4, 8, 16, 32, 64)] + [(64, 1, 0), (64, 2, 0), (64, 4, 0)]
 EDGE_LENGTHS = [1, 2, 3, 15, 16, 17, 31, 32, 33, 47, 48, 64, 65]
 
 
@@ -168,16 +169,16 @@ def test_every_msv_shape_bit_exact(orc, gctx, q, g):
 
 
 @pytest.mark.parametrize("stage", [1, 2, 3], ids=["viterbi", "forward", "backward"])
-@pytest.mark.parametrize("q,g", DP3_SHAPES, ids=[f"Q{q}G{g}" for q, g in DP3_SHAPES])
-def test_every_dp3_shape(orc, gctx, q, g, stage):
+@pytest.mark.parametrize("q,g,h", DP3_SHAPES, ids=[f"Q{q}G{g}{'R' if h else 'L'}" for q, g, h in DP3_SHAPES])
+def test_every_dp3_shape(orc, gctx, q, g, h, stage):
     if not lds_ok(stage, q, g):
         with pytest.raises(Exception):
-            gctx.set_shapes(stage, [(q * g, q, g)])
+            gctx.set_shapes(stage, [(q * g, q, g, h)])
         return
     hmms, profs, seqs = _shape_case(orc, q, g, 2000 + q * 131 + g)
     gctx.set_profiles(hmms)
     gctx.set_targets(seqs)
-    gctx.set_shapes(stage, [(q * g, q, g)])
+    gctx.set_shapes(stage, [(q * g, q, g, h)])
     got, _ = gctx.score_all(stage, 1)
     gctx.set_shapes(stage, [])
     batch = orc.Batch(profs, seqs)
@@ -199,3 +200,26 @@ def test_empty_target_scores_neg_inf(orc, gctx):
         got, _ = gctx.score_all(st, 1)
         assert np.isneginf(got[0, 0])
         assert np.isfinite(got[0, 1])
+
+
+def test_autotuned_tables_keep_parity(orc, gctx):
+    ms = [20, 60, 150, 150, 300, 700]
+    hmms = [random_hmm(orc, m, 900 + i) for i, m in enumerate(ms)]
+    seqs = make_targets(orc, hmms, 200, 9)
+    profs = [orc.configure(h) for h in hmms]
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    batch = orc.Batch(profs, seqs)
+    for st in (0, 1, 2):
+        gctx.autotune(st, 500.0)
+        table = gctx.get_shapes(st)
+        assert table and table[-1][0] >= max(ms)
+        for mmax, q, g, h in table:
+            assert q * g >= mmax
+    shape = (len(profs), len(seqs))
+    assert np.array_equal(gctx.score_all(0, 1)[0], batch.msv(nthreads=8).reshape(shape))
+    assert np.array_equal(gctx.score_all(1, 1)[0], batch.viterbi(nthreads=8).reshape(shape))
+    f = gctx.score_all(2, 1)[0]
+    assert np.abs(f - batch.forward64(nthreads=8).reshape(shape)).max() <= FWD_TOL
+    for st in range(4):
+        gctx.set_shapes(st, [])
