@@ -92,13 +92,25 @@ int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const
 int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp,
                 double* score);
 
-/* Filter cascade of score_target (pipeline.rs:159-189) over every pair, on the device:
- * MSV on all pairs, bias filter and Viterbi on survivors, Forward on Viterbi
- * survivors.  Thresholds f1, f2, f3 as PipelineConfig (pipeline.rs:27-42).  Profiles
- * must carry STATS (the reference skips the filters otherwise, pipeline.rs:164).
- * Results: one record per pair that reached the bias filter (MSV survivors).
- * stage = 1 rejected by the bias-corrected MSV test, 2 by Viterbi, 3 by Forward,
- * 4 passed every filter. */
+/* Filter cascade of score_target (pipeline.rs:159-189) over every (profile, target)
+ * pair: MSV on all pairs, the bias filter (pipeline.rs:107-150) on MSV survivors,
+ * Viterbi on bias-filter survivors, Forward on Viterbi survivors, with the thresholds
+ * f1, f2, f3 of PipelineConfig (pipeline.rs:27-42).  Every profile must carry STATS
+ * (the reference runs the filters only for calibrated models, pipeline.rs:164);
+ * otherwise SX_ERR_STATE.
+ *
+ * form = SX_FORM_BATCHED keeps the whole chain on the device: survivor lists are
+ * compacted in (model length, target length) order, the next stage's tiles are built
+ * on the device and consumed by persistent kernels from device-resident queues, and
+ * with st->use_graph the chain is captured once in a HIP graph and replayed.
+ * form = SX_FORM_PLAIN runs it one profile at a time (one launch per stage and
+ * profile, survivor tests on the host between launches).
+ *
+ * out receives one record per MSV survivor (pair that reached the bias filter), in
+ * (profile, target) order, up to cap records; *n_out is the total number, which may
+ * exceed cap (then only cap are written).  stage = 1 rejected by the bias-corrected MSV
+ * test, 2 by Viterbi, 3 by Forward, 4 passed every filter; vit / fwd are NaN for
+ * stages not reached. */
 typedef struct {
     uint32_t profile, target;
     float msv, bias, vit;
@@ -109,10 +121,12 @@ typedef struct {
 } sx_pair_scores;
 
 typedef struct {
-    double f1, f2, f3;
-    uint64_t n_pairs, n_msv_pass, n_bias_pass, n_vit_pass, n_fwd_pass;
-    double ms_msv, ms_bias, ms_vit, ms_fwd, ms_total;
-    uint64_t cells_msv, cells_vit, cells_fwd;
+    double f1, f2, f3;          /* in: PipelineConfig thresholds */
+    int32_t use_graph;          /* in: batched form replays a captured HIP graph */
+    int32_t pad;
+    uint64_t n_pairs, n_msv_pass, n_bias_pass, n_vit_pass, n_fwd_pass;  /* out */
+    double ms_msv, ms_bias, ms_vit, ms_fwd, ms_total;                   /* out: device time */
+    uint64_t cells_msv, cells_vit, cells_fwd;                           /* out: DP cells */
 } sx_cascade_stats;
 
 int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,

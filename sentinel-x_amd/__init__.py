@@ -59,7 +59,7 @@ class sx_pair_scores(ctypes.Structure):
 
 class sx_cascade_stats(ctypes.Structure):
     _fields_ = [("f1", ctypes.c_double), ("f2", ctypes.c_double), ("f3", ctypes.c_double),
-                ("n_pairs", ctypes.c_uint64), ("n_msv_pass", ctypes.c_uint64), ("n_bias_pass", ctypes.c_uint64),
+                ("use_graph", ctypes.c_int32), ("pad", ctypes.c_int32), ("n_pairs", ctypes.c_uint64), ("n_msv_pass", ctypes.c_uint64), ("n_bias_pass", ctypes.c_uint64),
                 ("n_vit_pass", ctypes.c_uint64), ("n_fwd_pass", ctypes.c_uint64),
                 ("ms_msv", ctypes.c_double), ("ms_bias", ctypes.c_double), ("ms_vit", ctypes.c_double),
                 ("ms_fwd", ctypes.c_double), ("ms_total", ctypes.c_double),
@@ -248,15 +248,21 @@ class Context:
             o += l + 1
         return out, sc
 
-    def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, cap: Optional[int] = None):
+    def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, use_graph: bool = True,
+                    cap: Optional[int] = None):
+        """Returns (stats, records) for every MSV survivor (see sx_run_cascade)."""
         st = sx_cascade_stats()
         st.f1, st.f2, st.f3 = f1, f2, f3
+        st.use_graph = 1 if use_graph else 0
         if cap is None:
-            cap = max(1024, self.n_profiles * len(self.lengths) // 10)
-        out = np.empty(cap, PAIR_DTYPE)
-        n = ctypes.c_uint64()
-        self._chk(self._L.sx_run_cascade(self._h, form, ctypes.byref(st), _ptr(out), cap, ctypes.byref(n)))
-        return st, out[:n.value].copy()
+            cap = 1 << 16
+        while True:
+            out = np.empty(cap, PAIR_DTYPE)
+            n = ctypes.c_uint64()
+            self._chk(self._L.sx_run_cascade(self._h, form, ctypes.byref(st), _ptr(out), cap, ctypes.byref(n)))
+            if n.value <= cap:
+                return st, out[:n.value].copy()
+            cap = int(n.value)
 
     def autotune(self, stage: int, budget_ms: float = 2000.0):
         self._chk(self._L.sx_autotune(self._h, stage, budget_ms))

@@ -1,0 +1,280 @@
+// sx_cascade.hip -- device side of the filter cascade of score_target
+// (sentinel-x src/hmmer/pipeline.rs:159-189): pass tests, order-preserving survivor
+// compaction, the two-state composition (bias) filter (pipeline.rs:107-150), tile
+// construction for the next stage, and the final record assembly.
+//
+// Pass tests.  The reference rejects a pair when a tail probability of the stage's
+// bit score y = ((score - ref) / ln 2) exceeds a threshold (gumbel_surv for MSV and
+// Viterbi, exp(exp_logsurv) for Forward).  Each of those tails is monotone in y, so
+// the host finds, per profile and with the same libm, the smallest f32 y that passes;
+// the device computes y with the reference's f32 operations and compares.  The
+// decision is therefore the reference's whenever the score is.
+//
+// Survivor lists keep the order (profile position, target length): every list is built
+// by a count / exclusive-scan / write sequence, never by atomic appends, so a stage's
+// tiles are contiguous runs of one profile.  Built without -ffp-contract, so the bias
+// filter's f64 sums round as the reference's; only its log() may differ in the last
+// bit.
+#include <hip/hip_runtime.h>
+
+#include "sx_dev.h"
+
+#define SXC_LN2F 0.693147182464599609375f
+#define SXC_BLOCK 256
+
+namespace sxc {
+
+__device__ __forceinline__ int block_rank(bool pass, int* s_wave, int* total) {
+    // exclusive rank of this thread's pass flag inside the block, and the block total
+    const unsigned long long m = __ballot(pass ? 1 : 0);
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < SXC_BLOCK / 64; w++) {
+        if (w < wave) base += s_wave[w];
+        tot += s_wave[w];
+    }
+    *total = tot;
+    return base + below;
+}
+
+}  // namespace sxc
+
+// MSV test over every pair, enumerated as (profile position pi, length-sorted target j).
+// Phase 0 writes block counts, phase 1 writes the survivors at their scanned offsets.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__ nullsc,
+             const float* __restrict__ thr, const int32_t* __restrict__ perm, const int32_t* __restrict__ sorted,
+             int64_t P, int64_t T, int32_t* counts, const int64_t* offsets, int64_t cap, int32_t* s_p, int32_t* s_pi,
+             int32_t* s_t, float* s_usc) {
+    __shared__ int s_wave[SXC_BLOCK / 64];
+    const int64_t pos = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    bool pass = false;
+    int32_t p = 0, pi = 0, t = 0;
+    float u = 0.0f;
+    if (pos < P * T) {
+        pi = (int32_t)(pos / T);
+        const int64_t j = pos - (int64_t)pi * T;
+        p = perm[pi];
+        t = sorted[j];
+        u = usc[(int64_t)p * T + t];
+        const float y = (u - nullsc[t]) / SXC_LN2F;
+        pass = y >= thr[p];
+    }
+    int tot;
+    const int r = sxc::block_rank(pass, s_wave, &tot);
+    if (phase == 0) {
+        if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+        return;
+    }
+    if (pass) {
+        const int64_t o = offsets[blockIdx.x] + r;
+        if (o < cap) {
+            s_p[o] = p;
+            s_pi[o] = pi;
+            s_t[o] = t;
+            s_usc[o] = u;
+        }
+    }
+}
+
+// Exclusive scan of n int32 counts into n+1 int64 offsets (out[n] = total), one
+// workgroup of 1024 threads; n may be a device value.
+extern "C" __global__ void __launch_bounds__(1024)
+sxc_scan(const int32_t* __restrict__ in, int64_t n_host, const int64_t* n_dev, int64_t* out) {
+    __shared__ int64_t s_part[1024];
+    const int64_t n = n_dev ? *n_dev : n_host;
+    const int tid = (int)threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t a = (int64_t)tid * per, b = a + per < n ? a + per : n;
+    int64_t sum = 0;
+    for (int64_t i = a; i < b; i++) sum += in[i];
+    s_part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = tid >= off ? s_part[tid - off] : 0;
+        __syncthreads();
+        s_part[tid] += v;
+        __syncthreads();
+    }
+    int64_t run = s_part[tid] - sum;
+    for (int64_t i = a; i < b; i++) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (tid == 1023) out[n] = s_part[1023];
+}
+
+// Two-state composition filter (pipeline.rs:107-150) on each MSV survivor, then the
+// bias-corrected MSV test (pipeline.rs:170-173).  eo1[p*25 + x] holds the model-state
+// emission odds computed on the host exactly as the reference (the background state's
+// odds are 1 for every code).
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_bias(const int64_t* __restrict__ n1, const int32_t* __restrict__ s_p, const int32_t* __restrict__ s_t,
+         const float* __restrict__ s_usc, const double* __restrict__ eo1, const int32_t* __restrict__ pm,
+         const float* __restrict__ thr, SxTargets T, float* s_bias, uint8_t* flag) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i >= *n1) return;
+    const int32_t p = s_p[i], t = s_t[i];
+    const uint8_t* seq = T.codes + T.off[t];
+    const int L = T.len[t];
+    const double* e1 = eo1 + (int64_t)p * 25;
+    const double l0 = 400.0;
+    const double l1 = (double)pm[p] / 8.0;
+    const double t00 = l0 / (l0 + 1.0), t01 = 1.0 / (l0 + 1.0);
+    const double t10 = 1.0 / (l1 + 1.0), t11 = l1 / (l1 + 1.0);
+    const double tiny = 2.2250738585072014e-308;
+    double logsc = 0.0;
+    double d0 = 1.0 * 0.999, d1 = e1[seq[0]] * 0.001;
+    double mx = fmax(fmax(d0, d1), tiny);
+    d0 = d0 / mx;
+    d1 = d1 / mx;
+    logsc += log(mx);
+    for (int k = 1; k < L; k++) {
+        const int x = seq[k];
+        const double n0 = (d0 * t00 + d1 * t10) * 1.0;
+        const double nn1 = (d0 * t01 + d1 * t11) * e1[x];
+        mx = fmax(fmax(n0, nn1), tiny);
+        d0 = n0 / mx;
+        d1 = nn1 / mx;
+        logsc += log(mx);
+    }
+    logsc += log(d0 + d1);
+    const double p1 = (double)L / ((double)L + 1.0);
+    const float fs = (float)(logsc + (double)L * log(p1) + log(1.0 - p1));
+    s_bias[i] = fs;
+    const float y = (s_usc[i] - fs) / SXC_LN2F;
+    flag[i] = y >= thr[p] ? 1 : 0;
+}
+
+// Order-preserving compaction of a flagged list: out[k] = src[i] for the k-th set flag.
+// Phase 0 counts per block, phase 1 writes and counts survivors per profile position.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_compact(int phase, const uint8_t* __restrict__ flag, const int64_t* __restrict__ n_dev,
+            const int32_t* __restrict__ src, const int32_t* __restrict__ s1_pi, int32_t* counts,
+            const int64_t* offsets, int32_t* out, int32_t* per_pi) {
+    __shared__ int s_wave[SXC_BLOCK / 64];
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    const bool pass = i < *n_dev && flag[i] != 0;
+    int tot;
+    const int r = sxc::block_rank(pass, s_wave, &tot);
+    if (phase == 0) {
+        if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+        return;
+    }
+    if (pass) {
+        const int32_t s1 = src ? src[i] : (int32_t)i;
+        out[offsets[blockIdx.x] + r] = s1;
+        atomicAdd(&per_pi[s1_pi[s1]], 1);
+    }
+}
+
+// Tiles for a stage: profile position pi gets ceil(cnt[pi] / ng[pi]) tiles of ng[pi]
+// consecutive list entries.  Phase 0 writes the per-position tile counts; phase 1
+// (after both scans) writes the tiles and the stage's target-id list.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_tiles(int phase, int64_t P, const int32_t* __restrict__ cnt, const int32_t* __restrict__ ng,
+          const int32_t* __restrict__ perm, int32_t* tcount, const int64_t* __restrict__ t_off,
+          const int64_t* __restrict__ e_off, SxTile* tiles) {
+    const int64_t pi = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (pi >= P) return;
+    const int c = cnt[pi], g = ng[pi];
+    if (phase == 0) {
+        tcount[pi] = (c + g - 1) / g;
+        return;
+    }
+    int64_t w = t_off[pi];
+    for (int a = 0; a < c; a += g) {
+        SxTile t;
+        t.prof = perm[pi];
+        t.count = c - a < g ? c - a : g;
+        t.first = e_off[pi] + a;
+        t.out = t.first;
+        tiles[w++] = t;
+    }
+}
+
+// Target ids of a list of MSV-survivor indices.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_gather_t(const int64_t* __restrict__ n_dev, const int32_t* __restrict__ list, const int32_t* __restrict__ s_t,
+             int32_t* tids) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i < *n_dev) tids[i] = s_t[list[i]];
+}
+
+// Viterbi test (pipeline.rs:174-178) on the Viterbi scores of list entries.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_vit_pass(const int64_t* __restrict__ n_dev, const int32_t* __restrict__ list, const float* __restrict__ vsc,
+             const float* __restrict__ s_bias, const int32_t* __restrict__ s_p, const float* __restrict__ thr,
+             float* res_vit, int32_t* res_stage, uint8_t* flag) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i >= *n_dev) return;
+    const int32_t s1 = list[i];
+    const float v = vsc[i];
+    const float y = (v - s_bias[s1]) / SXC_LN2F;
+    const bool pass = y >= thr[s_p[s1]];
+    res_vit[s1] = v;
+    res_stage[s1] = pass ? 3 : 2;
+    flag[i] = pass ? 1 : 0;
+}
+
+// Forward test (pipeline.rs:179-183): exp(exp_logsurv(y)) <= F3 passes.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_fwd_pass(const int64_t* __restrict__ n_dev, const int32_t* __restrict__ list, const double* __restrict__ fsc,
+             const float* __restrict__ s_bias, const int32_t* __restrict__ s_p, const float* __restrict__ thr,
+             double* res_fwd, int32_t* res_stage) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i >= *n_dev) return;
+    const int32_t s1 = list[i];
+    const double f = fsc[i];
+    const float ff = (float)f;
+    const float y = (ff - s_bias[s1]) / SXC_LN2F;
+    res_fwd[s1] = f;
+    res_stage[s1] = y >= thr[s_p[s1]] ? 4 : 3;
+}
+
+// Initialise the per-survivor result slots.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_res_init(const int64_t* __restrict__ n_dev, int64_t cap, float* res_vit, double* res_fwd, int32_t* res_stage) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i >= cap) return;
+    res_vit[i] = __builtin_nanf("");
+    res_fwd[i] = __builtin_nan("");
+    res_stage[i] = 1;
+}
+
+// Zero an int32 array (graph-safe: no memset nodes).
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK) sxc_zero_i32(int32_t* a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+struct SxPairRec {
+    uint32_t profile, target;
+    float msv, bias, vit, fwd;
+    double fwd64;
+    int32_t stage, pad;
+};
+
+// One record per MSV survivor, in list order.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_records(const int64_t* __restrict__ n_dev, int64_t cap, const int32_t* __restrict__ s_p,
+            const int32_t* __restrict__ s_t, const float* __restrict__ s_usc, const float* __restrict__ s_bias,
+            const float* __restrict__ res_vit, const double* __restrict__ res_fwd,
+            const int32_t* __restrict__ res_stage, SxPairRec* out) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i >= *n_dev || i >= cap) return;
+    SxPairRec r;
+    r.profile = (uint32_t)s_p[i];
+    r.target = (uint32_t)s_t[i];
+    r.msv = s_usc[i];
+    r.bias = s_bias[i];
+    r.vit = res_vit[i];
+    r.fwd64 = res_fwd[i];
+    r.fwd = (float)res_fwd[i];
+    r.stage = res_stage[i];
+    r.pad = 0;
+    out[i] = r;
+}

@@ -68,6 +68,10 @@ struct SxWork {
     int64_t out_stride;
     unsigned long long* head; // persistent-queue head; nullptr -> tile = blockIdx.x (+ base)
     int64_t tile_base;
+    // Data-dependent tile range (device cascade): when set, the launch covers explicit
+    // tiles [*dyn_lo, *dyn_hi), read from device memory at kernel start.
+    const int64_t* dyn_lo;
+    const int64_t* dyn_hi;
 };
 
 // Raw scaled special states per row, written for survivors that need the host-side

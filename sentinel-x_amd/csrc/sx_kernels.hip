@@ -704,6 +704,8 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
     const int gid = (int)(threadIdx.x / G);
     const int jl = (int)(threadIdx.x & (G - 1));
     int cur = -1;
+    const int64_t tlo = W.dyn_lo ? *W.dyn_lo : 0;
+    const int64_t thi = W.dyn_lo ? *W.dyn_hi : W.ntiles;
     int64_t t = (int64_t)blockIdx.x + W.tile_base;
     for (;;) {
         if (W.head) {
@@ -711,8 +713,8 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
             __syncthreads();
             t = s_tile;
         }
-        if (t >= W.ntiles) break;
-        const TileInfo ti = tile_of(W, t);
+        if (tlo + t >= thi) break;
+        const TileInfo ti = tile_of(W, tlo + t);
         if (ti.prof != cur) {
             __syncthreads();
             const int m = P.m[ti.prof];

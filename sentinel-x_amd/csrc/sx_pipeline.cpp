@@ -1,8 +1,632 @@
-// sx_pipeline.cpp -- device filter cascade and autotuning entry points.
+// sx_pipeline.cpp -- host orchestration of the filter cascade of score_target
+// (sentinel-x src/hmmer/pipeline.rs:159-189) on the device: per-profile pass
+// thresholds, the batched device-resident chain (optionally captured in a HIP graph)
+// and the plain one-profile-at-a-time form.
 #include "sx_pipeline.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
 
 #include "../../include/sx_hmmer.h"
 
-extern "C" int sx_run_cascade(sx_ctx*, int, sx_cascade_stats*, sx_pair_scores*, uint64_t, uint64_t*) {
-    return SX_ERR_STATE;
+namespace sx {
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw SxError(SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+// pipeline.rs:87-95
+double gumbel_surv(double x, double mu, double lambda) {
+    const double y = lambda * (x - mu);
+    const double ey = -std::exp(-y);
+    if (std::fabs(ey) < 1e-5) return -ey;
+    return 1.0 - std::exp(ey);
+}
+
+// pipeline.rs:98-104
+double exp_logsurv(double x, double tau, double lambda) {
+    if (x < tau) return 0.0;
+    return -lambda * (x - tau);
+}
+
+namespace {
+
+int64_t f32_ord(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? -(int64_t)(b & 0x7fffffffu) : (int64_t)b;
+}
+
+float ord_f32(int64_t o) {
+    uint32_t b = o >= 0 ? (uint32_t)o : (0x80000000u | (uint32_t)(-o));
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+}  // namespace
+
+// Smallest f32 y (from -inf to +inf) that `rejects` lets through; rejects must be true
+// up to some point and false after it (each reference tail test is monotone in y).
+// Checks the 256 neighbours on each side of the boundary and counts any that disagree.
+float pass_threshold(const std::function<bool(float)>& rejects, int* violations) {
+    int64_t lo = f32_ord(-INFINITY), hi = f32_ord(INFINITY);
+    if (!rejects(-INFINITY)) return -INFINITY;
+    if (rejects(INFINITY)) return INFINITY;
+    while (hi - lo > 1) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (rejects(ord_f32(mid)))
+            lo = mid;
+        else
+            hi = mid;
+    }
+    if (violations) {
+        for (int64_t o = std::max(hi - 256, f32_ord(-INFINITY)); o < hi; o++)
+            if (!rejects(ord_f32(o))) ++*violations;
+        for (int64_t o = hi; o < std::min(hi + 256, f32_ord(INFINITY)); o++)
+            if (rejects(ord_f32(o))) ++*violations;
+    }
+    return ord_f32(hi);
+}
+
+// Model-state odds of the two-state composition filter for every code
+// (pipeline.rs:116-131); the background state's odds are 1 for every code.
+static void bias_odds(const Profile& p, double* eo1) {
+    for (int x = 0; x < K; x++) eo1[x] = (double)p.compo[x] / (double)BG[x];
+    for (int code = 20; code <= 23; code++) {
+        const int* mem;
+        const int nm = degenerate_members(code, &mem);
+        double e1 = 0.0, z = 0.0;
+        for (int j = 0; j < nm; j++) {
+            e1 += (double)p.compo[mem[j]];
+            z += (double)BG[mem[j]];
+        }
+        eo1[code] = e1 / z;
+    }
+    eo1[CODE_STAR] = 1.0;
+}
+
+}  // namespace sx
+
+extern "C" {
+__global__ void sxc_msv_pass(int, const float*, const float*, const float*, const int32_t*, const int32_t*, int64_t,
+                             int64_t, int32_t*, const int64_t*, int64_t, int32_t*, int32_t*, int32_t*, float*);
+__global__ void sxc_scan(const int32_t*, int64_t, const int64_t*, int64_t*);
+__global__ void sxc_bias(const int64_t*, const int32_t*, const int32_t*, const float*, const double*, const int32_t*,
+                         const float*, SxTargets, float*, uint8_t*);
+__global__ void sxc_compact(int, const uint8_t*, const int64_t*, const int32_t*, const int32_t*, int32_t*,
+                            const int64_t*, int32_t*, int32_t*);
+__global__ void sxc_tiles(int, int64_t, const int32_t*, const int32_t*, const int32_t*, int32_t*, const int64_t*,
+                          const int64_t*, SxTile*);
+__global__ void sxc_gather_t(const int64_t*, const int32_t*, const int32_t*, int32_t*);
+__global__ void sxc_vit_pass(const int64_t*, const int32_t*, const float*, const float*, const int32_t*, const float*,
+                             float*, int32_t*, uint8_t*);
+__global__ void sxc_fwd_pass(const int64_t*, const int32_t*, const double*, const float*, const int32_t*,
+                             const float*, double*, int32_t*);
+__global__ void sxc_res_init(const int64_t*, int64_t, float*, double*, int32_t*);
+__global__ void sxc_zero_i32(int32_t*, int64_t);
+__global__ void sxc_records(const int64_t*, int64_t, const int32_t*, const int32_t*, const float*, const float*,
+                            const float*, const double*, const int32_t*, void*);
+}
+
+namespace sx {
+
+namespace {
+
+const int CB = 256;  // cascade kernels' block size
+
+int64_t nblk(int64_t n) { return (n + CB - 1) / CB; }
+
+template <class T>
+T* dptr(DeviceBuf& b, size_t n) {
+    return (T*)b.ensure(std::max<size_t>(n, 1) * sizeof(T));
+}
+
+}  // namespace
+
+struct CascadeState {
+    // identity of what the plan was built for
+    size_t nprof = 0;
+    uint32_t ntarg = 0;
+    double f1 = -1, f2 = -1, f3 = -1;
+    int64_t cap1 = 0;
+    // host-side tables
+    std::vector<float> thr_msv, thr_vit, thr_fwd, nullsc;
+    std::vector<int32_t> perm;           // profile ids by increasing model length
+    int threshold_violations = 0;
+    // device tables
+    DeviceBuf d_thr_msv, d_thr_vit, d_thr_fwd, d_nullsc, d_perm, d_eo1, d_ng_vit, d_ng_fwd;
+    std::vector<DeviceBuf> msv_plists;
+    struct Group {
+        Shape s;
+        int32_t* plist;
+        int n;
+    };
+    std::vector<Group> msv_groups;
+    struct Run {
+        Shape s;
+        int64_t a, b;  // profile positions [a, b)
+    };
+    std::vector<Run> vit_runs, fwd_runs;
+    // device work buffers
+    DeviceBuf usc, cnt1, off1, s_p, s_pi, s_t, s_usc, s_bias, flag, res_vit, res_fwd, res_stage;
+    DeviceBuf cnt2, off2, list2, per2, tcount, toff2, eoff2, tiles2, tids2, vsc, flag3;
+    DeviceBuf cnt3, off3, list3, per3, toff3, eoff3, tiles3, tids3, fsc, recs;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    ~CascadeState() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+void cascade_release(Ctx& c) { c.cascade.reset(); }
+
+static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3) {
+    const size_t P = c.profs.size();
+    const uint32_t T = c.nt;
+    for (const auto& p : c.profs)
+        if (!p.stats.calibrated())
+            throw SxError(SX_ERR_STATE, "profile '" + p.name +
+                                            "' has no STATS lines; the filter cascade needs calibrated models");
+    S.nprof = P;
+    S.ntarg = T;
+    S.f1 = f1;
+    S.f2 = f2;
+    S.f3 = f3;
+    S.thr_msv.resize(P);
+    S.thr_vit.resize(P);
+    S.thr_fwd.resize(P);
+    S.threshold_violations = 0;
+    for (size_t p = 0; p < P; p++) {
+        const Stats& st = c.profs[p].stats;
+        S.thr_msv[p] = pass_threshold(
+            [&](float y) { return gumbel_surv((double)y, st.msv[0], st.msv[1]) > f1; }, &S.threshold_violations);
+        S.thr_vit[p] = pass_threshold(
+            [&](float y) { return gumbel_surv((double)y, st.vit[0], st.vit[1]) > f2; }, &S.threshold_violations);
+        S.thr_fwd[p] = pass_threshold(
+            [&](float y) { return std::exp(exp_logsurv((double)y, st.fwd[0], st.fwd[1])) > f3; },
+            &S.threshold_violations);
+    }
+    S.nullsc.resize(T);
+    for (uint32_t t = 0; t < T; t++) S.nullsc[t] = null_score((size_t)c.tlen[t]);
+    std::vector<double> eo1(P * 25);
+    for (size_t p = 0; p < P; p++) bias_odds(c.profs[p], eo1.data() + p * 25);
+    S.perm.resize(P);
+    std::iota(S.perm.begin(), S.perm.end(), 0);
+    std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) { return c.profs[a].m < c.profs[b].m; });
+    std::vector<int32_t> ngv(P), ngf(P);
+    S.vit_runs.clear();
+    S.fwd_runs.clear();
+    for (size_t i = 0; i < P; i++) {
+        const int m = c.profs[S.perm[i]].m;
+        const Shape sv = c.shape_for(1, SX_FORM_BATCHED, m), sf = c.shape_for(2, SX_FORM_BATCHED, m);
+        ngv[i] = 256 / sv.g;
+        ngf[i] = 256 / sf.g;
+        auto add = [&](std::vector<CascadeState::Run>& runs, Shape s) {
+            if (!runs.empty() && runs.back().s.q == s.q && runs.back().s.g == s.g && runs.back().s.hoist == s.hoist)
+                runs.back().b = (int64_t)i + 1;
+            else
+                runs.push_back({s, (int64_t)i, (int64_t)i + 1});
+        };
+        add(S.vit_runs, sv);
+        add(S.fwd_runs, sf);
+    }
+    // MSV groups (all pairs, implicit tiles)
+    std::map<std::tuple<int, int, int>, std::vector<int32_t>> g;
+    for (size_t p = 0; p < P; p++) {
+        const Shape s = c.shape_for(0, SX_FORM_BATCHED, c.profs[p].m);
+        g[{s.q, s.g, s.hoist ? 1 : 0}].push_back((int32_t)p);
+    }
+    S.msv_plists.clear();
+    S.msv_plists.reserve(g.size());
+    S.msv_groups.clear();
+    for (auto& kv : g) {
+        S.msv_plists.emplace_back();
+        int32_t* d = dptr<int32_t>(S.msv_plists.back(), kv.second.size());
+        HIPCHK(hipMemcpy(d, kv.second.data(), kv.second.size() * 4, hipMemcpyHostToDevice));
+        S.msv_groups.push_back({Shape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first) != 0}, d,
+                                (int)kv.second.size()});
+    }
+    auto up = [](DeviceBuf& b, const void* src, size_t bytes) {
+        b.ensure(std::max<size_t>(bytes, 16));
+        HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    up(S.d_thr_msv, S.thr_msv.data(), P * 4);
+    up(S.d_thr_vit, S.thr_vit.data(), P * 4);
+    up(S.d_thr_fwd, S.thr_fwd.data(), P * 4);
+    up(S.d_nullsc, S.nullsc.data(), (size_t)T * 4);
+    up(S.d_perm, S.perm.data(), P * 4);
+    up(S.d_eo1, eo1.data(), eo1.size() * 8);
+    up(S.d_ng_vit, ngv.data(), P * 4);
+    up(S.d_ng_fwd, ngf.data(), P * 4);
+    for (auto& e : S.ev)
+        if (!e) HIPCHK(hipEventCreate(&e));
+}
+
+static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
+    const int64_t P = (int64_t)c.profs.size(), T = c.nt;
+    S.cap1 = cap1;
+    const int64_t nb1 = nblk(P * T), nbc = nblk(cap1);
+    dptr<float>(S.usc, (size_t)(P * T));
+    dptr<int32_t>(S.cnt1, nb1);
+    dptr<int64_t>(S.off1, nb1 + 1);
+    dptr<int32_t>(S.s_p, cap1);
+    dptr<int32_t>(S.s_pi, cap1);
+    dptr<int32_t>(S.s_t, cap1);
+    dptr<float>(S.s_usc, cap1);
+    dptr<float>(S.s_bias, cap1);
+    dptr<uint8_t>(S.flag, cap1);
+    dptr<float>(S.res_vit, cap1);
+    dptr<double>(S.res_fwd, cap1);
+    dptr<int32_t>(S.res_stage, cap1);
+    for (DeviceBuf* b : {&S.cnt2, &S.cnt3}) dptr<int32_t>(*b, nbc);
+    for (DeviceBuf* b : {&S.off2, &S.off3}) dptr<int64_t>(*b, nbc + 1);
+    for (DeviceBuf* b : {&S.list2, &S.list3, &S.tids2, &S.tids3}) dptr<int32_t>(*b, cap1);
+    for (DeviceBuf* b : {&S.per2, &S.per3, &S.tcount}) dptr<int32_t>(*b, P);
+    for (DeviceBuf* b : {&S.toff2, &S.eoff2, &S.toff3, &S.eoff3}) dptr<int64_t>(*b, P + 1);
+    for (DeviceBuf* b : {&S.tiles2, &S.tiles3}) dptr<SxTile>(*b, cap1 + P);
+    dptr<float>(S.vsc, cap1);
+    dptr<uint8_t>(S.flag3, cap1);
+    dptr<double>(S.fsc, cap1);
+    S.recs.ensure(std::max<int64_t>(cap1, 1) * sizeof(sx_pair_scores));
+    if (S.gexec) {
+        (void)hipGraphExecDestroy(S.gexec);
+        S.gexec = nullptr;
+    }
+}
+
+// The batched chain.  Everything is enqueued on c.stream with no host synchronisation,
+// so it can be captured.  with_events records stage boundaries for the timing split.
+static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
+    const int64_t P = (int64_t)c.profs.size(), T = c.nt, cap = S.cap1;
+    const int64_t nb1 = nblk(P * T), nbc = nblk(cap), nbp = nblk(P);
+    hipStream_t st = c.stream;
+    SxSpecialsOut nosp{};
+    if (with_events) HIPCHK(hipEventRecord(S.ev[0], st));
+    // 1. MSV over all pairs
+    int slot = 0;
+    for (auto& gr : S.msv_groups) {
+        const KernelInfo& k = c.kernel(0, gr.s.q, gr.s.g, false, gr.s.hoist);
+        SxWork w{};
+        w.profs = gr.plist;
+        w.tids = c.d_sorted;
+        w.chunk = 256 / gr.s.g;
+        w.ntids = (int32_t)T;
+        w.nchunks = (int32_t)((T + w.chunk - 1) / w.chunk);
+        w.ntiles = (int64_t)w.nchunks * gr.n;
+        w.out_stride = T;
+        const int grid = c.persistent_grid(k, Ctx::lds_bytes(0, k.wid), w.ntiles);
+        c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot++ % 32);
+    }
+    if (with_events) HIPCHK(hipEventRecord(S.ev[1], st));
+    // 2. MSV test and ordered compaction
+    const float* usc = (const float*)S.usc.p;
+    int32_t *s_p = (int32_t*)S.s_p.p, *s_pi = (int32_t*)S.s_pi.p, *s_t = (int32_t*)S.s_t.p;
+    float *s_usc = (float*)S.s_usc.p, *s_bias = (float*)S.s_bias.p;
+    hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 0, usc, (const float*)S.d_nullsc.p,
+                       (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
+                       (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
+    hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cnt1.p, nb1, (const int64_t*)nullptr,
+                       (int64_t*)S.off1.p);
+    hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 1, usc, (const float*)S.d_nullsc.p,
+                       (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
+                       (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
+    const int64_t* n1 = (const int64_t*)S.off1.p + nb1;
+    // 3. bias filter and its test
+    hipLaunchKernelGGL(sxc_res_init, dim3(nbc), dim3(CB), 0, st, n1, cap, (float*)S.res_vit.p, (double*)S.res_fwd.p,
+                       (int32_t*)S.res_stage.p);
+    hipLaunchKernelGGL(sxc_bias, dim3(nbc), dim3(CB), 0, st, n1, (const int32_t*)s_p, (const int32_t*)s_t,
+                       (const float*)s_usc, (const double*)S.d_eo1.p, (const int32_t*)c.d_m,
+                       (const float*)S.d_thr_msv.p, c.dtarg(), s_bias, (uint8_t*)S.flag.p);
+    // 4. survivors -> Viterbi tiles
+    auto compact = [&](const uint8_t* flag, const int64_t* n_in, const int32_t* src, DeviceBuf& cnt, DeviceBuf& off,
+                       DeviceBuf& list, DeviceBuf& per) {
+        hipLaunchKernelGGL(sxc_zero_i32, dim3(nbp), dim3(CB), 0, st, (int32_t*)per.p, P);
+        hipLaunchKernelGGL(sxc_compact, dim3(nbc), dim3(CB), 0, st, 0, flag, n_in, src, (const int32_t*)s_pi,
+                           (int32_t*)cnt.p, (const int64_t*)off.p, (int32_t*)list.p, (int32_t*)per.p);
+        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)cnt.p, nbc, (const int64_t*)nullptr,
+                           (int64_t*)off.p);
+        hipLaunchKernelGGL(sxc_compact, dim3(nbc), dim3(CB), 0, st, 1, flag, n_in, src, (const int32_t*)s_pi,
+                           (int32_t*)cnt.p, (const int64_t*)off.p, (int32_t*)list.p, (int32_t*)per.p);
+        return (const int64_t*)off.p + nbc;
+    };
+    auto tiles = [&](DeviceBuf& per, DeviceBuf& ng, DeviceBuf& toff, DeviceBuf& eoff, DeviceBuf& tl,
+                     const int64_t* n_list, DeviceBuf& list, DeviceBuf& tids) {
+        hipLaunchKernelGGL(sxc_tiles, dim3(nbp), dim3(CB), 0, st, 0, P, (const int32_t*)per.p,
+                           (const int32_t*)ng.p, (const int32_t*)S.d_perm.p, (int32_t*)S.tcount.p,
+                           (const int64_t*)toff.p, (const int64_t*)eoff.p, (SxTile*)tl.p);
+        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.tcount.p, P,
+                           (const int64_t*)nullptr, (int64_t*)toff.p);
+        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)per.p, P, (const int64_t*)nullptr,
+                           (int64_t*)eoff.p);
+        hipLaunchKernelGGL(sxc_tiles, dim3(nbp), dim3(CB), 0, st, 1, P, (const int32_t*)per.p,
+                           (const int32_t*)ng.p, (const int32_t*)S.d_perm.p, (int32_t*)S.tcount.p,
+                           (const int64_t*)toff.p, (const int64_t*)eoff.p, (SxTile*)tl.p);
+        hipLaunchKernelGGL(sxc_gather_t, dim3(nbc), dim3(CB), 0, st, n_list, (const int32_t*)list.p,
+                           (const int32_t*)s_t, (int32_t*)tids.p);
+    };
+    auto stage_runs = [&](int stage, std::vector<CascadeState::Run>& runs, DeviceBuf& tl, DeviceBuf& tids,
+                          DeviceBuf& toff, void* out) {
+        for (auto& r : runs) {
+            const KernelInfo& k = c.kernel(stage, r.s.q, r.s.g, false, r.s.hoist);
+            SxWork w{};
+            w.tiles = (const SxTile*)tl.p;
+            w.tids = (const int32_t*)tids.p;
+            w.dyn_lo = (const int64_t*)toff.p + r.a;
+            w.dyn_hi = (const int64_t*)toff.p + r.b;
+            w.ntiles = 0;
+            const int grid = c.persistent_grid(k, Ctx::lds_bytes(stage, k.wid), 1 << 30);
+            c.launch(k, stage, w, out, nosp, grid, true, slot++ % 32 + 32);
+        }
+    };
+    const int64_t* n2 = compact((const uint8_t*)S.flag.p, n1, nullptr, S.cnt2, S.off2, S.list2, S.per2);
+    tiles(S.per2, S.d_ng_vit, S.toff2, S.eoff2, S.tiles2, n2, S.list2, S.tids2);
+    if (with_events) HIPCHK(hipEventRecord(S.ev[2], st));
+    // 5. Viterbi and its test
+    stage_runs(1, S.vit_runs, S.tiles2, S.tids2, S.toff2, S.vsc.p);
+    hipLaunchKernelGGL(sxc_vit_pass, dim3(nbc), dim3(CB), 0, st, n2, (const int32_t*)S.list2.p,
+                       (const float*)S.vsc.p, (const float*)s_bias, (const int32_t*)s_p,
+                       (const float*)S.d_thr_vit.p, (float*)S.res_vit.p, (int32_t*)S.res_stage.p,
+                       (uint8_t*)S.flag3.p);
+    const int64_t* n3 =
+        compact((const uint8_t*)S.flag3.p, n2, (const int32_t*)S.list2.p, S.cnt3, S.off3, S.list3, S.per3);
+    tiles(S.per3, S.d_ng_fwd, S.toff3, S.eoff3, S.tiles3, n3, S.list3, S.tids3);
+    if (with_events) HIPCHK(hipEventRecord(S.ev[3], st));
+    // 6. Forward and its test
+    stage_runs(2, S.fwd_runs, S.tiles3, S.tids3, S.toff3, S.fsc.p);
+    hipLaunchKernelGGL(sxc_fwd_pass, dim3(nbc), dim3(CB), 0, st, n3, (const int32_t*)S.list3.p,
+                       (const double*)S.fsc.p, (const float*)s_bias, (const int32_t*)s_p,
+                       (const float*)S.d_thr_fwd.p, (double*)S.res_fwd.p, (int32_t*)S.res_stage.p);
+    hipLaunchKernelGGL(sxc_records, dim3(nbc), dim3(CB), 0, st, n1, cap, (const int32_t*)s_p, (const int32_t*)s_t,
+                       (const float*)s_usc, (const float*)s_bias, (const float*)S.res_vit.p,
+                       (const double*)S.res_fwd.p, (const int32_t*)S.res_stage.p, S.recs.p);
+    if (with_events) HIPCHK(hipEventRecord(S.ev[4], st));
+    HIPCHK(hipGetLastError());
+}
+
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// Batched form.  Returns the number of MSV survivors; records copied to out (<= cap).
+static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap) {
+    if (!c.cascade) c.cascade = std::make_shared<CascadeState>();
+    CascadeState& S = *c.cascade;
+    if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
+        c.cascade_dirty) {
+        build_plan(c, S, st->f1, st->f2, st->f3);
+        const int64_t P = (int64_t)c.profs.size(), T = c.nt;
+        alloc_buffers(c, S, std::max<int64_t>(1 << 16, (int64_t)((double)P * T * std::max(st->f1, 1e-4) * 2.0)));
+        c.cascade_dirty = false;
+    }
+    for (int attempt = 0; attempt < 2; attempt++) {
+        double total = 0.0;
+        if (st->use_graph) {
+            if (!S.gexec) {
+                hipGraph_t graph;
+                HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_chain(c, S, false);
+                } catch (...) {
+                    hipGraph_t g2;
+                    (void)hipStreamEndCapture(c.stream, &g2);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(c.stream, &graph));
+                HIPCHK(hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(graph);
+            }
+            HIPCHK(hipEventRecord(S.ev[0], c.stream));
+            HIPCHK(hipGraphLaunch(S.gexec, c.stream));
+            HIPCHK(hipEventRecord(S.ev[4], c.stream));
+            HIPCHK(hipEventSynchronize(S.ev[4]));
+            total = ev_ms(S.ev[0], S.ev[4]);
+            st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = 0.0;
+        } else {
+            enqueue_chain(c, S, true);
+            HIPCHK(hipEventSynchronize(S.ev[4]));
+            st->ms_msv = ev_ms(S.ev[0], S.ev[1]);
+            st->ms_bias = ev_ms(S.ev[1], S.ev[2]);
+            st->ms_vit = ev_ms(S.ev[2], S.ev[3]);
+            st->ms_fwd = ev_ms(S.ev[3], S.ev[4]);
+            total = ev_ms(S.ev[0], S.ev[4]);
+        }
+        st->ms_total = total;
+        const int64_t P = (int64_t)c.profs.size(), T = c.nt;
+        const int64_t nb1 = nblk(P * T), nbc = nblk(S.cap1);
+        int64_t n1 = 0, n2 = 0, n3 = 0;
+        HIPCHK(hipMemcpy(&n1, (int64_t*)S.off1.p + nb1, 8, hipMemcpyDeviceToHost));
+        if (n1 > S.cap1) {
+            // survivor list overflowed its capacity: grow to the exact size and run again
+            alloc_buffers(c, S, n1 + 1024);
+            continue;
+        }
+        HIPCHK(hipMemcpy(&n2, (int64_t*)S.off2.p + nbc, 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&n3, (int64_t*)S.off3.p + nbc, 8, hipMemcpyDeviceToHost));
+        st->n_pairs = (uint64_t)(P * T);
+        st->n_msv_pass = (uint64_t)n1;
+        st->n_bias_pass = (uint64_t)n2;
+        st->n_vit_pass = (uint64_t)n3;
+        std::vector<sx_pair_scores> recs((size_t)n1);
+        if (n1) HIPCHK(hipMemcpy(recs.data(), S.recs.p, (size_t)n1 * sizeof(sx_pair_scores), hipMemcpyDeviceToHost));
+        uint64_t npass = 0, cv = 0, cf = 0;
+        for (const auto& r : recs) {
+            const uint64_t cells = (uint64_t)c.profs[r.profile].m * (uint64_t)c.tlen[r.target];
+            if (r.stage >= 2) cv += cells;
+            if (r.stage >= 3) cf += cells;
+            if (r.stage == 4) npass++;
+        }
+        st->n_fwd_pass = npass;
+        st->cells_vit = cv;
+        st->cells_fwd = cf;
+        if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, (uint64_t)n1) * sizeof(sx_pair_scores));
+        return (uint64_t)n1;
+    }
+    throw SxError(SX_ERR_STATE, "cascade survivor list did not fit after growing it");
+}
+
+// Plain form: one profile at a time, one launch per stage, tests on the host.
+static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap) {
+    if (!c.cascade) c.cascade = std::make_shared<CascadeState>();
+    CascadeState& S = *c.cascade;
+    if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
+        c.cascade_dirty) {
+        build_plan(c, S, st->f1, st->f2, st->f3);
+        alloc_buffers(c, S, 1 << 16);
+        c.cascade_dirty = false;
+    }
+    c.ensure_pidx();
+    const uint32_t T = c.nt;
+    std::vector<sx_pair_scores> recs;
+    std::vector<float> urow(T), vrow;
+    std::vector<double> frow;
+    float* d_u = (float*)c.scratch_out.ensure((size_t)T * 8);
+    DeviceBuf d_pp, d_pt, d_pu, d_pb, d_fl, d_n;
+    std::vector<double> eo1(25);
+    double ms_total = 0.0;
+    uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0;
+    HIPCHK(hipEventRecord(S.ev[0], c.stream));
+    for (size_t p = 0; p < c.profs.size(); p++) {
+        const Profile& pr = c.profs[p];
+        // MSV: one launch over every target, in input order
+        std::vector<int32_t> one{(int32_t)p};
+        c.run_implicit(0, SX_FORM_PLAIN, one, c.d_identity, (int)T, d_u, 0);
+        HIPCHK(hipMemcpyAsync(urow.data(), d_u, (size_t)T * 4, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        std::vector<uint32_t> sp_t;
+        for (uint32_t t = 0; t < T; t++) {
+            const float y = (urow[t] - S.nullsc[t]) / LN2F;
+            if (y >= S.thr_msv[p]) sp_t.push_back(t);
+        }
+        if (sp_t.empty()) continue;
+        n1 += sp_t.size();
+        // bias filter on the device for this profile's survivors
+        const int64_t k = (int64_t)sp_t.size();
+        std::vector<int32_t> pp(k, (int32_t)p), pt(sp_t.begin(), sp_t.end());
+        std::vector<float> pu(k);
+        for (int64_t i = 0; i < k; i++) pu[i] = urow[sp_t[i]];
+        int64_t kk = k;
+        int32_t* dpp = dptr<int32_t>(d_pp, k);
+        int32_t* dpt = dptr<int32_t>(d_pt, k);
+        float* dpu = dptr<float>(d_pu, k);
+        float* dpb = dptr<float>(d_pb, k);
+        uint8_t* dfl = dptr<uint8_t>(d_fl, k);
+        int64_t* dn = dptr<int64_t>(d_n, 1);
+        HIPCHK(hipMemcpyAsync(dpp, pp.data(), k * 4, hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(dpt, pt.data(), k * 4, hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(dpu, pu.data(), k * 4, hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(dn, &kk, 8, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(sxc_bias, dim3(nblk(k)), dim3(CB), 0, c.stream, (const int64_t*)dn, (const int32_t*)dpp,
+                           (const int32_t*)dpt, (const float*)dpu, (const double*)S.d_eo1.p, (const int32_t*)c.d_m,
+                           (const float*)S.d_thr_msv.p, c.dtarg(), dpb, dfl);
+        std::vector<float> pb(k);
+        std::vector<uint8_t> fl(k);
+        HIPCHK(hipMemcpyAsync(pb.data(), dpb, k * 4, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipMemcpyAsync(fl.data(), dfl, k, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        std::vector<uint32_t> vp, vt;
+        std::vector<size_t> vidx;
+        const size_t base = recs.size();
+        for (int64_t i = 0; i < k; i++) {
+            sx_pair_scores r{};
+            r.profile = (uint32_t)p;
+            r.target = sp_t[i];
+            r.msv = pu[i];
+            r.bias = pb[i];
+            r.vit = NAN;
+            r.fwd = NAN;
+            r.fwd64 = NAN;
+            r.stage = 1;
+            recs.push_back(r);
+            if (fl[i]) {
+                vp.push_back((uint32_t)p);
+                vt.push_back(sp_t[i]);
+                vidx.push_back(base + i);
+            }
+        }
+        if (vp.empty()) continue;
+        n2 += vp.size();
+        // Viterbi: one launch for this profile's survivors
+        void* dv = c.scratch_x5.ensure(vp.size() * 8);
+        c.run_pairs(1, SX_FORM_PLAIN, vp.data(), vt.data(), vp.size(), dv, false, nullptr, nullptr, nullptr);
+        vrow.resize(vp.size());
+        {
+            std::vector<float> tmp(vp.size());
+            HIPCHK(hipMemcpyAsync(tmp.data(), dv, vp.size() * 4, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            for (size_t i = 0; i < vp.size(); i++) vrow[c.last_order[i]] = tmp[i];
+        }
+        std::vector<uint32_t> fp, ft;
+        std::vector<size_t> fidx;
+        for (size_t i = 0; i < vp.size(); i++) {
+            sx_pair_scores& r = recs[vidx[i]];
+            r.vit = vrow[i];
+            r.stage = 2;
+            cv += (uint64_t)pr.m * (uint64_t)c.tlen[vt[i]];
+            const float y = (vrow[i] - r.bias) / LN2F;
+            if (y >= S.thr_vit[p]) {
+                r.stage = 3;
+                fp.push_back((uint32_t)p);
+                ft.push_back(vt[i]);
+                fidx.push_back(vidx[i]);
+            }
+        }
+        if (fp.empty()) continue;
+        n3 += fp.size();
+        void* df = c.scratch_x5.ensure(fp.size() * 8);
+        c.run_pairs(2, SX_FORM_PLAIN, fp.data(), ft.data(), fp.size(), df, false, nullptr, nullptr, nullptr);
+        frow.resize(fp.size());
+        {
+            std::vector<double> tmp(fp.size());
+            HIPCHK(hipMemcpyAsync(tmp.data(), df, fp.size() * 8, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            for (size_t i = 0; i < fp.size(); i++) frow[c.last_order[i]] = tmp[i];
+        }
+        for (size_t i = 0; i < fp.size(); i++) {
+            sx_pair_scores& r = recs[fidx[i]];
+            r.fwd64 = frow[i];
+            r.fwd = (float)frow[i];
+            cf += (uint64_t)pr.m * (uint64_t)c.tlen[ft[i]];
+            const float y = (r.fwd - r.bias) / LN2F;
+            if (y >= S.thr_fwd[p]) {
+                r.stage = 4;
+                n4++;
+            }
+        }
+    }
+    HIPCHK(hipEventRecord(S.ev[4], c.stream));
+    HIPCHK(hipEventSynchronize(S.ev[4]));
+    ms_total = ev_ms(S.ev[0], S.ev[4]);
+    st->ms_total = ms_total;
+    st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = 0.0;
+    st->n_pairs = (uint64_t)c.profs.size() * c.nt;
+    st->n_msv_pass = n1;
+    st->n_bias_pass = n2;
+    st->n_vit_pass = n3;
+    st->n_fwd_pass = n4;
+    st->cells_vit = cv;
+    st->cells_fwd = cf;
+    if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, recs.size()) * sizeof(sx_pair_scores));
+    return recs.size();
+}
+
+}  // namespace sx
+
+extern "C" int sx_run_cascade_impl(sx::Ctx* c, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
+                                   uint64_t* n_out) {
+    uint64_t cells = 0;
+    for (const auto& p : c->profs) cells += (uint64_t)p.m * (uint64_t)c->total_len;
+    st->cells_msv = cells;
+    const uint64_t n = form == SX_FORM_PLAIN ? sx::run_plain(*c, st, out, cap) : sx::run_batched(*c, st, out, cap);
+    if (n_out) *n_out = n;
+    return SX_OK;
 }

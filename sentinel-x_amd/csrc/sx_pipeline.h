@@ -1,4 +1,19 @@
-// sx_pipeline.h -- device filter cascade (pipeline.rs:159-189) and the host-side
-// domain definition / reporting (pipeline.rs:190-294, search.rs:131-271).
+// sx_pipeline.h -- filter cascade of score_target (pipeline.rs:159-189) on the device.
 #pragma once
+#include <functional>
+
+#include "../../include/sx_hmmer.h"
 #include "sx_runtime.h"
+
+namespace sx {
+
+double gumbel_surv(double x, double mu, double lambda);   // pipeline.rs:87-95
+double exp_logsurv(double x, double tau, double lambda);  // pipeline.rs:98-104
+
+// Smallest f32 value that a monotone reject-test lets through (see sx_pipeline.cpp).
+float pass_threshold(const std::function<bool(float)>& rejects, int* violations);
+
+}  // namespace sx
+
+extern "C" int sx_run_cascade_impl(sx::Ctx* c, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
+                                   uint64_t* n_out);

@@ -15,6 +15,7 @@
 #include "sx_dev.h"
 #include "sx_model.h"
 #include "sx_runtime.h"
+#include "sx_pipeline.h"
 
 struct SxKernelEntry {
     int stage, q, g, spec, hoist, wid;
@@ -130,6 +131,7 @@ size_t Ctx::lds_bytes(int stage, int wid) { return (size_t)(SX_NCODE + (stage ==
 // ---------------------------------------------------------------------------
 
 void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n) {
+    cascade_dirty = true;
     free_targets();
     nt = n;
     tlen.assign(n, 0);
@@ -179,6 +181,7 @@ void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n)
 }
 
 void Ctx::set_profiles(std::vector<Profile>&& ps) {
+    cascade_dirty = true;
     free_profiles();
     profs = std::move(ps);
     std::vector<float> msc, mo, tsc, tp, tbm;
@@ -525,6 +528,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
     }
     pgroups_keep.clear();
     tuned[stage] = table;
+    cascade_dirty = true;
     return log;
 }
 
@@ -835,6 +839,7 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
             c.tuned.erase(stage);
         else
             c.tuned[stage] = tab;
+        c.cascade_dirty = true;
     });
 }
 
@@ -858,4 +863,17 @@ extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, 
 extern "C" int sx_autotune(sx_ctx* ctx, int stage, double budget_ms) {
     if (!ctx || stage < 0 || stage > 3) return SX_ERR_ARG;
     return guard(ctx, [&] { (void)ctx->c->autotune(stage, budget_ms); });
+}
+
+extern "C" int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
+                              uint64_t* n_out) {
+    if (!ctx || !st || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED) || (cap && !out)) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        if (ctx->c->profs.empty() || ctx->c->nt == 0) {
+            if (n_out) *n_out = 0;
+            return;
+        }
+        ctx->c->ensure_pidx();
+        (void)sx_run_cascade_impl(ctx->c, form, st, out, cap, n_out);
+    });
 }

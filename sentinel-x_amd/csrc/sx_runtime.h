@@ -5,6 +5,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <stdexcept>
 #include <string>
@@ -14,6 +15,8 @@
 #include "sx_model.h"
 
 namespace sx {
+
+struct CascadeState;
 
 struct SxError : std::runtime_error {
     int code;
@@ -115,6 +118,10 @@ struct Ctx {
     void run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* dout, bool spec,
                    const std::vector<int64_t>* row_off, float* dx5, double* dls);
     double timed(const std::function<void()>& f);
+
+    // device filter cascade (sx_pipeline.cpp); rebuilt when profiles, targets or shapes change
+    std::shared_ptr<CascadeState> cascade;
+    bool cascade_dirty = true;
 };
 
 }  // namespace sx

@@ -1,0 +1,102 @@
+"""GPU parity of the device filter cascade (pipeline.rs:159-189) against the oracle's
+per-pair cascade (sx_oracle.c sxo_cascade), plus batched vs plain vs graph replay."""
+import numpy as np
+import pytest
+
+from helpers import emit_homolog, random_hmm, random_seq
+
+pytestmark = pytest.mark.gpu
+
+F = dict(f1=0.3, f2=0.2, f3=0.1)  # loose thresholds so every stage sees work
+
+
+@pytest.fixture(scope="module")
+def calibrated(orc, sxa):
+    from sentinel_x_amd import synth
+    ms = [12, 40, 90, 150, 230, 333, 520]
+    hmms = [random_hmm(orc, m, 4000 + i) for i, m in enumerate(ms)]
+    rng = np.random.default_rng(11)
+    seqs = []
+    for i in range(160):
+        if i % 5 == 0:
+            seqs.append(emit_homolog(orc, hmms[i % len(hmms)], rng, int(rng.integers(40, 400))))
+        else:
+            seqs.append(random_seq(rng, int(rng.integers(1, 500)), star_frac=0.005, degen_frac=0.01))
+    profs = [orc.configure(h) for h in hmms]
+
+    def scorer(stage, ss):
+        b = orc.Batch(profs, ss)
+        fn = {0: b.msv, 1: b.viterbi, 2: b.forward}[stage]
+        return fn(nthreads=8).reshape(len(profs), len(ss))
+
+    synth.calibrate(hmms, scorer, n=60)
+    profs = [orc.configure(h) for h in hmms]
+    return hmms, profs, seqs
+
+
+def oracle_cascade(orc, profs, seqs):
+    want = {}
+    for p, pr in enumerate(profs):
+        for t, s in enumerate(seqs):
+            stage, out = orc.cascade(pr, s, **F)
+            want[(p, t)] = (stage, out)
+    return want
+
+
+def test_batched_cascade_matches_oracle(orc, gctx, calibrated):
+    hmms, profs, seqs = calibrated
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    st, rec = gctx.run_cascade(1, use_graph=False, **F)
+    want = oracle_cascade(orc, profs, seqs)
+    survivors = {k for k, v in want.items() if v[0] >= 1}
+    got = {(int(r["profile"]), int(r["target"])) for r in rec}
+    assert got == survivors
+    assert st.n_msv_pass == len(survivors)
+    assert sum(v[0] >= 2 for v in want.values()) > 5 and sum(v[0] == 4 for v in want.values()) > 2
+    same_bias = agree = 0
+    for r in rec:
+        stage, out = want[(int(r["profile"]), int(r["target"]))]
+        assert np.float32(r["msv"]) == out[0]
+        if np.float32(r["bias"]) == out[1]:
+            same_bias += 1
+            assert r["stage"] == stage
+            if stage >= 2:
+                assert np.float32(r["vit"]) == out[2]
+        else:
+            assert abs(float(r["bias"]) - float(out[1])) <= 1e-4 * max(1.0, abs(float(out[1])))
+        agree += int(r["stage"] == stage)
+        if stage >= 3 and r["stage"] >= 3:
+            f64 = orc.forward64(profs[int(r["profile"])], seqs[int(r["target"])])
+            assert abs(float(r["fwd64"]) - f64) <= 1e-3
+    assert agree == len(rec)
+    assert same_bias >= 0.99 * len(rec)
+
+
+def test_plain_and_graph_forms_agree_with_batched(gctx, calibrated):
+    hmms, profs, seqs = calibrated
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    _, eager = gctx.run_cascade(1, use_graph=False, **F)
+    _, plain = gctx.run_cascade(0, **F)
+    key = lambda a: np.lexsort((a["target"], a["profile"]))
+    eager, plain = eager[key(eager)], plain[key(plain)]
+    assert np.array_equal(eager[["profile", "target", "stage"]], plain[["profile", "target", "stage"]])
+    for f in ("msv", "bias", "vit"):
+        assert np.array_equal(eager[f], plain[f], equal_nan=True)
+    assert np.allclose(eager["fwd64"], plain["fwd64"], atol=1e-3, equal_nan=True)
+    # graph replays (several, with eager work in between) reproduce the eager run
+    for _ in range(3):
+        _, g = gctx.run_cascade(1, use_graph=True, **F)
+        g = g[key(g)]
+        assert np.array_equal(g[["profile", "target", "stage"]], eager[["profile", "target", "stage"]])
+        assert np.array_equal(g["vit"], eager["vit"], equal_nan=True)
+        gctx.score_all(0, 1)
+
+
+def test_cascade_requires_calibrated_profiles(orc, gctx, sxa):
+    h = random_hmm(orc, 30, 1, stats=False)
+    gctx.set_profiles([h])
+    gctx.set_targets([np.array([1, 2, 3], np.uint8)])
+    with pytest.raises(sxa.SxError):
+        gctx.run_cascade(1)
