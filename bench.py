@@ -1,15 +1,17 @@
-"""Throughput of the MI355X Plan7 filter path (MSV / Viterbi / Forward), batched vs plain.
+"""Throughput of the MI355X Plan7 search path: the MSV -> bias -> Viterbi -> Forward filter
+cascade of score_target (sentinel-x src/hmmer/pipeline.rs:159-189) over every
+(profile, protein) pair of a synthetic workload, batched (device-resident, graph-replayed)
+vs plain (one launch per profile and stage).
 
-A step is one pass of the hot path over the synthetic workload's pairs.  By default the
-workload is the Pfam-scale set (20,000 profiles x 50,000 proteins, 6.1e13 MSV cells); the
-timed stage is selected with --stage.  GCUPS = sum over pairs of M*L / stage time / 1e9.
-Prints one JSON line (see DESIGN.md for every field).
+A step is one cascade over all pairs.  GCUPS = sum over all pairs of M*L (the MSV cells)
+/ step time / 1e9; per-stage GCUPS divide each stage's own cells by its time.  The default
+workload is the Pfam-scale set: 20,000 profiles x 50,000 proteins, 6.1e13 MSV cells.
+Prints one JSON line; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -21,61 +23,85 @@ sys.path.insert(0, ROOT)
 
 from __graft_entry__ import load_package  # noqa: E402
 
-# Algorithmic VALU work per DP cell ("Roofline"): MSV 1 add + 2 max;
-# Viterbi 10 add + 6 max (+1 select for the I state); Forward 17 flops (FMA = 2).
-OPS_PER_CELL = {0: 3, 1: 17, 2: 17, 3: 19}
+# Algorithmic VALU work per DP cell: MSV 1 add + 2 max; Viterbi 10 add + 6 max + 1 select;
+# Forward 17 flops (an FMA counts 2).
+OPS_PER_CELL = {0: 3, 1: 17, 2: 17}
 # MI355X vector ALU peak, non-packed f32 lane operations: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
-HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(orc, wl, stage, budget_s=15.0, threads=None):
-    """Time the oracle's threaded pair driver on a bounded sample of the workload."""
+def calibrate(sxa, synth, hmms, n):
+    """HMMER-style STATS for every profile, scored by the device kernels themselves."""
+    with sxa.Context(0) as cal:
+        cal.set_profiles(hmms)
+
+        def scorer(stage, seqs):
+            cal.set_targets(seqs)
+            out, _ = cal.score_all(stage, sxa.FORM_BATCHED)
+            return out
+
+        synth.calibrate(hmms, scorer, n=n)
+
+
+def cpu_baseline(wl, recs, f, budget_s, threads=None):
+    """The oracle's threaded cascade on a bounded sample (first k profiles x all proteins)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker, timed here as the CPU baseline
     threads = threads or min(16, os.cpu_count() or 1)
-    hm = [orc.Hmm(name=h.name, m=h.m, mat=h.mat, t=h.t, compo=h.compo) for h in wl.hmms]
     lens = np.array([len(s) for s in wl.seqs], np.int64)
+    ms = np.array([h.m for h in wl.hmms], np.int64)
     tot_l = int(lens.sum())
-    # grow the profile sample until the estimated time reaches the budget
-    rate_guess = {0: 2.0e9, 1: 1.5e8, 2: 1.2e8}[stage] * threads
-    npf = 1
-    ms = np.array([h.m for h in hm], np.int64)
-    while npf < len(hm) and ms[:npf + 1].sum() * tot_l / rate_guess < budget_s:
-        npf += 1
-    profs = [orc.configure(h) for h in hm[:npf]]
-    batch = orc.Batch(profs, wl.seqs)
-    fn = {0: batch.msv, 1: batch.viterbi, 2: batch.forward}[stage]
-    fn(0, min(64, npf * len(wl.seqs)), threads)  # warm-up
+    rate_guess = 2.0e9 * threads  # MSV-dominated cells per second
+    k = 1
+    while k < len(ms) and ms[:k + 1].sum() * tot_l / rate_guess < budget_s:
+        k += 1
+    hm = [orc.Hmm(name=h.name, m=h.m, mat=h.mat, t=h.t, compo=h.compo, stats=h.stats) for h in wl.hmms[:k]]
+    batch = orc.Batch([orc.configure(h) for h in hm], wl.seqs)
+    batch.cascade(**f, begin=0, end=min(64, k * len(wl.seqs)), nthreads=threads)  # warm-up
     t0 = time.perf_counter()
-    fn(0, npf * len(wl.seqs), threads)
+    stage, sc = batch.cascade(**f, nthreads=threads)
     dt = time.perf_counter() - t0
-    cells = int(ms[:npf].sum()) * tot_l
-    return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
-            "sample": f"{npf} of {len(hm)} profiles x all {len(wl.seqs)} proteins ({cells:.3g} cells, {dt:.1f} s)"}
+    cells = int(ms[:k].sum()) * tot_l
+    # decision agreement with the device cascade on the same pairs
+    nt = len(wl.seqs)
+    sel = recs[recs["profile"] < k]
+    gkey = sel["profile"].astype(np.int64) * nt + sel["target"].astype(np.int64)
+    okey = np.flatnonzero(stage >= 1)
+    same_set = bool(np.array_equal(np.sort(gkey), okey))
+    order = np.argsort(gkey)
+    agree = int(np.sum(sel["stage"][order] == stage[np.sort(gkey)])) if same_set else 0
+    return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": f"cascade on {k} of {len(ms)} profiles x all {nt} proteins ({cells:.3g} MSV cells, "
+                      f"{dt:.1f} s)",
+            "check": {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--profile-scale", type=float, default=1.0)
     ap.add_argument("--target-scale", type=float, default=1.0)
-    ap.add_argument("--stage", type=int, default=0, help="0 MSV, 1 Viterbi, 2 Forward")
-    ap.add_argument("--plain-sample", type=float, default=0.05,
-                    help="fraction of the profiles timed in the plain form")
+    ap.add_argument("--plain-sample", type=float, default=0.02, help="fraction of profiles run in the plain form")
+    ap.add_argument("--tune-ms", type=float, default=3000.0, help="autotuner budget per stage (0 = built-in)")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune-ms", type=float, default=4000.0, help="autotuner budget (0 = built-in shapes)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--f1", type=float, default=0.02)
+    ap.add_argument("--f2", type=float, default=1e-3)
+    ap.add_argument("--f3", type=float, default=1e-5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+    f = dict(f1=args.f1, f2=args.f2, f3=args.f3)
 
     sxa = load_package()
     from sentinel_x_amd import synth
@@ -84,66 +110,89 @@ def main():
     ms = np.array([h.m for h in wl.hmms], np.int64)
     lens = np.array([len(s) for s in wl.seqs], np.int64)
     cells = int(ms.sum()) * int(lens.sum())
-    log(f"[bench] workload {wl.name}: {len(ms)} profiles x {len(lens)} proteins, {cells:.3g} cells "
+    log(f"[bench] workload {wl.name}: {len(ms)} profiles x {len(lens)} proteins, {cells:.3g} MSV cells "
         f"(generated in {time.time() - t0:.1f} s)")
+    t0 = time.time()
+    calibrate(sxa, synth, wl.hmms, 200)
+    log(f"[bench] calibrated STATS on the device in {time.time() - t0:.1f} s")
+
     ctx = sxa.Context(0)
     ctx.set_profiles(wl.hmms)
     ctx.set_targets(wl.seqs)
-    stage = args.stage
     if args.tune_ms > 0:
-        t0 = time.time()
-        ctx.autotune(stage, args.tune_ms)
-        log(f"[bench] autotuned shapes (max_m, Q, G, registers) in {time.time() - t0:.1f} s: {ctx.get_shapes(stage)}")
-
+        for st in (0, 1, 2):
+            t0 = time.time()
+            ctx.autotune(st, args.tune_ms)
+            log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
+    graph = not args.no_graph
     for i in range(args.warmup):
-        _, ms_w = ctx.score_all(stage, sxa.FORM_BATCHED, want=False)
-        log(f"[bench] warmup {i}: {ms_w:.1f} ms")
+        s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        log(f"[bench] warmup {i}: {s.ms_total:.1f} ms")
     times = []
     for i in range(args.steps):
-        _, ms_s = ctx.score_all(stage, sxa.FORM_BATCHED, want=False)
-        times.append(ms_s)
-        log(f"[bench] step {i}: {ms_s:.1f} ms, {cells / ms_s / 1e6:.1f} GCUPS")
+        s, recs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        times.append(s.ms_total)
+        log(f"[bench] step {i}: {s.ms_total:.1f} ms = {cells / s.ms_total / 1e6:.1f} GCUPS; survivors "
+            f"MSV {s.n_msv_pass} bias {s.n_bias_pass} Viterbi {s.n_vit_pass} Forward {s.n_fwd_pass}")
     ms_step = float(np.median(times))
     gcups = cells / ms_step / 1e6
+    # per-stage split from one eager (event-timed) run of the same chain
+    e, erecs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, **f)
+    stages = {
+        "msv": {"ms": round(e.ms_msv, 3), "cells": int(e.cells_msv), "gcups": round(e.cells_msv / e.ms_msv / 1e6, 1)},
+        "bias+compaction": {"ms": round(e.ms_bias, 3), "pairs": int(e.n_msv_pass)},
+        "viterbi": {"ms": round(e.ms_vit, 3), "cells": int(e.cells_vit),
+                    "gcups": round(e.cells_vit / max(e.ms_vit, 1e-9) / 1e6, 1)},
+        "forward": {"ms": round(e.ms_fwd, 3), "cells": int(e.cells_fwd),
+                    "gcups": round(e.cells_fwd / max(e.ms_fwd, 1e-9) / 1e6, 1)},
+    }
+    log(f"[bench] eager split: {stages}")
+    key = lambda a: np.lexsort((a["target"], a["profile"]))
+    same_as_eager = bool(np.array_equal(recs[key(recs)][["profile", "target", "stage"]],
+                                        erecs[key(erecs)][["profile", "target", "stage"]]))
 
-    # plain form (one launch per profile, one wavefront per pair) on a profile sample
-    n_plain = max(1, int(round(len(wl.hmms) * args.plain_sample)))
-    ctx_p = sxa.Context(0)
-    ctx_p.set_profiles(wl.hmms[:n_plain])
-    ctx_p.set_targets(wl.seqs)
-    if args.tune_ms > 0:
-        ctx_p.set_shapes(stage, ctx.get_shapes(stage))
-    plain_cells = int(ms[:n_plain].sum()) * int(lens.sum())
-    ctx_p.score_all(stage, sxa.FORM_PLAIN, want=False)
-    _, ms_plain = ctx_p.score_all(stage, sxa.FORM_PLAIN, want=False)
-    _, ms_pb = ctx_p.score_all(stage, sxa.FORM_BATCHED, want=False)
-    plain_gcups = plain_cells / ms_plain / 1e6
-    log(f"[bench] plain form on {n_plain} profiles: {ms_plain:.1f} ms = {plain_gcups:.1f} GCUPS "
-        f"(batched on the same sample {plain_cells / ms_pb / 1e6:.1f} GCUPS)")
-    ctx_p.close()
+    # plain form on a profile sample, and the batched form on exactly the same pairs
+    n_plain = max(1, int(round(len(ms) * args.plain_sample)))
+    with sxa.Context(0) as cp:
+        cp.set_profiles(wl.hmms[:n_plain])
+        cp.set_targets(wl.seqs)
+        for st in (0, 1, 2):
+            cp.set_shapes(st, ctx.get_shapes(st))
+        sp_cells = int(ms[:n_plain].sum()) * int(lens.sum())
+        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, **f)
+        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        plain_same = bool(np.array_equal(prec[key(prec)][["profile", "target", "stage"]],
+                                         brec[key(brec)][["profile", "target", "stage"]]))
+    plain = {"profiles": n_plain, "cells": sp_cells, "ms": round(pl.ms_total, 3),
+             "gcups": round(sp_cells / pl.ms_total / 1e6, 2),
+             "batched_same_pairs_gcups": round(sp_cells / bt.ms_total / 1e6, 2),
+             "ratio_batched_over_plain": round(pl.ms_total / bt.ms_total, 2),
+             "identical_decisions": plain_same}
+    log(f"[bench] plain form: {plain}")
 
-    achieved = gcups * OPS_PER_CELL[stage] / 1e3  # T lane-ops/s
+    achieved = e.cells_msv * OPS_PER_CELL[0] / (e.ms_msv * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-            "ops_per_cell": OPS_PER_CELL[stage]}
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None, "kernel": "MSV stage (k_tiles, stage 0)",
+            "ops_per_cell": OPS_PER_CELL[0]}
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as orc  # the checker, here timed as the CPU baseline
-        cpu = cpu_baseline(orc, wl, stage, args.cpu_budget)
+        cpu = cpu_baseline(wl, recs, f, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu}")
 
-    name = sxa.STAGE_NAMES[stage]
     out = {
-        "metric": f"GCUPS {name.upper()} batched on 1 MI355X (all pairs)",
+        "metric": "GCUPS of the MSV->bias->Viterbi->Forward cascade over all pairs, batched, 1 MI355X",
         "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": wl.name, "profiles": len(ms), "proteins": len(lens), "cells": cells,
-                   "stage": name, "form": "batched"},
-        "plain": {"gcups": round(plain_gcups, 2), "profiles": n_plain, "ratio_batched_over_plain":
-                  round(gcups / plain_gcups, 2)},
+        "config": {"workload": f"{wl.name}: {len(ms)} profiles x {len(lens)} proteins", "cells_msv": cells,
+                   "f1": args.f1, "f2": args.f2, "f3": args.f3, "graph": graph},
+        "stages": stages,
+        "survivors": {"msv": int(e.n_msv_pass), "bias": int(e.n_bias_pass), "viterbi": int(e.n_vit_pass),
+                      "forward": int(e.n_fwd_pass)},
+        "graph_matches_eager": same_as_eager,
+        "plain": plain,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

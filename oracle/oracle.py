@@ -76,6 +76,10 @@ def lib():
         L.sxo_cascade.restype = ctypes.c_int
         L.sxo_cascade.argtypes = [ctypes.c_int, f32p, f32p, f32p, f32p, ctypes.c_float, f32p, f64p,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, u8p, i64, f32p]
+        L.sxo_batch_cascade.restype = ctypes.c_int
+        L.sxo_batch_cascade.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, i64, i64, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int]
         for name in ("sxo_batch_msv", "sxo_batch_viterbi", "sxo_batch_forward", "sxo_batch_forward64"):
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
@@ -458,3 +462,15 @@ class Batch:
 
     def forward64(self, begin=0, end=None, nthreads=1):
         return self._run(lib().sxo_batch_forward64, np.float64, begin, end, nthreads)
+
+    def cascade(self, f1=0.02, f2=1e-3, f3=1e-5, begin=0, end=None, nthreads=1):
+        """Per pair: (stage reached 0..4, [msv, filtersc, vit, fwd]) of pipeline.rs:159-189."""
+        if end is None:
+            end = self.np_ * self.nt
+        stats = np.ascontiguousarray(np.stack([p.stats6 for p in self.profiles]), np.float64)
+        compo = np.ascontiguousarray(np.stack([p.compo for p in self.profiles]), np.float32)
+        stage = np.empty(end - begin, np.int32)
+        sc = np.empty((end - begin, 4), np.float32)
+        lib().sxo_batch_cascade(ctypes.byref(self._s), stats.ctypes.data, compo.ctypes.data, f1, f2, f3, begin, end,
+                                stage.ctypes.data, sc.ctypes.data, nthreads)
+        return stage, sc

@@ -608,11 +608,15 @@ typedef struct {
     atomic_llong next;
     int stage;
     void* out;
+    const double* stats;
+    const float* compo;
+    double f[3];
+    float* sc;
 } pool_job;
 
 #define CHUNK 16
 
-static void run_pair(const sxo_batch* b, int stage, int64_t idx, int64_t begin, void* out) {
+static void run_pair(const pool_job* j, const sxo_batch* b, int stage, int64_t idx, int64_t begin, void* out) {
     int p = (int)(idx / b->nt), t = (int)(idx % b->nt);
     int m = b->m[p];
     const uint8_t* dsq = b->codes + b->seq_off[t];
@@ -629,8 +633,14 @@ static void run_pair(const sxo_batch* b, int stage, int64_t idx, int64_t begin, 
         case 2:
             ((double*)out)[idx - begin] = sxo_forward(m, b->mo + b->off[p], b->tp + b->toff[p], xs, dsq, l, NULL);
             break;
-        default:
+        case 3:
             ((double*)out)[idx - begin] = sxo_forward64(m, b->mo + b->off[p], b->tp + b->toff[p], xs, dsq, l);
+            break;
+        default:
+            ((int32_t*)out)[idx - begin] =
+                sxo_cascade(m, b->msc + b->off[p], b->tsc + b->toff[p], b->mo + b->off[p], b->tp + b->toff[p],
+                            b->msv_tbm[p], j->compo + (size_t)p * SXO_K, j->stats + (size_t)p * 6, j->f[0], j->f[1],
+                            j->f[2], dsq, l, j->sc + 4 * (idx - begin));
             break;
     }
 }
@@ -641,23 +651,36 @@ static void* pool_worker(void* arg) {
         int64_t s = atomic_fetch_add(&j->next, CHUNK);
         if (s >= j->end) break;
         int64_t e = s + CHUNK < j->end ? s + CHUNK : j->end;
-        for (int64_t i = s; i < e; i++) run_pair(j->b, j->stage, i, j->begin, j->out);
+        for (int64_t i = s; i < e; i++) run_pair(j, j->b, j->stage, i, j->begin, j->out);
     }
     return NULL;
 }
 
-static int run_pool(const sxo_batch* b, int64_t begin, int64_t end, void* out, int nthreads, int stage) {
+static int run_pool_ex(const sxo_batch* b, int64_t begin, int64_t end, void* out, int nthreads, int stage,
+                       const double* stats, const float* compo, const double* f, float* sc) {
     if (end <= begin) return 0;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     pool_job j;
     j.b = b; j.begin = begin; j.end = end; j.stage = stage; j.out = out;
+    j.stats = stats; j.compo = compo; j.sc = sc;
+    j.f[0] = f ? f[0] : 0; j.f[1] = f ? f[1] : 0; j.f[2] = f ? f[2] : 0;
     atomic_init(&j.next, begin);
     pthread_t th[256];
     for (int i = 1; i < nthreads; i++) pthread_create(&th[i], NULL, pool_worker, &j);
     pool_worker(&j);
     for (int i = 1; i < nthreads; i++) pthread_join(th[i], NULL);
     return 0;
+}
+
+static int run_pool(const sxo_batch* b, int64_t begin, int64_t end, void* out, int nthreads, int stage) {
+    return run_pool_ex(b, begin, end, out, nthreads, stage, NULL, NULL, NULL, NULL);
+}
+
+int sxo_batch_cascade(const sxo_batch* b, const double* stats, const float* compo, double f1, double f2, double f3,
+                      int64_t pb, int64_t pe, int32_t* stage_out, float* sc_out, int nt) {
+    const double f[3] = {f1, f2, f3};
+    return run_pool_ex(b, pb, pe, stage_out, nt, 4, stats, compo, f, sc_out);
 }
 
 int sxo_batch_msv(const sxo_batch* b, int64_t pb, int64_t pe, float* out, int nt) { return run_pool(b, pb, pe, out, nt, 0); }

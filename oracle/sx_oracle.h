@@ -126,6 +126,11 @@ int sxo_batch_msv(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, floa
 int sxo_batch_viterbi(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, float* out, int nthreads);
 int sxo_batch_forward(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
 int sxo_batch_forward64(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
+/* sxo_cascade over many pairs (the reference's search_profiles pair loop with
+ * filters on, search.rs:217-224 + pipeline.rs:159-189).  stats: 6 doubles per profile;
+ * compo: 20 floats per profile.  stage_out: one int per pair; sc_out: 4 floats per pair. */
+int sxo_batch_cascade(const sxo_batch* b, const double* stats, const float* compo, double f1, double f2, double f3,
+                      int64_t pair_begin, int64_t pair_end, int32_t* stage_out, float* sc_out, int nthreads);
 
 #ifdef __cplusplus
 }
