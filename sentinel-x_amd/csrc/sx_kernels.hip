@@ -259,14 +259,26 @@ struct RevStream {
     }
 };
 
-// LDS row width for a (Q, G) configuration: columns 0 .. Q*G+1 (the extra column lets the
-// last lane read position k+1), rounded up to an odd count so that lanes reading the same
-// column for different residues hit different banks.
+// LDS layout of a profile table row for a (Q, G) shape.  Lane j of a group owns the
+// Q+1 slots [j*(Q+1), (j+1)*(Q+1)): slot s holds model position k = j*Q + s, so slot 0
+// duplicates the previous lane's last position (the k-1 operand of the lane's first
+// cell) and slots 1..Q are the lane's own positions.  The odd stride Q+1 puts the lanes
+// of a group on distinct banks when they read the same slot (a stride of Q, always even
+// here, made them collide).  Two extra slots let the last lane read position k+1, and
+// the row width is odd so that groups reading different residue rows spread over banks.
 template <int Q, int G>
 struct Geo {
     static constexpr int CAP = Q * G;
-    static constexpr int WID = ((CAP + 2) & 1) ? (CAP + 2) : (CAP + 3);
+    static constexpr int STRIDE = Q + 1;
+    static constexpr int WID = ((G * STRIDE + 2) & 1) ? (G * STRIDE + 2) : (G * STRIDE + 3);
 };
+
+// Model position stored at LDS slot `pos` of a row.
+template <int Q, int G>
+__device__ __forceinline__ int slot_k(int pos) {
+    const int j = pos / Geo<Q, G>::STRIDE;
+    return j * Q + (pos - j * Geo<Q, G>::STRIDE);
+}
 
 // ---------------------------------------------------------------------------
 // MSV (dp.rs:92-145)
@@ -276,7 +288,7 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tb
                                           float tloop, float tmove) {
     constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
-    const int kb = j * Q + 1;
+    const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
     float row[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) row[q] = SX_NEG_INF;
@@ -317,7 +329,7 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, const fl
                                           const uint8_t* seq, int L, float lloop, float lmove) {
     constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
-    const int kb = j * Q + 1;
+    const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
     // tr rows: MM MI MD IM II DM DD BM; Tx(k) = tr[x*WID + k]
     float mr[Q], ir[Q], dr[Q];
 #pragma unroll
@@ -393,7 +405,7 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const 
                                            double* lsrow, bool writer) {
     constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
-    const int kb = j * Q + 1;
+    const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
     float blk = 1.0f;  // product of p_dd over the lane's block: the D-chain map's slope
     if constexpr (G > 1) {
         const float* __restrict__ pdd0 = tr + SX_TDD * WID + kb;
@@ -513,7 +525,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
                                            float lmove, float* x5, double* lsrow, bool writer) {
     constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
-    const int kb = j * Q + 1;
+    const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
     const float* __restrict__ pmd = tr + SX_TMD * WID + kb;
     const float* __restrict__ pdd = tr + SX_TDD * WID + kb;
     const float* __restrict__ pbm = tr + SX_TBM * WID + kb;
@@ -587,7 +599,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
         const float mR = grp_from_next<G>(mr[0], 0.0f);  // mn at the next lane's first k
         // D chain (descending): dc[k] = mnext*p_dm[k] + dc[k+1]*p_dd[k] + xe
         {
-            const float mnx = mR * eo[Q];
+            const float mnx = mR * eo[Q + 1];
             dr[Q - 1] = fmaf(mnx, pdm[Q - 1], xe);
         }
 #pragma unroll
@@ -611,7 +623,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
         float lmax = SX_NEG_INF;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            const float mnx = (q < Q - 1 ? mr[q + 1] : mR) * eo[q + 1];
+            const float mnx = q < Q - 1 ? mr[q + 1] * eo[q + 1] : mR * eo[Q + 1];
             const float iv = ir[q] * ins;
             const float dn1 = q < Q - 1 ? dr[q + 1] : dR;
             const float mv = fmaf(dn1, pmd[q], fmaf(mnx, pmm[q], iv * pmi[q]) + xe);
@@ -722,13 +734,13 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
             if (STAGE == ST_MSV || STAGE == ST_VIT) {
                 const float* src = P.msc + P.off[ti.prof];
                 for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
-                    const int x = idx / WID, k = idx - x * WID;
+                    const int x = idx / WID, k = slot_k<Q, G>(idx - x * WID);
                     lds[idx] = k < w ? src[x * w + k] : SX_NEG_INF;
                 }
             } else {
                 const float* src = P.mo + P.off[ti.prof];
                 for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
-                    const int x = idx / WID, k = idx - x * WID;
+                    const int x = idx / WID, k = slot_k<Q, G>(idx - x * WID);
                     lds[idx] = k < w ? src[x * w + k] : 0.0f;
                 }
             }
@@ -736,7 +748,7 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
                 const float* src = (STAGE == ST_VIT ? P.tsc : P.tp) + P.toff[ti.prof];
                 const float pad = STAGE == ST_VIT ? SX_NEG_INF : 0.0f;
                 for (int idx = threadIdx.x; idx < SX_NT * WID; idx += blockDim.x) {
-                    const int r = idx / WID, k = idx - r * WID;
+                    const int r = idx / WID, k = slot_k<Q, G>(idx - r * WID);
                     lds[NTAB_E + idx] = k < w ? src[r * w + k] : pad;
                 }
             }
@@ -811,9 +823,10 @@ struct SxKernelEntry {
         SXK_E(ST, Q, 16, SP, H), SXK_E(ST, Q, 32, SP, H), SXK_E(ST, Q, 64, SP, H)
 
 // Viterbi / Forward / Backward shapes: transitions in registers (HOIST) for Q <= 16, in LDS
-// for Q >= 16; the autotuner picks among them per model length.
+// for Q >= 16; the autotuner picks among them per model length (and MSV's wider menu).
 #define SXK_DP3(ST)                                                                             \
-    SXK_ALLG(ST, 4, false, true), SXK_ALLG(ST, 8, false, true), SXK_ALLG(ST, 16, false, true),  \
+    SXK_ALLG(ST, 4, false, true), SXK_ALLG(ST, 8, false, true), SXK_ALLG(ST, 12, false, true),  \
+        SXK_ALLG(ST, 16, false, true),                                                          \
         SXK_ALLG(ST, 16, false, false), SXK_ALLG(ST, 32, false, false), SXK_ALLG(ST, 48, false, false), \
         SXK_E(ST, 64, 1, false, false), SXK_E(ST, 64, 2, false, false), SXK_E(ST, 64, 4, false, false)
 
@@ -825,6 +838,17 @@ static const SxKernelEntry k_table[] = {
     SXK_E(0, 160, 1, false, false), SXK_E(0, 192, 1, false, false), SXK_E(0, 256, 1, false, false),
     SXK_E(0, 48, 2, false, false), SXK_E(0, 96, 2, false, false), SXK_E(0, 128, 2, false, false),
     SXK_E(0, 48, 4, false, false),
+    // finer capacities for the common model lengths
+    SXK_E(0, 40, 1, false, false), SXK_E(0, 56, 1, false, false), SXK_E(0, 72, 1, false, false),
+    SXK_E(0, 80, 1, false, false), SXK_E(0, 112, 1, false, false), SXK_E(0, 144, 1, false, false),
+    SXK_E(0, 176, 1, false, false), SXK_E(0, 208, 1, false, false), SXK_E(0, 224, 1, false, false),
+    SXK_E(0, 24, 4, false, false), SXK_E(0, 40, 4, false, false), SXK_E(0, 56, 4, false, false),
+    SXK_E(0, 72, 4, false, false), SXK_E(0, 80, 4, false, false),
+    SXK_E(0, 24, 8, false, false), SXK_E(0, 40, 8, false, false), SXK_E(0, 48, 8, false, false),
+    SXK_E(0, 56, 8, false, false),
+    SXK_E(0, 12, 16, false, false), SXK_E(0, 20, 16, false, false), SXK_E(0, 24, 16, false, false),
+    SXK_E(0, 40, 16, false, false), SXK_E(0, 48, 16, false, false),
+    SXK_E(0, 24, 32, false, false), SXK_E(0, 40, 32, false, false), SXK_E(0, 48, 32, false, false),
     // Viterbi, Forward, Backward: three rows per lane
     SXK_DP3(1), SXK_DP3(2), SXK_DP3(3),
     // special-row writers for the host-side domain definition

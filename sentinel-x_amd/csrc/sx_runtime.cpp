@@ -460,8 +460,10 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
     std::vector<TuneRecord> log;
     if (profs.empty() || nt == 0) return log;
     ensure_pidx();
-    // target sample: every k-th id of the length-sorted order, at most 16384 ids
-    const int nts = (int)std::min<uint32_t>(nt, 16384);
+    // target sample: every k-th id of the length-sorted order (all of them up to 2^20):
+    // a small sample would leave the persistent grid under-filled for shapes with many
+    // pairs per tile and misjudge them
+    const int nts = (int)std::min<uint32_t>(nt, 1u << 20);
     std::vector<int32_t> ts(nts);
     int64_t sum_l = 0;
     for (int i = 0; i < nts; i++) {
@@ -488,7 +490,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         if (inb.empty()) continue;
         // up to 16 profiles, spread over the bucket, repeated if the bucket is small
         std::vector<int32_t> sample;
-        const int want = (int)std::clamp<int64_t>((int64_t)(4e10 / ((double)mmax * (double)sum_l)), 1, 16);
+        const int want = (int)std::clamp<int64_t>((int64_t)(6e10 / ((double)mmax * (double)sum_l)), 1, 16);
         for (int i = 0; i < want; i++) sample.push_back(inb[(size_t)((double)i * inb.size() / want)]);
         int64_t cells = 0;
         for (int32_t p : sample) cells += (int64_t)profs[p].m * sum_l;

@@ -13,16 +13,18 @@ pytestmark = pytest.mark.gpu
 
 FWD_TOL = 1e-3  # nats, absolute, against the FP64 oracle
 
-MSV_SHAPES = [(q, g) for q in (4, 8, 16, 32, 64) for g in (1, 2, 4, 8, 16, 32, 64)] + [(48, 1), (96, 1), (48, 2), (96, 2), (48, 4)] + \
-    [(128, 1), (128, 2), (160, 1), (192, 1), (256, 1)]
-DP3_SHAPES = [(q, g, h) for q, h in ((4, 1), (8, 1), (16, 1), (16, 0), (32, 0), (48, 0))
+MSV_SHAPES = [(q, g) for q in (4, 8, 16, 32, 64) for g in (1, 2, 4, 8, 16, 32, 64)] + \
+    [(48, 1), (96, 1), (48, 2), (96, 2), (48, 4), (128, 1), (128, 2), (160, 1), (192, 1), (256, 1)] + \
+    [(q, 1) for q in (40, 56, 72, 80, 112, 144, 176, 208, 224)] + [(q, 4) for q in (24, 40, 56, 72, 80)] + \
+    [(q, 8) for q in (24, 40, 48, 56)] + [(q, 16) for q in (12, 20, 24, 40, 48)] + [(q, 32) for q in (24, 40, 48)]
+DP3_SHAPES = [(q, g, h) for q, h in ((4, 1), (8, 1), (12, 1), (16, 1), (16, 0), (32, 0), (48, 0))
               for g in (1, 2, 4, 8, 16, 32, 64)] + [(64, 1, 0), (64, 2, 0), (64, 4, 0)]
 EDGE_LENGTHS = [1, 2, 3, 15, 16, 17, 31, 32, 33, 47, 48, 64, 65]
 
 
 def lds_ok(stage, q, g):
-    cap = q * g
-    wid = cap + 2 if (cap + 2) % 2 else cap + 3
+    w0 = g * (q + 1) + 2
+    wid = w0 if w0 % 2 else w0 + 1
     rows = 25 if stage == 0 else 33
     return rows * wid * 4 <= 160 * 1024
 
