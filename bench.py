@@ -124,6 +124,9 @@ def main():
             t0 = time.time()
             ctx.autotune(st, args.tune_ms)
             log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
+            for row in ctx.autotune_log(st):
+                log(f"[tune] stage {st} bucket<= {row[0]:5d} Q={row[1]:3d} G={row[2]:2d} reg={row[3]} "
+                    f"{row[4]:9.1f} GCUPS")
     graph = not args.no_graph
     for i in range(args.warmup):
         s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)

@@ -150,6 +150,10 @@ int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, cons
 /* Read the installed table: up to cap rows; n receives the row count. */
 int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q, int32_t* g,
                   int32_t* hoist);
+/* The measurements of the last sx_autotune of `stage`: one row per (bucket, shape)
+ * timed, with the bucket's max_m and the measured GCUPS; n receives the row count. */
+int sx_autotune_log(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q, int32_t* g,
+                    int32_t* hoist, double* gcups);
 /* Shape used for a model length (form = SX_FORM_PLAIN or SX_FORM_BATCHED). */
 int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g, int32_t* hoist);
 

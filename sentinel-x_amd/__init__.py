@@ -26,6 +26,7 @@ EXPORTS = [
     "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
     "sx_run_cascade", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
+    "sx_autotune_log",
 ]
 
 
@@ -103,6 +104,7 @@ def lib():
                                    ctypes.POINTER(i32)]
         L.sx_set_shapes.argtypes = [vp, ctypes.c_int, u32, vp, vp, vp, vp]
         L.sx_get_shapes.argtypes = [vp, ctypes.c_int, u32, ctypes.POINTER(u32), vp, vp, vp, vp]
+        L.sx_autotune_log.argtypes = [vp, ctypes.c_int, u32, ctypes.POINTER(u32), vp, vp, vp, vp, vp]
         for name in EXPORTS:
             getattr(L, name)  # raises AttributeError if the library lacks a declared entry point
         _lib = L
@@ -282,6 +284,17 @@ class Context:
         self._chk(self._L.sx_get_shapes(self._h, stage, cap, ctypes.byref(n), _ptr(mm), _ptr(q), _ptr(g), _ptr(h)))
         k = min(n.value, cap)
         return [(int(mm[i]), int(q[i]), int(g[i]), int(h[i])) for i in range(k)]
+
+    def autotune_log(self, stage: int):
+        """Rows (bucket max_m, q, g, hoist, gcups) measured by the last autotune of `stage`."""
+        n = ctypes.c_uint32()
+        self._chk(self._L.sx_autotune_log(self._h, stage, 0, ctypes.byref(n), None, None, None, None, None))
+        k = n.value
+        mm, q, g, h = (np.zeros(max(k, 1), np.int32) for _ in range(4))
+        gc = np.zeros(max(k, 1), np.float64)
+        self._chk(self._L.sx_autotune_log(self._h, stage, k, ctypes.byref(n), _ptr(mm), _ptr(q), _ptr(g), _ptr(h),
+                                          _ptr(gc)))
+        return [(int(mm[i]), int(q[i]), int(g[i]), int(h[i]), float(gc[i])) for i in range(k)]
 
     def shape_for(self, stage: int, m: int, form: int = FORM_BATCHED):
         q, g, h = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()

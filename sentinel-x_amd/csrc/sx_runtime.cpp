@@ -476,7 +476,20 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
     DeviceBuf dout;
     dout.ensure((size_t)nt * 8);
     std::vector<std::pair<int, Shape>> table;
-    double spent = 0.0;
+    // buckets present in this profile set share the budget equally
+    std::vector<int> present;
+    {
+        int lo0 = 0;
+        for (int hi : edges) {
+            for (const auto& p : profs)
+                if (p.m > lo0 && p.m <= hi) {
+                    present.push_back(hi);
+                    break;
+                }
+            lo0 = hi;
+        }
+    }
+    const double per_bucket = budget_ms / std::max<size_t>(1, present.size());
     int lo = 0;
     for (int hi : edges) {
         std::vector<int32_t> inb;
@@ -490,23 +503,36 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         if (inb.empty()) continue;
         // up to 16 profiles, spread over the bucket, repeated if the bucket is small
         std::vector<int32_t> sample;
-        const int want = (int)std::clamp<int64_t>((int64_t)(6e10 / ((double)mmax * (double)sum_l)), 1, 16);
+        const int want = (int)std::clamp<int64_t>((int64_t)(4e10 / ((double)mmax * (double)sum_l)), 1, 16);
         for (int i = 0; i < want; i++) sample.push_back(inb[(size_t)((double)i * inb.size() / want)]);
         int64_t cells = 0;
         for (int32_t p : sample) cells += (int64_t)profs[p].m * sum_l;
+        // candidates: every shape covering the bucket's longest model with at most twice its
+        // capacity (at least the three tightest), tightest first, then fewer lanes per pair
         std::vector<Shape> cands;
         for (const auto& kv : kernels) {
             const KernelInfo& k = kv.second;
             if (k.stage != stage || k.spec) continue;
-            const long cap = (long)k.q * k.g;
-            if (cap < mmax || cap > 4L * std::max(mmax, 16)) continue;
+            if ((long)k.q * k.g < mmax) continue;
             if (lds_bytes(stage, k.wid) > LDS_MAX) continue;
             cands.push_back({k.q, k.g, k.hoist != 0});
         }
+        std::sort(cands.begin(), cands.end(), [](const Shape& a, const Shape& b) {
+            const long ca = (long)a.q * a.g, cb = (long)b.q * b.g;
+            if (ca != cb) return ca < cb;
+            if (a.g != b.g) return a.g < b.g;
+            return a.hoist > b.hoist;
+        });
+        {
+            std::vector<Shape> keep;
+            for (size_t i = 0; i < cands.size(); i++)
+                if (i < 3 || (long)cands[i].q * cands[i].g <= 2L * std::max(mmax, 16)) keep.push_back(cands[i]);
+            cands.swap(keep);
+        }
         Shape best = default_shape(stage, SX_FORM_BATCHED, mmax);
-        double best_ms = 1e300;
+        double best_ms = 1e300, spent = 0.0;
         for (const Shape& s : cands) {
-            if (spent > budget_ms && best_ms < 1e300) break;
+            if (spent > per_bucket && best_ms < 1e300) break;
             forced[stage] = s;
             try {
                 run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);  // warm-up
@@ -524,6 +550,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
                 throw;
             }
             forced.erase(stage);
+            while (pgroups_keep.size() > 64) pgroups_keep.pop_front();
         }
         // a row never claims more than the shape covers; longer models fall through
         table.push_back({std::min<int>(hi, best.q * best.g), best});
@@ -864,7 +891,22 @@ extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, 
 
 extern "C" int sx_autotune(sx_ctx* ctx, int stage, double budget_ms) {
     if (!ctx || stage < 0 || stage > 3) return SX_ERR_ARG;
-    return guard(ctx, [&] { (void)ctx->c->autotune(stage, budget_ms); });
+    return guard(ctx, [&] { ctx->c->last_tune[stage] = ctx->c->autotune(stage, budget_ms); });
+}
+
+extern "C" int sx_autotune_log(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q,
+                               int32_t* g, int32_t* hoist, double* gcups) {
+    if (!ctx || !n || stage < 0 || stage > 3) return SX_ERR_ARG;
+    const auto& L = ctx->c->last_tune[stage];
+    *n = (uint32_t)L.size();
+    for (uint32_t i = 0; i < *n && i < cap; i++) {
+        if (max_m) max_m[i] = L[i].max_m;
+        if (q) q[i] = L[i].shape.q;
+        if (g) g[i] = L[i].shape.g;
+        if (hoist) hoist[i] = L[i].shape.hoist ? 1 : 0;
+        if (gcups) gcups[i] = L[i].gcups;
+    }
+    return SX_OK;
 }
 
 extern "C" int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,

@@ -100,6 +100,7 @@ struct Ctx {
     std::map<int, std::vector<std::pair<int, Shape>>> tuned;
     std::map<int, Shape> forced;  // autotuner: every model of the stage uses this shape
     std::vector<TuneRecord> autotune(int stage, double budget_ms);
+    std::vector<TuneRecord> last_tune[4];
     Shape default_shape(int stage, int form, int m) const;
     Shape shape_for(int stage, int form, int m) const;
     Shape shape_for_spec(int stage, int m) const;
