@@ -80,6 +80,28 @@ def cpu_baseline(wl, recs, f, budget_s, threads=None):
             "check": {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}}
 
 
+def plain_leg(sxa, ctx, wl, n_plain, f, graph):
+    """The plain form (one launch per profile and stage) on the first n_plain profiles, and
+    the batched form on exactly the same pairs, with the tuned shapes of ctx."""
+    key = lambda a: np.lexsort((a["target"], a["profile"]))
+    with sxa.Context(0) as cp:
+        cp.set_profiles(wl.hmms[:n_plain])
+        cp.set_targets(wl.seqs)
+        for st in (0, 1, 2):
+            cp.set_shapes(st, ctx.get_shapes(st))
+        sp_cells = sum(h.m for h in wl.hmms[:n_plain]) * sum(len(s) for s in wl.seqs)
+        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, **f)
+        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        same = bool(np.array_equal(prec[key(prec)][["profile", "target", "stage"]],
+                                   brec[key(brec)][["profile", "target", "stage"]]))
+    return {"profiles": n_plain, "cells": sp_cells, "ms": round(pl.ms_total, 3),
+            "gcups": round(sp_cells / pl.ms_total / 1e6, 2),
+            "batched_same_pairs_gcups": round(sp_cells / bt.ms_total / 1e6, 2),
+            "ratio_batched_over_plain": round(pl.ms_total / bt.ms_total, 2),
+            "identical_decisions": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +110,7 @@ def main():
     ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--profile-scale", type=float, default=1.0)
     ap.add_argument("--target-scale", type=float, default=1.0)
-    ap.add_argument("--plain-sample", type=float, default=0.02, help="fraction of profiles run in the plain form")
+    ap.add_argument("--plain-sample", type=float, default=0.02, help="fraction of profiles run in the plain form (0 = skip)")
     ap.add_argument("--tune-ms", type=float, default=3000.0, help="autotuner budget per stage (0 = built-in)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -155,24 +177,10 @@ def main():
                                         erecs[key(erecs)][["profile", "target", "stage"]]))
 
     # plain form on a profile sample, and the batched form on exactly the same pairs
-    n_plain = max(1, int(round(len(ms) * args.plain_sample)))
-    with sxa.Context(0) as cp:
-        cp.set_profiles(wl.hmms[:n_plain])
-        cp.set_targets(wl.seqs)
-        for st in (0, 1, 2):
-            cp.set_shapes(st, ctx.get_shapes(st))
-        sp_cells = int(ms[:n_plain].sum()) * int(lens.sum())
-        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, **f)
-        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
-        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
-        plain_same = bool(np.array_equal(prec[key(prec)][["profile", "target", "stage"]],
-                                         brec[key(brec)][["profile", "target", "stage"]]))
-    plain = {"profiles": n_plain, "cells": sp_cells, "ms": round(pl.ms_total, 3),
-             "gcups": round(sp_cells / pl.ms_total / 1e6, 2),
-             "batched_same_pairs_gcups": round(sp_cells / bt.ms_total / 1e6, 2),
-             "ratio_batched_over_plain": round(pl.ms_total / bt.ms_total, 2),
-             "identical_decisions": plain_same}
-    log(f"[bench] plain form: {plain}")
+    plain = None
+    if args.plain_sample > 0:
+        plain = plain_leg(sxa, ctx, wl, max(1, int(round(len(ms) * args.plain_sample))), f, graph)
+        log(f"[bench] plain form: {plain}")
 
     achieved = e.cells_msv * OPS_PER_CELL[0] / (e.ms_msv * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",

@@ -106,9 +106,9 @@ int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tg
  * form = SX_FORM_PLAIN runs it one profile at a time (one launch per stage and
  * profile, survivor tests on the host between launches).
  *
- * out receives one record per MSV survivor (pair that reached the bias filter), in
- * (profile, target) order, up to cap records; *n_out is the total number, which may
- * exceed cap (then only cap are written).  stage = 1 rejected by the bias-corrected MSV
+ * out receives one record per MSV survivor (pair that reached the bias filter), up to
+ * cap records, in no particular order (sort on (profile, target) for a canonical one);
+ * *n_out is the total number, which may exceed cap (then only cap are written).  stage = 1 rejected by the bias-corrected MSV
  * test, 2 by Viterbi, 3 by Forward, 4 passed every filter; vit / fwd are NaN for
  * stages not reached. */
 typedef struct {
@@ -131,6 +131,10 @@ typedef struct {
 
 int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
                    uint64_t* n_out);
+/* The records of the last sx_run_cascade on ctx, without running it again: up to cap
+ * of them into out; *n_out is the total.  A caller that does not know the survivor
+ * count runs the cascade with cap = 0, sizes out from *n_out and fetches them here. */
+int sx_cascade_records(sx_ctx* ctx, sx_pair_scores* out, uint64_t cap, uint64_t* n_out);
 
 /* Lane-group shapes.  A pair is scored by a group of G lanes of one wavefront, each lane
  * holding Q consecutive model positions of the DP row in registers (capacity Q*G >= M);

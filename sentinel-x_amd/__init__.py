@@ -25,7 +25,7 @@ STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward"}
 EXPORTS = [
     "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
-    "sx_run_cascade", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
+    "sx_run_cascade", "sx_cascade_records", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
     "sx_autotune_log",
 ]
 
@@ -99,6 +99,7 @@ def lib():
         L.sx_specials.argtypes = [vp, ctypes.c_int, vp, vp, u64, vp, vp]
         L.sx_run_cascade.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sx_cascade_stats), vp, u64,
                                      ctypes.POINTER(u64)]
+        L.sx_cascade_records.argtypes = [vp, vp, u64, ctypes.POINTER(u64)]
         L.sx_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]
@@ -252,19 +253,28 @@ class Context:
 
     def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, use_graph: bool = True,
                     cap: Optional[int] = None):
-        """Returns (stats, records) for every MSV survivor (see sx_run_cascade)."""
+        """Returns (stats, records) for every MSV survivor (see sx_run_cascade).  With cap the
+        records are truncated to cap; without it all of them are fetched after the run."""
         st = sx_cascade_stats()
         st.f1, st.f2, st.f3 = f1, f2, f3
         st.use_graph = 1 if use_graph else 0
-        if cap is None:
-            cap = 1 << 16
-        while True:
+        n = ctypes.c_uint64()
+        if cap is not None:
             out = np.empty(cap, PAIR_DTYPE)
-            n = ctypes.c_uint64()
             self._chk(self._L.sx_run_cascade(self._h, form, ctypes.byref(st), _ptr(out), cap, ctypes.byref(n)))
-            if n.value <= cap:
-                return st, out[:n.value].copy()
+            return st, out[:min(cap, n.value)].copy()
+        self._chk(self._L.sx_run_cascade(self._h, form, ctypes.byref(st), None, 0, ctypes.byref(n)))
+        return st, self.cascade_records(int(n.value))
+
+    def cascade_records(self, cap: Optional[int] = None):
+        """Records of the last run_cascade (all of them, or the first cap)."""
+        n = ctypes.c_uint64()
+        if cap is None:
+            self._chk(self._L.sx_cascade_records(self._h, None, 0, ctypes.byref(n)))
             cap = int(n.value)
+        out = np.empty(cap, PAIR_DTYPE)
+        self._chk(self._L.sx_cascade_records(self._h, _ptr(out), cap, ctypes.byref(n)))
+        return out[:min(cap, n.value)]
 
     def autotune(self, stage: int, budget_ms: float = 2000.0):
         self._chk(self._L.sx_autotune(self._h, stage, budget_ms))

@@ -43,6 +43,7 @@ __device__ __forceinline__ int block_rank(bool pass, int* s_wave, int* total) {
 }  // namespace sxc
 
 // MSV test over every pair, enumerated as (profile position pi, length-sorted target j).
+// usc is indexed [profile][j] and nullsc [j], so the reads are coalesced.
 // Phase 0 writes block counts, phase 1 writes the survivors at their scanned offsets.
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__ nullsc,
@@ -59,8 +60,8 @@ sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__
         const int64_t j = pos - (int64_t)pi * T;
         p = perm[pi];
         t = sorted[j];
-        u = usc[(int64_t)p * T + t];
-        const float y = (u - nullsc[t]) / SXC_LN2F;
+        u = usc[(int64_t)p * T + j];
+        const float y = (u - nullsc[j]) / SXC_LN2F;
         pass = y >= thr[p];
     }
     int tot;

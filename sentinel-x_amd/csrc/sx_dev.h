@@ -64,7 +64,8 @@ struct SxWork {
     int32_t nchunks;
     int32_t chunk;
     int32_t ntids;
-    int32_t pad;
+    int32_t out_by_pos;       // implicit form: result index = profile * out_stride + position
+                              // in tids (0: + target id)
     int64_t out_stride;
     unsigned long long* head; // persistent-queue head; nullptr -> tile = blockIdx.x (+ base)
     int64_t tile_base;

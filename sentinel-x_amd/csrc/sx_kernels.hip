@@ -42,6 +42,13 @@ __device__ __forceinline__ float dppf(float old, float src) {
         __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, RM, BM, BC));
 }
 
+// Full-permutation DPP read (every lane has a source lane): no old value, so the
+// compiler can fold it into the consuming VALU instruction as a DPP operand.
+template <int CTRL>
+__device__ __forceinline__ float dppm(float src) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), CTRL, 0xF, 0xF, true));
+}
+
 __device__ __forceinline__ float rdlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -68,10 +75,10 @@ struct OpAdd {
 // (each butterfly stage pairs lanes symmetrically and + / max are commutative).
 template <int G, class Op>
 __device__ __forceinline__ float grp_reduce(float v, Op op) {
-    if constexpr (G >= 2) v = op(v, dppf<0xB1>(v, v));   // quad_perm [1,0,3,2]: xor 1
-    if constexpr (G >= 4) v = op(v, dppf<0x4E>(v, v));   // quad_perm [2,3,0,1]: xor 2
-    if constexpr (G >= 8) v = op(v, dppf<0x141>(v, v));  // row_half_mirror
-    if constexpr (G >= 16) v = op(v, dppf<0x140>(v, v)); // row_mirror
+    if constexpr (G >= 2) v = op(v, dppm<0xB1>(v));   // quad_perm [1,0,3,2]: xor 1
+    if constexpr (G >= 4) v = op(v, dppm<0x4E>(v));   // quad_perm [2,3,0,1]: xor 2
+    if constexpr (G >= 8) v = op(v, dppm<0x141>(v));  // row_half_mirror
+    if constexpr (G >= 16) v = op(v, dppm<0x140>(v)); // row_mirror
     if constexpr (G == 32) {
         float a = rdlane(v, 0), b = rdlane(v, 16), c = rdlane(v, 32), d = rdlane(v, 48);
         v = lane64() < 32 ? op(a, b) : op(c, d);
@@ -292,11 +299,12 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tb
     float row[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) row[q] = SX_NEG_INF;
-    float xn = 0.0f, xb = tmove, xj = SX_NEG_INF, xc = SX_NEG_INF;
-    FwdStream rs;
-    rs.init(seq);
-    for (int i = 0; i < L; i++) {
-        const int x = rs.next();
+    // dp.rs:102 gives E->J and E->C the same score (-ln 2) and J, C start at -inf, so
+    // xJ and xC follow one recurrence and are equal on every row: xC is not kept.
+    // xB = max(xN + tmove, xJ + tmove) is computed as max(xN, xJ) + tmove, which is the
+    // same f32 value because rounding is monotone.
+    float xn = 0.0f, xb = tmove, xj = SX_NEG_INF;
+    auto step = [&](int x) {
         const float* __restrict__ sc = em + x * WID + kb;
         const float be = xb + tbm;
         const float left = grp_from_prev<G>(row[Q - 1], SX_NEG_INF);
@@ -314,11 +322,36 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tb
         }
         const float xe = grp_reduce<G>(e, OpMax());
         xj = fmaxf(xj + tloop, xe - SX_LN2F);
-        xc = fmaxf(xc + tloop, xe - SX_LN2F);
         xn = xn + tloop;
-        xb = fmaxf(xn + tmove, xj + tmove);
+        xb = fmaxf(xn, xj) + tmove;
+    };
+    // Residues come one dword (four rows) at a time with the next dword loaded a dword
+    // ahead; targets start 16-byte aligned and the code buffer has 64 bytes of padding
+    // after the last one, so the look-ahead stays inside it.
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(seq);
+    if constexpr (Q < 96) {
+        // four rows per loop trip: the byte position is fixed in each copy of the step
+        uint32_t nxt = wp[0];
+        for (int i = 0;;) {
+            const uint32_t w = nxt;
+            nxt = wp[1];
+            ++wp;
+            step((int)(w & 0xFFu));
+            if (++i == L) break;
+            step((int)((w >> 8) & 0xFFu));
+            if (++i == L) break;
+            step((int)((w >> 16) & 0xFFu));
+            if (++i == L) break;
+            step((int)(w >> 24));
+            if (++i == L) break;
+        }
+    } else {
+        // one row per trip: four copies of a wide row would not fit the register file
+        FwdStream rs;
+        rs.init(seq);
+        for (int i = 0; i < L; i++) step(rs.next());
     }
-    return xc + tmove;
+    return xj + tmove;
 }
 
 // ---------------------------------------------------------------------------
@@ -682,7 +715,7 @@ enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3 };
 struct TileInfo {
     int32_t prof, count;
     int64_t first, out;
-    bool by_tid;  // implicit all-pairs form: result index = out + target id
+    bool by_tid;  // result index = out + target id (else out + lane group)
 };
 
 __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
@@ -701,8 +734,8 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
         ti.first = (int64_t)c * w.chunk;
         const int rem = w.ntids - (int)ti.first;
         ti.count = rem < w.chunk ? rem : w.chunk;
-        ti.out = (int64_t)ti.prof * w.out_stride;
-        ti.by_tid = true;
+        ti.out = (int64_t)ti.prof * w.out_stride + (w.out_by_pos ? ti.first : 0);
+        ti.by_tid = !w.out_by_pos;
     }
     return ti;
 }

@@ -135,6 +135,8 @@ struct CascadeState {
     uint32_t ntarg = 0;
     double f1 = -1, f2 = -1, f3 = -1;
     int64_t cap1 = 0;
+    // records of the last run (one per MSV survivor), kept for sx_cascade_records
+    std::vector<sx_pair_scores> last;
     // host-side tables
     std::vector<float> thr_msv, thr_vit, thr_fwd, nullsc;
     std::vector<int32_t> perm;           // profile ids by increasing model length
@@ -241,7 +243,12 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     up(S.d_thr_msv, S.thr_msv.data(), P * 4);
     up(S.d_thr_vit, S.thr_vit.data(), P * 4);
     up(S.d_thr_fwd, S.thr_fwd.data(), P * 4);
-    up(S.d_nullsc, S.nullsc.data(), (size_t)T * 4);
+    {
+        // null scores in the length-sorted order the MSV test walks
+        std::vector<float> ns(T);
+        for (uint32_t j = 0; j < T; j++) ns[j] = S.nullsc[c.sorted[j]];
+        up(S.d_nullsc, ns.data(), (size_t)T * 4);
+    }
     up(S.d_perm, S.perm.data(), P * 4);
     up(S.d_eo1, eo1.data(), eo1.size() * 8);
     up(S.d_ng_vit, ngv.data(), P * 4);
@@ -302,6 +309,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         w.nchunks = (int32_t)((T + w.chunk - 1) / w.chunk);
         w.ntiles = (int64_t)w.nchunks * gr.n;
         w.out_stride = T;
+        w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
         const int grid = c.persistent_grid(k, Ctx::lds_bytes(0, k.wid), w.ntiles);
         c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot++ % 32);
     }
@@ -456,7 +464,8 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
         st->n_msv_pass = (uint64_t)n1;
         st->n_bias_pass = (uint64_t)n2;
         st->n_vit_pass = (uint64_t)n3;
-        std::vector<sx_pair_scores> recs((size_t)n1);
+        std::vector<sx_pair_scores>& recs = S.last;
+        recs.resize((size_t)n1);
         if (n1) HIPCHK(hipMemcpy(recs.data(), S.recs.p, (size_t)n1 * sizeof(sx_pair_scores), hipMemcpyDeviceToHost));
         uint64_t npass = 0, cv = 0, cf = 0;
         for (const auto& r : recs) {
@@ -616,10 +625,22 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     st->cells_vit = cv;
     st->cells_fwd = cf;
     if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, recs.size()) * sizeof(sx_pair_scores));
-    return recs.size();
+    S.last.swap(recs);
+    return S.last.size();
 }
 
 }  // namespace sx
+
+extern "C" uint64_t sx_cascade_records_impl(sx::Ctx* c, sx_pair_scores* out, uint64_t cap) {
+    if (!c->cascade) return 0;
+    const std::vector<sx_pair_scores>& r = c->cascade->last;
+    if (out) std::memcpy(out, r.data(), std::min<uint64_t>(cap, r.size()) * sizeof(sx_pair_scores));
+    return r.size();
+}
+
+extern "C" void sx_cascade_records_clear(sx::Ctx* c) {
+    if (c->cascade) c->cascade->last.clear();
+}
 
 extern "C" int sx_run_cascade_impl(sx::Ctx* c, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
                                    uint64_t* n_out) {

@@ -17,3 +17,6 @@ float pass_threshold(const std::function<bool(float)>& rejects, int* violations)
 
 extern "C" int sx_run_cascade_impl(sx::Ctx* c, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
                                    uint64_t* n_out);
+// Records of the last cascade run on c: copies up to cap of them, returns how many there are.
+extern "C" uint64_t sx_cascade_records_impl(sx::Ctx* c, sx_pair_scores* out, uint64_t cap);
+extern "C" void sx_cascade_records_clear(sx::Ctx* c);

@@ -168,10 +168,11 @@ void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n)
     d_soff = dupload(off);
     d_len = dupload(tlen);
     d_lm = dupload(lm);
-    // length-sorted processing order (stable), used by the batched form
+    // processing order of the batched form: longest targets first (stable), so that a
+    // persistent launch ends on its shortest tiles and drains with a short tail
     sorted.resize(n);
     std::iota(sorted.begin(), sorted.end(), 0);
-    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return tlen[a] < tlen[b]; });
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return tlen[a] > tlen[b]; });
     d_sorted = dupload(sorted);
     std::vector<int32_t> ident(n);
     std::iota(ident.begin(), ident.end(), 0);
@@ -340,18 +341,27 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         const Shape s = shape_for(stage, form, profs[p].m);
         groups[{s.q, s.g, s.hoist ? 1 : 0}].push_back(p);
     }
+    // every group's profile list goes up in one copy, which completes before the host
+    // lists go out of scope (an async copy reads its host source when it executes)
+    std::vector<int32_t> flat;
+    std::vector<size_t> goff;
+    for (auto& kv : groups) {
+        goff.push_back(flat.size());
+        flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+    }
+    pgroups_keep.emplace_back();
+    DeviceBuf& pb = pgroups_keep.back();
+    pb.ensure(flat.size() * sizeof(int32_t));
+    HIPCHK(hipMemcpyAsync(pb.p, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     int slot = 0;
+    size_t gi = 0;
     for (auto& kv : groups) {
         const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
         const bool h = std::get<2>(kv.first) != 0;
         const KernelInfo& k = kernel(stage, q, g, false, h);
-        pgroups_keep.emplace_back();
-        DeviceBuf& pb = pgroups_keep.back();
-        pb.ensure(kv.second.size() * sizeof(int32_t));
-        HIPCHK(hipMemcpyAsync(pb.p, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                              stream));
         SxWork w{};
-        w.profs = (const int32_t*)pb.p;
+        w.profs = (const int32_t*)pb.p + goff[gi++];
         w.tids = dtids;
         w.chunk = BLOCK / g;
         w.ntids = ntids;
@@ -385,7 +395,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     if (form == SX_FORM_BATCHED)
         std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
             if (prof[a] != prof[b]) return prof[a] < prof[b];
-            return tlen[tgt[a]] < tlen[tgt[b]];
+            return tlen[tgt[a]] > tlen[tgt[b]];
         });
     else
         std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return prof[a] < prof[b]; });
@@ -424,9 +434,12 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     }
     // results come back in the sorted order; remember it for the caller
     last_order = order;
-    size_t ttot = 0;
-    for (auto& kv : tiles) ttot += kv.second.size();
-    scratch_tiles.ensure(ttot * sizeof(SxTile));
+    std::vector<SxTile> flat;
+    for (auto& kv : tiles) flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+    scratch_tiles.ensure(flat.size() * sizeof(SxTile));
+    HIPCHK(hipMemcpyAsync(scratch_tiles.p, flat.data(), flat.size() * sizeof(SxTile), hipMemcpyHostToDevice, stream));
+    // the host arrays above die with this call: the copies complete before it returns
+    HIPCHK(hipStreamSynchronize(stream));
     size_t toff = 0;
     int slot = 0;
     for (auto& kv : tiles) {
@@ -434,7 +447,6 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         const bool h = std::get<2>(kv.first) != 0;
         const KernelInfo& k = kernel(stage, q, g, spec, h);
         SxTile* dt = (SxTile*)scratch_tiles.p + toff;
-        HIPCHK(hipMemcpyAsync(dt, kv.second.data(), kv.second.size() * sizeof(SxTile), hipMemcpyHostToDevice, stream));
         toff += kv.second.size();
         SxWork w{};
         w.tiles = dt;
@@ -507,6 +519,9 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         for (int i = 0; i < want; i++) sample.push_back(inb[(size_t)((double)i * inb.size() / want)]);
         int64_t cells = 0;
         for (int32_t p : sample) cells += (int64_t)profs[p].m * sum_l;
+        // the same profiles twice: twice the cells with the same mix of lengths
+        std::vector<int32_t> sample2(sample);
+        sample2.insert(sample2.end(), sample.begin(), sample.end());
         // candidates: every shape covering the bucket's longest model with at most twice its
         // capacity (at least the three tightest), tightest first, then fewer lanes per pair
         std::vector<Shape> cands;
@@ -536,10 +551,24 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             forced[stage] = s;
             try {
                 run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);  // warm-up
-                const double ms = timed([&] {
+                const double t1 = timed([&] {
                     run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);
                 });
-                spent += 2 * ms;
+                double ms = t1;
+                spent += 2 * t1;
+                if (stage == 0) {
+                    // The MSV launches of a search cover thousands of profiles x every
+                    // target: rank shapes by the marginal time of a second copy of the
+                    // sample, in which the launch, the stream synchronisation and the
+                    // persistent grid's drain tail cancel.  The later stages' launches
+                    // cover survivor lists about the size of the sample, where the tail
+                    // is part of the cost, so they are ranked by the plain time.
+                    const double t2 = timed([&] {
+                        run_implicit(stage, SX_FORM_BATCHED, sample2, (const int32_t*)dts.p, nts, dout.p, 0);
+                    });
+                    spent += t2;
+                    ms = (t2 - t1 > 0.25 * t1) ? t2 - t1 : t2 / 2;
+                }
                 log.push_back({hi, s, ms, cells / ms / 1e6});
                 if (ms < best_ms) {
                     best_ms = ms;
@@ -914,10 +943,19 @@ extern "C" int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pa
     if (!ctx || !st || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED) || (cap && !out)) return SX_ERR_ARG;
     return guard(ctx, [&] {
         if (ctx->c->profs.empty() || ctx->c->nt == 0) {
+            sx_cascade_records_clear(ctx->c);
             if (n_out) *n_out = 0;
             return;
         }
         ctx->c->ensure_pidx();
         (void)sx_run_cascade_impl(ctx->c, form, st, out, cap, n_out);
+    });
+}
+
+extern "C" int sx_cascade_records(sx_ctx* ctx, sx_pair_scores* out, uint64_t cap, uint64_t* n_out) {
+    if (!ctx || (cap && !out)) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        const uint64_t n = sx_cascade_records_impl(ctx->c, out, cap);
+        if (n_out) *n_out = n;
     });
 }

@@ -94,6 +94,22 @@ def test_plain_and_graph_forms_agree_with_batched(gctx, calibrated):
         gctx.score_all(0, 1)
 
 
+def test_records_fetch_matches_truncated_run(gctx, calibrated):
+    hmms, _, seqs = calibrated
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    same = lambda a, b: a.tobytes() == b.tobytes()  # records hold NaN for stages not reached
+    st, full = gctx.run_cascade(1, use_graph=False, **F)
+    assert len(full) == st.n_msv_pass > 10
+    assert same(gctx.cascade_records(), full)
+    assert same(gctx.cascade_records(7), full[:7])
+    st2, cut = gctx.run_cascade(1, use_graph=False, cap=5, **F)
+    assert st2.n_msv_pass == st.n_msv_pass and len(cut) == 5
+    assert same(cut, full[:5])
+    _, plain = gctx.run_cascade(0, **F)
+    assert same(gctx.cascade_records(), plain)
+
+
 def test_cascade_requires_calibrated_profiles(orc, gctx, sxa):
     h = random_hmm(orc, 30, 1, stats=False)
     gctx.set_profiles([h])
