@@ -23,10 +23,12 @@ sys.path.insert(0, ROOT)
 
 from __graft_entry__ import load_package  # noqa: E402
 
-# Algorithmic VALU work per DP cell: MSV 1 add + 2 max; Viterbi 10 add + 6 max + 1 select;
-# Forward 17 flops (an FMA counts 2).
-OPS_PER_CELL = {0: 3, 1: 17, 2: 17}
-# MI355X vector ALU peak, non-packed f32 lane operations: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
+# Fewest VALU instructions an MSV cell can take on gfx950: one f32 add and one f32 max
+# (sc + max(prev, xB); no instruction fuses them), plus half a v_max3 for the running row
+# maximum xE.  Packed f32 forms are not used (and there is no packed max).
+MSV_INSTR_PER_CELL = 2.5
+# MI355X vector ALU issue peak: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-instructions/s
+# (each wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles).
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 
 
@@ -182,10 +184,22 @@ def main():
         plain = plain_leg(sxa, ctx, wl, max(1, int(round(len(ms) * args.plain_sample))), f, graph)
         log(f"[bench] plain form: {plain}")
 
-    achieved = e.cells_msv * OPS_PER_CELL[0] / (e.ms_msv * 1e-3) / 1e12
+    # dominant kernel: the longest MSV launch of the eager run (HIP events on the launch stream)
+    ls = ctx.cascade_launches()
+    msv = ls[ls["stage"] == 0]
+    d = msv[int(np.argmax(msv["ms"]))]
+    achieved = float(d["cells"]) * MSV_INSTR_PER_CELL / (float(d["ms"]) * 1e-3) / 1e12
+    stage_achieved = e.cells_msv * MSV_INSTR_PER_CELL / (e.ms_msv * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None, "kernel": "MSV stage (k_tiles, stage 0)",
-            "ops_per_cell": OPS_PER_CELL[0]}
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+            "kernel": f"k_tiles<Q={int(d['q'])},G={int(d['g'])},MSV>", "launch_ms": round(float(d["ms"]), 3),
+            "cells": int(d["cells"]), "instr_per_cell": MSV_INSTR_PER_CELL,
+            "msv_stage": {"launches": int(len(msv)), "achieved": round(stage_achieved, 3),
+                          "frac": round(stage_achieved / VALU_PEAK_TOPS, 4)}}
+    launches = [{"stage": int(r["stage"]), "q": int(r["q"]), "g": int(r["g"]), "reg": int(r["hoist"]),
+                 "ms": round(float(r["ms"]), 3), "gcups": round(float(r["cells"]) / max(float(r["ms"]), 1e-9) / 1e6, 1)}
+                for r in ls]
+    log(f"[bench] launches: {launches}")
 
     cpu = None
     if not args.no_cpu:
@@ -203,6 +217,7 @@ def main():
         "survivors": {"msv": int(e.n_msv_pass), "bias": int(e.n_bias_pass), "viterbi": int(e.n_vit_pass),
                       "forward": int(e.n_fwd_pass)},
         "graph_matches_eager": same_as_eager,
+        "launches": launches,
         "plain": plain,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -1,7 +1,8 @@
 /*
  * sx_hmmer.h -- C ABI of the MI355X implementation of sentinel-x's src/hmmer Plan7
- * search path (MSV -> bias -> Viterbi -> Forward/Backward filter cascade, plus the
- * host-side domain definition and hmmsearch-style reporting).
+ * search path: the MSV -> bias -> Viterbi -> Forward filter cascade over every
+ * (profile, target) pair, the DP stages on their own (MSV, Viterbi, Forward, Backward)
+ * and the Forward/Backward special-state rows that the host domain definition reads.
  *
  * The reference is a Rust module with no FFI (src/hmmer/mod.rs:23-30).  Each entry
  * point below names the reference interface it replaces; a Rust caller binds them with
@@ -135,6 +136,20 @@ int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* 
  * of them into out; *n_out is the total.  A caller that does not know the survivor
  * count runs the cascade with cap = 0, sizes out from *n_out and fetches them here. */
 int sx_cascade_records(sx_ctx* ctx, sx_pair_scores* out, uint64_t cap, uint64_t* n_out);
+
+/* Device time of every DP-stage kernel launch of the last batched run made without the
+ * graph (st->use_graph = 0), in launch order: the MSV launches (one per shape group,
+ * each over all its profiles x all targets), then the Viterbi and Forward launches (one
+ * per run of consecutive same-shape profiles, over the survivors).  ms is the time
+ * between HIP events recorded on the launch stream around the launch; cells counts
+ * M x L over the pairs the launch scored.  n receives the count (0 after a graph run).
+ * No counterpart in the reference (a CPU loop has no launches); used for measurement. */
+typedef struct {
+    int32_t stage, q, g, hoist;
+    double ms;
+    uint64_t cells;
+} sx_launch_stat;
+int sx_cascade_launches(sx_ctx* ctx, uint32_t cap, uint32_t* n, sx_launch_stat* out);
 
 /* Lane-group shapes.  A pair is scored by a group of G lanes of one wavefront, each lane
  * holding Q consecutive model positions of the DP row in registers (capacity Q*G >= M);

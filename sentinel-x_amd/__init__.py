@@ -25,7 +25,7 @@ STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward"}
 EXPORTS = [
     "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
-    "sx_run_cascade", "sx_cascade_records", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
+    "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
     "sx_autotune_log",
 ]
 
@@ -70,6 +70,9 @@ class sx_cascade_stats(ctypes.Structure):
 PAIR_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("msv", "<f4"), ("bias", "<f4"), ("vit", "<f4"),
                        ("fwd", "<f4"), ("fwd64", "<f8"), ("stage", "<i4"), ("pad", "<i4")])
 
+LAUNCH_DTYPE = np.dtype([("stage", "<i4"), ("q", "<i4"), ("g", "<i4"), ("hoist", "<i4"), ("ms", "<f8"),
+                         ("cells", "<u8")])
+
 _lib = None
 
 
@@ -100,6 +103,7 @@ def lib():
         L.sx_run_cascade.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sx_cascade_stats), vp, u64,
                                      ctypes.POINTER(u64)]
         L.sx_cascade_records.argtypes = [vp, vp, u64, ctypes.POINTER(u64)]
+        L.sx_cascade_launches.argtypes = [vp, u32, ctypes.POINTER(u32), vp]
         L.sx_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]
@@ -275,6 +279,15 @@ class Context:
         out = np.empty(cap, PAIR_DTYPE)
         self._chk(self._L.sx_cascade_records(self._h, _ptr(out), cap, ctypes.byref(n)))
         return out[:min(cap, n.value)]
+
+    def cascade_launches(self):
+        """Per-launch device time of the last eager batched run_cascade (see sx_cascade_launches)."""
+        n = ctypes.c_uint32()
+        self._chk(self._L.sx_cascade_launches(self._h, 0, ctypes.byref(n), None))
+        out = np.zeros(n.value, LAUNCH_DTYPE)
+        if n.value:
+            self._chk(self._L.sx_cascade_launches(self._h, n.value, ctypes.byref(n), _ptr(out)))
+        return out
 
     def autotune(self, stage: int, budget_ms: float = 2000.0):
         self._chk(self._L.sx_autotune(self._h, stage, budget_ms))

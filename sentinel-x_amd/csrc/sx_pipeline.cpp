@@ -148,6 +148,7 @@ struct CascadeState {
         Shape s;
         int32_t* plist;
         int n;
+        uint64_t cells;  // sum of M * target length over the group's pairs
     };
     std::vector<Group> msv_groups;
     struct Run {
@@ -161,9 +162,14 @@ struct CascadeState {
     DeviceBuf cnt3, off3, list3, per3, toff3, eoff3, tiles3, tids3, fsc, recs;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // one event after every kernel launch of the three DP stages (eager runs only)
+    std::vector<hipEvent_t> lev;
+    std::vector<sx_launch_stat> launches;  // per-launch timing of the last eager run
     ~CascadeState() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : lev)
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -233,8 +239,10 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
         S.msv_plists.emplace_back();
         int32_t* d = dptr<int32_t>(S.msv_plists.back(), kv.second.size());
         HIPCHK(hipMemcpy(d, kv.second.data(), kv.second.size() * 4, hipMemcpyHostToDevice));
+        uint64_t cells = 0;
+        for (int32_t p : kv.second) cells += (uint64_t)c.profs[p].m * (uint64_t)c.total_len;
         S.msv_groups.push_back({Shape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first) != 0}, d,
-                                (int)kv.second.size()});
+                                (int)kv.second.size(), cells});
     }
     auto up = [](DeviceBuf& b, const void* src, size_t bytes) {
         b.ensure(std::max<size_t>(bytes, 16));
@@ -255,6 +263,12 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     up(S.d_ng_fwd, ngf.data(), P * 4);
     for (auto& e : S.ev)
         if (!e) HIPCHK(hipEventCreate(&e));
+    const size_t nl = S.msv_groups.size() + S.vit_runs.size() + S.fwd_runs.size();
+    while (S.lev.size() < nl) {
+        S.lev.push_back(nullptr);
+        HIPCHK(hipEventCreate(&S.lev.back()));
+    }
+    S.launches.clear();
 }
 
 static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
@@ -311,7 +325,9 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         w.out_stride = T;
         w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
         const int grid = c.persistent_grid(k, Ctx::lds_bytes(0, k.wid), w.ntiles);
-        c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot++ % 32);
+        c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot % 32);
+        if (with_events) HIPCHK(hipEventRecord(S.lev[slot], st));
+        slot++;
     }
     if (with_events) HIPCHK(hipEventRecord(S.ev[1], st));
     // 2. MSV test and ordered compaction
@@ -360,6 +376,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         hipLaunchKernelGGL(sxc_gather_t, dim3(nbc), dim3(CB), 0, st, n_list, (const int32_t*)list.p,
                            (const int32_t*)s_t, (int32_t*)tids.p);
     };
+    int lslot = (int)S.msv_groups.size();
     auto stage_runs = [&](int stage, std::vector<CascadeState::Run>& runs, DeviceBuf& tl, DeviceBuf& tids,
                           DeviceBuf& toff, void* out) {
         for (auto& r : runs) {
@@ -372,6 +389,8 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
             w.ntiles = 0;
             const int grid = c.persistent_grid(k, Ctx::lds_bytes(stage, k.wid), 1 << 30);
             c.launch(k, stage, w, out, nosp, grid, true, slot++ % 32 + 32);
+            if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], st));
+            lslot++;
         }
     };
     const int64_t* n2 = compact((const uint8_t*)S.flag.p, n1, nullptr, S.cnt2, S.off2, S.list2, S.per2);
@@ -477,6 +496,43 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
         st->n_fwd_pass = npass;
         st->cells_vit = cv;
         st->cells_fwd = cf;
+        S.launches.clear();
+        if (!st->use_graph) {
+            // per-launch times between consecutive events; the Viterbi and Forward cells
+            // of each run come from the records of the pairs that reached the stage
+            std::vector<int32_t> pos(c.profs.size());
+            for (size_t i = 0; i < S.perm.size(); i++) pos[S.perm[i]] = (int32_t)i;
+            auto run_cells = [&](const std::vector<CascadeState::Run>& runs, int min_stage) {
+                std::vector<uint64_t> v(runs.size(), 0);
+                for (const auto& r : recs) {
+                    if (r.stage < min_stage) continue;
+                    const int64_t pi = pos[r.profile];
+                    for (size_t k = 0; k < runs.size(); k++)
+                        if (pi >= runs[k].a && pi < runs[k].b) {
+                            v[k] += (uint64_t)c.profs[r.profile].m * (uint64_t)c.tlen[r.target];
+                            break;
+                        }
+                }
+                return v;
+            };
+            const std::vector<uint64_t> vc = run_cells(S.vit_runs, 2), fc = run_cells(S.fwd_runs, 3);
+            size_t e = 0;
+            auto add = [&](int stage, const Shape& sh, hipEvent_t from, uint64_t cells) {
+                sx_launch_stat l{};
+                l.stage = stage;
+                l.q = sh.q;
+                l.g = sh.g;
+                l.hoist = sh.hoist ? 1 : 0;
+                l.ms = ev_ms(from, S.lev[e]);
+                l.cells = cells;
+                S.launches.push_back(l);
+                e++;
+            };
+            for (size_t i = 0; i < S.msv_groups.size(); i++)
+                add(0, S.msv_groups[i].s, i ? S.lev[e - 1] : S.ev[0], S.msv_groups[i].cells);
+            for (size_t i = 0; i < S.vit_runs.size(); i++) add(1, S.vit_runs[i].s, i ? S.lev[e - 1] : S.ev[2], vc[i]);
+            for (size_t i = 0; i < S.fwd_runs.size(); i++) add(2, S.fwd_runs[i].s, i ? S.lev[e - 1] : S.ev[3], fc[i]);
+        }
         if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, (uint64_t)n1) * sizeof(sx_pair_scores));
         return (uint64_t)n1;
     }
@@ -636,6 +692,13 @@ extern "C" uint64_t sx_cascade_records_impl(sx::Ctx* c, sx_pair_scores* out, uin
     const std::vector<sx_pair_scores>& r = c->cascade->last;
     if (out) std::memcpy(out, r.data(), std::min<uint64_t>(cap, r.size()) * sizeof(sx_pair_scores));
     return r.size();
+}
+
+extern "C" uint32_t sx_cascade_launches_impl(sx::Ctx* c, sx_launch_stat* out, uint32_t cap) {
+    if (!c->cascade) return 0;
+    const std::vector<sx_launch_stat>& v = c->cascade->launches;
+    if (out) std::memcpy(out, v.data(), std::min<size_t>(cap, v.size()) * sizeof(sx_launch_stat));
+    return (uint32_t)v.size();
 }
 
 extern "C" void sx_cascade_records_clear(sx::Ctx* c) {

@@ -20,3 +20,5 @@ extern "C" int sx_run_cascade_impl(sx::Ctx* c, int form, sx_cascade_stats* st, s
 // Records of the last cascade run on c: copies up to cap of them, returns how many there are.
 extern "C" uint64_t sx_cascade_records_impl(sx::Ctx* c, sx_pair_scores* out, uint64_t cap);
 extern "C" void sx_cascade_records_clear(sx::Ctx* c);
+// Per-launch timing of the last eager batched run: copies up to cap, returns the count.
+extern "C" uint32_t sx_cascade_launches_impl(sx::Ctx* c, sx_launch_stat* out, uint32_t cap);

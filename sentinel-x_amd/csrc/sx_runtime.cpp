@@ -952,6 +952,14 @@ extern "C" int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pa
     });
 }
 
+extern "C" int sx_cascade_launches(sx_ctx* ctx, uint32_t cap, uint32_t* n, sx_launch_stat* out) {
+    if (!ctx || (cap && !out)) return SX_ERR_ARG;
+    return guard(ctx, [&] {
+        const uint32_t k = sx_cascade_launches_impl(ctx->c, out, cap);
+        if (n) *n = k;
+    });
+}
+
 extern "C" int sx_cascade_records(sx_ctx* ctx, sx_pair_scores* out, uint64_t cap, uint64_t* n_out) {
     if (!ctx || (cap && !out)) return SX_ERR_ARG;
     return guard(ctx, [&] {

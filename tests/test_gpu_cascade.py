@@ -110,6 +110,23 @@ def test_records_fetch_matches_truncated_run(gctx, calibrated):
     assert same(gctx.cascade_records(), plain)
 
 
+def test_launch_timing_covers_each_stage(gctx, calibrated):
+    hmms, _, seqs = calibrated
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    st, _ = gctx.run_cascade(1, use_graph=False, **F)
+    ls = gctx.cascade_launches()
+    assert len(ls) >= 3 and set(ls["stage"]) == {0, 1, 2}
+    assert int(ls["cells"][ls["stage"] == 0].sum()) == st.cells_msv
+    assert int(ls["cells"][ls["stage"] == 1].sum()) == st.cells_vit
+    assert int(ls["cells"][ls["stage"] == 2].sum()) == st.cells_fwd
+    assert np.all(ls["ms"] >= 0)
+    # the MSV launches fill the MSV stage's event window
+    assert abs(float(ls["ms"][ls["stage"] == 0].sum()) - st.ms_msv) <= 0.05 + 0.01 * st.ms_msv
+    gctx.run_cascade(1, use_graph=True, **F)
+    assert len(gctx.cascade_launches()) == 0
+
+
 def test_cascade_requires_calibrated_profiles(orc, gctx, sxa):
     h = random_hmm(orc, 30, 1, stats=False)
     gctx.set_profiles([h])
