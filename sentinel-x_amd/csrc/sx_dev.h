@@ -35,7 +35,22 @@ struct SxProfiles {
     const int64_t* toff;   // per profile: offset of its 8 x (M+1) block in tsc / tp
     const int32_t* m;
     const float* msv_tbm;
+    // Long models (M > SX_GEM_MIN_M): padded copies of the emission rows read straight
+    // from global memory.  Row x of profile p starts at gmsc/gmo + goff[p] + x * gw, and
+    // position k sits at index k + 3; positions 0 and M+1 .. gw-4 hold the pad value
+    // (-inf in gmsc, 0 in gmo).  goff[p] = -1 for the other profiles.
+    const float* gmsc;
+    const float* gmo;
+    const int64_t* goff;
+    int32_t gw;
+    int32_t pad;
 };
+
+// Models longer than this use the global-emission kernels (64 lanes per pair); the
+// longest supported model is 64 x 64 = 4096 positions.
+#define SX_GEM_MIN_M 1024
+#define SX_MAX_M 4096
+#define SX_GEM_WIDTH 4104  // >= SX_MAX_M + 2 + 3, a multiple of 8
 
 struct SxTargets {
     const uint8_t* codes;

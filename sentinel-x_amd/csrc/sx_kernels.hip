@@ -287,15 +287,50 @@ __device__ __forceinline__ int slot_k(int pos) {
     return j * Q + (pos - j * Geo<Q, G>::STRIDE);
 }
 
+// Table modes of the Viterbi/Forward/Backward kernels (MSV has no transition rows):
+//   TM_LDS  emission and transition rows staged in LDS, transitions re-read every row;
+//   TM_REG  as TM_LDS, but the lane's transitions stay in registers across rows;
+//   TM_GEM  long models: emission rows read from a padded copy in global memory
+//           (through L1/L2), transitions staged in LDS and re-read every row.
+enum { TM_LDS = 0, TM_REG = 1, TM_GEM = 2 };
+
+// Padded global emission rows: position k of residue row x sits at x * ew + k + 3, so a
+// lane's block of Q positions (Q a multiple of 4) starts 16-byte aligned; positions 0
+// and beyond M hold the pad value (-inf scores, 0 odds).
+constexpr int GEM_SHIFT = 3;
+
+// Emission row access for one lane: row(x)[q] is the lane's q-th position of residue
+// row x; row(x)[NEXT] is the first position of the next lane of the group.
+template <int Q, int G, int TM>
+struct EmRows {
+    static constexpr int NEXT = TM == TM_GEM ? Q : Q + 1;
+    const float* base;
+    int width;
+    __device__ __forceinline__ EmRows(const float* em, int ew, int j) {
+        if constexpr (TM == TM_GEM) {
+            base = em + j * Q + 1 + GEM_SHIFT;
+            width = ew;
+        } else {
+            base = em + j * Geo<Q, G>::STRIDE + 1;
+            width = Geo<Q, G>::WID;
+        }
+    }
+    __device__ __forceinline__ const float* row(int x) const {
+        if constexpr (TM == TM_GEM)
+            return static_cast<const float*>(__builtin_assume_aligned(base + x * width, 16));
+        else
+            return base + x * Geo<Q, G>::WID;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // MSV (dp.rs:92-145)
 // ---------------------------------------------------------------------------
-template <int Q, int G>
-__device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tbm, const uint8_t* seq, int L,
+template <int Q, int G, int TM>
+__device__ __forceinline__ float msv_pair(const float* __restrict__ em, int ew, float tbm, const uint8_t* seq, int L,
                                           float tloop, float tmove) {
-    constexpr int WID = Geo<Q, G>::WID;
     const int j = (int)(threadIdx.x & (G - 1));
-    const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
+    const EmRows<Q, G, TM> er(em, ew, j);
     float row[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) row[q] = SX_NEG_INF;
@@ -305,7 +340,7 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tb
     // same f32 value because rounding is monotone.
     float xn = 0.0f, xb = tmove, xj = SX_NEG_INF;
     auto step = [&](int x) {
-        const float* __restrict__ sc = em + x * WID + kb;
+        const float* __restrict__ sc = er.row(x);
         const float be = xb + tbm;
         const float left = grp_from_prev<G>(row[Q - 1], SX_NEG_INF);
         float e = SX_NEG_INF;
@@ -357,12 +392,14 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, float tb
 // ---------------------------------------------------------------------------
 // Viterbi score (dp.rs:152-189)
 // ---------------------------------------------------------------------------
-template <int Q, int G, bool HOIST>
-__device__ __forceinline__ float vit_pair(const float* __restrict__ em, const float* __restrict__ tr,
+template <int Q, int G, int TM>
+__device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, const float* __restrict__ tr,
                                           const uint8_t* seq, int L, float lloop, float lmove) {
     constexpr int WID = Geo<Q, G>::WID;
+    constexpr bool HOIST = TM == TM_REG;
     const int j = (int)(threadIdx.x & (G - 1));
     const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
+    const EmRows<Q, G, TM> er(em, ew, j);
     // tr rows: MM MI MD IM II DM DD BM; Tx(k) = tr[x*WID + k]
     float mr[Q], ir[Q], dr[Q];
 #pragma unroll
@@ -386,7 +423,7 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, const fl
         const float* __restrict__ tdm = tr + SX_TDM * WID + kbo;
         const float* __restrict__ tdd = tr + SX_TDD * WID + kbo;
         const float* __restrict__ tbm = tr + SX_TBM * WID + kbo;
-        const float* __restrict__ sc = em + x * WID + kb;
+        const float* __restrict__ sc = er.row(x);
         const float mL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
         const float iL = grp_from_prev<G>(ir[Q - 1], SX_NEG_INF);
         const float dL = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
@@ -432,13 +469,15 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, const fl
 // ---------------------------------------------------------------------------
 // Forward (dp.rs:229-298), multihit probabilities e_move = e_loop = 1/2
 // ---------------------------------------------------------------------------
-template <int Q, int G, bool SPEC, bool HOIST>
-__device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const float* __restrict__ tr,
+template <int Q, int G, bool SPEC, int TM>
+__device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew, const float* __restrict__ tr,
                                            const uint8_t* seq, int L, float ploop, float pmove, float* x5,
                                            double* lsrow, bool writer) {
     constexpr int WID = Geo<Q, G>::WID;
+    constexpr bool HOIST = TM == TM_REG;
     const int j = (int)(threadIdx.x & (G - 1));
     const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
+    const EmRows<Q, G, TM> er(eot, ew, j);
     float blk = 1.0f;  // product of p_dd over the lane's block: the D-chain map's slope
     if constexpr (G > 1) {
         const float* __restrict__ pdd0 = tr + SX_TDD * WID + kb;
@@ -474,7 +513,7 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, const 
         const float* __restrict__ pdm = tr + SX_TDM * WID + kbo;
         const float* __restrict__ pdd = tr + SX_TDD * WID + kbo;
         const float* __restrict__ pbm = tr + SX_TBM * WID + kbo;
-        const float* __restrict__ eo = eot + x * WID + kb;
+        const float* __restrict__ eo = er.row(x);
         const float mL = grp_from_prev<G>(mr[Q - 1], 0.0f);
         const float iL = grp_from_prev<G>(ir[Q - 1], 0.0f);
         const float dL = grp_from_prev<G>(dr[Q - 1], 0.0f);
@@ -552,13 +591,16 @@ __device__ __forceinline__ double logsum_d(double a, double b) {
     return hi + log1p(exp(lo - hi));
 }
 
-template <int Q, int G, bool SPEC, bool HOIST>
-__device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const float* __restrict__ tr,
+template <int Q, int G, bool SPEC, int TM>
+__device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, int ew, const float* __restrict__ tr,
                                            const uint8_t* seq, int L, float ploop, float pmove, float lloop,
                                            float lmove, float* x5, double* lsrow, bool writer) {
     constexpr int WID = Geo<Q, G>::WID;
+    constexpr bool HOIST = TM == TM_REG;
+    constexpr int NX = EmRows<Q, G, TM>::NEXT;  // the next lane's first position, from row(x)
     const int j = (int)(threadIdx.x & (G - 1));
     const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
+    const EmRows<Q, G, TM> er(eot, ew, j);
     const float* __restrict__ pmd = tr + SX_TMD * WID + kb;
     const float* __restrict__ pdd = tr + SX_TDD * WID + kb;
     const float* __restrict__ pbm = tr + SX_TBM * WID + kb;
@@ -618,7 +660,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
         const float* __restrict__ pdm = tr + SX_TDM * WID + kbo;
         const float* __restrict__ pdd = tr + SX_TDD * WID + kbo;
         const float* __restrict__ pbm = tr + SX_TBM * WID + kbo;
-        const float* __restrict__ eo = eot + xnext * WID + kb;  // emissions of residue i+1
+        const float* __restrict__ eo = er.row(xnext);  // emissions of residue i+1
         const float ins = xnext != SX_CODE_STAR ? 1.0f : 0.0f;
         float bl = 0.0f;
 #pragma unroll
@@ -632,7 +674,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
         const float mR = grp_from_next<G>(mr[0], 0.0f);  // mn at the next lane's first k
         // D chain (descending): dc[k] = mnext*p_dm[k] + dc[k+1]*p_dd[k] + xe
         {
-            const float mnx = mR * eo[Q + 1];
+            const float mnx = mR * eo[NX];
             dr[Q - 1] = fmaf(mnx, pdm[Q - 1], xe);
         }
 #pragma unroll
@@ -656,7 +698,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
         float lmax = SX_NEG_INF;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            const float mnx = q < Q - 1 ? mr[q + 1] * eo[q + 1] : mR * eo[Q + 1];
+            const float mnx = q < Q - 1 ? mr[q + 1] * eo[q + 1] : mR * eo[NX];
             const float iv = ir[q] * ins;
             const float dn1 = q < Q - 1 ? dr[q + 1] : dR;
             const float mv = fmaf(dn1, pmd[q], fmaf(mnx, pmm[q], iv * pmi[q]) + xe);
@@ -691,7 +733,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, const 
         xnext = rs.next();
     }
     // row 0: B(0) from residue 1
-    const float* __restrict__ eo = eot + xnext * WID + kb;
+    const float* __restrict__ eo = er.row(xnext);
     float bl = 0.0f;
 #pragma unroll
     for (int q = 0; q < Q; q++) bl = fmaf(mr[q] * pbm[q], eo[q], bl);
@@ -740,10 +782,10 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
     return ti;
 }
 
-template <int Q, int G, int STAGE, bool SPEC, bool HOIST>
+template <int Q, int G, int STAGE, bool SPEC, int TM>
 __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out, SxSpecialsOut SP) {
     constexpr int WID = Geo<Q, G>::WID;
-    constexpr int NTAB_E = SX_NCODE * WID;
+    constexpr int NTAB_E = TM == TM_GEM ? 0 : SX_NCODE * WID;  // emission rows staged in LDS
     extern __shared__ float lds[];
     __shared__ long long s_tile;
     const int gid = (int)(threadIdx.x / G);
@@ -764,7 +806,9 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
             __syncthreads();
             const int m = P.m[ti.prof];
             const int w = m + 1;
-            if (STAGE == ST_MSV || STAGE == ST_VIT) {
+            if constexpr (TM == TM_GEM) {
+                // emission rows stay in global memory (padded copy)
+            } else if (STAGE == ST_MSV || STAGE == ST_VIT) {
                 const float* src = P.msc + P.off[ti.prof];
                 for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
                     const int x = idx / WID, k = slot_k<Q, G>(idx - x * WID);
@@ -790,6 +834,10 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
         } else if (W.head) {
             __syncthreads();  // s_tile is rewritten only after every lane has read it
         }
+        // emission rows: LDS, or the profile's padded global copy for long models
+        const float* em = lds;
+        if constexpr (TM == TM_GEM)
+            em = (STAGE == ST_MSV || STAGE == ST_VIT ? P.gmsc : P.gmo) + P.goff[ti.prof];
         if (gid < ti.count) {
             const int tid = W.tids[ti.first + gid];
             const int L = T.len[tid];
@@ -798,11 +846,11 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
             const int64_t o = ti.out + (ti.by_tid ? (int64_t)tid : (int64_t)gid);
             if (STAGE == ST_MSV) {
                 float r = SX_NEG_INF;
-                if (L > 0) r = msv_pair<Q, G>(lds, P.msv_tbm[ti.prof], seq, L, lm.msv_tloop, lm.msv_tmove);
+                if (L > 0) r = msv_pair<Q, G, TM>(em, P.gw, P.msv_tbm[ti.prof], seq, L, lm.msv_tloop, lm.msv_tmove);
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_VIT) {
                 float r = SX_NEG_INF;
-                if (L > 0) r = vit_pair<Q, G, HOIST>(lds, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
+                if (L > 0) r = vit_pair<Q, G, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_FWD) {
                 double r = -(double)__builtin_inf();
@@ -813,7 +861,8 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
                     lsr = SP.ls + SP.row_off[o];
                 }
                 if (L > 0)
-                    r = fwd_pair<Q, G, SPEC, HOIST>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, x5, lsr, jl == 0);
+                    r = fwd_pair<Q, G, SPEC, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, x5, lsr,
+                                                 jl == 0);
                 if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
             } else {
                 double r = -(double)__builtin_inf();
@@ -824,8 +873,8 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
                     lsr = SP.ls + SP.row_off[o];
                 }
                 if (L > 0)
-                    r = bwd_pair<Q, G, SPEC, HOIST>(lds, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, lm.lloop, lm.lmove,
-                                             x5, lsr, jl == 0);
+                    r = bwd_pair<Q, G, SPEC, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, lm.lloop,
+                                                 lm.lmove, x5, lsr, jl == 0);
                 if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
             }
         }
@@ -849,7 +898,8 @@ struct SxKernelEntry {
 };
 
 #define SXK_E(ST, Q, G, SP, H) \
-    {ST, Q, G, SP, H, sxk::Geo<Q, G>::WID, reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, H>)}
+    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::Geo<Q, G>::WID, \
+     reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, (int)(H)>)}
 
 #define SXK_ALLG(ST, Q, SP, H)                                                                          \
     SXK_E(ST, Q, 1, SP, H), SXK_E(ST, Q, 2, SP, H), SXK_E(ST, Q, 4, SP, H), SXK_E(ST, Q, 8, SP, H),     \
@@ -862,6 +912,13 @@ struct SxKernelEntry {
         SXK_ALLG(ST, 16, false, true),                                                          \
         SXK_ALLG(ST, 16, false, false), SXK_ALLG(ST, 32, false, false), SXK_ALLG(ST, 48, false, false), \
         SXK_E(ST, 64, 1, false, false), SXK_E(ST, 64, 2, false, false), SXK_E(ST, 64, 4, false, false)
+
+#define SXK_GEM(ST)                                                                                        \
+    SXK_E(ST, 24, 64, false, 2), SXK_E(ST, 32, 64, false, 2), SXK_E(ST, 40, 64, false, 2),                \
+        SXK_E(ST, 48, 64, false, 2), SXK_E(ST, 64, 64, false, 2)
+#define SXK_GEMS(ST)                                                                                       \
+    SXK_E(ST, 24, 64, true, 2), SXK_E(ST, 32, 64, true, 2), SXK_E(ST, 40, 64, true, 2),                   \
+        SXK_E(ST, 48, 64, true, 2), SXK_E(ST, 64, 64, true, 2)
 
 static const SxKernelEntry k_table[] = {
     // MSV: one f32 row per lane
@@ -889,6 +946,9 @@ static const SxKernelEntry k_table[] = {
     SXK_ALLG(2, 32, true, false), SXK_ALLG(2, 48, true, false),
     SXK_ALLG(3, 4, true, true), SXK_ALLG(3, 8, true, true), SXK_ALLG(3, 16, true, false),
     SXK_ALLG(3, 32, true, false), SXK_ALLG(3, 48, true, false),
+    // long models (M > 1024): 64 lanes per pair, emission rows from global memory
+    SXK_GEM(0), SXK_GEM(1), SXK_GEM(2), SXK_GEM(3),
+    SXK_GEMS(2), SXK_GEMS(3),
 };
 
 extern "C" const SxKernelEntry* sx_kernel_table(int* n) {

@@ -230,7 +230,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     std::map<std::tuple<int, int, int>, std::vector<int32_t>> g;
     for (size_t p = 0; p < P; p++) {
         const Shape s = c.shape_for(0, SX_FORM_BATCHED, c.profs[p].m);
-        g[{s.q, s.g, s.hoist ? 1 : 0}].push_back((int32_t)p);
+        g[{s.q, s.g, s.hoist}].push_back((int32_t)p);
     }
     S.msv_plists.clear();
     S.msv_plists.reserve(g.size());
@@ -241,7 +241,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
         HIPCHK(hipMemcpy(d, kv.second.data(), kv.second.size() * 4, hipMemcpyHostToDevice));
         uint64_t cells = 0;
         for (int32_t p : kv.second) cells += (uint64_t)c.profs[p].m * (uint64_t)c.total_len;
-        S.msv_groups.push_back({Shape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first) != 0}, d,
+        S.msv_groups.push_back({Shape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)}, d,
                                 (int)kv.second.size(), cells});
     }
     auto up = [](DeviceBuf& b, const void* src, size_t bytes) {
@@ -324,7 +324,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         w.ntiles = (int64_t)w.nchunks * gr.n;
         w.out_stride = T;
         w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
-        const int grid = c.persistent_grid(k, Ctx::lds_bytes(0, k.wid), w.ntiles);
+        const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), w.ntiles);
         c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot % 32);
         if (with_events) HIPCHK(hipEventRecord(S.lev[slot], st));
         slot++;
@@ -387,7 +387,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
             w.dyn_lo = (const int64_t*)toff.p + r.a;
             w.dyn_hi = (const int64_t*)toff.p + r.b;
             w.ntiles = 0;
-            const int grid = c.persistent_grid(k, Ctx::lds_bytes(stage, k.wid), 1 << 30);
+            const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), 1 << 30);
             c.launch(k, stage, w, out, nosp, grid, true, slot++ % 32 + 32);
             if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], st));
             lslot++;
@@ -522,7 +522,7 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
                 l.stage = stage;
                 l.q = sh.q;
                 l.g = sh.g;
-                l.hoist = sh.hoist ? 1 : 0;
+                l.hoist = sh.hoist;
                 l.ms = ev_ms(from, S.lev[e]);
                 l.cells = cells;
                 S.launches.push_back(l);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -93,7 +94,10 @@ Ctx::Ctx(int dev) : device(dev) {
         ki.hoist = tab[i].hoist;
         ki.wid = tab[i].wid;
         ki.fn = tab[i].fn;
-        kernels[key(ki.stage, ki.q, ki.g, ki.spec, ki.hoist)] = ki;
+        // every (stage, Q, G, special rows, table mode) names exactly one kernel
+        if (!kernels.emplace(key(ki.stage, ki.q, ki.g, ki.spec, ki.hoist), ki).second)
+            throw SxError(SX_ERR_STATE, "kernel table has two entries for stage " + std::to_string(ki.stage) +
+                                            " Q=" + std::to_string(ki.q) + " G=" + std::to_string(ki.g));
     }
 }
 
@@ -110,21 +114,28 @@ Ctx::~Ctx() {
 void Ctx::free_profiles() {
     dfree(d_msc); dfree(d_mo); dfree(d_tsc); dfree(d_tp);
     dfree(d_off); dfree(d_toff); dfree(d_m); dfree(d_tbm);
+    dfree(d_gmsc); dfree(d_gmo); dfree(d_goff);
 }
 
 void Ctx::free_targets() {
     dfree(d_codes); dfree(d_soff); dfree(d_len); dfree(d_lm); dfree(d_sorted); dfree(d_identity);
 }
 
-const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec, bool hoist) const {
-    auto it = kernels.find(key(stage, q, g, spec ? 1 : 0, hoist ? 1 : 0));
+const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec, int hoist) const {
+    auto it = kernels.find(key(stage, q, g, spec ? 1 : 0, hoist));
     if (it == kernels.end())
         throw SxError(SX_ERR_STATE, "no kernel for stage " + std::to_string(stage) + " Q=" + std::to_string(q) +
-                                        " G=" + std::to_string(g) + (hoist ? " (registers)" : " (LDS)"));
+                                        " G=" + std::to_string(g) +
+                                        (hoist == 2 ? " (global emissions)" : hoist ? " (registers)" : " (LDS)"));
     return it->second;
 }
 
-size_t Ctx::lds_bytes(int stage, int wid) { return (size_t)(SX_NCODE + (stage == 0 ? 0 : SX_NT)) * wid * sizeof(float); }
+// LDS of one workgroup: the emission rows (unless they are read from global memory)
+// and, past MSV, the eight transition rows.
+size_t Ctx::lds_bytes(int stage, int wid, int hoist) {
+    const int rows = (hoist == 2 ? 0 : SX_NCODE) + (stage == 0 ? 0 : SX_NT);
+    return (size_t)rows * wid * sizeof(float);
+}
 
 // ---------------------------------------------------------------------------
 // data upload
@@ -182,6 +193,11 @@ void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n)
 }
 
 void Ctx::set_profiles(std::vector<Profile>&& ps) {
+    for (const auto& p : ps)
+        if (p.m > SX_MAX_M)
+            throw SxError(SX_ERR_ARG, "model '" + p.name + "' has M = " + std::to_string(p.m) +
+                                          "; the longest supported model has " + std::to_string(SX_MAX_M) +
+                                          " positions");
     cascade_dirty = true;
     free_profiles();
     profs = std::move(ps);
@@ -206,12 +222,34 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     d_toff = dupload(toff);
     d_m = dupload(ms);
     d_tbm = dupload(tbm);
+    // padded emission rows of the long models (read from global memory by their kernels)
+    std::vector<float> gmsc, gmo;
+    std::vector<int64_t> goff(profs.size(), -1);
+    for (size_t i = 0; i < profs.size(); i++) {
+        const Profile& p = profs[i];
+        if (p.m <= SX_GEM_MIN_M) continue;
+        goff[i] = (int64_t)gmsc.size();
+        const size_t base = gmsc.size(), w = (size_t)p.m + 1;
+        gmsc.resize(base + (size_t)SX_NCODE * SX_GEM_WIDTH, -std::numeric_limits<float>::infinity());
+        gmo.resize(base + (size_t)SX_NCODE * SX_GEM_WIDTH, 0.0f);
+        for (int x = 0; x < SX_NCODE; x++)
+            for (int k = 1; k <= p.m; k++) {
+                gmsc[base + (size_t)x * SX_GEM_WIDTH + k + 3] = p.msc[x * w + k];
+                gmo[base + (size_t)x * SX_GEM_WIDTH + k + 3] = p.mo[x * w + k];
+            }
+    }
+    if (!gmsc.empty()) {
+        d_gmsc = dupload(gmsc);
+        d_gmo = dupload(gmo);
+    }
+    d_goff = dupload(goff);
 }
 
 SxProfiles Ctx::dprof() const {
     SxProfiles P;
     P.msc = d_msc; P.mo = d_mo; P.tsc = d_tsc; P.tp = d_tp;
     P.off = d_off; P.toff = d_toff; P.m = d_m; P.msv_tbm = d_tbm;
+    P.gmsc = d_gmsc; P.gmo = d_gmo; P.goff = d_goff; P.gw = SX_GEM_WIDTH; P.pad = 0;
     return P;
 }
 
@@ -229,27 +267,33 @@ SxTargets Ctx::dtarg() const {
 // smallest capacity Q*G >= M among shapes whose LDS tables fit, preferring (at equal
 // capacity) register-resident transitions and then fewer lanes per pair.  Viterbi,
 // Forward and Backward avoid Q > 16 per lane, whose three register rows cut occupancy.
+Shape Ctx::gem_shape(int m) {
+    for (int q : {24, 32, 40, 48, 64})
+        if (q * 64 >= m) return {q, 64, 2};
+    throw SxError(SX_ERR_ARG, "model length " + std::to_string(m) + " exceeds the longest supported (" +
+                                  std::to_string(SX_MAX_M) + ")");
+}
+
 Shape Ctx::default_shape(int stage, int form, int m) const {
+    if (m > SX_GEM_MIN_M) return gem_shape(m);
     if (form == SX_FORM_PLAIN) {
-        for (int q : {4, 8, 16, 32, 48})
-            if (q * 64 >= m) return {q, 64, stage != 0 && q <= 8};
-        throw SxError(SX_ERR_ARG, "model too long for the plain form (M > 3072)");
+        for (int q : {4, 8, 16})
+            if (q * 64 >= m) return {q, 64, stage != 0 && q <= 8 ? 1 : 0};
     }
-    Shape best{0, 0, false};
+    Shape best{0, 0, 0};
     long best_cap = 1L << 40;
     for (const auto& kv : kernels) {
         const KernelInfo& k = kv.second;
-        if (k.stage != stage || k.spec) continue;
+        if (k.stage != stage || k.spec || k.hoist == 2) continue;
         const long cap = (long)k.q * k.g;
         if (cap < m) continue;
-        if (lds_bytes(stage, k.wid) > LDS_MAX) continue;
+        if (lds_bytes(k) > LDS_MAX) continue;
         if (stage != 0 && k.q > 16 && cap < 1024) continue;
         const bool better = cap < best_cap ||
-                            (cap == best_cap && (k.hoist > (int)best.hoist ||
-                                                 (k.hoist == (int)best.hoist && k.g < best.g)));
+                            (cap == best_cap && (k.hoist > best.hoist || (k.hoist == best.hoist && k.g < best.g)));
         if (better) {
             best_cap = cap;
-            best = {k.q, k.g, k.hoist != 0};
+            best = {k.q, k.g, k.hoist};
         }
     }
     if (!best.q) throw SxError(SX_ERR_ARG, "no kernel shape covers model length " + std::to_string(m));
@@ -257,6 +301,7 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
 }
 
 Shape Ctx::shape_for(int stage, int form, int m) const {
+    if (m > SX_GEM_MIN_M) return gem_shape(m);  // long models: one fixed shape, never tuned
     if (form == SX_FORM_BATCHED) {
         if (forced.count(stage)) return forced.at(stage);
         auto it = tuned.find(stage);
@@ -269,15 +314,16 @@ Shape Ctx::shape_for(int stage, int form, int m) const {
 }
 
 Shape Ctx::shape_for_spec(int stage, int m) const {
+    if (m > SX_GEM_MIN_M) return gem_shape(m);
     // special-row writers: Q in {4, 8} with register-resident transitions, {16, 32, 48} in LDS
     for (int g : {1, 2, 4, 8, 16, 32, 64})
         for (int q : {4, 8, 16, 32, 48}) {
             if ((long)q * g < m) continue;
             if (g == 1 && q > 16) continue;
-            auto it = kernels.find(key(stage, q, g, 1, q <= 8));
+            auto it = kernels.find(key(stage, q, g, 1, q <= 8 ? 1 : 0));
             if (it == kernels.end()) continue;
-            if (lds_bytes(stage, it->second.wid) > LDS_MAX) continue;
-            return {q, g, q <= 8};
+            if (lds_bytes(it->second) > LDS_MAX) continue;
+            return {q, g, q <= 8 ? 1 : 0};
         }
     throw SxError(SX_ERR_ARG, "no special-row kernel covers model length " + std::to_string(m));
 }
@@ -296,7 +342,7 @@ int Ctx::persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const 
 
 void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
                  int head_slot) {
-    const size_t lds = lds_bytes(stage, k.wid);
+    const size_t lds = lds_bytes(k);
     if (lds > LDS_MAX) throw SxError(SX_ERR_ARG, "profile tables exceed the LDS budget");
     if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (persistent) {
@@ -339,7 +385,7 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
     std::map<std::tuple<int, int, int>, std::vector<int32_t>> groups;
     for (int32_t p : plist) {
         const Shape s = shape_for(stage, form, profs[p].m);
-        groups[{s.q, s.g, s.hoist ? 1 : 0}].push_back(p);
+        groups[{s.q, s.g, s.hoist}].push_back(p);
     }
     // every group's profile list goes up in one copy, which completes before the host
     // lists go out of scope (an async copy reads its host source when it executes)
@@ -357,8 +403,7 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
     int slot = 0;
     size_t gi = 0;
     for (auto& kv : groups) {
-        const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
-        const bool h = std::get<2>(kv.first) != 0;
+        const int q = std::get<0>(kv.first), g = std::get<1>(kv.first), h = std::get<2>(kv.first);
         const KernelInfo& k = kernel(stage, q, g, false, h);
         SxWork w{};
         w.profs = (const int32_t*)pb.p + goff[gi++];
@@ -369,7 +414,7 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         w.ntiles = (int64_t)w.nchunks * (int64_t)kv.second.size();
         w.out_stride = out_stride;
         if (w.ntiles == 0) continue;
-        const int grid = persistent_grid(k, lds_bytes(stage, k.wid), w.ntiles);
+        const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
         launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
     }
 }
@@ -416,7 +461,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
             t.count = (int32_t)std::min<uint64_t>(ng, e - a);
             t.first = (int64_t)a;
             t.out = (int64_t)a;
-            tiles[{s.q, s.g, s.hoist ? 1 : 0}].push_back(t);
+            tiles[{s.q, s.g, s.hoist}].push_back(t);
         }
         i = e;
     }
@@ -444,7 +489,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     int slot = 0;
     for (auto& kv : tiles) {
         const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
-        const bool h = std::get<2>(kv.first) != 0;
+        const int h = std::get<2>(kv.first);
         const KernelInfo& k = kernel(stage, q, g, spec, h);
         SxTile* dt = (SxTile*)scratch_tiles.p + toff;
         toff += kv.second.size();
@@ -453,7 +498,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         w.tids = (const int32_t*)scratch_tids.p;
         w.ntiles = (int64_t)kv.second.size();
         if (form == SX_FORM_BATCHED) {
-            const int grid = persistent_grid(k, lds_bytes(stage, k.wid), w.ntiles);
+            const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
             launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
         } else {
             launch(k, stage, w, dout, sp, (int)w.ntiles, false, 0);
@@ -513,6 +558,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             }
         lo = hi;
         if (inb.empty()) continue;
+        if (mmax > SX_GEM_MIN_M) break;  // long models have one fixed shape
         // up to 16 profiles, spread over the bucket, repeated if the bucket is small
         std::vector<int32_t> sample;
         const int want = (int)std::clamp<int64_t>((int64_t)(4e10 / ((double)mmax * (double)sum_l)), 1, 16);
@@ -527,10 +573,10 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         std::vector<Shape> cands;
         for (const auto& kv : kernels) {
             const KernelInfo& k = kv.second;
-            if (k.stage != stage || k.spec) continue;
+            if (k.stage != stage || k.spec || k.hoist == 2) continue;
             if ((long)k.q * k.g < mmax) continue;
-            if (lds_bytes(stage, k.wid) > LDS_MAX) continue;
-            cands.push_back({k.q, k.g, k.hoist != 0});
+            if (lds_bytes(k) > LDS_MAX) continue;
+            cands.push_back({k.q, k.g, k.hoist});
         }
         std::sort(cands.begin(), cands.end(), [](const Shape& a, const Shape& b) {
             const long ca = (long)a.q * a.g, cb = (long)b.q * b.g;
@@ -872,7 +918,7 @@ extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, i
         const sx::Shape s = ctx->c->shape_for(stage, form, m);
         *q = s.q;
         *g = s.g;
-        if (hoist) *hoist = s.hoist ? 1 : 0;
+        if (hoist) *hoist = s.hoist;
         return SX_OK;
     } catch (const SxError& e) {
         return e.code;
@@ -886,10 +932,14 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
         Ctx& c = *ctx->c;
         std::vector<std::pair<int, sx::Shape>> tab;
         for (uint32_t i = 0; i < n; i++) {
-            const bool h = hoist ? hoist[i] != 0 : false;
+            const int h = hoist ? hoist[i] : 0;
+            if (h != 0 && h != 1) throw SxError(SX_ERR_ARG, "hoist must be 0 or 1 (long models have a fixed shape)");
+            if (max_m[i] > SX_GEM_MIN_M)
+                throw SxError(SX_ERR_ARG, "rows cover models up to " + std::to_string(SX_GEM_MIN_M) +
+                                              "; longer models use a fixed shape");
             const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false, h);
             if ((long)q[i] * g[i] < max_m[i]) throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
-            if (Ctx::lds_bytes(stage, k.wid) > 160 * 1024) throw SxError(SX_ERR_ARG, "shape exceeds the LDS budget");
+            if (Ctx::lds_bytes(k) > 160 * 1024) throw SxError(SX_ERR_ARG, "shape exceeds the LDS budget");
             if (i && max_m[i] <= max_m[i - 1]) throw SxError(SX_ERR_ARG, "max_m must increase");
             tab.push_back({max_m[i], sx::Shape{q[i], g[i], h}});
         }
@@ -913,7 +963,7 @@ extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, 
             if (max_m) max_m[i] = it->second[i].first;
             if (q) q[i] = it->second[i].second.q;
             if (g) g[i] = it->second[i].second.g;
-            if (hoist) hoist[i] = it->second[i].second.hoist ? 1 : 0;
+            if (hoist) hoist[i] = it->second[i].second.hoist;
         }
     });
 }
@@ -932,7 +982,7 @@ extern "C" int sx_autotune_log(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n
         if (max_m) max_m[i] = L[i].max_m;
         if (q) q[i] = L[i].shape.q;
         if (g) g[i] = L[i].shape.g;
-        if (hoist) hoist[i] = L[i].shape.hoist ? 1 : 0;
+        if (hoist) hoist[i] = L[i].shape.hoist;
         if (gcups) gcups[i] = L[i].gcups;
     }
     return SX_OK;

@@ -42,7 +42,9 @@ struct KernelInfo {
 
 struct Shape {
     int q, g;
-    bool hoist;  // transition tables kept in registers across rows (else read from LDS)
+    // table mode: 0 = tables in LDS; 1 = transitions kept in registers across rows;
+    // 2 = long models, emission rows read from global memory (transitions in LDS)
+    int hoist;
 };
 
 struct TuneRecord {
@@ -61,15 +63,19 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int n_cu = 0;
     std::map<long, KernelInfo> kernels;
+    // distinct fields: stage | Q (20 bits) | G (16 bits) | special rows (1 bit) | table mode (3 bits)
     static long key(int st, int q, int g, int sp, int h) {
-        return ((long)st << 40) | ((long)q << 20) | ((long)g << 2) | ((long)sp << 1) | h;
+        return ((long)st << 40) | ((long)q << 20) | ((long)g << 4) | ((long)sp << 3) | (long)h;
     }
-    const KernelInfo& kernel(int stage, int q, int g, bool spec, bool hoist) const;
-    static size_t lds_bytes(int stage, int wid);
+    const KernelInfo& kernel(int stage, int q, int g, bool spec, int hoist) const;
+    static size_t lds_bytes(int stage, int wid, int hoist);
+    static size_t lds_bytes(const KernelInfo& k) { return lds_bytes(k.stage, k.wid, k.hoist); }
 
     // profiles
     std::vector<Profile> profs;
     float *d_msc = nullptr, *d_mo = nullptr, *d_tsc = nullptr, *d_tp = nullptr, *d_tbm = nullptr;
+    float *d_gmsc = nullptr, *d_gmo = nullptr;  // padded emission rows of long models
+    int64_t* d_goff = nullptr;
     int64_t *d_off = nullptr, *d_toff = nullptr;
     int32_t* d_m = nullptr;
     int32_t* d_pidx = nullptr;
@@ -104,6 +110,8 @@ struct Ctx {
     Shape default_shape(int stage, int form, int m) const;
     Shape shape_for(int stage, int form, int m) const;
     Shape shape_for_spec(int stage, int m) const;
+    // the fixed shape of a long model (M > SX_GEM_MIN_M): 64 lanes, the smallest Q that covers M
+    static Shape gem_shape(int m);
 
     // launches
     unsigned long long* d_heads = nullptr;

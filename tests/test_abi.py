@@ -36,3 +36,31 @@ def test_create_without_device_fails_loudly(sxa):
         pytest.skip("a device is present")
     with pytest.raises(sxa.SxError):
         sxa.Context(0)
+
+
+class _KernelEntry(ctypes.Structure):
+    _fields_ = [("stage", ctypes.c_int), ("q", ctypes.c_int), ("g", ctypes.c_int), ("spec", ctypes.c_int),
+                ("hoist", ctypes.c_int), ("wid", ctypes.c_int), ("fn", ctypes.c_void_p)]
+
+
+def test_kernel_table_names_each_shape_once(sxa):
+    """Each (stage, Q, G, special rows, table mode) is one distinct kernel; the special-row
+    writers are separate kernels from the score-only ones (a launch must never pick one for
+    the other)."""
+    if not os.path.exists(sxa.LIB_PATH):
+        pytest.skip("libsxhmmer.so not built")
+    lib = ctypes.CDLL(sxa.LIB_PATH)
+    lib.sx_kernel_table.restype = ctypes.POINTER(_KernelEntry)
+    n = ctypes.c_int()
+    tab = lib.sx_kernel_table(ctypes.byref(n))
+    rows = [(tab[i].stage, tab[i].q, tab[i].g, tab[i].spec, tab[i].hoist) for i in range(n.value)]
+    fns = [tab[i].fn for i in range(n.value)]
+    assert len(set(rows)) == len(rows)
+    assert len(set(fns)) == len(fns)
+    assert all(r[4] in (0, 1, 2) and r[3] in (0, 1) for r in rows)
+    long_rows = {r for r in rows if r[4] == 2}
+    for st in (0, 1, 2, 3):
+        for q in (24, 32, 40, 48, 64):
+            assert (st, q, 64, 0, 2) in long_rows
+            if st >= 2:
+                assert (st, q, 64, 1, 2) in long_rows

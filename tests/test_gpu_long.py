@@ -1,0 +1,145 @@
+"""GPU parity for long models (M > 1024), which take the global-emission kernels: MSV and
+Viterbi bit-identical to the oracle, Forward/Backward within FWD_TOL nats of the FP64
+oracle, for both launch forms, special rows, pair lists and the filter cascade; models
+longer than the supported maximum are rejected."""
+import numpy as np
+import pytest
+
+from helpers import emit_homolog, random_hmm, random_seq
+from test_gpu_stages import FWD_TOL, _spec_close
+
+pytestmark = pytest.mark.gpu
+
+LONG_MS = [1025, 1536, 1537, 2100, 2560, 3000, 4096]
+
+
+@pytest.fixture(scope="module")
+def long_case(orc):
+    hmms = [random_hmm(orc, m, 7000 + i) for i, m in enumerate(LONG_MS)]
+    rng = np.random.default_rng(71)
+    seqs = [random_seq(rng, L, star_frac=0.01, degen_frac=0.02) for L in (1, 2, 3, 17, 64, 65)]
+    for i in range(14):
+        if i % 3 == 0:
+            seqs.append(emit_homolog(orc, hmms[i % len(hmms)], rng, int(rng.integers(200, 900))))
+        else:
+            seqs.append(random_seq(rng, int(rng.integers(20, 400)), star_frac=0.01, degen_frac=0.02))
+    # one long target, so that Forward and Backward rescale many times
+    seqs.append(emit_homolog(orc, hmms[3], rng, 6000))
+    profs = [orc.configure(h) for h in hmms]
+    return hmms, profs, seqs, orc.Batch(profs, seqs)
+
+
+@pytest.fixture(scope="module")
+def want(long_case):
+    hmms, profs, seqs, batch = long_case
+    shape = (len(profs), len(seqs))
+    return {
+        0: batch.msv(nthreads=8).reshape(shape),
+        1: batch.viterbi(nthreads=8).reshape(shape),
+        2: batch.forward64(nthreads=8).reshape(shape),
+    }
+
+
+@pytest.fixture
+def loaded(gctx, long_case):
+    hmms, profs, seqs, _ = long_case
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    for st in range(4):
+        gctx.set_shapes(st, [])
+    return gctx
+
+
+def test_long_shapes_are_fixed_global_emission_shapes(loaded):
+    for st in range(4):
+        for m in LONG_MS:
+            for form in (0, 1):
+                q, g, h = loaded.shape_for(st, m, form)
+                assert (g, h) == (64, 2)
+                assert q == min(c for c in (24, 32, 40, 48, 64) if 64 * c >= m)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "batched"])
+def test_long_msv_viterbi_bit_exact(loaded, want, form):
+    for st in (0, 1):
+        got, _ = loaded.score_all(st, form)
+        assert np.array_equal(got, want[st])
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "batched"])
+def test_long_forward_within_tolerance(loaded, want, form):
+    got, _ = loaded.score_all(2, form)
+    w = want[2]
+    fin = np.isfinite(w)
+    assert np.array_equal(np.isfinite(got), fin)
+    assert np.abs(got[fin] - w[fin]).max() <= FWD_TOL
+
+
+def test_long_backward_matches_oracle(orc, loaded, long_case):
+    hmms, profs, seqs, _ = long_case
+    got, _ = loaded.score_all(3, 1)
+    for p in range(len(profs)):
+        for t in range(len(seqs)):
+            w = orc.backward64(profs[p], seqs[t])
+            if np.isfinite(w):
+                assert abs(got[p, t] - w) <= FWD_TOL + 1e-4 * abs(w)
+            else:
+                assert not np.isfinite(got[p, t])
+
+
+def test_long_pair_list_and_specials(orc, loaded, long_case, want):
+    hmms, profs, seqs, _ = long_case
+    rng = np.random.default_rng(3)
+    pi = rng.integers(0, len(profs), 24).astype(np.uint32)
+    ti = rng.integers(0, len(seqs), 24).astype(np.uint32)
+    for st in (0, 1):
+        got, _ = loaded.score_pairs(st, pi, ti)
+        assert np.array_equal(got, want[st][pi, ti])
+    fsp, fsc = loaded.specials(2, pi, ti)
+    bsp, bsc = loaded.specials(3, pi, ti)
+    for k in range(len(pi)):
+        p, d = profs[pi[k]], seqs[ti[k]]
+        _, osp = orc.forward(p, d, specials=True)
+        assert _spec_close(fsp[k], osp)
+        assert abs(fsc[k] - orc.forward64(p, d)) <= FWD_TOL
+        _, obsp = orc.backward(p, d)
+        assert _spec_close(bsp[k], obsp)
+
+
+def test_long_and_short_models_in_one_cascade(orc, gctx, long_case):
+    from sentinel_x_amd import synth
+    hmms, _, seqs, _ = long_case
+    mixed = [random_hmm(orc, m, 7100 + i) for i, m in enumerate((90, 700, 1300, 2200))]
+
+    def scorer(stage, ss):
+        b = orc.Batch([orc.configure(h) for h in mixed], ss)
+        fn = {0: b.msv, 1: b.viterbi, 2: b.forward}[stage]
+        return fn(nthreads=8).reshape(len(mixed), len(ss))
+
+    synth.calibrate(mixed, scorer, n=40)
+    profs = [orc.configure(h) for h in mixed]
+    f = dict(f1=0.3, f2=0.2, f3=0.1)
+    gctx.set_profiles(mixed)
+    gctx.set_targets(seqs)
+    _, rec = gctx.run_cascade(1, use_graph=False, **f)
+    _, plain = gctx.run_cascade(0, **f)
+    key = lambda a: np.lexsort((a["target"], a["profile"]))
+    rec, plain = rec[key(rec)], plain[key(plain)]
+    assert np.array_equal(rec[["profile", "target", "stage"]], plain[["profile", "target", "stage"]])
+    got = {(int(r["profile"]), int(r["target"])): r for r in rec}
+    for p, pr in enumerate(profs):
+        for t, s in enumerate(seqs):
+            stage, out = orc.cascade(pr, s, **f)
+            assert ((p, t) in got) == (stage >= 1)
+            if stage >= 1:
+                r = got[(p, t)]
+                assert np.float32(r["msv"]) == out[0]
+                if np.float32(r["bias"]) == out[1]:
+                    assert r["stage"] == stage
+                    if stage >= 2:
+                        assert np.float32(r["vit"]) == out[2]
+
+
+def test_model_longer_than_supported_is_rejected(orc, gctx, sxa):
+    with pytest.raises(sxa.SxError):
+        gctx.set_profiles([random_hmm(orc, 4097, 1)])

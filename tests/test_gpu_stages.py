@@ -20,6 +20,7 @@ MSV_SHAPES = [(q, g) for q in (4, 8, 16, 32, 64) for g in (1, 2, 4, 8, 16, 32, 6
 DP3_SHAPES = [(q, g, h) for q, h in ((4, 1), (8, 1), (12, 1), (16, 1), (16, 0), (32, 0), (48, 0))
               for g in (1, 2, 4, 8, 16, 32, 64)] + [(64, 1, 0), (64, 2, 0), (64, 4, 0)]
 EDGE_LENGTHS = [1, 2, 3, 15, 16, 17, 31, 32, 33, 47, 48, 64, 65]
+LONG_M = 1024  # models longer than this use the fixed global-emission shape
 
 
 def lds_ok(stage, q, g):
@@ -146,8 +147,7 @@ def test_forward_backward_specials_match_oracle(orc, gctx, mixed):
         assert abs(bsc[k] - orc.backward64(p, d)) <= FWD_TOL + 1e-4 * abs(bsc[k])
 
 
-def _shape_case(orc, q, g, seed):
-    cap = q * g
+def _shape_case(orc, cap, seed):
     ms = sorted({max(1, cap // 2 + 1), max(1, (3 * cap) // 4), cap})
     hmms = [random_hmm(orc, m, seed + i) for i, m in enumerate(ms)]
     seqs = make_targets(orc, hmms, 24, seed, lmax=160)
@@ -156,14 +156,15 @@ def _shape_case(orc, q, g, seed):
 
 @pytest.mark.parametrize("q,g", MSV_SHAPES, ids=[f"Q{q}G{g}" for q, g in MSV_SHAPES])
 def test_every_msv_shape_bit_exact(orc, gctx, q, g):
+    cap = min(q * g, LONG_M)  # longer models always take the fixed long-model shape
     if not lds_ok(0, q, g):
         with pytest.raises(Exception):
-            gctx.set_shapes(0, [(q * g, q, g)])
+            gctx.set_shapes(0, [(cap, q, g)])
         return
-    hmms, profs, seqs = _shape_case(orc, q, g, 1000 + q * 131 + g)
+    hmms, profs, seqs = _shape_case(orc, cap, 1000 + q * 131 + g)
     gctx.set_profiles(hmms)
     gctx.set_targets(seqs)
-    gctx.set_shapes(0, [(q * g, q, g)])
+    gctx.set_shapes(0, [(cap, q, g)])
     got, _ = gctx.score_all(0, 1)
     gctx.set_shapes(0, [])
     want = orc.Batch(profs, seqs).msv(nthreads=8).reshape(len(profs), len(seqs))
@@ -173,14 +174,15 @@ def test_every_msv_shape_bit_exact(orc, gctx, q, g):
 @pytest.mark.parametrize("stage", [1, 2, 3], ids=["viterbi", "forward", "backward"])
 @pytest.mark.parametrize("q,g,h", DP3_SHAPES, ids=[f"Q{q}G{g}{'R' if h else 'L'}" for q, g, h in DP3_SHAPES])
 def test_every_dp3_shape(orc, gctx, q, g, h, stage):
+    cap = min(q * g, LONG_M)
     if not lds_ok(stage, q, g):
         with pytest.raises(Exception):
-            gctx.set_shapes(stage, [(q * g, q, g, h)])
+            gctx.set_shapes(stage, [(cap, q, g, h)])
         return
-    hmms, profs, seqs = _shape_case(orc, q, g, 2000 + q * 131 + g)
+    hmms, profs, seqs = _shape_case(orc, cap, 2000 + q * 131 + g)
     gctx.set_profiles(hmms)
     gctx.set_targets(seqs)
-    gctx.set_shapes(stage, [(q * g, q, g, h)])
+    gctx.set_shapes(stage, [(cap, q, g, h)])
     got, _ = gctx.score_all(stage, 1)
     gctx.set_shapes(stage, [])
     batch = orc.Batch(profs, seqs)
