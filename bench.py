@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -104,6 +105,91 @@ def plain_leg(sxa, ctx, wl, n_plain, f, graph):
             "identical_decisions": same}
 
 
+def kernel_mode(sxa, synth, tune_ms, reps=5):
+    """Every pair through every stage (no filters) on the marker-set workload (400 profiles,
+    M 80-1500, x 4,000 proteins): batched and plain GCUPS per stage, on identical pairs.
+    Returns the workload, the per-stage figures and the batched Forward scores."""
+    wl = synth.make_workload(1)
+    ms = np.array([h.m for h in wl.hmms], np.int64)
+    cells = int(ms.sum()) * sum(len(s) for s in wl.seqs)
+    calibrate(sxa, synth, wl.hmms, 200)  # STATS, for the E-values of the Forward-error report
+    out = {"workload": f"{wl.name}: {len(ms)} profiles (M {ms.min()}-{ms.max()}) x {len(wl.seqs)} proteins",
+           "cells_per_stage": cells}
+    with sxa.Context(0) as kc:
+        kc.set_profiles(wl.hmms)
+        kc.set_targets(wl.seqs)
+        for st in (0, 1, 2):
+            kc.autotune(st, tune_ms)
+        for st, name in ((0, "msv"), (1, "viterbi"), (2, "forward")):
+            kc.score_all(st, sxa.FORM_BATCHED, want=False)
+            tb = float(np.median([kc.score_all(st, sxa.FORM_BATCHED, want=False)[1] for _ in range(reps)]))
+            kc.score_all(st, sxa.FORM_PLAIN, want=False)
+            tp = float(np.median([kc.score_all(st, sxa.FORM_PLAIN, want=False)[1] for _ in range(3)]))
+            out[name] = {"batched_ms": round(tb, 3), "batched_gcups": round(cells / tb / 1e6, 1),
+                         "plain_ms": round(tp, 3), "plain_gcups": round(cells / tp / 1e6, 1),
+                         "ratio_batched_over_plain": round(tp / tb, 2)}
+        fwd, _ = kc.score_all(2, sxa.FORM_BATCHED)
+    return wl, out, fwd
+
+
+def precision_gpu(sxa, synth):
+    """Forward on all 200 profiles of the precision-stress workload (M 1500-2500) against
+    20 of its proteins (L 8,000-35,000 with low-complexity runs): the device scores and
+    the rate (4,000 pairs, enough to fill the device)."""
+    wl = synth.make_workload(3, profile_scale=1.0, target_scale=0.01)
+    ms = np.array([h.m for h in wl.hmms], np.int64)
+    cells = int(ms.sum()) * sum(len(s) for s in wl.seqs)
+    with sxa.Context(0) as pc:
+        pc.set_profiles(wl.hmms)
+        pc.set_targets(wl.seqs)
+        pc.score_all(2, sxa.FORM_BATCHED, want=False)
+        fwd, ms_ = pc.score_all(2, sxa.FORM_BATCHED)
+    return wl, fwd, {"workload": f"{wl.name}: {len(wl.hmms)} profiles (M {ms.min()}-{ms.max()}) x "
+                                 f"{len(wl.seqs)} proteins (L {min(len(x) for x in wl.seqs)}-"
+                                 f"{max(len(x) for x in wl.seqs)})",
+                     "cells": cells, "forward_ms": round(ms_, 3), "forward_gcups": round(cells / ms_ / 1e6, 1)}
+
+
+def forward_error(wl, fwd_gpu, k, threads, z=None):
+    """Device Forward (f32 score, as the reference returns it) against the oracle's FP64
+    Forward on the first k profiles x all proteins: nats error, and the inclusion decisions
+    each score gives on the Forward bit score (Forward minus the null score, in bits) at the
+    models' GA / TC / NC sequence cutoffs and at E <= 10 (E from the model's Forward STATS
+    line, Z = the number of proteins).  Decisions after null2 and domain definition are not
+    part of this library."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker
+    hm = [orc.Hmm(name=h.name, m=h.m, mat=h.mat, t=h.t, compo=h.compo, stats=h.stats) for h in wl.hmms[:k]]
+    t0 = time.perf_counter()
+    ref = orc.Batch([orc.configure(h) for h in hm], wl.seqs).forward64(nthreads=threads).reshape(k, len(wl.seqs))
+    dt = time.perf_counter() - t0
+    gpu = fwd_gpu[:k].astype(np.float32).astype(np.float64)
+    fin = np.isfinite(ref)
+    err = np.abs(gpu[fin] - ref[fin])
+    out = {"pairs": int(fin.sum()), "max_nats": float(err.max()), "median_nats": float(np.median(err)),
+           "p99_nats": float(np.quantile(err, 0.99)), "within_1e-3": bool(err.max() <= 1e-3),
+           "same_finiteness": bool(np.array_equal(np.isfinite(gpu), fin)), "oracle_s": round(dt, 2)}
+    nulls = np.array([float(orc.null_score(len(x))) for x in wl.seqs])
+    bits_g = (gpu - nulls[None, :]) / math.log(2.0)
+    bits_r = (ref - nulls[None, :]) / math.log(2.0)
+    dis = {}
+    for cut in ("ga", "tc", "nc"):
+        c = np.array([getattr(h, cut)[0] for h in wl.hmms[:k]])[:, None]
+        dis[cut] = int(np.sum((bits_g >= c) != (bits_r >= c)))
+    if all("FORWARD" in h.stats for h in wl.hmms[:k]):
+        zz = z or len(wl.seqs)
+        disagree = 0
+        for p in range(k):
+            tau, lam = wl.hmms[p].stats["FORWARD"]
+            for t in np.flatnonzero(fin[p]):
+                eg = zz * math.exp(orc.exp_logsurv(float(bits_g[p, t]), tau, lam))
+                er = zz * math.exp(orc.exp_logsurv(float(bits_r[p, t]), tau, lam))
+                disagree += int((eg <= 10.0) != (er <= 10.0))
+        dis["evalue_10"] = disagree
+    out["decision_disagreements"] = dis
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +206,10 @@ def main():
     ap.add_argument("--f1", type=float, default=0.02)
     ap.add_argument("--f2", type=float, default=1e-3)
     ap.add_argument("--f3", type=float, default=1e-5)
+    ap.add_argument("--no-kernel-mode", action="store_true",
+                    help="skip the every-pair-every-stage runs and the Forward-error report")
+    ap.add_argument("--fwd-err-profiles", type=int, default=40,
+                    help="marker-set profiles (x all proteins) checked against the FP64 Forward")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -201,10 +291,26 @@ def main():
                 for r in ls]
     log(f"[bench] launches: {launches}")
 
-    cpu = None
+    kmode = prec = None
+    if not args.no_kernel_mode:
+        t0 = time.time()
+        wl1, kmode, fwd1 = kernel_mode(sxa, synth, min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0)
+        log(f"[bench] kernel mode ({time.time() - t0:.1f} s): {kmode}")
+        wl3, fwd3, prec = precision_gpu(sxa, synth)
+        log(f"[bench] precision slice: {prec}")
+
+    cpu = fwd_err = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, recs, f, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu}")
+        if kmode is not None:
+            threads = min(16, os.cpu_count() or 1)
+            k1, k3 = min(args.fwd_err_profiles, len(wl1.hmms)), min(20, len(wl3.hmms))
+            fwd_err = {"marker_set": forward_error(wl1, fwd1, k1, threads),
+                       "precision": forward_error(wl3, fwd3, k3, threads)}
+            fwd_err["marker_set"]["sample"] = f"first {k1} profiles x all {len(wl1.seqs)} proteins"
+            fwd_err["precision"]["sample"] = f"first {k3} profiles x all {len(wl3.seqs)} proteins of the slice"
+            log(f"[bench] forward error vs FP64: {fwd_err}")
 
     out = {
         "metric": "GCUPS of the MSV->bias->Viterbi->Forward cascade over all pairs, batched, 1 MI355X",
@@ -221,6 +327,9 @@ def main():
         "plain": plain,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "kernel_mode": kmode,
+        "precision_forward": prec,
+        "forward_error_vs_fp64": fwd_err,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
