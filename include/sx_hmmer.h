@@ -151,6 +151,68 @@ typedef struct {
 } sx_launch_stat;
 int sx_cascade_launches(sx_ctx* ctx, uint32_t cap, uint32_t* n, sx_launch_stat* out);
 
+/* hmmsearch-equivalent search (search.rs:131-271): the filter cascade on the device,
+ * Forward and Backward special rows on the device for the pairs that pass it, then the
+ * rest of score_target on the host (pipeline.rs:186-294: posterior domain decoding,
+ * region finding, stochastic-trace clustering of multi-domain regions, envelope
+ * rescoring, null2, optimal-accuracy coordinates, sequence and domain scores) and
+ * `report` (search.rs:160-210: sequence E-value = P x Z, domain c-Evalue = P x the
+ * profile's number of reported hits, inclusion by E-value, bits or GA/NC/TC).
+ *
+ * With filters = 0, or for a profile without STATS lines, every pair goes through
+ * Forward and the domain definition, as score_target does (pipeline.rs:164, 184-186).
+ * Hits come out ordered by profile, then target (search.rs:225-233).  A GA/NC/TC
+ * cutoff on a profile that lacks it fails with SX_ERR_CUTOFF (search.rs:131-148). */
+enum { SX_CUT_EVALUE = 0, SX_CUT_BITS = 1, SX_CUT_GA = 2, SX_CUT_NC = 3, SX_CUT_TC = 4 };
+
+typedef struct {
+    int32_t cutoff;        /* SX_CUT_* (Cutoff, search.rs:26-32) */
+    int32_t filters;       /* PipelineConfig.filters (pipeline.rs:27-42) */
+    int32_t null2;         /* PipelineConfig.null2 */
+    int32_t threads;       /* host threads for the domain definition; 0 = all cores */
+    double seq_e, dom_e;   /* SX_CUT_EVALUE thresholds */
+    float seq_t, dom_t;    /* SX_CUT_BITS thresholds */
+    double f1, f2, f3;     /* filter thresholds */
+    double z;              /* database size for E-values (hmmsearch -Z); <= 0: number of targets */
+} sx_search_cfg;
+
+typedef struct {           /* Hit (search.rs:101-116), by index */
+    uint32_t profile, target;
+    uint32_t qlen, tlen;
+    float score, bias;     /* bits */
+    double evalue;
+    float nexpected;
+    uint32_t ndom;         /* this hit's domains: domains[dom_first .. dom_first + ndom) */
+    uint64_t dom_first;
+} sx_hit;
+
+typedef struct {           /* DomainHit (search.rs:84-97) */
+    float score, bias;
+    double c_evalue, i_evalue;
+    uint32_t hmm_from, hmm_to, ali_from, ali_to, env_from, env_to;
+    int32_t reported;
+    int32_t pad;
+} sx_domain;
+
+typedef struct sx_results sx_results;
+
+/* Search the context's profiles against its targets. */
+int sx_search(sx_ctx* ctx, const sx_search_cfg* cfg, sx_results** out);
+
+/* The host part alone, for callers that already hold the device results: the rest of
+ * score_target and `report` for the listed pairs, given their whole-sequence Forward
+ * and Backward scores and special rows (5 x (L+1) floats per pair, rows N B E J C,
+ * pairs back to back, as sx_specials writes them).  Models are configured from hmms;
+ * targets are codes[offsets[i] .. offsets[i+1]).  Needs no device.  Errors are read
+ * with sx_last_error(NULL). */
+int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,
+                   const sx_search_cfg* cfg, uint64_t npairs, const uint32_t* prof, const uint32_t* tgt,
+                   const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results** out);
+
+int sx_results_size(const sx_results* r, uint64_t* nhits, uint64_t* ndomains);
+int sx_results_copy(const sx_results* r, sx_hit* hits, sx_domain* domains);
+void sx_results_free(sx_results* r);
+
 /* Lane-group shapes.  A pair is scored by a group of G lanes of one wavefront, each lane
  * holding Q consecutive model positions of the DP row in registers (capacity Q*G >= M);
  * `hoist` = 1 keeps the profile's transition scores in registers across rows instead of

@@ -26,7 +26,7 @@ EXPORTS = [
     "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
     "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
-    "sx_autotune_log",
+    "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_copy", "sx_results_free",
 ]
 
 
@@ -67,6 +67,20 @@ class sx_cascade_stats(ctypes.Structure):
                 ("cells_msv", ctypes.c_uint64), ("cells_vit", ctypes.c_uint64), ("cells_fwd", ctypes.c_uint64)]
 
 
+class sx_search_cfg(ctypes.Structure):
+    _fields_ = [("cutoff", ctypes.c_int32), ("filters", ctypes.c_int32), ("null2", ctypes.c_int32),
+                ("threads", ctypes.c_int32), ("seq_e", ctypes.c_double), ("dom_e", ctypes.c_double),
+                ("seq_t", ctypes.c_float), ("dom_t", ctypes.c_float), ("f1", ctypes.c_double),
+                ("f2", ctypes.c_double), ("f3", ctypes.c_double), ("z", ctypes.c_double)]
+
+
+HIT_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("qlen", "<u4"), ("tlen", "<u4"), ("score", "<f4"),
+                      ("bias", "<f4"), ("evalue", "<f8"), ("nexpected", "<f4"), ("ndom", "<u4"),
+                      ("dom_first", "<u8")])
+DOMAIN_DTYPE = np.dtype([("score", "<f4"), ("bias", "<f4"), ("c_evalue", "<f8"), ("i_evalue", "<f8"),
+                         ("hmm_from", "<u4"), ("hmm_to", "<u4"), ("ali_from", "<u4"), ("ali_to", "<u4"),
+                         ("env_from", "<u4"), ("env_to", "<u4"), ("reported", "<i4"), ("pad", "<i4")])
+
 PAIR_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("msv", "<f4"), ("bias", "<f4"), ("vit", "<f4"),
                        ("fwd", "<f4"), ("fwd64", "<f8"), ("stage", "<i4"), ("pad", "<i4")])
 
@@ -105,6 +119,13 @@ def lib():
         L.sx_cascade_records.argtypes = [vp, vp, u64, ctypes.POINTER(u64)]
         L.sx_cascade_launches.argtypes = [vp, u32, ctypes.POINTER(u32), vp]
         L.sx_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sx_search.argtypes = [vp, ctypes.POINTER(sx_search_cfg), ctypes.POINTER(vp)]
+        L.sx_search_host.argtypes = [ctypes.POINTER(sx_hmm), u32, vp, vp, u32, ctypes.POINTER(sx_search_cfg), u64,
+                                     vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
+        L.sx_results_size.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.sx_results_copy.argtypes = [vp, vp, vp]
+        L.sx_results_free.argtypes = [vp]
+        L.sx_results_free.restype = None
         L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]
         L.sx_set_shapes.argtypes = [vp, ctypes.c_int, u32, vp, vp, vp, vp]
@@ -118,6 +139,99 @@ def lib():
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def _hmm_array(hmms):
+    """An sx_hmm array viewing the models' arrays (and the list of objects that must stay alive)."""
+    arr = (sx_hmm * max(1, len(hmms)))()
+    keep = [arr]
+    for i, h in enumerate(hmms):
+        mat = np.ascontiguousarray(h.mat, np.float32)
+        t = np.ascontiguousarray(h.t, np.float32)
+        keep += [mat, t]
+        e = arr[i]
+        nm = h.name.encode()
+        keep.append(nm)
+        e.name = nm
+        e.acc = getattr(h, "acc", None).encode() if getattr(h, "acc", None) else None
+        e.desc = getattr(h, "desc", None).encode() if getattr(h, "desc", None) else None
+        e.m = h.m
+        e.mat = mat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        e.t = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        if getattr(h, "compo", None) is not None:
+            c = np.ascontiguousarray(h.compo, np.float32)
+            keep.append(c)
+            e.compo = c.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        for nm2 in ("ga", "tc", "nc"):
+            v = getattr(h, nm2, None)
+            if v is not None:
+                setattr(e, "has_" + nm2, 1)
+                getattr(e, nm2)[0], getattr(e, nm2)[1] = v
+        st = getattr(h, "stats", {}) or {}
+        bits = 0
+        for b, key in enumerate(("MSV", "VITERBI", "FORWARD")):
+            if key in st:
+                bits |= 1 << b
+                e.stats[2 * b], e.stats[2 * b + 1] = st[key]
+        e.has_stats = bits
+    return arr, keep
+
+
+CUTOFFS = {"evalue": 0, "bits": 1, "ga": 2, "nc": 3, "tc": 4}
+
+
+def search_config(cutoff="evalue", seq_e=10.0, dom_e=10.0, seq_t=0.0, dom_t=0.0, filters=True, null2=True,
+                  f1=0.02, f2=1e-3, f3=1e-5, z=None, threads=0):
+    """SearchConfig (search.rs:42-47) with PipelineConfig (pipeline.rs:27-42) as an sx_search_cfg."""
+    c = sx_search_cfg()
+    c.cutoff = CUTOFFS[cutoff]
+    c.filters, c.null2, c.threads = int(bool(filters)), int(bool(null2)), int(threads)
+    c.seq_e, c.dom_e, c.seq_t, c.dom_t = seq_e, dom_e, seq_t, dom_t
+    c.f1, c.f2, c.f3 = f1, f2, f3
+    c.z = float(z) if z else 0.0
+    return c
+
+
+def _take_results(L, r):
+    nh, nd = ctypes.c_uint64(), ctypes.c_uint64()
+    try:
+        L.sx_results_size(r, ctypes.byref(nh), ctypes.byref(nd))
+        hits = np.zeros(nh.value, HIT_DTYPE)
+        doms = np.zeros(nd.value, DOMAIN_DTYPE)
+        L.sx_results_copy(r, _ptr(hits), _ptr(doms))
+    finally:
+        L.sx_results_free(r)
+    return hits, doms
+
+
+def search_host(hmms, seqs, prof, tgt, fwd, bwd, fsp, bsp, **cfg):
+    """The host part of the search alone (sx_search_host): pairs (prof[i], tgt[i]) with their
+    whole-sequence Forward / Backward scores and special rows (fsp / bsp: for each pair a
+    (5, L+1) array, rows N B E J C).  Needs no device."""
+    L = lib()
+    arr, keep = _hmm_array(hmms)
+    lens = np.array([len(x) for x in seqs], np.uint64)
+    off = np.zeros(len(seqs) + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    codes = np.concatenate([np.asarray(x, np.uint8) for x in seqs]) if len(seqs) else np.zeros(1, np.uint8)
+    if codes.size == 0:
+        codes = np.zeros(1, np.uint8)
+    prof = np.ascontiguousarray(prof, np.uint32)
+    tgt = np.ascontiguousarray(tgt, np.uint32)
+    fwd = np.ascontiguousarray(fwd, np.float64)
+    bwd = np.ascontiguousarray(bwd, np.float64)
+    f = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).ravel() for x in fsp]) if len(fsp)
+                             else np.zeros(1, np.float32))
+    b = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).ravel() for x in bsp]) if len(bsp)
+                             else np.zeros(1, np.float32))
+    c = search_config(**cfg)
+    r = ctypes.c_void_p()
+    rc = L.sx_search_host(arr, len(hmms), _ptr(codes), _ptr(off), len(seqs), ctypes.byref(c), len(prof), _ptr(prof),
+                          _ptr(tgt), _ptr(fwd), _ptr(bwd), _ptr(f), _ptr(b), ctypes.byref(r))
+    if rc != 0:
+        raise SxError(f"error {rc}: {L.sx_last_error(None).decode()}")
+    return _take_results(L, r)
+
 
 
 class Context:
@@ -168,39 +282,18 @@ class Context:
 
     def set_profiles(self, hmms):
         """hmms: objects with name, m, mat (M+1,20), t (M+1,7), compo, ga/tc/nc, stats dict."""
-        arr = (sx_hmm * max(1, len(hmms)))()
-        keep = []
-        for i, h in enumerate(hmms):
-            mat = np.ascontiguousarray(h.mat, np.float32)
-            t = np.ascontiguousarray(h.t, np.float32)
-            keep += [mat, t]
-            e = arr[i]
-            nm = h.name.encode()
-            keep.append(nm)
-            e.name = nm
-            e.acc = getattr(h, "acc", None).encode() if getattr(h, "acc", None) else None
-            e.desc = getattr(h, "desc", None).encode() if getattr(h, "desc", None) else None
-            e.m = h.m
-            e.mat = mat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-            e.t = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-            if getattr(h, "compo", None) is not None:
-                c = np.ascontiguousarray(h.compo, np.float32)
-                keep.append(c)
-                e.compo = c.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-            for nm2 in ("ga", "tc", "nc"):
-                v = getattr(h, nm2, None)
-                if v is not None:
-                    setattr(e, "has_" + nm2, 1)
-                    getattr(e, nm2)[0], getattr(e, nm2)[1] = v
-            st = getattr(h, "stats", {}) or {}
-            bits = 0
-            for b, key in enumerate(("MSV", "VITERBI", "FORWARD")):
-                if key in st:
-                    bits |= 1 << b
-                    e.stats[2 * b], e.stats[2 * b + 1] = st[key]
-            e.has_stats = bits
+        arr, keep = _hmm_array(hmms)
         self._chk(self._L.sx_set_profiles(self._h, arr, len(hmms)))
         self.n_profiles = len(hmms)
+
+    def search(self, **cfg):
+        """hmmsearch-equivalent search of the loaded profiles against the loaded targets
+        (see sx_search).  Keywords as search_config().  Returns (hits, domains) structured
+        arrays; hit h's domains are domains[h['dom_first'] : h['dom_first'] + h['ndom']]."""
+        c = search_config(**cfg)
+        r = ctypes.c_void_p()
+        self._chk(self._L.sx_search(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return _take_results(self._L, r)
 
     def load_hmm_file(self, path: str) -> int:
         n = ctypes.c_uint32()

@@ -1,0 +1,96 @@
+// sx_domain.h -- host-side remainder of score_target (sentinel-x src/hmmer/pipeline.rs:186-294):
+// posterior domain decoding from the Forward/Backward special rows, region finding,
+// multi-domain resolution by a stochastic-trace ensemble, envelope rescoring with
+// full-matrix Forward/Backward, null2 and optimal-accuracy coordinates, and the
+// sequence / domain scores.  This part runs on the host, as in the reference; the
+// special rows and whole-sequence scores it starts from come from the device.
+//
+// Arithmetic follows the reference's f32/f64 operation order (built without FP
+// contraction, glibc logf/expf/log/exp) so that scores agree to f32 rounding and
+// coordinates agree exactly.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "sx_model.h"
+
+namespace sx {
+
+// Special-state rows N, B, E, J, C for rows 0..L as natural-log values (dp.rs:211-226).
+struct SpecRows {
+    std::vector<float> n, b, e, j, c;
+    void reset(int l);
+};
+
+// Full DP matrix in probability space, rows 0..L x columns 0..M, with the cumulative
+// ln scale per row, the log special rows, and the scaled special values and row
+// scale factor that the stochastic traceback reads (dp.rs:398-440).
+struct DpMatrix {
+    int m = 0, l = 0;
+    std::vector<float> mm, im, dm;
+    std::vector<double> lnscale;
+    SpecRows sp;
+    std::vector<float> xn, xb, xe, xj, xc, scale;
+    void reset(int m, int l);
+};
+
+float forward_matrix(const Profile& p, const uint8_t* dsq, int l, const Xsc& xs, DpMatrix& fx);   // dp.rs:443-522
+float backward_matrix(const Profile& p, const uint8_t* dsq, int l, const Xsc& xs, DpMatrix& bx);  // dp.rs:525-601
+
+struct Posterior {  // dp.rs:1262-1272
+    int m = 0, l = 0;
+    std::vector<float> pm, pi, pn, pj, pc;
+};
+void posterior_decode(const Profile& p, const Xsc& xs, const DpMatrix& fx, const DpMatrix& bx, Posterior& pp);
+void null2_from_posterior(const Profile& p, const Posterior& pp, float out[NCODE]);
+
+struct AliCoords {
+    int iali, jali, kali, mali;
+};
+bool optimal_accuracy_coords(const Profile& p, const Xsc& xs, const Posterior& pp, AliCoords& out);
+std::vector<AliCoords> viterbi_trace(const Profile& p, const uint8_t* dsq, int l, const Xsc& xs);
+
+// Easel's fast generator (esl_randomness_CreateFast): 32-bit LCG x = 69069 x + 1,
+// seeded through esl_mix3 (dp.rs:962-1024).
+class EaselRng {
+public:
+    explicit EaselRng(uint32_t seed);
+    void reseed();
+    double next();
+
+private:
+    uint32_t seed_, x_;
+};
+
+struct SampledDomain {
+    int sqfrom = 0, sqto = 0, hmmfrom = 0, hmmto = 0;
+    std::vector<int> nodes;  // model node of each M or I emission
+};
+std::vector<SampledDomain> stochastic_trace(const Profile& p, const DpMatrix& fx, const Xsc& xs, EaselRng& rng);
+
+struct RawDomain {
+    int ienv, jenv, iali, jali, kali, mali;
+    float bits, bias;
+    double lnp;
+};
+struct RawHit {
+    float bits, bias;
+    double lnp;
+    float nexpected;
+    int nregions;
+    std::vector<RawDomain> domains;
+};
+
+// Scratch reused across pairs by one worker.
+struct TailWork {
+    DpMatrix fx, bx;
+    Posterior pp;
+};
+
+// pipeline.rs:186-294, given the whole-sequence Forward and Backward scores and their
+// special rows.  Returns false where score_target returns None.
+bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, float bcksc,
+                const SpecRows& f, const SpecRows& b, TailWork& ws, RawHit& out);
+
+}  // namespace sx

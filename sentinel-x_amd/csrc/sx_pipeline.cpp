@@ -180,7 +180,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     const size_t P = c.profs.size();
     const uint32_t T = c.nt;
     for (const auto& p : c.profs)
-        if (!p.stats.calibrated())
+        if (!p.stats.calibrated() && !c.cascade_pass_uncalibrated)
             throw SxError(SX_ERR_STATE, "profile '" + p.name +
                                             "' has no STATS lines; the filter cascade needs calibrated models");
     S.nprof = P;
@@ -194,6 +194,11 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     S.threshold_violations = 0;
     for (size_t p = 0; p < P; p++) {
         const Stats& st = c.profs[p].stats;
+        if (!st.calibrated()) {
+            // the reference runs no filter for an uncalibrated model (pipeline.rs:164)
+            S.thr_msv[p] = S.thr_vit[p] = S.thr_fwd[p] = NEG_INF;
+            continue;
+        }
         S.thr_msv[p] = pass_threshold(
             [&](float y) { return gumbel_surv((double)y, st.msv[0], st.msv[1]) > f1; }, &S.threshold_violations);
         S.thr_vit[p] = pass_threshold(
@@ -683,6 +688,35 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, recs.size()) * sizeof(sx_pair_scores));
     S.last.swap(recs);
     return S.last.size();
+}
+
+}  // namespace sx
+
+namespace sx {
+
+std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3) {
+    sx_cascade_stats st{};
+    st.f1 = f1;
+    st.f2 = f2;
+    st.f3 = f3;
+    st.use_graph = 1;
+    c.ensure_pidx();
+    c.cascade_pass_uncalibrated = true;
+    c.cascade_dirty = true;
+    try {
+        (void)run_batched(c, &st, nullptr, 0);
+    } catch (...) {
+        c.cascade_pass_uncalibrated = false;
+        c.cascade_dirty = true;
+        throw;
+    }
+    c.cascade_pass_uncalibrated = false;
+    c.cascade_dirty = true;
+    std::vector<std::pair<uint32_t, uint32_t>> out;
+    for (const auto& r : c.cascade->last)
+        if (r.stage == 4) out.push_back({r.profile, r.target});
+    std::sort(out.begin(), out.end());
+    return out;
 }
 
 }  // namespace sx

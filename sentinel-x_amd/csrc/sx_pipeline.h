@@ -1,6 +1,8 @@
 // sx_pipeline.h -- filter cascade of score_target (pipeline.rs:159-189) on the device.
 #pragma once
 #include <functional>
+#include <utility>
+#include <vector>
 
 #include "../../include/sx_hmmer.h"
 #include "sx_runtime.h"
@@ -12,6 +14,10 @@ double exp_logsurv(double x, double tau, double lambda);  // pipeline.rs:98-104
 
 // Smallest f32 value that a monotone reject-test lets through (see sx_pipeline.cpp).
 float pass_threshold(const std::function<bool(float)>& rejects, int* violations);
+
+// The batched cascade over every pair, with no filter for profiles lacking STATS lines
+// (as score_target, pipeline.rs:164); returns the pairs that pass every filter, sorted.
+std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3);
 
 }  // namespace sx
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -17,6 +18,7 @@
 #include "sx_model.h"
 #include "sx_runtime.h"
 #include "sx_pipeline.h"
+#include "sx_search.h"
 
 struct SxKernelEntry {
     int stage, q, g, spec, hoist, wid;
@@ -176,6 +178,7 @@ void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n)
     }
     h_toff = off;
     d_codes = dupload(packed);
+    h_codes.swap(packed);  // host copy, read by the host-side domain definition
     d_soff = dupload(off);
     d_len = dupload(tlen);
     d_lm = dupload(lm);
@@ -655,6 +658,76 @@ double Ctx::timed(const std::function<void()>& f) {
     return ms;
 }
 
+// Forward or Backward special rows of a pair list: the device writes the scaled special
+// values and the f64 row scale; the host takes logs exactly as the reference does.
+void Ctx::special_rows(int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp, double* score) {
+    Ctx& c = *this;
+    if (n == 0) return;
+    std::vector<int64_t> roff(n);
+    int64_t rows = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        if (tgt[i] >= c.nt || prof[i] >= c.profs.size()) throw SxError(SX_ERR_ARG, "pair index out of range");
+        roff[i] = rows;
+        rows += c.tlen[tgt[i]] + 1;
+    }
+    void* d = c.scratch_out.ensure(n * 8);
+    float* dx5 = (float*)c.scratch_x5.ensure((size_t)rows * 5 * sizeof(float));
+    double* dls = (double*)c.scratch_ls.ensure((size_t)rows * sizeof(double));
+    c.run_pairs(stage, SX_FORM_BATCHED, prof, tgt, n, d, true, &roff, dx5, dls);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    std::vector<float> x5((size_t)rows * 5);
+    std::vector<double> ls((size_t)rows);
+    std::vector<double> dev(n);
+    HIPCHK(hipMemcpy(x5.data(), dx5, x5.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ls.data(), dls, ls.size() * 8, hipMemcpyDeviceToHost));
+    {
+        std::vector<double> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), d, n * 8, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; i++) dev[c.last_order[i]] = tmp[i];
+    }
+    // ln of the stored values plus the row scale, with the host libm, as
+    // dp.rs:284-292 (Forward) and dp.rs:372-377, 383-390 (Backward) do
+    auto lnf = [](float x) { return x > 0.0f ? std::log(x) : sx::NEG_INF; };
+    for (uint64_t i = 0; i < n; i++) {
+        const int L = c.tlen[tgt[i]];
+        const float* r = x5.data() + (size_t)roff[i] * 5;
+        const double* lr = ls.data() + roff[i];
+        float* o = sp ? sp + (size_t)roff[i] * 5 : nullptr;
+        if (L == 0) {
+            if (o)
+                for (int q = 0; q < 5; q++) o[q] = sx::NEG_INF;
+            if (score) score[i] = -INFINITY;
+            continue;
+        }
+        const sx::Xsc xs = sx::Xsc::make((size_t)L, true);
+        if (o) {
+            for (int row = 0; row <= L; row++) {
+                const float l = (float)lr[row];
+                for (int q = 0; q < 5; q++) o[(size_t)q * (L + 1) + row] = lnf(r[row * 5 + q]) + l;
+            }
+            if (stage == SX_STAGE_FORWARD) {
+                o[0] = 0.0f;  // N(0) = ln 1
+                for (int q = 2; q < 5; q++) o[(size_t)q * (L + 1)] = sx::NEG_INF;
+            } else {
+                o[(size_t)2 * (L + 1)] = o[(size_t)3 * (L + 1)] = o[(size_t)4 * (L + 1)] = sx::NEG_INF;
+                o[L] = sx::NEG_INF;                                   // N(L)
+                o[(size_t)1 * (L + 1) + L] = sx::NEG_INF;             // B(L)
+                o[(size_t)3 * (L + 1) + L] = sx::NEG_INF;             // J(L)
+                const float n1 = L > 1 ? o[1] : sx::NEG_INF;
+                o[0] = sx::logsum(n1 + xs.n_loop, o[(size_t)1 * (L + 1)] + xs.n_move);
+            }
+        }
+        if (score) {
+            if (stage == SX_STAGE_FORWARD) {
+                const float fin = r[L * 5 + 4] * xs.pc_move;
+                score[i] = (double)lnf(fin) + lr[L];
+            } else {
+                score[i] = o ? (double)o[0] : dev[i];
+            }
+        }
+    }
+    }
+
 }  // namespace sx
 
 // ---------------------------------------------------------------------------
@@ -723,7 +796,7 @@ extern "C" int sx_set_targets(sx_ctx* ctx, const uint8_t* codes, const uint64_t*
     return guard(ctx, [&] { ctx->c->set_targets(codes, offsets, n); });
 }
 
-static sx::Hmm hmm_from_c(const sx_hmm& h) {
+sx::Hmm sx::hmm_from_c(const sx_hmm& h) {
     sx::Hmm o;
     if (h.m <= 0 || !h.mat || !h.t) throw SxError(SX_ERR_ARG, "model needs m > 0, mat and t");
     o.name = h.name ? h.name : "";
@@ -761,7 +834,7 @@ extern "C" int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n) {
     return guard(ctx, [&] {
         std::vector<sx::Profile> ps;
         ps.reserve(n);
-        for (uint32_t i = 0; i < n; i++) ps.push_back(sx::configure(hmm_from_c(hmms[i])));
+        for (uint32_t i = 0; i < n; i++) ps.push_back(sx::configure(sx::hmm_from_c(hmms[i])));
         ctx->c->set_profiles(std::move(ps));
     });
 }
@@ -842,73 +915,7 @@ extern "C" int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* 
 extern "C" int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp,
                            double* score) {
     if (!ctx || (stage != SX_STAGE_FORWARD && stage != SX_STAGE_BACKWARD) || (n && (!prof || !tgt))) return SX_ERR_ARG;
-    return guard(ctx, [&] {
-        Ctx& c = *ctx->c;
-        if (n == 0) return;
-        std::vector<int64_t> roff(n);
-        int64_t rows = 0;
-        for (uint64_t i = 0; i < n; i++) {
-            if (tgt[i] >= c.nt || prof[i] >= c.profs.size()) throw SxError(SX_ERR_ARG, "pair index out of range");
-            roff[i] = rows;
-            rows += c.tlen[tgt[i]] + 1;
-        }
-        void* d = c.scratch_out.ensure(n * 8);
-        float* dx5 = (float*)c.scratch_x5.ensure((size_t)rows * 5 * sizeof(float));
-        double* dls = (double*)c.scratch_ls.ensure((size_t)rows * sizeof(double));
-        c.run_pairs(stage, SX_FORM_BATCHED, prof, tgt, n, d, true, &roff, dx5, dls);
-        HIPCHK(hipStreamSynchronize(c.stream));
-        std::vector<float> x5((size_t)rows * 5);
-        std::vector<double> ls((size_t)rows);
-        std::vector<double> dev(n);
-        HIPCHK(hipMemcpy(x5.data(), dx5, x5.size() * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(ls.data(), dls, ls.size() * 8, hipMemcpyDeviceToHost));
-        {
-            std::vector<double> tmp(n);
-            HIPCHK(hipMemcpy(tmp.data(), d, n * 8, hipMemcpyDeviceToHost));
-            for (uint64_t i = 0; i < n; i++) dev[c.last_order[i]] = tmp[i];
-        }
-        // ln of the stored values plus the row scale, with the host libm, as
-        // dp.rs:284-292 (Forward) and dp.rs:372-377, 383-390 (Backward) do
-        auto lnf = [](float x) { return x > 0.0f ? std::log(x) : sx::NEG_INF; };
-        for (uint64_t i = 0; i < n; i++) {
-            const int L = c.tlen[tgt[i]];
-            const float* r = x5.data() + (size_t)roff[i] * 5;
-            const double* lr = ls.data() + roff[i];
-            float* o = sp ? sp + (size_t)roff[i] * 5 : nullptr;
-            if (L == 0) {
-                if (o)
-                    for (int q = 0; q < 5; q++) o[q] = sx::NEG_INF;
-                if (score) score[i] = -INFINITY;
-                continue;
-            }
-            const sx::Xsc xs = sx::Xsc::make((size_t)L, true);
-            if (o) {
-                for (int row = 0; row <= L; row++) {
-                    const float l = (float)lr[row];
-                    for (int q = 0; q < 5; q++) o[(size_t)q * (L + 1) + row] = lnf(r[row * 5 + q]) + l;
-                }
-                if (stage == SX_STAGE_FORWARD) {
-                    o[0] = 0.0f;  // N(0) = ln 1
-                    for (int q = 2; q < 5; q++) o[(size_t)q * (L + 1)] = sx::NEG_INF;
-                } else {
-                    o[(size_t)2 * (L + 1)] = o[(size_t)3 * (L + 1)] = o[(size_t)4 * (L + 1)] = sx::NEG_INF;
-                    o[L] = sx::NEG_INF;                                   // N(L)
-                    o[(size_t)1 * (L + 1) + L] = sx::NEG_INF;             // B(L)
-                    o[(size_t)3 * (L + 1) + L] = sx::NEG_INF;             // J(L)
-                    const float n1 = L > 1 ? o[1] : sx::NEG_INF;
-                    o[0] = sx::logsum(n1 + xs.n_loop, o[(size_t)1 * (L + 1)] + xs.n_move);
-                }
-            }
-            if (score) {
-                if (stage == SX_STAGE_FORWARD) {
-                    const float fin = r[L * 5 + 4] * xs.pc_move;
-                    score[i] = (double)lnf(fin) + lr[L];
-                } else {
-                    score[i] = o ? (double)o[0] : dev[i];
-                }
-            }
-        }
-    });
+    return guard(ctx, [&] { ctx->c->special_rows(stage, prof, tgt, n, sp, score); });
 }
 
 extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g,
@@ -1017,3 +1024,53 @@ extern "C" int sx_cascade_records(sx_ctx* ctx, sx_pair_scores* out, uint64_t cap
         if (n_out) *n_out = n;
     });
 }
+
+extern "C" int sx_search(sx_ctx* ctx, const sx_search_cfg* cfg, sx_results** out) {
+    if (!ctx || !cfg || !out) return SX_ERR_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        auto r = std::make_unique<sx_results>();
+        sx::search(*ctx->c, *cfg, *r);
+        *out = r.release();
+    });
+}
+
+extern "C" int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets,
+                              uint32_t nt, const sx_search_cfg* cfg, uint64_t npairs, const uint32_t* prof,
+                              const uint32_t* tgt, const double* fwd, const double* bwd, const float* fsp,
+                              const float* bsp, sx_results** out) {
+    if (!out || !cfg || (np && !hmms) || !offsets || (npairs && (!prof || !tgt || !fwd || !bwd || !fsp || !bsp)))
+        return SX_ERR_ARG;
+    *out = nullptr;
+    try {
+        auto r = std::make_unique<sx_results>();
+        sx::search_host(hmms, np, codes, offsets, nt, *cfg, npairs, prof, tgt, fwd, bwd, fsp, bsp, *r);
+        *out = r.release();
+        return SX_OK;
+    } catch (const SxError& e) {
+        g_create_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_create_err = "out of host memory";
+        return SX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return SX_ERR_PARSE;
+    }
+}
+
+extern "C" int sx_results_size(const sx_results* r, uint64_t* nhits, uint64_t* ndomains) {
+    if (!r) return SX_ERR_ARG;
+    if (nhits) *nhits = r->hits.size();
+    if (ndomains) *ndomains = r->doms.size();
+    return SX_OK;
+}
+
+extern "C" int sx_results_copy(const sx_results* r, sx_hit* hits, sx_domain* domains) {
+    if (!r) return SX_ERR_ARG;
+    if (hits && !r->hits.empty()) std::memcpy(hits, r->hits.data(), r->hits.size() * sizeof(sx_hit));
+    if (domains && !r->doms.empty()) std::memcpy(domains, r->doms.data(), r->doms.size() * sizeof(sx_domain));
+    return SX_OK;
+}
+
+extern "C" void sx_results_free(sx_results* r) { delete r; }

@@ -71,6 +71,9 @@ struct Ctx {
     static size_t lds_bytes(int stage, int wid, int hoist);
     static size_t lds_bytes(const KernelInfo& k) { return lds_bytes(k.stage, k.wid, k.hoist); }
 
+    // the search's cascade lets profiles without STATS lines through every filter
+    bool cascade_pass_uncalibrated = false;
+
     // profiles
     std::vector<Profile> profs;
     float *d_msc = nullptr, *d_mo = nullptr, *d_tsc = nullptr, *d_tp = nullptr, *d_tbm = nullptr;
@@ -87,7 +90,8 @@ struct Ctx {
     // targets
     uint32_t nt = 0;
     std::vector<int32_t> tlen;
-    std::vector<int64_t> h_toff;
+    std::vector<int64_t> h_toff;   // start of each target in h_codes
+    std::vector<uint8_t> h_codes;  // host copy of the packed codes (the device buffer's layout)
     std::vector<int32_t> sorted;
     int64_t total_len = 0;
     uint8_t* d_codes = nullptr;
@@ -127,6 +131,9 @@ struct Ctx {
     void run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* dout, bool spec,
                    const std::vector<int64_t>* row_off, float* dx5, double* dls);
     double timed(const std::function<void()>& f);
+    // Forward (stage 2) or Backward (stage 3) special rows of n pairs: sp receives
+    // 5 x (L+1) floats per pair (rows N B E J C, pairs back to back), score one value each
+    void special_rows(int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp, double* score);
 
     // device filter cascade (sx_pipeline.cpp); rebuilt when profiles, targets or shapes change
     std::shared_ptr<CascadeState> cascade;

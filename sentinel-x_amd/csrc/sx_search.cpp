@@ -1,0 +1,284 @@
+// sx_search.cpp -- hmmsearch-equivalent search over the device cascade (sentinel-x
+// src/hmmer/search.rs:131-271): pairs that pass the filters get Forward and Backward
+// special rows from the device, the rest of score_target runs on host threads
+// (sx_domain.cpp), and `report` applies the E-value / bit / GA-NC-TC thresholds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "sx_search.h"
+
+#include "sx_domain.h"
+#include "sx_pipeline.h"
+
+namespace sx {
+
+namespace {
+
+struct Threshold {
+    bool by_evalue;
+    double seq_e, dom_e;
+    float seq_t, dom_t;
+};
+
+// thresholds (search.rs:131-148)
+std::vector<Threshold> resolve_thresholds(const std::vector<Profile>& profs, const sx_search_cfg& cfg) {
+    std::vector<Threshold> out;
+    for (const Profile& p : profs) {
+        Threshold t{};
+        switch (cfg.cutoff) {
+            case SX_CUT_EVALUE:
+                t = {true, cfg.seq_e, cfg.dom_e, 0.0f, 0.0f};
+                break;
+            case SX_CUT_BITS:
+                t = {false, 0.0, 0.0, cfg.seq_t, cfg.dom_t};
+                break;
+            case SX_CUT_GA:
+            case SX_CUT_NC:
+            case SX_CUT_TC: {
+                const auto& c = cfg.cutoff == SX_CUT_GA ? p.ga : cfg.cutoff == SX_CUT_NC ? p.nc : p.tc;
+                const char* nm = cfg.cutoff == SX_CUT_GA ? "GA" : cfg.cutoff == SX_CUT_NC ? "NC" : "TC";
+                if (!c) throw SxError(SX_ERR_CUTOFF, std::string(nm) + " bit thresholds unavailable on model " + p.name);
+                t = {false, 0.0, 0.0, c->first, c->second};
+                break;
+            }
+            default:
+                throw SxError(SX_ERR_ARG, "unknown cutoff kind");
+        }
+        out.push_back(t);
+    }
+    return out;
+}
+
+// The host part of score_target for a list of pairs on host worker threads.
+// fsp / bsp hold 5 x (L+1) rows per pair starting at row_off[i].
+void run_tails(const std::vector<Profile>& profs, const std::vector<const uint8_t*>& seq, const std::vector<int>& len,
+               const uint32_t* prof, const uint32_t* tgt, uint64_t n, const double* fwd, const double* bwd,
+               const float* fsp, const float* bsp, const std::vector<int64_t>& row_off, bool null2, int threads,
+               std::vector<RawHit>& hit, std::vector<char>& ok) {
+    hit.assign(n, RawHit{});
+    ok.assign(n, 0);
+    if (n == 0) return;
+    int nth = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nth = (int)std::min<uint64_t>((uint64_t)nth, n);
+    std::atomic<uint64_t> next{0};
+    std::vector<std::string> err((size_t)nth);
+    auto worker = [&](int w) {
+        TailWork ws;
+        SpecRows f, b;
+        try {
+            for (uint64_t i; (i = next.fetch_add(1)) < n;) {
+                const int L = len[tgt[i]];
+                const size_t r0 = (size_t)row_off[i] * 5, W = (size_t)L + 1;
+                auto load = [&](const float* src, SpecRows& d) {
+                    d.n.assign(src + r0, src + r0 + W);
+                    d.b.assign(src + r0 + W, src + r0 + 2 * W);
+                    d.e.assign(src + r0 + 2 * W, src + r0 + 3 * W);
+                    d.j.assign(src + r0 + 3 * W, src + r0 + 4 * W);
+                    d.c.assign(src + r0 + 4 * W, src + r0 + 5 * W);
+                };
+                load(fsp, f);
+                load(bsp, b);
+                ok[i] = score_tail(profs[prof[i]], seq[tgt[i]], L, null2, (float)fwd[i], (float)bwd[i], f, b, ws, hit[i]);
+            }
+        } catch (const std::exception& e) {
+            err[(size_t)w] = e.what();
+            next.store(n);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nth; w++) pool.emplace_back(worker, w);
+    worker(0);
+    for (auto& t : pool) t.join();
+    for (const auto& e : err)
+        if (!e.empty()) throw SxError(SX_ERR_STATE, "domain definition: " + e);
+}
+
+// report (search.rs:160-210) for one profile; raws in target order
+void report(uint32_t pi, const Profile& p, const std::vector<std::pair<uint32_t, const RawHit*>>& raws,
+            const std::vector<int>& len, const Threshold& th, double z, sx_results& res) {
+    const size_t first_hit = res.hits.size();
+    for (const auto& tr : raws) {
+        const RawHit& r = *tr.second;
+        const double evalue = std::exp(r.lnp) * z;
+        const bool keep = th.by_evalue ? evalue <= th.seq_e : r.bits >= th.seq_t;
+        if (!keep) continue;
+        sx_hit h{};
+        h.profile = pi;
+        h.target = tr.first;
+        h.qlen = (uint32_t)p.m;
+        h.tlen = (uint32_t)len[tr.first];
+        h.score = r.bits;
+        h.bias = r.bias;
+        h.evalue = evalue;
+        h.nexpected = r.nexpected;
+        h.ndom = (uint32_t)r.domains.size();
+        h.dom_first = res.doms.size();
+        for (const RawDomain& d : r.domains) {
+            sx_domain o{};
+            o.score = d.bits;
+            o.bias = d.bias;
+            o.c_evalue = std::numeric_limits<double>::quiet_NaN();
+            o.i_evalue = std::exp(d.lnp) * z;
+            o.hmm_from = (uint32_t)d.kali;
+            o.hmm_to = (uint32_t)d.mali;
+            o.ali_from = (uint32_t)d.iali;
+            o.ali_to = (uint32_t)d.jali;
+            o.env_from = (uint32_t)d.ienv;
+            o.env_to = (uint32_t)d.jenv;
+            o.reported = 0;
+            res.doms.push_back(o);
+        }
+        res.hits.push_back(h);
+    }
+    const double dom_z = (double)std::max<size_t>(res.hits.size() - first_hit, 1);
+    for (size_t hi = first_hit; hi < res.hits.size(); hi++) {
+        const sx_hit& h = res.hits[hi];
+        for (uint64_t di = h.dom_first; di < h.dom_first + h.ndom; di++) {
+            sx_domain& d = res.doms[di];
+            d.c_evalue = d.i_evalue / z * dom_z;
+            d.reported = th.by_evalue ? d.c_evalue <= th.dom_e : d.score >= th.dom_t;
+        }
+    }
+}
+
+// Tails and reports for sorted pairs whose scores and special rows are given.
+void tails_and_report(const std::vector<Profile>& profs, const std::vector<int>& len,
+                      const std::vector<Threshold>& ths, double z, const std::vector<uint32_t>& pp, const std::vector<uint32_t>& pt,
+                      const std::vector<RawHit>& hit, const std::vector<char>& ok, sx_results& res) {
+    size_t i = 0;
+    while (i < pp.size()) {
+        const uint32_t p = pp[i];
+        std::vector<std::pair<uint32_t, const RawHit*>> raws;
+        for (; i < pp.size() && pp[i] == p; i++)
+            if (ok[i]) raws.push_back({pt[i], &hit[i]});
+        report(p, profs[p], raws, len, ths[p], z, res);
+    }
+}
+
+}  // namespace
+
+void check_search_cfg(const sx_search_cfg* cfg) {
+    if (!cfg) throw SxError(SX_ERR_ARG, "search configuration missing");
+    if (cfg->cutoff < SX_CUT_EVALUE || cfg->cutoff > SX_CUT_TC) throw SxError(SX_ERR_ARG, "unknown cutoff kind");
+}
+
+// The device search: cascade, special rows in chunks, host tails, report.
+void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
+    check_search_cfg(&cfg);
+    const std::vector<Threshold> ths = resolve_thresholds(c.profs, cfg);
+    const double z = std::max(cfg.z > 0.0 ? cfg.z : (double)c.nt, 1.0);
+    if (c.profs.empty() || c.nt == 0) return;
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    if (cfg.filters) {
+        pairs = cascade_pass_pairs(c, cfg.f1, cfg.f2, cfg.f3);
+    } else {
+        pairs.reserve((size_t)c.profs.size() * c.nt);
+        for (uint32_t p = 0; p < c.profs.size(); p++)
+            for (uint32_t t = 0; t < c.nt; t++) pairs.push_back({p, t});
+    }
+    std::vector<const uint8_t*> seq(c.nt);
+    for (uint32_t t = 0; t < c.nt; t++) seq[t] = c.h_codes.data() + c.h_toff[t];
+    const std::vector<int>& len = c.tlen;
+    // chunks of at most 16M special rows per direction (320 MB of floats each)
+    const int64_t max_rows = 16LL << 20;
+    size_t a = 0;
+    std::vector<uint32_t> cp, ct;
+    std::vector<double> fwd, bwd;
+    std::vector<float> fsp, bsp;
+    std::vector<int64_t> roff;
+    std::vector<RawHit> hit;
+    std::vector<char> ok;
+    std::vector<uint32_t> all_p, all_t;
+    std::vector<RawHit> all_hit;
+    std::vector<char> all_ok;
+    while (a < pairs.size()) {
+        cp.clear();
+        ct.clear();
+        roff.clear();
+        int64_t rows = 0;
+        size_t b = a;
+        for (; b < pairs.size(); b++) {
+            const int64_t r = (int64_t)len[pairs[b].second] + 1;
+            if (b > a && rows + r > max_rows) break;
+            cp.push_back(pairs[b].first);
+            ct.push_back(pairs[b].second);
+            roff.push_back(rows);
+            rows += r;
+        }
+        const uint64_t n = cp.size();
+        fwd.assign(n, 0.0);
+        bwd.assign(n, 0.0);
+        fsp.assign((size_t)rows * 5, 0.0f);
+        bsp.assign((size_t)rows * 5, 0.0f);
+        c.special_rows(SX_STAGE_FORWARD, cp.data(), ct.data(), n, fsp.data(), fwd.data());
+        c.special_rows(SX_STAGE_BACKWARD, cp.data(), ct.data(), n, bsp.data(), bwd.data());
+        run_tails(c.profs, seq, len, cp.data(), ct.data(), n, fwd.data(), bwd.data(), fsp.data(), bsp.data(), roff,
+                  cfg.null2 != 0, cfg.threads, hit, ok);
+        all_p.insert(all_p.end(), cp.begin(), cp.end());
+        all_t.insert(all_t.end(), ct.begin(), ct.end());
+        all_hit.insert(all_hit.end(), std::make_move_iterator(hit.begin()), std::make_move_iterator(hit.end()));
+        all_ok.insert(all_ok.end(), ok.begin(), ok.end());
+        a = b;
+    }
+    tails_and_report(c.profs, len, ths, z, all_p, all_t, all_hit, all_ok, res);
+}
+
+void search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,
+                 const sx_search_cfg& cfg, uint64_t npairs, const uint32_t* prof, const uint32_t* tgt,
+                 const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results& res) {
+    check_search_cfg(&cfg);
+    std::vector<Profile> profs;
+    profs.reserve(np);
+    for (uint32_t i = 0; i < np; i++) profs.push_back(configure(hmm_from_c(hmms[i])));
+    const std::vector<Threshold> ths = resolve_thresholds(profs, cfg);
+    const double z = std::max(cfg.z > 0.0 ? cfg.z : (double)nt, 1.0);
+    std::vector<const uint8_t*> seq(nt);
+    std::vector<int> len(nt);
+    for (uint32_t t = 0; t < nt; t++) {
+        if (offsets[t + 1] < offsets[t]) throw SxError(SX_ERR_ARG, "target offsets must be non-decreasing");
+        seq[t] = codes + offsets[t];
+        len[t] = (int)(offsets[t + 1] - offsets[t]);
+        for (int k = 0; k < len[t]; k++)
+            if (seq[t][k] >= NCODE) throw SxError(SX_ERR_ARG, "residue code out of range (expected 0..24)");
+    }
+    // pairs in (profile, target) order, as the reference regroups them
+    std::vector<uint64_t> order(npairs);
+    for (uint64_t i = 0; i < npairs; i++) {
+        if (prof[i] >= np || tgt[i] >= nt) throw SxError(SX_ERR_ARG, "pair index out of range");
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+        return prof[x] != prof[y] ? prof[x] < prof[y] : tgt[x] < tgt[y];
+    });
+    std::vector<int64_t> roff_in(npairs);
+    int64_t rows = 0;
+    for (uint64_t i = 0; i < npairs; i++) {
+        roff_in[i] = rows;
+        rows += (int64_t)len[tgt[i]] + 1;
+    }
+    std::vector<uint32_t> sp(npairs), st(npairs);
+    std::vector<double> sf(npairs), sb(npairs);
+    std::vector<int64_t> roff(npairs);
+    for (uint64_t i = 0; i < npairs; i++) {
+        const uint64_t o = order[i];
+        sp[i] = prof[o];
+        st[i] = tgt[o];
+        sf[i] = fwd[o];
+        sb[i] = bwd[o];
+        roff[i] = roff_in[o];
+    }
+    std::vector<RawHit> hit;
+    std::vector<char> ok;
+    run_tails(profs, seq, len, sp.data(), st.data(), npairs, sf.data(), sb.data(), fsp, bsp, roff, cfg.null2 != 0,
+              cfg.threads, hit, ok);
+    tails_and_report(profs, len, ths, z, sp, st, hit, ok, res);
+}
+
+}  // namespace sx

@@ -1,0 +1,25 @@
+// sx_search.h -- hmmsearch-equivalent search (sentinel-x src/hmmer/search.rs:131-271) over the
+// device cascade and the host-side domain definition.
+#pragma once
+#include <vector>
+
+#include "../../include/sx_hmmer.h"
+#include "sx_runtime.h"
+
+struct sx_results {
+    std::vector<sx_hit> hits;
+    std::vector<sx_domain> doms;
+};
+
+namespace sx {
+
+Hmm hmm_from_c(const sx_hmm& h);  // sx_hmm (caller's model view) -> parsed model
+void check_search_cfg(const sx_search_cfg* cfg);
+// Device search of the context's profiles against its targets.
+void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res);
+// The host part alone for pairs whose scores and special rows are given.
+void search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,
+                 const sx_search_cfg& cfg, uint64_t npairs, const uint32_t* prof, const uint32_t* tgt,
+                 const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results& res);
+
+}  // namespace sx
