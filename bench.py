@@ -28,9 +28,12 @@ from __graft_entry__ import load_package  # noqa: E402
 # (sc + max(prev, xB); no instruction fuses them), plus half a v_max3 for the running row
 # maximum xE.  Packed f32 forms are not used (and there is no packed max).
 MSV_INSTR_PER_CELL = 2.5
-# MI355X vector ALU issue peak: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-instructions/s
-# (each wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles).
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# MI355X's published FP32 vector rate, 157.3 TFLOP/s, counts a fused multiply-add as two
+# flops: 78.6 T lane-operations/s (256 CU x 4 SIMD x 32 lanes x 2.4 GHz).  On gfx950
+# v_add_f32 issues in about 2.3-2.6 cycles per wave64 instruction but v_max_f32 and
+# v_max3_f32 take about 4, so an MSV cell (add + max + half a max3) needs 8 SIMD cycles per
+# 64 cells at best: 19.7 T cells/s, 0.625 of this peak.
+VALU_PEAK_TOPS = 157.3 / 2
 
 
 def log(*a):

@@ -214,9 +214,12 @@ int sx_results_copy(const sx_results* r, sx_hit* hits, sx_domain* domains);
 void sx_results_free(sx_results* r);
 
 /* Lane-group shapes.  A pair is scored by a group of G lanes of one wavefront, each lane
- * holding Q consecutive model positions of the DP row in registers (capacity Q*G >= M);
- * `hoist` = 1 keeps the profile's transition scores in registers across rows instead of
- * reading them from LDS.  The batched form looks a model's shape up in a per-stage
+ * holding Q consecutive model positions of the DP row in registers (capacity Q*G >= M).
+ * `hoist` names the table mode: 0 = profile tables in LDS; 1 = the transition scores
+ * stay in registers across rows instead of being re-read from LDS; 2 = long models
+ * (M > 1024, fixed shapes), emission rows read from global memory; 3 and 4 = MSV with
+ * G >= 16, emission rows in LDS as 16-byte-aligned lane blocks read four positions
+ * per load, 512 (3) or 1024 (4) threads per workgroup.  The batched form looks a model's shape up in a per-stage
  * table of rows (max_m[i], q[i], g[i], hoist[i]) in increasing max_m; a model of length
  * M uses the first row with M <= max_m.  These calls have no counterpart in the
  * reference (a CPU loop has no launch shape).

@@ -292,7 +292,44 @@ __device__ __forceinline__ int slot_k(int pos) {
 //   TM_REG  as TM_LDS, but the lane's transitions stay in registers across rows;
 //   TM_GEM  long models: emission rows read from a padded copy in global memory
 //           (through L1/L2), transitions staged in LDS and re-read every row.
-enum { TM_LDS = 0, TM_REG = 1, TM_GEM = 2 };
+// and two MSV-only modes for groups of G >= 16 lanes:
+//   TM_ALN  emission rows in LDS as 16-byte-aligned lane blocks (GeoA), read four
+//           positions per load; 512-thread workgroups share one staged table;
+//   TM_ALN1K  as TM_ALN with 1024-thread workgroups.
+enum { TM_LDS = 0, TM_REG = 1, TM_GEM = 2, TM_ALN = 3, TM_ALN1K = 4 };
+
+template <int TM>
+constexpr bool tm_aligned() {
+    return TM == TM_ALN || TM == TM_ALN1K;
+}
+// threads per workgroup of a table mode
+template <int TM>
+constexpr int tm_block() {
+    return TM == TM_ALN ? 512 : TM == TM_ALN1K ? 1024 : 256;
+}
+
+// Aligned LDS layout (TM_ALN*): lane j of a group owns the STRIDE slots
+// [j*STRIDE, (j+1)*STRIDE) of a row, slot s < Q holding model position k = j*Q + 1 + s
+// (the rest is padding, never read).  A 128-bit LDS read serves 16 lanes per cycle
+// from 64 four-byte banks; those 16 lanes belong to one group (G >= 16), so they read
+// the same residue row, and with STRIDE/4 odd their 16-byte blocks start on 16
+// distinct multiples of four banks: each read is conflict-free whatever the residues
+// of the other groups.  Rows are WID = G*STRIDE floats apart (a multiple of four).
+template <int Q, int G>
+struct GeoA {
+    static_assert(Q % 4 == 0 && G >= 16, "aligned lane blocks need Q % 4 == 0 and G >= 16");
+    static constexpr int STRIDE = (Q % 8 == 0) ? Q + 4 : Q + 8;
+    static constexpr int WID = G * STRIDE;
+};
+
+// LDS row width of a shape in a table mode
+template <int Q, int G, int TM>
+constexpr int tm_wid() {
+    if constexpr (tm_aligned<TM>())
+        return GeoA<Q, G>::WID;
+    else
+        return Geo<Q, G>::WID;
+}
 
 // Padded global emission rows: position k of residue row x sits at x * ew + k + 3, so a
 // lane's block of Q positions (Q a multiple of 4) starts 16-byte aligned; positions 0
@@ -310,6 +347,9 @@ struct EmRows {
         if constexpr (TM == TM_GEM) {
             base = em + j * Q + 1 + GEM_SHIFT;
             width = ew;
+        } else if constexpr (tm_aligned<TM>()) {
+            base = em + j * GeoA<Q, G>::STRIDE;
+            width = GeoA<Q, G>::WID;
         } else {
             base = em + j * Geo<Q, G>::STRIDE + 1;
             width = Geo<Q, G>::WID;
@@ -318,6 +358,8 @@ struct EmRows {
     __device__ __forceinline__ const float* row(int x) const {
         if constexpr (TM == TM_GEM)
             return static_cast<const float*>(__builtin_assume_aligned(base + x * width, 16));
+        else if constexpr (tm_aligned<TM>())
+            return static_cast<const float*>(__builtin_assume_aligned(base + x * GeoA<Q, G>::WID, 16));
         else
             return base + x * Geo<Q, G>::WID;
     }
@@ -783,10 +825,12 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
 }
 
 template <int Q, int G, int STAGE, bool SPEC, int TM>
-__global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out, SxSpecialsOut SP) {
-    constexpr int WID = Geo<Q, G>::WID;
+__global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out,
+                                                          SxSpecialsOut SP) {
+    static_assert(STAGE == ST_MSV || !tm_aligned<TM>(), "aligned lane blocks are an MSV layout");
+    constexpr int WID = tm_wid<Q, G, TM>();
     constexpr int NTAB_E = TM == TM_GEM ? 0 : SX_NCODE * WID;  // emission rows staged in LDS
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];
     __shared__ long long s_tile;
     const int gid = (int)(threadIdx.x / G);
     const int jl = (int)(threadIdx.x & (G - 1));
@@ -808,6 +852,15 @@ __global__ void __launch_bounds__(256) k_tiles(SxProfiles P, SxTargets T, SxWork
             const int w = m + 1;
             if constexpr (TM == TM_GEM) {
                 // emission rows stay in global memory (padded copy)
+            } else if constexpr (tm_aligned<TM>()) {
+                const float* src = P.msc + P.off[ti.prof];
+                constexpr int S = GeoA<Q, G>::STRIDE;
+                for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
+                    const int x = idx / WID, pos = idx - x * WID;
+                    const int j = pos / S, s = pos - j * S;
+                    const int k = j * Q + 1 + s;
+                    lds[idx] = s < Q && k < w ? src[x * w + k] : SX_NEG_INF;
+                }
             } else if (STAGE == ST_MSV || STAGE == ST_VIT) {
                 const float* src = P.msc + P.off[ti.prof];
                 for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
@@ -894,11 +947,12 @@ struct SxKernelEntry {
     int spec;
     int hoist;
     int wid;
+    int block;
     const void* fn;
 };
 
 #define SXK_E(ST, Q, G, SP, H) \
-    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::Geo<Q, G>::WID, \
+    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::tm_wid<Q, G, (int)(H)>(), sxk::tm_block<(int)(H)>(), \
      reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, (int)(H)>)}
 
 #define SXK_ALLG(ST, Q, SP, H)                                                                          \
@@ -920,6 +974,8 @@ struct SxKernelEntry {
     SXK_E(ST, 24, 64, true, 2), SXK_E(ST, 32, 64, true, 2), SXK_E(ST, 40, 64, true, 2),                   \
         SXK_E(ST, 48, 64, true, 2), SXK_E(ST, 64, 64, true, 2)
 
+#define SXK_ALN(G, Q) SXK_E(0, Q, G, false, sxk::TM_ALN), SXK_E(0, Q, G, false, sxk::TM_ALN1K)
+
 static const SxKernelEntry k_table[] = {
     // MSV: one f32 row per lane
     SXK_ALLG(0, 4, false, false), SXK_ALLG(0, 8, false, false), SXK_ALLG(0, 16, false, false),
@@ -939,6 +995,9 @@ static const SxKernelEntry k_table[] = {
     SXK_E(0, 12, 16, false, false), SXK_E(0, 20, 16, false, false), SXK_E(0, 24, 16, false, false),
     SXK_E(0, 40, 16, false, false), SXK_E(0, 48, 16, false, false),
     SXK_E(0, 24, 32, false, false), SXK_E(0, 40, 32, false, false), SXK_E(0, 48, 32, false, false),
+    // aligned lane blocks, 512- and 1024-thread workgroups
+    SXK_ALN(16, 12), SXK_ALN(16, 16), SXK_ALN(16, 20), SXK_ALN(16, 24), SXK_ALN(16, 32), SXK_ALN(16, 40),
+    SXK_ALN(16, 48), SXK_E(0, 64, 16, false, sxk::TM_ALN), SXK_ALN(32, 16), SXK_ALN(32, 24), SXK_ALN(32, 32),
     // Viterbi, Forward, Backward: three rows per lane
     SXK_DP3(1), SXK_DP3(2), SXK_DP3(3),
     // special-row writers for the host-side domain definition

@@ -220,8 +220,8 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     for (size_t i = 0; i < P; i++) {
         const int m = c.profs[S.perm[i]].m;
         const Shape sv = c.shape_for(1, SX_FORM_BATCHED, m), sf = c.shape_for(2, SX_FORM_BATCHED, m);
-        ngv[i] = 256 / sv.g;
-        ngf[i] = 256 / sf.g;
+        ngv[i] = c.kernel(1, sv.q, sv.g, false, sv.hoist).block / sv.g;
+        ngf[i] = c.kernel(2, sf.q, sf.g, false, sf.hoist).block / sf.g;
         auto add = [&](std::vector<CascadeState::Run>& runs, Shape s) {
             if (!runs.empty() && runs.back().s.q == s.q && runs.back().s.g == s.g && runs.back().s.hoist == s.hoist)
                 runs.back().b = (int64_t)i + 1;
@@ -323,7 +323,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         SxWork w{};
         w.profs = gr.plist;
         w.tids = c.d_sorted;
-        w.chunk = 256 / gr.s.g;
+        w.chunk = k.block / gr.s.g;
         w.ntids = (int32_t)T;
         w.nchunks = (int32_t)((T + w.chunk - 1) / w.chunk);
         w.ntiles = (int64_t)w.nchunks * gr.n;

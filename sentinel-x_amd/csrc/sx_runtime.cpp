@@ -21,7 +21,7 @@
 #include "sx_search.h"
 
 struct SxKernelEntry {
-    int stage, q, g, spec, hoist, wid;
+    int stage, q, g, spec, hoist, wid, block;
     const void* fn;
 };
 extern "C" const SxKernelEntry* sx_kernel_table(int* n);
@@ -35,7 +35,6 @@ namespace sx {
             throw SxError(SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
-static const int BLOCK = 256;
 static const size_t LDS_MAX = 160 * 1024;
 
 template <class T>
@@ -95,6 +94,7 @@ Ctx::Ctx(int dev) : device(dev) {
         ki.spec = tab[i].spec;
         ki.hoist = tab[i].hoist;
         ki.wid = tab[i].wid;
+        ki.block = tab[i].block;
         ki.fn = tab[i].fn;
         // every (stage, Q, G, special rows, table mode) names exactly one kernel
         if (!kernels.emplace(key(ki.stage, ki.q, ki.g, ki.spec, ki.hoist), ki).second)
@@ -128,7 +128,10 @@ const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec, int hoist) con
     if (it == kernels.end())
         throw SxError(SX_ERR_STATE, "no kernel for stage " + std::to_string(stage) + " Q=" + std::to_string(q) +
                                         " G=" + std::to_string(g) +
-                                        (hoist == 2 ? " (global emissions)" : hoist ? " (registers)" : " (LDS)"));
+                                        (hoist == 2   ? " (global emissions)"
+                                         : hoist >= 3 ? " (aligned lane blocks)"
+                                         : hoist      ? " (registers)"
+                                                      : " (LDS)"));
     return it->second;
 }
 
@@ -270,6 +273,8 @@ SxTargets Ctx::dtarg() const {
 // smallest capacity Q*G >= M among shapes whose LDS tables fit, preferring (at equal
 // capacity) register-resident transitions and then fewer lanes per pair.  Viterbi,
 // Forward and Backward avoid Q > 16 per lane, whose three register rows cut occupancy.
+// MSV models longer than 192 take 16-lane groups over aligned lane blocks (512-thread
+// workgroups), which measured fastest from M = 256 to 1024.
 Shape Ctx::gem_shape(int m) {
     for (int q : {24, 32, 40, 48, 64})
         if (q * 64 >= m) return {q, 64, 2};
@@ -283,11 +288,16 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
         for (int q : {4, 8, 16})
             if (q * 64 >= m) return {q, 64, stage != 0 && q <= 8 ? 1 : 0};
     }
+    if (stage == 0 && m > 192) {
+        // MSV past the one-lane shapes: 16-lane groups over aligned lane blocks
+        for (int q : {16, 20, 24, 32, 40, 48, 64})
+            if (16 * q >= m && kernels.count(key(0, q, 16, 0, 3))) return {q, 16, 3};
+    }
     Shape best{0, 0, 0};
     long best_cap = 1L << 40;
     for (const auto& kv : kernels) {
         const KernelInfo& k = kv.second;
-        if (k.stage != stage || k.spec || k.hoist == 2) continue;
+        if (k.stage != stage || k.spec || k.hoist >= 2) continue;
         const long cap = (long)k.q * k.g;
         if (cap < m) continue;
         if (lds_bytes(k) > LDS_MAX) continue;
@@ -337,7 +347,7 @@ Shape Ctx::shape_for_spec(int stage, int m) const {
 
 int Ctx::persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, BLOCK, lds) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, k.block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     int64_t g = (int64_t)per_cu * n_cu;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
@@ -358,7 +368,7 @@ void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecials
     SxProfiles P = dprof();
     SxTargets T = dtarg();
     void* args[] = {&P, &T, &w, &out, &sp};
-    HIPCHK(hipLaunchKernel(k.fn, dim3(grid), dim3(BLOCK), args, lds, stream));
+    HIPCHK(hipLaunchKernel(k.fn, dim3(grid), dim3(k.block), args, lds, stream));
 }
 
 // Implicit all-pairs launches for a set of profiles against a target-id list (device
@@ -367,14 +377,14 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
                        void* dout, int64_t out_stride) {
     SxSpecialsOut sp{};
     if (form == SX_FORM_PLAIN) {
-        // one launch per profile; each workgroup takes 256/G pairs in input order
+        // one launch per profile; each workgroup takes block/G pairs in input order
         for (int32_t p : plist) {
             const Shape s = shape_for(stage, form, profs[p].m);
             const KernelInfo& k = kernel(stage, s.q, s.g, false, s.hoist);
             SxWork w{};
             w.profs = d_pidx + p;
             w.tids = dtids;
-            w.chunk = BLOCK / s.g;
+            w.chunk = k.block / s.g;
             w.ntids = ntids;
             w.nchunks = (ntids + w.chunk - 1) / w.chunk;
             w.ntiles = w.nchunks;
@@ -411,7 +421,7 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         SxWork w{};
         w.profs = (const int32_t*)pb.p + goff[gi++];
         w.tids = dtids;
-        w.chunk = BLOCK / g;
+        w.chunk = k.block / g;
         w.ntids = ntids;
         w.nchunks = (ntids + w.chunk - 1) / w.chunk;
         w.ntiles = (int64_t)w.nchunks * (int64_t)kv.second.size();
@@ -457,7 +467,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         uint64_t e = i;
         while (e < n && prof[order[e]] == p) e++;
         const Shape s = spec ? shape_for_spec(stage, profs[p].m) : shape_for(stage, form, profs[p].m);
-        const int ng = BLOCK / s.g;
+        const int ng = kernel(stage, s.q, s.g, spec, s.hoist).block / s.g;
         for (uint64_t a = i; a < e; a += ng) {
             SxTile t;
             t.prof = (int32_t)p;
@@ -940,7 +950,8 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
         std::vector<std::pair<int, sx::Shape>> tab;
         for (uint32_t i = 0; i < n; i++) {
             const int h = hoist ? hoist[i] : 0;
-            if (h != 0 && h != 1) throw SxError(SX_ERR_ARG, "hoist must be 0 or 1 (long models have a fixed shape)");
+            if (h < 0 || h > 4 || h == 2)
+                throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (long models have a fixed shape)");
             if (max_m[i] > SX_GEM_MIN_M)
                 throw SxError(SX_ERR_ARG, "rows cover models up to " + std::to_string(SX_GEM_MIN_M) +
                                               "; longer models use a fixed shape");

@@ -37,13 +37,15 @@ struct DeviceBuf {
 
 struct KernelInfo {
     int stage = 0, q = 0, g = 0, spec = 0, hoist = 0, wid = 0;
+    int block = 256;  // threads per workgroup; a tile holds block / g pairs
     const void* fn = nullptr;
 };
 
 struct Shape {
     int q, g;
     // table mode: 0 = tables in LDS; 1 = transitions kept in registers across rows;
-    // 2 = long models, emission rows read from global memory (transitions in LDS)
+    // 2 = long models, emission rows read from global memory (transitions in LDS);
+    // 3, 4 = MSV with G >= 16: 16-byte-aligned lane blocks in LDS, 512 / 1024 threads
     int hoist;
 };
 

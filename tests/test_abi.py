@@ -40,7 +40,7 @@ def test_create_without_device_fails_loudly(sxa):
 
 class _KernelEntry(ctypes.Structure):
     _fields_ = [("stage", ctypes.c_int), ("q", ctypes.c_int), ("g", ctypes.c_int), ("spec", ctypes.c_int),
-                ("hoist", ctypes.c_int), ("wid", ctypes.c_int), ("fn", ctypes.c_void_p)]
+                ("hoist", ctypes.c_int), ("wid", ctypes.c_int), ("block", ctypes.c_int), ("fn", ctypes.c_void_p)]
 
 
 def test_kernel_table_names_each_shape_once(sxa):
@@ -57,7 +57,18 @@ def test_kernel_table_names_each_shape_once(sxa):
     fns = [tab[i].fn for i in range(n.value)]
     assert len(set(rows)) == len(rows)
     assert len(set(fns)) == len(fns)
-    assert all(r[4] in (0, 1, 2) and r[3] in (0, 1) for r in rows)
+    assert all(r[4] in (0, 1, 2, 3, 4) and r[3] in (0, 1) for r in rows)
+    for i in range(n.value):
+        e = tab[i]
+        assert e.block == {3: 512, 4: 1024}.get(e.hoist, 256)
+        if e.hoist in (3, 4):
+            # aligned lane blocks: MSV only, G >= 16, blocks of STRIDE floats with STRIDE/4
+            # odd, so the 16 lanes that one 128-bit LDS read serves start on 16 distinct
+            # four-bank slots of the 64 banks
+            assert e.stage == 0 and e.spec == 0 and e.g >= 16 and e.q % 4 == 0
+            stride = e.wid // e.g
+            assert e.wid % e.g == 0 and stride >= e.q and stride % 4 == 0 and (stride // 4) % 2 == 1
+            assert sorted((j * stride) % 64 for j in range(16)) == list(range(0, 64, 4))
     long_rows = {r for r in rows if r[4] == 2}
     for st in (0, 1, 2, 3):
         for q in (24, 32, 40, 48, 64):

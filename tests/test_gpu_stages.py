@@ -23,6 +23,11 @@ EDGE_LENGTHS = [1, 2, 3, 15, 16, 17, 31, 32, 33, 47, 48, 64, 65]
 LONG_M = 1024  # models longer than this use the fixed global-emission shape
 
 
+# MSV with 16-byte-aligned lane blocks: (q, g, table mode 3 = 512 / 4 = 1024 threads)
+MSV_ALN_SHAPES = [(q, 16, h) for q in (12, 16, 20, 24, 32, 40, 48) for h in (3, 4)] + [(64, 16, 3)] + \
+    [(q, 32, h) for q in (16, 24, 32) for h in (3, 4)]
+
+
 def lds_ok(stage, q, g):
     w0 = g * (q + 1) + 2
     wid = w0 if w0 % 2 else w0 + 1
@@ -169,6 +174,39 @@ def test_every_msv_shape_bit_exact(orc, gctx, q, g):
     gctx.set_shapes(0, [])
     want = orc.Batch(profs, seqs).msv(nthreads=8).reshape(len(profs), len(seqs))
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("q,g,h", MSV_ALN_SHAPES, ids=[f"Q{q}G{g}M{h}" for q, g, h in MSV_ALN_SHAPES])
+def test_aligned_msv_shapes_bit_exact(orc, gctx, q, g, h):
+    """Aligned lane blocks, 512 / 1024-thread workgroups: enough targets for several
+    tiles per profile, the last one partial."""
+    cap = min(q * g, LONG_M)
+    rng = np.random.default_rng(q * 7 + g + h)
+    ms = sorted({cap // 2 + 1, (3 * cap) // 4, cap, int(rng.integers(cap // 2 + 1, cap + 1))})
+    hmms = [random_hmm(orc, m, 3000 + q * 131 + g + i) for i, m in enumerate(ms)]
+    seqs = make_targets(orc, hmms, 150, 3000 + q + g + h, lmax=200)
+    profs = [orc.configure(x) for x in hmms]
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(0, [(cap, q, g, h)])
+    got, _ = gctx.score_all(0, 1)
+    pairs = (np.arange(40) % len(hmms), (np.arange(40) * 7) % len(seqs))
+    gp, _ = gctx.score_pairs(0, *pairs)
+    gctx.set_shapes(0, [])
+    want = orc.Batch(profs, seqs).msv(nthreads=8).reshape(len(profs), len(seqs))
+    assert np.array_equal(got, want)
+    assert np.array_equal(gp, want[pairs])
+
+
+def test_aligned_modes_are_msv_only(gctx, orc):
+    gctx.set_profiles([random_hmm(orc, 100, 5)])
+    for stage in (1, 2, 3):
+        with pytest.raises(Exception):
+            gctx.set_shapes(stage, [(256, 16, 16, 3)])
+    with pytest.raises(Exception):
+        gctx.set_shapes(0, [(256, 32, 8, 3)])  # needs G >= 16
+    with pytest.raises(Exception):
+        gctx.set_shapes(0, [(256, 16, 16, 2)])  # the long-model mode is fixed
 
 
 @pytest.mark.parametrize("stage", [1, 2, 3], ids=["viterbi", "forward", "backward"])
