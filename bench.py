@@ -36,6 +36,21 @@ MSV_INSTR_PER_CELL = 2.5
 VALU_PEAK_TOPS = 157.3 / 2
 
 
+def measured_traffic(workload, kernel, mode):
+    """HBM bytes per launch of the dominant MSV kernel, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs recorded under profiles/ (scaled by the copy-kernel calibration given
+    there); None when the record is for another workload or kernel."""
+    p = os.path.join(ROOT, "profiles", "msv_pfam_scale_traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if t.get("workload") != workload or t.get("kernel") != kernel or t.get("table_mode") != mode:
+        return None, None
+    return int(t["bytes_read"]) + int(t["bytes_written"]), "profiles/msv_pfam_scale_traffic.json"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -202,7 +217,8 @@ def main():
     ap.add_argument("--profile-scale", type=float, default=1.0)
     ap.add_argument("--target-scale", type=float, default=1.0)
     ap.add_argument("--plain-sample", type=float, default=0.02, help="fraction of profiles run in the plain form (0 = skip)")
-    ap.add_argument("--tune-ms", type=float, default=3000.0, help="autotuner budget per stage (0 = built-in)")
+    ap.add_argument("--tune-ms", type=float, default=3000.0,
+                    help="autotuner budget per stage, three times this for MSV (0 = built-in shapes)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -239,7 +255,7 @@ def main():
     if args.tune_ms > 0:
         for st in (0, 1, 2):
             t0 = time.time()
-            ctx.autotune(st, args.tune_ms)
+            ctx.autotune(st, args.tune_ms * (3 if st == 0 else 1))  # MSV times 3x larger samples
             log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
             for row in ctx.autotune_log(st):
                 log(f"[tune] stage {st} bucket<= {row[0]:5d} Q={row[1]:3d} G={row[2]:2d} reg={row[3]} "
@@ -283,9 +299,12 @@ def main():
     d = msv[int(np.argmax(msv["ms"]))]
     achieved = float(d["cells"]) * MSV_INSTR_PER_CELL / (float(d["ms"]) * 1e-3) / 1e12
     stage_achieved = e.cells_msv * MSV_INSTR_PER_CELL / (e.ms_msv * 1e-3) / 1e12
+    kname = f"k_tiles<Q={int(d['q'])},G={int(d['g'])},MSV>"
+    traffic, traffic_src = measured_traffic(f"{wl.name}: {len(ms)} profiles x {len(lens)} proteins", kname,
+                                            int(d["hoist"]))
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-            "kernel": f"k_tiles<Q={int(d['q'])},G={int(d['g'])},MSV>", "launch_ms": round(float(d["ms"]), 3),
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": kname, "launch_ms": round(float(d["ms"]), 3),
             "cells": int(d["cells"]), "instr_per_cell": MSV_INSTR_PER_CELL,
             "msv_stage": {"launches": int(len(msv)), "achieved": round(stage_achieved, 3),
                           "frac": round(stage_achieved / VALU_PEAK_TOPS, 4)}}

@@ -218,8 +218,8 @@ void sx_results_free(sx_results* r);
  * `hoist` names the table mode: 0 = profile tables in LDS; 1 = the transition scores
  * stay in registers across rows instead of being re-read from LDS; 2 = long models
  * (M > 1024, fixed shapes), emission rows read from global memory; 3 and 4 = MSV with
- * G >= 16, emission rows in LDS as 16-byte-aligned lane blocks read four positions
- * per load, 512 (3) or 1024 (4) threads per workgroup.  The batched form looks a model's shape up in a per-stage
+ * G = 8 or G >= 16, emission rows in LDS as 16-byte-aligned lane blocks read four
+ * positions per load, 512 (3) or 1024 (4) threads per workgroup.  The batched form looks a model's shape up in a per-stage
  * table of rows (max_m[i], q[i], g[i], hoist[i]) in increasing max_m; a model of length
  * M uses the first row with M <= max_m.  These calls have no counterpart in the
  * reference (a CPU loop has no launch shape).

@@ -292,7 +292,7 @@ __device__ __forceinline__ int slot_k(int pos) {
 //   TM_REG  as TM_LDS, but the lane's transitions stay in registers across rows;
 //   TM_GEM  long models: emission rows read from a padded copy in global memory
 //           (through L1/L2), transitions staged in LDS and re-read every row.
-// and two MSV-only modes for groups of G >= 16 lanes:
+// and two MSV-only modes for groups of G = 8 or G >= 16 lanes:
 //   TM_ALN  emission rows in LDS as 16-byte-aligned lane blocks (GeoA), read four
 //           positions per load; 512-thread workgroups share one staged table;
 //   TM_ALN1K  as TM_ALN with 1024-thread workgroups.
@@ -311,15 +311,24 @@ constexpr int tm_block() {
 // Aligned LDS layout (TM_ALN*): lane j of a group owns the STRIDE slots
 // [j*STRIDE, (j+1)*STRIDE) of a row, slot s < Q holding model position k = j*Q + 1 + s
 // (the rest is padding, never read).  A 128-bit LDS read serves 16 lanes per cycle
-// from 64 four-byte banks; those 16 lanes belong to one group (G >= 16), so they read
-// the same residue row, and with STRIDE/4 odd their 16-byte blocks start on 16
-// distinct multiples of four banks: each read is conflict-free whatever the residues
-// of the other groups.  Rows are WID = G*STRIDE floats apart (a multiple of four).
+// from 64 four-byte banks, so its 16 lanes must start on 16 distinct multiples of
+// four banks.  With STRIDE/4 odd, the lanes j..j+15 of one group do (j*STRIDE mod 64
+// runs over all 16 slots); a group of G >= 16 lanes fills the 16 lanes with one residue
+// row, so every read is conflict-free whatever the residues of the other groups.
+// Groups of 8 lanes share a quarter-wave in pairs that read different rows: the table
+// is then kept twice, the second copy 32 banks (SKEW words) further round, rows are
+// padded to a multiple of 64 floats so the row index moves no bank, and the odd group
+// of each pair reads the second copy.  The 8 slots {j*STRIDE mod 64 : j < 8} and the
+// same set turned 32 banks are disjoint (j - j' would have to be 8 mod 16).
 template <int Q, int G>
 struct GeoA {
-    static_assert(Q % 4 == 0 && G >= 16, "aligned lane blocks need Q % 4 == 0 and G >= 16");
+    static_assert(Q % 4 == 0 && (G == 8 || G >= 16), "aligned lane blocks need Q % 4 == 0 and G = 8 or G >= 16");
     static constexpr int STRIDE = (Q % 8 == 0) ? Q + 4 : Q + 8;
-    static constexpr int WID = G * STRIDE;
+    static constexpr int COPIES = G >= 16 ? 1 : 2;
+    static constexpr int WID = COPIES == 1 ? G * STRIDE : (G * STRIDE + 63) / 64 * 64;
+    static constexpr int SKEW = COPIES == 1 ? 0 : 32;
+    static constexpr int COPY = SX_NCODE * WID + SKEW;        // floats from one copy to the next
+    static constexpr int TAB = COPIES * SX_NCODE * WID + (COPIES - 1) * SKEW;  // floats staged
 };
 
 // LDS row width of a shape in a table mode
@@ -348,7 +357,9 @@ struct EmRows {
             base = em + j * Q + 1 + GEM_SHIFT;
             width = ew;
         } else if constexpr (tm_aligned<TM>()) {
-            base = em + j * GeoA<Q, G>::STRIDE;
+            // the odd group of each pair of 8-lane groups reads the second copy
+            const int c = GeoA<Q, G>::COPIES == 1 ? 0 : (int)((threadIdx.x / G) & 1);
+            base = em + c * GeoA<Q, G>::COPY + j * GeoA<Q, G>::STRIDE;
             width = GeoA<Q, G>::WID;
         } else {
             base = em + j * Geo<Q, G>::STRIDE + 1;
@@ -796,6 +807,22 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, int ew
 // ---------------------------------------------------------------------------
 enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3 };
 
+// Floats of a kernel's dynamic LDS: the staged emission rows (none for TM_GEM) and,
+// past MSV, the eight transition rows after them.
+template <int Q, int G, int TM>
+constexpr int lds_emission_floats() {
+    if constexpr (TM == TM_GEM)
+        return 0;
+    else if constexpr (tm_aligned<TM>())
+        return GeoA<Q, G>::TAB;
+    else
+        return SX_NCODE * Geo<Q, G>::WID;
+}
+template <int Q, int G, int ST, int TM>
+constexpr int lds_floats() {
+    return lds_emission_floats<Q, G, TM>() + (ST == ST_MSV ? 0 : SX_NT * tm_wid<Q, G, TM>());
+}
+
 struct TileInfo {
     int32_t prof, count;
     int64_t first, out;
@@ -829,7 +856,7 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                                                           SxSpecialsOut SP) {
     static_assert(STAGE == ST_MSV || !tm_aligned<TM>(), "aligned lane blocks are an MSV layout");
     constexpr int WID = tm_wid<Q, G, TM>();
-    constexpr int NTAB_E = TM == TM_GEM ? 0 : SX_NCODE * WID;  // emission rows staged in LDS
+    constexpr int NTAB_E = lds_emission_floats<Q, G, TM>();  // emission rows staged in LDS
     extern __shared__ __align__(16) float lds[];
     __shared__ long long s_tile;
     const int gid = (int)(threadIdx.x / G);
@@ -854,12 +881,13 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                 // emission rows stay in global memory (padded copy)
             } else if constexpr (tm_aligned<TM>()) {
                 const float* src = P.msc + P.off[ti.prof];
-                constexpr int S = GeoA<Q, G>::STRIDE;
-                for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
-                    const int x = idx / WID, pos = idx - x * WID;
-                    const int j = pos / S, s = pos - j * S;
+                using GA = GeoA<Q, G>;
+                for (int idx = threadIdx.x; idx < GA::TAB; idx += blockDim.x) {
+                    const int r = idx - (idx / GA::COPY) * GA::COPY;  // offset inside its copy
+                    const int x = r / WID, pos = r - x * WID;
+                    const int j = pos / GA::STRIDE, s = pos - j * GA::STRIDE;
                     const int k = j * Q + 1 + s;
-                    lds[idx] = s < Q && k < w ? src[x * w + k] : SX_NEG_INF;
+                    lds[idx] = x < SX_NCODE && j < G && s < Q && k < w ? src[x * w + k] : SX_NEG_INF;
                 }
             } else if (STAGE == ST_MSV || STAGE == ST_VIT) {
                 const float* src = P.msc + P.off[ti.prof];
@@ -948,11 +976,13 @@ struct SxKernelEntry {
     int hoist;
     int wid;
     int block;
+    int lds;  // bytes of dynamic LDS per workgroup
     const void* fn;
 };
 
 #define SXK_E(ST, Q, G, SP, H) \
     {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::tm_wid<Q, G, (int)(H)>(), sxk::tm_block<(int)(H)>(), \
+     4 * sxk::lds_floats<Q, G, ST, (int)(H)>(), \
      reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, (int)(H)>)}
 
 #define SXK_ALLG(ST, Q, SP, H)                                                                          \
@@ -998,6 +1028,7 @@ static const SxKernelEntry k_table[] = {
     // aligned lane blocks, 512- and 1024-thread workgroups
     SXK_ALN(16, 12), SXK_ALN(16, 16), SXK_ALN(16, 20), SXK_ALN(16, 24), SXK_ALN(16, 32), SXK_ALN(16, 40),
     SXK_ALN(16, 48), SXK_E(0, 64, 16, false, sxk::TM_ALN), SXK_ALN(32, 16), SXK_ALN(32, 24), SXK_ALN(32, 32),
+    SXK_ALN(8, 16), SXK_ALN(8, 20), SXK_ALN(8, 24), SXK_ALN(8, 32), SXK_ALN(8, 40), SXK_ALN(8, 48),
     // Viterbi, Forward, Backward: three rows per lane
     SXK_DP3(1), SXK_DP3(2), SXK_DP3(3),
     // special-row writers for the host-side domain definition

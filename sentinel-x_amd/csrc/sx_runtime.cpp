@@ -21,7 +21,7 @@
 #include "sx_search.h"
 
 struct SxKernelEntry {
-    int stage, q, g, spec, hoist, wid, block;
+    int stage, q, g, spec, hoist, wid, block, lds;
     const void* fn;
 };
 extern "C" const SxKernelEntry* sx_kernel_table(int* n);
@@ -95,6 +95,7 @@ Ctx::Ctx(int dev) : device(dev) {
         ki.hoist = tab[i].hoist;
         ki.wid = tab[i].wid;
         ki.block = tab[i].block;
+        ki.lds = tab[i].lds;
         ki.fn = tab[i].fn;
         // every (stage, Q, G, special rows, table mode) names exactly one kernel
         if (!kernels.emplace(key(ki.stage, ki.q, ki.g, ki.spec, ki.hoist), ki).second)
@@ -133,13 +134,6 @@ const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec, int hoist) con
                                          : hoist      ? " (registers)"
                                                       : " (LDS)"));
     return it->second;
-}
-
-// LDS of one workgroup: the emission rows (unless they are read from global memory)
-// and, past MSV, the eight transition rows.
-size_t Ctx::lds_bytes(int stage, int wid, int hoist) {
-    const int rows = (hoist == 2 ? 0 : SX_NCODE) + (stage == 0 ? 0 : SX_NT);
-    return (size_t)rows * wid * sizeof(float);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,8 +267,8 @@ SxTargets Ctx::dtarg() const {
 // smallest capacity Q*G >= M among shapes whose LDS tables fit, preferring (at equal
 // capacity) register-resident transitions and then fewer lanes per pair.  Viterbi,
 // Forward and Backward avoid Q > 16 per lane, whose three register rows cut occupancy.
-// MSV models longer than 192 take 16-lane groups over aligned lane blocks (512-thread
-// workgroups), which measured fastest from M = 256 to 1024.
+// MSV models longer than 128 take 8-lane (M <= 320) or 16-lane groups over aligned lane
+// blocks with 512-thread workgroups, which measured fastest from M = 160 to 1024.
 Shape Ctx::gem_shape(int m) {
     for (int q : {24, 32, 40, 48, 64})
         if (q * 64 >= m) return {q, 64, 2};
@@ -288,9 +282,11 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
         for (int q : {4, 8, 16})
             if (q * 64 >= m) return {q, 64, stage != 0 && q <= 8 ? 1 : 0};
     }
-    if (stage == 0 && m > 192) {
-        // MSV past the one-lane shapes: 16-lane groups over aligned lane blocks
-        for (int q : {16, 20, 24, 32, 40, 48, 64})
+    if (stage == 0 && m > 128) {
+        // MSV past the one-lane shapes: 8- then 16-lane groups over aligned lane blocks
+        for (int q : {20, 24, 32, 40})
+            if (8 * q >= m && kernels.count(key(0, q, 8, 0, 3))) return {q, 8, 3};
+        for (int q : {24, 32, 40, 48, 64})
             if (16 * q >= m && kernels.count(key(0, q, 16, 0, 3))) return {q, 16, 3};
     }
     Shape best{0, 0, 0};
@@ -572,9 +568,14 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         lo = hi;
         if (inb.empty()) continue;
         if (mmax > SX_GEM_MIN_M) break;  // long models have one fixed shape
-        // up to 16 profiles, spread over the bucket, repeated if the bucket is small
+        // profiles spread over the bucket, repeated if the bucket is small: about 4e10
+        // cells (up to 16 profiles) for the later stages; three times that (up to 64) for
+        // MSV, whose one-lane shapes run a whole pair per lane, so that a tile lasts long
+        // enough for the persistent grid's drain tail to swing a short sample's time
         std::vector<int32_t> sample;
-        const int want = (int)std::clamp<int64_t>((int64_t)(4e10 / ((double)mmax * (double)sum_l)), 1, 16);
+        const double cells_want = stage == 0 ? 1.2e11 : 4e10;
+        const int want = (int)std::clamp<int64_t>((int64_t)(cells_want / ((double)mmax * (double)sum_l)), 1,
+                                                  stage == 0 ? 64 : 16);
         for (int i = 0; i < want; i++) sample.push_back(inb[(size_t)((double)i * inb.size() / want)]);
         int64_t cells = 0;
         for (int32_t p : sample) cells += (int64_t)profs[p].m * sum_l;
@@ -588,6 +589,9 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             const KernelInfo& k = kv.second;
             if (k.stage != stage || k.spec || k.hoist == 2) continue;
             if ((long)k.q * k.g < mmax) continue;
+            // an MSV shape with an aligned-block twin: the twin reads LDS without bank
+            // conflicts and is always the faster of the two
+            if (stage == 0 && k.hoist == 0 && kernels.count(key(0, k.q, k.g, 0, 3))) continue;
             if (lds_bytes(k) > LDS_MAX) continue;
             cands.push_back({k.q, k.g, k.hoist});
         }

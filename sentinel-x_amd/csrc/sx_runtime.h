@@ -38,6 +38,7 @@ struct DeviceBuf {
 struct KernelInfo {
     int stage = 0, q = 0, g = 0, spec = 0, hoist = 0, wid = 0;
     int block = 256;  // threads per workgroup; a tile holds block / g pairs
+    int lds = 0;      // bytes of dynamic LDS per workgroup (the staged profile tables)
     const void* fn = nullptr;
 };
 
@@ -45,7 +46,7 @@ struct Shape {
     int q, g;
     // table mode: 0 = tables in LDS; 1 = transitions kept in registers across rows;
     // 2 = long models, emission rows read from global memory (transitions in LDS);
-    // 3, 4 = MSV with G >= 16: 16-byte-aligned lane blocks in LDS, 512 / 1024 threads
+    // 3, 4 = MSV with G = 8 or G >= 16: 16-byte-aligned lane blocks in LDS, 512 / 1024 threads
     int hoist;
 };
 
@@ -70,8 +71,7 @@ struct Ctx {
         return ((long)st << 40) | ((long)q << 20) | ((long)g << 4) | ((long)sp << 3) | (long)h;
     }
     const KernelInfo& kernel(int stage, int q, int g, bool spec, int hoist) const;
-    static size_t lds_bytes(int stage, int wid, int hoist);
-    static size_t lds_bytes(const KernelInfo& k) { return lds_bytes(k.stage, k.wid, k.hoist); }
+    static size_t lds_bytes(const KernelInfo& k) { return (size_t)k.lds; }
 
     // the search's cascade lets profiles without STATS lines through every filter
     bool cascade_pass_uncalibrated = false;

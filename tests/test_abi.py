@@ -40,7 +40,34 @@ def test_create_without_device_fails_loudly(sxa):
 
 class _KernelEntry(ctypes.Structure):
     _fields_ = [("stage", ctypes.c_int), ("q", ctypes.c_int), ("g", ctypes.c_int), ("spec", ctypes.c_int),
-                ("hoist", ctypes.c_int), ("wid", ctypes.c_int), ("block", ctypes.c_int), ("fn", ctypes.c_void_p)]
+                ("hoist", ctypes.c_int), ("wid", ctypes.c_int), ("block", ctypes.c_int), ("lds", ctypes.c_int),
+                ("fn", ctypes.c_void_p)]
+
+
+def _check_aligned_layout(e):
+    """Aligned lane blocks (MSV only, G = 8 or G >= 16): a 128-bit LDS read serves 16 lanes
+    a cycle from 64 four-byte banks, so the 16 lanes of every quarter-wave must start on 16
+    distinct four-bank slots whatever residue row each group reads."""
+    assert e.stage == 0 and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
+    stride = e.q + 4 if e.q % 8 == 0 else e.q + 8
+    copies = 1 if e.g >= 16 else 2
+    skew = 0 if copies == 1 else 32
+    assert (stride // 4) % 2 == 1 and e.wid >= e.g * stride
+    if copies > 1:
+        assert e.wid % 64 == 0  # the residue row moves no bank
+    assert e.lds == 4 * (copies * 25 * e.wid + (copies - 1) * skew)
+    copy_stride = 25 * e.wid + skew
+    for quarter in range(4):
+        slots = set()
+        for t in range(16):
+            lane = quarter * 16 + t
+            grp, j = divmod(lane, e.g)
+            c = grp % copies
+            # any residue row x adds x * wid, a multiple of 64 when groups share a quarter
+            start = c * copy_stride + j * stride
+            assert start % 4 == 0
+            slots.add((start % 64) // 4)
+        assert len(slots) == 16, (e.q, e.g, quarter, sorted(slots))
 
 
 def test_kernel_table_names_each_shape_once(sxa):
@@ -62,13 +89,11 @@ def test_kernel_table_names_each_shape_once(sxa):
         e = tab[i]
         assert e.block == {3: 512, 4: 1024}.get(e.hoist, 256)
         if e.hoist in (3, 4):
-            # aligned lane blocks: MSV only, G >= 16, blocks of STRIDE floats with STRIDE/4
-            # odd, so the 16 lanes that one 128-bit LDS read serves start on 16 distinct
-            # four-bank slots of the 64 banks
-            assert e.stage == 0 and e.spec == 0 and e.g >= 16 and e.q % 4 == 0
-            stride = e.wid // e.g
-            assert e.wid % e.g == 0 and stride >= e.q and stride % 4 == 0 and (stride // 4) % 2 == 1
-            assert sorted((j * stride) % 64 for j in range(16)) == list(range(0, 64, 4))
+            _check_aligned_layout(e)
+        elif e.hoist == 2:
+            assert e.lds == (0 if e.stage == 0 else 4 * 8 * e.wid)
+        else:
+            assert e.lds == 4 * (25 if e.stage == 0 else 33) * e.wid
     long_rows = {r for r in rows if r[4] == 2}
     for st in (0, 1, 2, 3):
         for q in (24, 32, 40, 48, 64):

@@ -25,7 +25,7 @@ LONG_M = 1024  # models longer than this use the fixed global-emission shape
 
 # MSV with 16-byte-aligned lane blocks: (q, g, table mode 3 = 512 / 4 = 1024 threads)
 MSV_ALN_SHAPES = [(q, 16, h) for q in (12, 16, 20, 24, 32, 40, 48) for h in (3, 4)] + [(64, 16, 3)] + \
-    [(q, 32, h) for q in (16, 24, 32) for h in (3, 4)]
+    [(q, 32, h) for q in (16, 24, 32) for h in (3, 4)] + [(q, 8, h) for q in (16, 20, 24, 32, 40, 48) for h in (3, 4)]
 
 
 def lds_ok(stage, q, g):
@@ -204,7 +204,7 @@ def test_aligned_modes_are_msv_only(gctx, orc):
         with pytest.raises(Exception):
             gctx.set_shapes(stage, [(256, 16, 16, 3)])
     with pytest.raises(Exception):
-        gctx.set_shapes(0, [(256, 32, 8, 3)])  # needs G >= 16
+        gctx.set_shapes(0, [(256, 64, 4, 3)])  # needs G = 8 or G >= 16
     with pytest.raises(Exception):
         gctx.set_shapes(0, [(256, 16, 16, 2)])  # the long-model mode is fixed
 
