@@ -125,8 +125,9 @@ def plain_leg(sxa, ctx, wl, n_plain, f, graph):
 
 def kernel_mode(sxa, synth, tune_ms, reps=5):
     """Every pair through every stage (no filters) on the marker-set workload (400 profiles,
-    M 80-1500, x 4,000 proteins): batched and plain GCUPS per stage, on identical pairs.
-    Returns the workload, the per-stage figures and the batched Forward scores."""
+    M 80-1500, x 4,000 proteins): batched and plain GCUPS per stage, on identical pairs;
+    then the full search (filters on) with its phase times.  Returns the workload, the
+    per-stage figures and the batched Forward scores."""
     wl = synth.make_workload(1)
     ms = np.array([h.m for h in wl.hmms], np.int64)
     cells = int(ms.sum()) * sum(len(s) for s in wl.seqs)
@@ -147,6 +148,12 @@ def kernel_mode(sxa, synth, tune_ms, reps=5):
                          "plain_ms": round(tp, 3), "plain_gcups": round(cells / tp / 1e6, 1),
                          "ratio_batched_over_plain": round(tp / tb, 2)}
         fwd, _ = kc.score_all(2, sxa.FORM_BATCHED)
+        # the whole hmmsearch-equivalent search (filters on, E <= 10): where its time goes
+        for _ in range(2):
+            hits, doms = kc.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True)
+        st = kc.last_search_stats
+        out["search"] = {"hits": int(len(hits)), "domains": int(len(doms)), "pairs_in_tail": int(st["n_pairs"]),
+                         **{k: round(st[k], 2) for k in ("ms_filters", "ms_specials", "ms_tail", "ms_report")}}
     return wl, out, fwd
 
 

@@ -196,6 +196,17 @@ typedef struct {           /* DomainHit (search.rs:84-97) */
 
 typedef struct sx_results sx_results;
 
+/* Where an sx_search call spent its time (host wall clock, ms) and how much reached the
+ * host part: n_pairs pairs got special rows and domain definition, n_rows = their
+ * summed L+1.  No counterpart in the reference (it does not time its phases). */
+typedef struct {
+    uint64_t n_pairs, n_rows;
+    double ms_filters;   /* the device cascade (0 with filters off) */
+    double ms_specials;  /* device Forward + Backward special rows, with their copies */
+    double ms_tail;      /* the rest of score_target on the host threads */
+    double ms_report;    /* report: thresholds, E-values, domain flags */
+} sx_search_stats;
+
 /* Search the context's profiles against its targets. */
 int sx_search(sx_ctx* ctx, const sx_search_cfg* cfg, sx_results** out);
 
@@ -210,6 +221,7 @@ int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const 
                    const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results** out);
 
 int sx_results_size(const sx_results* r, uint64_t* nhits, uint64_t* ndomains);
+int sx_results_stats(const sx_results* r, sx_search_stats* st);
 int sx_results_copy(const sx_results* r, sx_hit* hits, sx_domain* domains);
 void sx_results_free(sx_results* r);
 

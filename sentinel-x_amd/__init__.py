@@ -26,7 +26,8 @@ EXPORTS = [
     "sx_create", "sx_destroy", "sx_last_error", "sx_set_targets", "sx_set_profiles", "sx_load_hmm_file",
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
     "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
-    "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_copy", "sx_results_free",
+    "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_stats", "sx_results_copy",
+    "sx_results_free",
 ]
 
 
@@ -124,6 +125,7 @@ def lib():
                                      vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
         L.sx_results_size.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.sx_results_copy.argtypes = [vp, vp, vp]
+        L.sx_results_stats.argtypes = [vp, ctypes.POINTER(SearchStats)]
         L.sx_results_free.argtypes = [vp]
         L.sx_results_free.restype = None
         L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
@@ -192,13 +194,24 @@ def search_config(cutoff="evalue", seq_e=10.0, dom_e=10.0, seq_t=0.0, dom_t=0.0,
     return c
 
 
-def _take_results(L, r):
+class SearchStats(ctypes.Structure):
+    """sx_search_stats: where a search spent its time (ms) and how much reached the host."""
+    _fields_ = [("n_pairs", ctypes.c_uint64), ("n_rows", ctypes.c_uint64), ("ms_filters", ctypes.c_double),
+                ("ms_specials", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_report", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _take_results(L, r, stats=None):
     nh, nd = ctypes.c_uint64(), ctypes.c_uint64()
     try:
         L.sx_results_size(r, ctypes.byref(nh), ctypes.byref(nd))
         hits = np.zeros(nh.value, HIT_DTYPE)
         doms = np.zeros(nd.value, DOMAIN_DTYPE)
         L.sx_results_copy(r, _ptr(hits), _ptr(doms))
+        if stats is not None:
+            L.sx_results_stats(r, ctypes.byref(stats))
     finally:
         L.sx_results_free(r)
     return hits, doms
@@ -289,11 +302,15 @@ class Context:
     def search(self, **cfg):
         """hmmsearch-equivalent search of the loaded profiles against the loaded targets
         (see sx_search).  Keywords as search_config().  Returns (hits, domains) structured
-        arrays; hit h's domains are domains[h['dom_first'] : h['dom_first'] + h['ndom']]."""
+        arrays; hit h's domains are domains[h['dom_first'] : h['dom_first'] + h['ndom']].
+        The call's phase times (sx_search_stats) are left in self.last_search_stats."""
         c = search_config(**cfg)
         r = ctypes.c_void_p()
         self._chk(self._L.sx_search(self._h, ctypes.byref(c), ctypes.byref(r)))
-        return _take_results(self._L, r)
+        st = SearchStats()
+        out = _take_results(self._L, r, st)
+        self.last_search_stats = st.as_dict()
+        return out
 
     def load_hmm_file(self, path: str) -> int:
         n = ctypes.c_uint32()

@@ -292,8 +292,8 @@ __device__ __forceinline__ int slot_k(int pos) {
 //   TM_REG  as TM_LDS, but the lane's transitions stay in registers across rows;
 //   TM_GEM  long models: emission rows read from a padded copy in global memory
 //           (through L1/L2), transitions staged in LDS and re-read every row.
-// and two MSV-only modes for groups of G = 8 or G >= 16 lanes:
-//   TM_ALN  emission rows in LDS as 16-byte-aligned lane blocks (GeoA), read four
+// and two modes of MSV and Viterbi for groups of G = 8 or G >= 16 lanes:
+//   TM_ALN  profile rows in LDS as 16-byte-aligned lane blocks (GeoA), read four
 //           positions per load; 512-thread workgroups share one staged table;
 //   TM_ALN1K  as TM_ALN with 1024-thread workgroups.
 enum { TM_LDS = 0, TM_REG = 1, TM_GEM = 2, TM_ALN = 3, TM_ALN1K = 4 };
@@ -328,7 +328,10 @@ struct GeoA {
     static constexpr int WID = COPIES == 1 ? G * STRIDE : (G * STRIDE + 63) / 64 * 64;
     static constexpr int SKEW = COPIES == 1 ? 0 : 32;
     static constexpr int COPY = SX_NCODE * WID + SKEW;        // floats from one copy to the next
-    static constexpr int TAB = COPIES * SX_NCODE * WID + (COPIES - 1) * SKEW;  // floats staged
+    static constexpr int TAB = COPIES * SX_NCODE * WID + (COPIES - 1) * SKEW;  // emission floats staged
+    // Viterbi's eight transition rows follow the emission rows, one copy (every group of
+    // a quarter-wave reads the same addresses of them: a broadcast), TWID floats apart.
+    static constexpr int TWID = G * STRIDE;
 };
 
 // LDS row width of a shape in a table mode
@@ -505,6 +508,88 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, 
                 dr[0] = fmaxf(dr[0], c);
 #pragma unroll
                 for (int q = 1; q < Q; q++) dr[q] = fmaxf(dr[q], dr[q - 1] + tdd[q - 1]);
+            }
+        }
+        float e = SX_NEG_INF;
+#pragma unroll
+        for (int q = 0; q < Q; q++) e = fmaxf(e, fmaxf(mr[q], dr[q]));
+        const float xe = grp_reduce<G>(e, OpMax());
+        xj = fmaxf(xj + lloop, xe - SX_LN2F);
+        xc = fmaxf(xc + lloop, xe - SX_LN2F);
+        xn = xn + lloop;
+        xb = fmaxf(xn + lmove, xj + lmove);
+    }
+    return xc + lmove;
+}
+
+// Viterbi over aligned lane blocks (TM_ALN*): the same recurrence as vit_pair, with
+// every profile row read four positions per LDS load.  Transition rows are staged
+// already shifted to the cell they feed (position k-1 for MM IM DM MD DD, k for MI II
+// BM), so the lane's operands for cell q all sit at slot q of its block.  Nothing is
+// kept in registers across rows but the three DP rows.
+template <int Q, int G, int TM>
+__device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, const float* __restrict__ tr,
+                                              const uint8_t* seq, int L, float lloop, float lmove) {
+    using GA = GeoA<Q, G>;
+    const int j = (int)(threadIdx.x & (G - 1));
+    const EmRows<Q, G, TM> er(em, 0, j);
+    const int tj = j * GA::STRIDE;
+    float mr[Q], ir[Q], dr[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        mr[q] = SX_NEG_INF;
+        ir[q] = SX_NEG_INF;
+        dr[q] = SX_NEG_INF;
+    }
+    float xn = 0.0f, xb = lmove, xj = SX_NEG_INF, xc = SX_NEG_INF;
+    FwdStream rs;
+    rs.init(seq);
+    for (int i = 0; i < L; i++) {
+        const int x = rs.next();
+        const bool ins_ok = x != SX_CODE_STAR;
+        // re-read every row (an opaque offset keeps the compiler from hoisting the rows
+        // into 8*Q registers, which is what costs TM_REG its occupancy)
+        const float* tb = tr + opaque(tj >> 2) * 4;  // a multiple of four floats: stays aligned
+        auto trow = [&](int r) {
+            return static_cast<const float*>(__builtin_assume_aligned(tb + r * GA::TWID, 16));
+        };
+        const float* __restrict__ tmm = trow(SX_TMM);
+        const float* __restrict__ tmi = trow(SX_TMI);
+        const float* __restrict__ tmd = trow(SX_TMD);
+        const float* __restrict__ tim = trow(SX_TIM);
+        const float* __restrict__ tii = trow(SX_TII);
+        const float* __restrict__ tdm = trow(SX_TDM);
+        const float* __restrict__ tdd = trow(SX_TDD);
+        const float* __restrict__ tbm = trow(SX_TBM);
+        const float* __restrict__ sc = er.row(x);
+        const float mL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
+        const float iL = grp_from_prev<G>(ir[Q - 1], SX_NEG_INF);
+        const float dL = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
+        // M and I, descending so that position q-1 still holds the previous row
+#pragma unroll
+        for (int q = Q - 1; q >= 0; q--) {
+            const float mp = q ? mr[q - 1] : mL;
+            const float ip = q ? ir[q - 1] : iL;
+            const float dp = q ? dr[q - 1] : dL;
+            const float s = fmaxf(fmaxf(mp + tmm[q], ip + tim[q]), fmaxf(xb + tbm[q], dp + tdm[q]));
+            const float iv = fmaxf(mr[q] + tmi[q], ir[q] + tii[q]);
+            mr[q] = s + sc[q];
+            ir[q] = ins_ok ? iv : SX_NEG_INF;
+        }
+        // D chain, ascending; the value entering from the left lane is resolved below
+        const float mcL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
+        dr[0] = mcL + tmd[0];
+#pragma unroll
+        for (int q = 1; q < Q; q++) dr[q] = fmaxf(mr[q - 1] + tmd[q], dr[q - 1] + tdd[q]);
+        if constexpr (G > 1) {
+            // lazy-F: re-run the chain from each lane's true incoming D until nothing changes
+            for (int it = 1; it < G; it++) {
+                const float din = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
+                const float c = din + tdd[0];
+                if (!__any(c > dr[0])) break;
+                dr[0] = fmaxf(dr[0], c);
+#pragma unroll
+                for (int q = 1; q < Q; q++) dr[q] = fmaxf(dr[q], dr[q - 1] + tdd[q]);
             }
         }
         float e = SX_NEG_INF;
@@ -820,7 +905,10 @@ constexpr int lds_emission_floats() {
 }
 template <int Q, int G, int ST, int TM>
 constexpr int lds_floats() {
-    return lds_emission_floats<Q, G, TM>() + (ST == ST_MSV ? 0 : SX_NT * tm_wid<Q, G, TM>());
+    if constexpr (tm_aligned<TM>())
+        return GeoA<Q, G>::TAB + (ST == ST_MSV ? 0 : SX_NT * GeoA<Q, G>::TWID);
+    else
+        return lds_emission_floats<Q, G, TM>() + (ST == ST_MSV ? 0 : SX_NT * tm_wid<Q, G, TM>());
 }
 
 struct TileInfo {
@@ -854,7 +942,8 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
 template <int Q, int G, int STAGE, bool SPEC, int TM>
 __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out,
                                                           SxSpecialsOut SP) {
-    static_assert(STAGE == ST_MSV || !tm_aligned<TM>(), "aligned lane blocks are an MSV layout");
+    static_assert(STAGE == ST_MSV || STAGE == ST_VIT || !tm_aligned<TM>(),
+                  "aligned lane blocks are an MSV and Viterbi layout");
     constexpr int WID = tm_wid<Q, G, TM>();
     constexpr int NTAB_E = lds_emission_floats<Q, G, TM>();  // emission rows staged in LDS
     extern __shared__ __align__(16) float lds[];
@@ -889,6 +978,18 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                     const int k = j * Q + 1 + s;
                     lds[idx] = x < SX_NCODE && j < G && s < Q && k < w ? src[x * w + k] : SX_NEG_INF;
                 }
+                if constexpr (STAGE == ST_VIT) {
+                    // slot s of lane j's block holds position j*Q + s (the k-1 operand of
+                    // the lane's cell s) in the rows read at k-1 (MM IM DM MD DD) and
+                    // position j*Q + 1 + s in those read at k (MI II BM)
+                    const float* tsrc = P.tsc + P.toff[ti.prof];
+                    for (int idx = threadIdx.x; idx < SX_NT * GA::TWID; idx += blockDim.x) {
+                        const int r = idx / GA::TWID, pos = idx - r * GA::TWID;
+                        const int j = pos / GA::STRIDE, s = pos - j * GA::STRIDE;
+                        const int k = j * Q + s + (r == SX_TMI || r == SX_TII || r == SX_TBM ? 1 : 0);
+                        lds[GA::TAB + idx] = s < Q && k < w ? tsrc[r * w + k] : SX_NEG_INF;
+                    }
+                }
             } else if (STAGE == ST_MSV || STAGE == ST_VIT) {
                 const float* src = P.msc + P.off[ti.prof];
                 for (int idx = threadIdx.x; idx < NTAB_E; idx += blockDim.x) {
@@ -902,7 +1003,7 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                     lds[idx] = k < w ? src[x * w + k] : 0.0f;
                 }
             }
-            if (STAGE != ST_MSV) {
+            if (STAGE != ST_MSV && !tm_aligned<TM>()) {
                 const float* src = (STAGE == ST_VIT ? P.tsc : P.tp) + P.toff[ti.prof];
                 const float pad = STAGE == ST_VIT ? SX_NEG_INF : 0.0f;
                 for (int idx = threadIdx.x; idx < SX_NT * WID; idx += blockDim.x) {
@@ -931,7 +1032,11 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_VIT) {
                 float r = SX_NEG_INF;
-                if (L > 0) r = vit_pair<Q, G, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
+                if constexpr (tm_aligned<TM>()) {
+                    if (L > 0) r = vit_pair_aln<Q, G, TM>(em, lds + GeoA<Q, G>::TAB, seq, L, lm.lloop, lm.lmove);
+                } else {
+                    if (L > 0) r = vit_pair<Q, G, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
+                }
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_FWD) {
                 double r = -(double)__builtin_inf();
@@ -1031,6 +1136,11 @@ static const SxKernelEntry k_table[] = {
     SXK_ALN(8, 16), SXK_ALN(8, 20), SXK_ALN(8, 24), SXK_ALN(8, 32), SXK_ALN(8, 40), SXK_ALN(8, 48),
     // Viterbi, Forward, Backward: three rows per lane
     SXK_DP3(1), SXK_DP3(2), SXK_DP3(3),
+    // Viterbi over aligned lane blocks, 512-thread workgroups
+    SXK_E(1, 8, 8, false, sxk::TM_ALN), SXK_E(1, 12, 8, false, sxk::TM_ALN), SXK_E(1, 16, 8, false, sxk::TM_ALN),
+    SXK_E(1, 8, 16, false, sxk::TM_ALN), SXK_E(1, 12, 16, false, sxk::TM_ALN), SXK_E(1, 16, 16, false, sxk::TM_ALN),
+    SXK_E(1, 24, 16, false, sxk::TM_ALN), SXK_E(1, 12, 32, false, sxk::TM_ALN), SXK_E(1, 16, 32, false, sxk::TM_ALN),
+    SXK_E(1, 24, 32, false, sxk::TM_ALN), SXK_E(1, 32, 32, false, sxk::TM_ALN),
     // special-row writers for the host-side domain definition
     SXK_ALLG(2, 4, true, true), SXK_ALLG(2, 8, true, true), SXK_ALLG(2, 16, true, false),
     SXK_ALLG(2, 32, true, false), SXK_ALLG(2, 48, true, false),

@@ -1081,6 +1081,12 @@ extern "C" int sx_results_size(const sx_results* r, uint64_t* nhits, uint64_t* n
     return SX_OK;
 }
 
+extern "C" int sx_results_stats(const sx_results* r, sx_search_stats* st) {
+    if (!r || !st) return SX_ERR_ARG;
+    *st = r->stats;
+    return SX_OK;
+}
+
 extern "C" int sx_results_copy(const sx_results* r, sx_hit* hits, sx_domain* domains) {
     if (!r) return SX_ERR_ARG;
     if (hits && !r->hits.empty()) std::memcpy(hits, r->hits.data(), r->hits.size() * sizeof(sx_hit));

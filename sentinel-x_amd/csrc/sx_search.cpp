@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <limits>
 #include <string>
@@ -171,13 +172,19 @@ void check_search_cfg(const sx_search_cfg* cfg) {
 
 // The device search: cascade, special rows in chunks, host tails, report.
 void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t0) {
+        return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    };
     check_search_cfg(&cfg);
     const std::vector<Threshold> ths = resolve_thresholds(c.profs, cfg);
     const double z = std::max(cfg.z > 0.0 ? cfg.z : (double)c.nt, 1.0);
     if (c.profs.empty() || c.nt == 0) return;
     std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    auto t0 = clk::now();
     if (cfg.filters) {
         pairs = cascade_pass_pairs(c, cfg.f1, cfg.f2, cfg.f3);
+        res.stats.ms_filters = ms_since(t0);
     } else {
         pairs.reserve((size_t)c.profs.size() * c.nt);
         for (uint32_t p = 0; p < c.profs.size(); p++)
@@ -217,17 +224,25 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
         bwd.assign(n, 0.0);
         fsp.assign((size_t)rows * 5, 0.0f);
         bsp.assign((size_t)rows * 5, 0.0f);
+        t0 = clk::now();
         c.special_rows(SX_STAGE_FORWARD, cp.data(), ct.data(), n, fsp.data(), fwd.data());
         c.special_rows(SX_STAGE_BACKWARD, cp.data(), ct.data(), n, bsp.data(), bwd.data());
+        res.stats.ms_specials += ms_since(t0);
+        t0 = clk::now();
         run_tails(c.profs, seq, len, cp.data(), ct.data(), n, fwd.data(), bwd.data(), fsp.data(), bsp.data(), roff,
                   cfg.null2 != 0, cfg.threads, hit, ok);
+        res.stats.ms_tail += ms_since(t0);
+        res.stats.n_pairs += n;
+        res.stats.n_rows += (uint64_t)rows;
         all_p.insert(all_p.end(), cp.begin(), cp.end());
         all_t.insert(all_t.end(), ct.begin(), ct.end());
         all_hit.insert(all_hit.end(), std::make_move_iterator(hit.begin()), std::make_move_iterator(hit.end()));
         all_ok.insert(all_ok.end(), ok.begin(), ok.end());
         a = b;
     }
+    t0 = clk::now();
     tails_and_report(c.profs, len, ths, z, all_p, all_t, all_hit, all_ok, res);
+    res.stats.ms_report = ms_since(t0);
 }
 
 void search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,

@@ -9,6 +9,7 @@
 struct sx_results {
     std::vector<sx_hit> hits;
     std::vector<sx_domain> doms;
+    sx_search_stats stats{};
 };
 
 namespace sx {

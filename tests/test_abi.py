@@ -45,17 +45,20 @@ class _KernelEntry(ctypes.Structure):
 
 
 def _check_aligned_layout(e):
-    """Aligned lane blocks (MSV only, G = 8 or G >= 16): a 128-bit LDS read serves 16 lanes
-    a cycle from 64 four-byte banks, so the 16 lanes of every quarter-wave must start on 16
-    distinct four-bank slots whatever residue row each group reads."""
-    assert e.stage == 0 and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
+    """Aligned lane blocks (MSV and Viterbi, G = 8 or G >= 16): a 128-bit LDS read serves 16
+    lanes a cycle from 64 four-byte banks, so the 16 lanes of every quarter-wave must start
+    on 16 distinct four-bank slots whatever residue row each group reads.  Viterbi's
+    transition rows (one copy, the same addresses for every group) follow."""
+    assert e.stage in (0, 1) and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
     stride = e.q + 4 if e.q % 8 == 0 else e.q + 8
     copies = 1 if e.g >= 16 else 2
     skew = 0 if copies == 1 else 32
     assert (stride // 4) % 2 == 1 and e.wid >= e.g * stride
     if copies > 1:
         assert e.wid % 64 == 0  # the residue row moves no bank
-    assert e.lds == 4 * (copies * 25 * e.wid + (copies - 1) * skew)
+    tab = copies * 25 * e.wid + (copies - 1) * skew
+    assert e.lds == 4 * (tab + (8 * e.g * stride if e.stage == 1 else 0))
+    assert tab % 4 == 0 and (e.g * stride) % 4 == 0  # transition rows stay 16-byte aligned
     copy_stride = 25 * e.wid + skew
     for quarter in range(4):
         slots = set()

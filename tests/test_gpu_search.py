@@ -44,8 +44,15 @@ def test_filtered_search_keeps_scores_of_the_hits_it_reports(gctx, fixture34):
     gctx.set_profiles(hmms)
     gctx.set_targets(seqs)
     h0, d0 = gctx.search(filters=False)
+    st0 = dict(gctx.last_search_stats)
     h1, d1 = gctx.search(filters=True)
+    st1 = dict(gctx.last_search_stats)
     assert 0 < len(h1) <= len(h0)
+    # phase accounting: every pair reaches the host part without filters, fewer with them
+    assert st0["n_pairs"] == len(hmms) * len(seqs) and st0["ms_filters"] == 0.0
+    assert len(h1) <= st1["n_pairs"] < st0["n_pairs"] and st1["ms_filters"] > 0.0
+    assert st0["n_rows"] == len(hmms) * sum(len(x) + 1 for x in seqs)
+    assert all(st1[k] >= 0.0 for k in ("ms_specials", "ms_tail", "ms_report"))
     full = {(int(h["profile"]), int(h["target"])): h for h in h0}
     for h in h1:
         g = full[(int(h["profile"]), int(h["target"]))]

@@ -198,15 +198,38 @@ def test_aligned_msv_shapes_bit_exact(orc, gctx, q, g, h):
     assert np.array_equal(gp, want[pairs])
 
 
-def test_aligned_modes_are_msv_only(gctx, orc):
+def test_aligned_modes_are_msv_and_viterbi_only(gctx, orc):
     gctx.set_profiles([random_hmm(orc, 100, 5)])
-    for stage in (1, 2, 3):
+    for stage in (2, 3):
         with pytest.raises(Exception):
             gctx.set_shapes(stage, [(256, 16, 16, 3)])
     with pytest.raises(Exception):
         gctx.set_shapes(0, [(256, 64, 4, 3)])  # needs G = 8 or G >= 16
     with pytest.raises(Exception):
         gctx.set_shapes(0, [(256, 16, 16, 2)])  # the long-model mode is fixed
+
+
+VIT_ALN_SHAPES = [(8, 8), (12, 8), (16, 8), (8, 16), (12, 16), (16, 16), (24, 16), (12, 32), (16, 32), (24, 32),
+                  (32, 32)]
+
+
+@pytest.mark.parametrize("q,g", VIT_ALN_SHAPES, ids=[f"Q{q}G{g}" for q, g in VIT_ALN_SHAPES])
+def test_aligned_viterbi_shapes_bit_exact(orc, gctx, q, g):
+    """Viterbi over aligned lane blocks (transition rows staged shifted to the cell they
+    feed), 512-thread workgroups: several tiles per profile, the last one partial; '*'
+    residues (no insert emission) and degenerate codes among the targets."""
+    cap = min(q * g, LONG_M)
+    ms = sorted({cap // 2 + 1, (3 * cap) // 4, cap})
+    hmms = [random_hmm(orc, m, 4000 + q * 131 + g + i) for i, m in enumerate(ms)]
+    seqs = make_targets(orc, hmms, 150, 4000 + q + g, lmax=200)
+    profs = [orc.configure(x) for x in hmms]
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(1, [(cap, q, g, 3)])
+    got, _ = gctx.score_all(1, 1)
+    gctx.set_shapes(1, [])
+    want = orc.Batch(profs, seqs).viterbi(nthreads=8).reshape(len(profs), len(seqs))
+    assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize("stage", [1, 2, 3], ids=["viterbi", "forward", "backward"])
