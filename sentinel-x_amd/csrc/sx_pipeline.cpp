@@ -162,14 +162,17 @@ struct CascadeState {
     DeviceBuf cnt3, off3, list3, per3, toff3, eoff3, tiles3, tids3, fsc, recs;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    // one event after every kernel launch of the three DP stages (eager runs only)
-    std::vector<hipEvent_t> lev;
+    // an event before (lev0) and after (lev) every kernel launch of the three DP stages,
+    // on the launch's own stream (eager runs only)
+    std::vector<hipEvent_t> lev0, lev;
     std::vector<sx_launch_stat> launches;  // per-launch timing of the last eager run
     ~CascadeState() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : lev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : lev0)
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -272,6 +275,8 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     while (S.lev.size() < nl) {
         S.lev.push_back(nullptr);
         HIPCHK(hipEventCreate(&S.lev.back()));
+        S.lev0.push_back(nullptr);
+        HIPCHK(hipEventCreate(&S.lev0.back()));
     }
     S.launches.clear();
 }
@@ -330,6 +335,8 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         w.out_stride = T;
         w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
         const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), w.ntiles);
+        // MSV launches run one after the other: each fills the device on its own
+        if (with_events) HIPCHK(hipEventRecord(S.lev0[slot], st));
         c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot % 32);
         if (with_events) HIPCHK(hipEventRecord(S.lev[slot], st));
         slot++;
@@ -382,8 +389,12 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
                            (const int32_t*)s_t, (int32_t*)tids.p);
     };
     int lslot = (int)S.msv_groups.size();
+    // A stage's runs cover disjoint survivor tiles: they go out side by side on the side
+    // streams (the small runs of a stage no longer wait for the large ones), each with its
+    // own head slot (slots repeat only every 32 launches, on the same side stream).
     auto stage_runs = [&](int stage, std::vector<CascadeState::Run>& runs, DeviceBuf& tl, DeviceBuf& tids,
                           DeviceBuf& toff, void* out) {
+        Fanout fo(c, st);
         for (auto& r : runs) {
             const KernelInfo& k = c.kernel(stage, r.s.q, r.s.g, false, r.s.hoist);
             SxWork w{};
@@ -393,10 +404,13 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
             w.dyn_hi = (const int64_t*)toff.p + r.b;
             w.ntiles = 0;
             const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), 1 << 30);
-            c.launch(k, stage, w, out, nosp, grid, true, slot++ % 32 + 32);
-            if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], st));
+            hipStream_t ls = fo.next();
+            if (with_events) HIPCHK(hipEventRecord(S.lev0[lslot], ls));
+            c.launch(k, stage, w, out, nosp, grid, true, slot++ % 32 + 32, ls);
+            if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], ls));
             lslot++;
         }
+        fo.join();
     };
     const int64_t* n2 = compact((const uint8_t*)S.flag.p, n1, nullptr, S.cnt2, S.off2, S.list2, S.per2);
     tiles(S.per2, S.d_ng_vit, S.toff2, S.eoff2, S.tiles2, n2, S.list2, S.tids2);
@@ -503,8 +517,8 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
         st->cells_fwd = cf;
         S.launches.clear();
         if (!st->use_graph) {
-            // per-launch times between consecutive events; the Viterbi and Forward cells
-            // of each run come from the records of the pairs that reached the stage
+            // per-launch times between each launch's own events; the Viterbi and Forward
+            // cells of each run come from the records of the pairs that reached the stage
             std::vector<int32_t> pos(c.profs.size());
             for (size_t i = 0; i < S.perm.size(); i++) pos[S.perm[i]] = (int32_t)i;
             auto run_cells = [&](const std::vector<CascadeState::Run>& runs, int min_stage) {
@@ -522,21 +536,20 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
             };
             const std::vector<uint64_t> vc = run_cells(S.vit_runs, 2), fc = run_cells(S.fwd_runs, 3);
             size_t e = 0;
-            auto add = [&](int stage, const Shape& sh, hipEvent_t from, uint64_t cells) {
+            auto add = [&](int stage, const Shape& sh, uint64_t cells) {
                 sx_launch_stat l{};
                 l.stage = stage;
                 l.q = sh.q;
                 l.g = sh.g;
                 l.hoist = sh.hoist;
-                l.ms = ev_ms(from, S.lev[e]);
+                l.ms = ev_ms(S.lev0[e], S.lev[e]);
                 l.cells = cells;
                 S.launches.push_back(l);
                 e++;
             };
-            for (size_t i = 0; i < S.msv_groups.size(); i++)
-                add(0, S.msv_groups[i].s, i ? S.lev[e - 1] : S.ev[0], S.msv_groups[i].cells);
-            for (size_t i = 0; i < S.vit_runs.size(); i++) add(1, S.vit_runs[i].s, i ? S.lev[e - 1] : S.ev[2], vc[i]);
-            for (size_t i = 0; i < S.fwd_runs.size(); i++) add(2, S.fwd_runs[i].s, i ? S.lev[e - 1] : S.ev[3], fc[i]);
+            for (size_t i = 0; i < S.msv_groups.size(); i++) add(0, S.msv_groups[i].s, S.msv_groups[i].cells);
+            for (size_t i = 0; i < S.vit_runs.size(); i++) add(1, S.vit_runs[i].s, vc[i]);
+            for (size_t i = 0; i < S.fwd_runs.size(); i++) add(2, S.fwd_runs[i].s, fc[i]);
         }
         if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, (uint64_t)n1) * sizeof(sx_pair_scores));
         return (uint64_t)n1;

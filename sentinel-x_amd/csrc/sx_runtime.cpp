@@ -80,6 +80,11 @@ Ctx::Ctx(int dev) : device(dev) {
     HIPCHK(hipStreamCreate(&stream));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    for (int i = 0; i < NSIDE; i++) {
+        HIPCHK(hipStreamCreate(&side[i]));
+        HIPCHK(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     n_cu = prop.multiProcessorCount;
@@ -111,6 +116,12 @@ Ctx::~Ctx() {
     dfree(d_heads);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (int i = 0; i < NSIDE; i++) {
+        if (side[i]) (void)hipStreamSynchronize(side[i]);
+        if (side[i]) (void)hipStreamDestroy(side[i]);
+        if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -349,14 +360,44 @@ int Ctx::persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const 
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
 }
 
+Fanout::Fanout(Ctx& c_, hipStream_t o) : c(c_), origin(o) { HIPCHK(hipEventRecord(c.fork_ev, origin)); }
+
+hipStream_t Fanout::next() {
+    static_assert(Ctx::NSIDE == sizeof(used) / sizeof(used[0]), "one flag per side stream");
+    const int i = n++ % Ctx::NSIDE;
+    if (!used[i]) {
+        HIPCHK(hipStreamWaitEvent(c.side[i], c.fork_ev, 0));
+        used[i] = true;
+    }
+    return c.side[i];
+}
+
+Fanout::~Fanout() {
+    for (int i = 0; i < Ctx::NSIDE; i++)
+        if (used[i]) {
+            (void)hipEventRecord(c.join_ev[i], c.side[i]);
+            (void)hipStreamWaitEvent(origin, c.join_ev[i], 0);
+        }
+}
+
+void Fanout::join() {
+    for (int i = 0; i < Ctx::NSIDE; i++)
+        if (used[i]) {
+            HIPCHK(hipEventRecord(c.join_ev[i], c.side[i]));
+            HIPCHK(hipStreamWaitEvent(origin, c.join_ev[i], 0));
+            used[i] = false;
+        }
+}
+
 void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
-                 int head_slot) {
+                 int head_slot, hipStream_t s) {
+    if (!s) s = stream;
     const size_t lds = lds_bytes(k);
     if (lds > LDS_MAX) throw SxError(SX_ERR_ARG, "profile tables exceed the LDS budget");
     if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (persistent) {
         w.head = d_heads + head_slot;
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, w.head, 1);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, w.head, 1);
         HIPCHK(hipGetLastError());
     } else {
         w.head = nullptr;
@@ -364,7 +405,7 @@ void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecials
     SxProfiles P = dprof();
     SxTargets T = dtarg();
     void* args[] = {&P, &T, &w, &out, &sp};
-    HIPCHK(hipLaunchKernel(k.fn, dim3(grid), dim3(k.block), args, lds, stream));
+    HIPCHK(hipLaunchKernel(k.fn, dim3(grid), dim3(k.block), args, lds, s));
 }
 
 // Implicit all-pairs launches for a set of profiles against a target-id list (device
@@ -411,6 +452,7 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
     HIPCHK(hipStreamSynchronize(stream));
     int slot = 0;
     size_t gi = 0;
+    Fanout fo(*this, stream);  // the shape groups side by side (head slots differ within 64)
     for (auto& kv : groups) {
         const int q = std::get<0>(kv.first), g = std::get<1>(kv.first), h = std::get<2>(kv.first);
         const KernelInfo& k = kernel(stage, q, g, false, h);
@@ -424,8 +466,9 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         w.out_stride = out_stride;
         if (w.ntiles == 0) continue;
         const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
-        launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
+        launch(k, stage, w, dout, sp, grid, true, slot++ % 64, fo.next());
     }
+    fo.join();
 }
 
 // All pairs of every profile against every target.  out is a device buffer of P*T
@@ -496,6 +539,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     HIPCHK(hipStreamSynchronize(stream));
     size_t toff = 0;
     int slot = 0;
+    Fanout fo(*this, stream);  // batched: the shape groups side by side
     for (auto& kv : tiles) {
         const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
         const int h = std::get<2>(kv.first);
@@ -508,11 +552,12 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         w.ntiles = (int64_t)kv.second.size();
         if (form == SX_FORM_BATCHED) {
             const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
-            launch(k, stage, w, dout, sp, grid, true, slot++ % 64);
+            launch(k, stage, w, dout, sp, grid, true, slot++ % 64, fo.next());
         } else {
             launch(k, stage, w, dout, sp, (int)w.ntiles, false, 0);
         }
     }
+    fo.join();
 }
 
 // ---------------------------------------------------------------------------

@@ -57,6 +57,26 @@ struct TuneRecord {
     double gcups;
 };
 
+struct Ctx;
+
+// Fan independent launches out over the side streams after everything already queued
+// on `origin`, and join them back: launches on different side streams may run at the
+// same time.  Works eagerly and under stream capture (the joins become graph edges).
+// The i-th launch goes to side stream i % NSIDE, so launches i and i + NSIDE*k share a
+// stream and run in order: head slots need only differ modulo NSIDE*k.
+struct Fanout {
+    Fanout(Ctx& c, hipStream_t origin);
+    ~Fanout();  // joins what join() has not (an exception between launches)
+    Fanout(const Fanout&) = delete;
+    Fanout& operator=(const Fanout&) = delete;
+    hipStream_t next();
+    void join();
+    Ctx& c;
+    hipStream_t origin;
+    int n = 0;
+    bool used[4] = {};
+};
+
 struct Ctx {
     explicit Ctx(int dev);
     ~Ctx();
@@ -64,6 +84,11 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // side streams (default priority) on which a stage's independent per-shape launches
+    // run side by side; fork_ev / join_ev order them after / before the main stream
+    static constexpr int NSIDE = 4;
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[NSIDE] = {};
     int n_cu = 0;
     std::map<long, KernelInfo> kernels;
     // distinct fields: stage | Q (20 bits) | G (16 bits) | special rows (1 bit) | table mode (3 bits)
@@ -125,8 +150,10 @@ struct Ctx {
     std::deque<DeviceBuf> pgroups_keep;
     std::vector<uint64_t> last_order;
     int persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const;
+    // s = nullptr: the context's stream.  Two launches that may run at the same time
+    // need different head slots.
     void launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
-                int head_slot);
+                int head_slot, hipStream_t s = nullptr);
     void run_all(int stage, int form, void* dout);
     void run_implicit(int stage, int form, const std::vector<int32_t>& plist, const int32_t* dtids, int ntids,
                       void* dout, int64_t out_stride);
