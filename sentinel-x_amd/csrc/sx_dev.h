@@ -35,6 +35,10 @@ struct SxProfiles {
     const int64_t* toff;   // per profile: offset of its 8 x (M+1) block in tsc / tp
     const int32_t* m;
     const float* msv_tbm;
+    // per profile: 1 if every M->D and D->D score is <= 0 (true of any probability
+    // model), so that no D state can exceed the best M state of its row and Viterbi's
+    // row maximum xE can skip the D row; 0 otherwise (then D is included, as written)
+    const int32_t* vit_m_only;
     // Long models (M > SX_GEM_MIN_M): padded copies of the emission rows read straight
     // from global memory.  Row x of profile p starts at gmsc/gmo + goff[p] + x * gw, and
     // position k sits at index k + 3; positions 0 and M+1 .. gw-4 hold the pad value

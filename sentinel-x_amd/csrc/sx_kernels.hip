@@ -448,7 +448,10 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, int ew, 
 // ---------------------------------------------------------------------------
 // Viterbi score (dp.rs:152-189)
 // ---------------------------------------------------------------------------
-template <int Q, int G, int TM>
+// MONLY: the profile's M->D and D->D scores are all <= 0, so no D value of a row exceeds
+// the row's best M value and the row maximum xE is taken over M alone (the same value as
+// over M and D; see SxProfiles::vit_m_only).
+template <int Q, int G, int TM, bool MONLY>
 __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, const float* __restrict__ tr,
                                           const uint8_t* seq, int L, float lloop, float lmove) {
     constexpr int WID = Geo<Q, G>::WID;
@@ -511,8 +514,13 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, 
             }
         }
         float e = SX_NEG_INF;
+        if constexpr (MONLY) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) e = fmaxf(e, fmaxf(mr[q], dr[q]));
+            for (int q = 0; q < Q; q++) e = fmaxf(e, mr[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; q++) e = fmaxf(e, fmaxf(mr[q], dr[q]));
+        }
         const float xe = grp_reduce<G>(e, OpMax());
         xj = fmaxf(xj + lloop, xe - SX_LN2F);
         xc = fmaxf(xc + lloop, xe - SX_LN2F);
@@ -527,7 +535,7 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, 
 // already shifted to the cell they feed (position k-1 for MM IM DM MD DD, k for MI II
 // BM), so the lane's operands for cell q all sit at slot q of its block.  Nothing is
 // kept in registers across rows but the three DP rows.
-template <int Q, int G, int TM>
+template <int Q, int G, int TM, bool MONLY>
 __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, const float* __restrict__ tr,
                                               const uint8_t* seq, int L, float lloop, float lmove) {
     using GA = GeoA<Q, G>;
@@ -593,8 +601,13 @@ __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, cons
             }
         }
         float e = SX_NEG_INF;
+        if constexpr (MONLY) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) e = fmaxf(e, fmaxf(mr[q], dr[q]));
+            for (int q = 0; q < Q; q++) e = fmaxf(e, mr[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; q++) e = fmaxf(e, fmaxf(mr[q], dr[q]));
+        }
         const float xe = grp_reduce<G>(e, OpMax());
         xj = fmaxf(xj + lloop, xe - SX_LN2F);
         xc = fmaxf(xc + lloop, xe - SX_LN2F);
@@ -1032,10 +1045,17 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_VIT) {
                 float r = SX_NEG_INF;
+                const bool monly = __builtin_amdgcn_readfirstlane(P.vit_m_only[ti.prof]) != 0;  // tile-uniform
                 if constexpr (tm_aligned<TM>()) {
-                    if (L > 0) r = vit_pair_aln<Q, G, TM>(em, lds + GeoA<Q, G>::TAB, seq, L, lm.lloop, lm.lmove);
+                    const float* tr = lds + GeoA<Q, G>::TAB;
+                    if (L > 0)
+                        r = monly ? vit_pair_aln<Q, G, TM, true>(em, tr, seq, L, lm.lloop, lm.lmove)
+                                  : vit_pair_aln<Q, G, TM, false>(em, tr, seq, L, lm.lloop, lm.lmove);
                 } else {
-                    if (L > 0) r = vit_pair<Q, G, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.lloop, lm.lmove);
+                    const float* tr = lds + NTAB_E;
+                    if (L > 0)
+                        r = monly ? vit_pair<Q, G, TM, true>(em, P.gw, tr, seq, L, lm.lloop, lm.lmove)
+                                  : vit_pair<Q, G, TM, false>(em, P.gw, tr, seq, L, lm.lloop, lm.lmove);
                 }
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
             } else if (STAGE == ST_FWD) {

@@ -127,7 +127,7 @@ Ctx::~Ctx() {
 
 void Ctx::free_profiles() {
     dfree(d_msc); dfree(d_mo); dfree(d_tsc); dfree(d_tp);
-    dfree(d_off); dfree(d_toff); dfree(d_m); dfree(d_tbm);
+    dfree(d_off); dfree(d_toff); dfree(d_m); dfree(d_tbm); dfree(d_vmo);
     dfree(d_gmsc); dfree(d_gmo); dfree(d_goff);
 }
 
@@ -214,8 +214,18 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     profs = std::move(ps);
     std::vector<float> msc, mo, tsc, tp, tbm;
     std::vector<int64_t> off, toff;
-    std::vector<int32_t> ms;
+    std::vector<int32_t> ms, vmo;
     for (const auto& p : profs) {
+        // D(k) = max(M(k-1) + tMD, D(k-1) + tDD) with D(1) = -inf: when no tMD, tDD is
+        // positive, every D value is at most an earlier M value of the same row (f32
+        // rounding is monotone and x + t <= x for t <= 0), so max over M and D of a row
+        // equals max over M.  NaN fails the test and keeps D in the maximum.
+        bool m_only = true;
+        for (int k = 0; k <= p.m; k++) {
+            const float md = p.t_row(SX_TMD)[k], dd = p.t_row(SX_TDD)[k];
+            if (!(md <= 0.0f) || !(dd <= 0.0f)) m_only = false;
+        }
+        vmo.push_back(m_only ? 1 : 0);
         off.push_back((int64_t)msc.size());
         toff.push_back((int64_t)tsc.size());
         ms.push_back(p.m);
@@ -233,6 +243,7 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     d_toff = dupload(toff);
     d_m = dupload(ms);
     d_tbm = dupload(tbm);
+    d_vmo = dupload(vmo);
     // padded emission rows of the long models (read from global memory by their kernels)
     std::vector<float> gmsc, gmo;
     std::vector<int64_t> goff(profs.size(), -1);
@@ -259,7 +270,7 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
 SxProfiles Ctx::dprof() const {
     SxProfiles P;
     P.msc = d_msc; P.mo = d_mo; P.tsc = d_tsc; P.tp = d_tp;
-    P.off = d_off; P.toff = d_toff; P.m = d_m; P.msv_tbm = d_tbm;
+    P.off = d_off; P.toff = d_toff; P.m = d_m; P.msv_tbm = d_tbm; P.vit_m_only = d_vmo;
     P.gmsc = d_gmsc; P.gmo = d_gmo; P.goff = d_goff; P.gw = SX_GEM_WIDTH; P.pad = 0;
     return P;
 }
@@ -653,17 +664,35 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             cands.swap(keep);
         }
         Shape best = default_shape(stage, SX_FORM_BATCHED, mmax);
+        // the built-in choice is timed first: when the budget runs out on a bucket whose
+        // tightest shapes are slow ones, the table still holds a measured good shape
+        {
+            auto it = std::find_if(cands.begin(), cands.end(), [&](const Shape& x) {
+                return x.q == best.q && x.g == best.g && x.hoist == best.hoist;
+            });
+            if (it != cands.end()) std::rotate(cands.begin(), it, it + 1);
+        }
         double best_ms = 1e300, spent = 0.0;
         for (const Shape& s : cands) {
             if (spent > per_bucket && best_ms < 1e300) break;
             forced[stage] = s;
             try {
-                run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);  // warm-up
+                const double t0 = timed([&] {  // warm-up
+                    run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);
+                });
+                spent += t0;
+                if (t0 > 3.0 * best_ms) {
+                    // far slower than the best so far even counting first-launch costs:
+                    // logged from the warm-up alone, not timed again
+                    log.push_back({hi, s, t0, cells / t0 / 1e6});
+                    forced.erase(stage);
+                    continue;
+                }
                 const double t1 = timed([&] {
                     run_implicit(stage, SX_FORM_BATCHED, sample, (const int32_t*)dts.p, nts, dout.p, 0);
                 });
                 double ms = t1;
-                spent += 2 * t1;
+                spent += t1;
                 if (stage == 0) {
                     // The MSV launches of a search cover thousands of profiles x every
                     // target: rank shapes by the marginal time of a second copy of the

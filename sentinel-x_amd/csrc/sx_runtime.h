@@ -108,6 +108,7 @@ struct Ctx {
     int64_t* d_goff = nullptr;
     int64_t *d_off = nullptr, *d_toff = nullptr;
     int32_t* d_m = nullptr;
+    int32_t* d_vmo = nullptr;  // SxProfiles::vit_m_only
     int32_t* d_pidx = nullptr;
     size_t d_pidx_n = 0;
     void set_profiles(std::vector<Profile>&& ps);

@@ -209,6 +209,34 @@ def test_aligned_modes_are_msv_and_viterbi_only(gctx, orc):
         gctx.set_shapes(0, [(256, 16, 16, 2)])  # the long-model mode is fixed
 
 
+@pytest.mark.parametrize("q,g,h", [(8, 8, 1), (16, 16, 1), (16, 16, 0), (12, 16, 3), (16, 8, 3), (24, 64, 2)],
+                         ids=["Q8G8R", "Q16G16R", "Q16G16L", "Q12G16A", "Q16G8A", "Q24G64GEM"])
+def test_viterbi_row_max_keeps_d_for_positive_delete_scores(orc, gctx, q, g, h):
+    """A model whose M->D or D->D probabilities exceed 1 (parseable, though not a valid
+    model) has positive delete scores: a D value can then beat every M value of its row, so
+    Viterbi's xE must keep the D row.  Valid models take the M-only maximum; both must
+    match the oracle bit for bit."""
+    cap = q * g if h != 2 else 1400
+    ms = [max(2, cap // 2 + 1), cap]
+    hmms = [random_hmm(orc, m, 6000 + m) for m in ms]
+    for x in hmms[:1]:
+        t = np.array(x.t, np.float32)
+        t[1:x.m:3, 2] = 20.0  # MD: ln 20 = +3.0
+        t[2:x.m:2, 6] = 20.0  # DD
+        x.t = t
+    seqs = make_targets(orc, hmms, 60, 6000 + q + g, lmax=220)
+    profs = [orc.configure(x) for x in hmms]
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    if h != 2:
+        gctx.set_shapes(1, [(cap, q, g, h)])
+    got, _ = gctx.score_all(1, 1)
+    gctx.set_shapes(1, [])
+    want = orc.Batch(profs, seqs).viterbi(nthreads=8).reshape(len(profs), len(seqs))
+    assert np.array_equal(got, want)
+    assert np.isfinite(want[0]).all()  # the planted model scores every target
+
+
 VIT_ALN_SHAPES = [(8, 8), (12, 8), (16, 8), (8, 16), (12, 16), (16, 16), (24, 16), (12, 32), (16, 32), (24, 32),
                   (32, 32)]
 
