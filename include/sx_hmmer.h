@@ -31,8 +31,12 @@ extern "C" {
 
 typedef struct sx_ctx sx_ctx;
 
-/* DP stages (dp.rs:69 msv, :152 viterbi_score, :229 forward, :301 backward). */
-enum { SX_STAGE_MSV = 0, SX_STAGE_VITERBI = 1, SX_STAGE_FORWARD = 2, SX_STAGE_BACKWARD = 3 };
+/* DP stages (dp.rs:69 msv, :152 viterbi_score, :229 forward, :301 backward).
+ * SX_STAGE_MSV16 has no reference counterpart: the int16 MSV prefilter.  It computes an
+ * upper bound of the f32 MSV score with saturating packed int16 cells, as an int32 in
+ * units of 2^-22 nat: score(MSV16) >= 2^22 * score(MSV) for every pair, or INT32_MAX when
+ * the bound saturated (INT32_MIN for an empty target).  Models up to 1023 positions. */
+enum { SX_STAGE_MSV = 0, SX_STAGE_VITERBI = 1, SX_STAGE_FORWARD = 2, SX_STAGE_BACKWARD = 3, SX_STAGE_MSV16 = 4 };
 /* Launch forms: PLAIN = one launch per profile over blocks of targets in input order,
  * one wavefront per pair; BATCHED = one persistent launch per stage and lane-group
  * shape, length-bucketed tiles from a device-resident queue. */
@@ -124,10 +128,19 @@ typedef struct {
 typedef struct {
     double f1, f2, f3;          /* in: PipelineConfig thresholds */
     int32_t use_graph;          /* in: batched form replays a captured HIP graph */
-    int32_t pad;
+    /* in: 1 = the MSV stage first runs the int16 MSV bound (SX_STAGE_MSV16) on every
+     * pair and computes the f32 MSV only for the pairs whose bound reaches the pair's
+     * threshold; the bound never lies below the f32 score, so the survivors, their
+     * scores and every decision are the same as with 0 (f32 MSV on every pair). */
+    int32_t msv_bound;
     uint64_t n_pairs, n_msv_pass, n_bias_pass, n_vit_pass, n_fwd_pass;  /* out */
     double ms_msv, ms_bias, ms_vit, ms_fwd, ms_total;                   /* out: device time */
     uint64_t cells_msv, cells_vit, cells_fwd;                           /* out: DP cells */
+    /* out (msv_bound = 1): pairs whose bound reached the threshold (rescored in f32),
+     * device time of the bound's launches alone, and the rescored cells (eager runs) */
+    uint64_t n_msv_cand;
+    double ms_msv_bound;
+    uint64_t cells_msv_exact;
 } sx_cascade_stats;
 
 int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,

@@ -55,6 +55,12 @@ def lib():
         L.sxo_null_score.argtypes = [i64]
         L.sxo_msv.restype = ctypes.c_float
         L.sxo_msv.argtypes = [ctypes.c_int, f32p, ctypes.c_float, u8p, i64]
+        L.sxo_msv16.restype = ctypes.c_int32
+        L.sxo_msv16.argtypes = [ctypes.c_int, f32p, ctypes.c_float, u8p, i64]
+        L.sxo_q_emit.restype = ctypes.c_int16
+        L.sxo_q_emit.argtypes = [ctypes.c_float]
+        L.sxo_q_const.restype = ctypes.c_int32
+        L.sxo_q_const.argtypes = [ctypes.c_float]
         L.sxo_viterbi.restype = ctypes.c_float
         L.sxo_viterbi.argtypes = [ctypes.c_int, f32p, f32p, f32p, u8p, i64]
         L.sxo_forward.restype = ctypes.c_double
@@ -80,7 +86,8 @@ def lib():
         L.sxo_batch_cascade.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, i64, i64, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int]
-        for name in ("sxo_batch_msv", "sxo_batch_viterbi", "sxo_batch_forward", "sxo_batch_forward64"):
+        for name in ("sxo_batch_msv", "sxo_batch_viterbi", "sxo_batch_forward", "sxo_batch_forward64",
+                     "sxo_batch_msv16"):
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_void_p, i64, i64, ctypes.c_void_p, ctypes.c_int]
@@ -339,6 +346,16 @@ def msv(p: Profile, dsq) -> np.float32:
     return np.float32(lib().sxo_msv(p.m, fp(p.msc), p.msv_tbm, bp(d), len(d)))
 
 
+# Units of the quantised MSV bound (sxo_msv16): special states in 2^-22 nat.
+Q_UNITS_PER_NAT = 4194304.0
+
+
+def msv16(p: Profile, dsq) -> int:
+    """Quantised MSV upper bound (no reference counterpart; see sx_oracle.h)."""
+    d = _dsq(dsq)
+    return int(lib().sxo_msv16(p.m, fp(p.msc), p.msv_tbm, bp(d), len(d)))
+
+
 def viterbi(p: Profile, dsq) -> np.float32:
     d = _dsq(dsq)
     xs = xsc(len(d))
@@ -453,6 +470,9 @@ class Batch:
 
     def msv(self, begin=0, end=None, nthreads=1):
         return self._run(lib().sxo_batch_msv, np.float32, begin, end, nthreads)
+
+    def msv16(self, begin=0, end=None, nthreads=1):
+        return self._run(lib().sxo_batch_msv16, np.int32, begin, end, nthreads)
 
     def viterbi(self, begin=0, end=None, nthreads=1):
         return self._run(lib().sxo_batch_viterbi, np.float32, begin, end, nthreads)

@@ -232,6 +232,65 @@ float sxo_msv(int m, const float* msc, float tbmk, const uint8_t* dsq, int64_t l
     return xc + tmove;
 }
 
+/* Quantised MSV upper bound (see sx_oracle.h).  No reference counterpart: the
+ * recurrence is dp.rs:92-145 (as sxo_msv) with every score rounded up. */
+#define SXQ_S 1024.0          /* emission units per nat */
+#define SXQ_F 4194304.0       /* 2^22 special-state units per nat (SXQ_S * 4096) */
+#define SXQ_SLACK_S 0.03125   /* SXQ_S * 2^-24 * 512: f32 rounding allowance per step */
+#define SXQ_SLACK_F 128.0     /* SXQ_F * 2^-24 * 512 */
+
+int16_t sxo_q_emit(float sc) {
+    if (sc == NEG_INF) return -32768;
+    double v = ceil((double)sc * SXQ_S + SXQ_SLACK_S);
+    if (v < -32768.0) v = -32768.0;
+    if (v > 32767.0) v = 32767.0;
+    return (int16_t)v;
+}
+
+int32_t sxo_q_const(float v) { return (int32_t)ceil((double)v * SXQ_F + SXQ_SLACK_F); }
+
+static inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+static inline int32_t sat16(int32_t v) { return v < -32768 ? -32768 : v > 32767 ? 32767 : v; }
+/* ceil(v / 4096) for any sign */
+static inline int32_t ceil4096(int32_t v) { return (int32_t)floor(((double)v + 4095.0) / 4096.0); }
+
+int32_t sxo_msv16(int m, const float* msc, float tbmk, const uint8_t* dsq, int64_t l) {
+    if (l == 0) return INT32_MIN;
+    const int w = m + 1;
+    const float lf = (float)l;
+    const int32_t tl = sxo_q_const(logf(lf / (lf + 3.0f)));
+    const int32_t tm = sxo_q_const(logf(3.0f / (lf + 3.0f)));
+    const int32_t tb = sxo_q_const(tbmk);
+    const int32_t te = sxo_q_const(-LN2F);
+    /* positions 0..M+1; position 0 stays at the floor, M+1 has the score -32768 */
+    int32_t* prev = (int32_t*)malloc(sizeof(int32_t) * (size_t)(m + 2) * 2);
+    int32_t* cur = prev + (m + 2);
+    for (int k = 0; k < 2 * (m + 2); k++) prev[k] = -32768;
+    int32_t xn = 0, xj = -(1 << 30), xb = imax(xn, xj) + tm;
+    int sat = 0;
+    for (int64_t i = 0; i < l; i++) {
+        const float* rs = msc + (size_t)dsq[i] * w;
+        int32_t be = ceil4096(xb + tb);
+        if (be > 32767) sat = 1;
+        be = sat16(be);
+        int32_t xe = -32768;
+        for (int k = 1; k <= m + 1; k++) {
+            const int32_t e = k <= m ? sxo_q_emit(rs[k]) : -32768;
+            const int32_t v = sat16(imax(prev[k - 1], be) + e);
+            cur[k] = v;
+            xe = imax(xe, v);
+        }
+        cur[0] = -32768;
+        if (xe >= 32767) sat = 1;
+        xj = imax(xj + tl, xe * 4096 + te);
+        xn += tl;
+        xb = imax(xn, xj) + tm;
+        int32_t* s = prev; prev = cur; cur = s;
+    }
+    free(prev < cur ? prev : cur);
+    return sat ? INT32_MAX : xj + tm;
+}
+
 /* dp.rs:152-189 */
 float sxo_viterbi(int m, const float* msc, const float* tsc, const float* xs, const uint8_t* dsq,
                   int64_t l) {
@@ -636,6 +695,9 @@ static void run_pair(const pool_job* j, const sxo_batch* b, int stage, int64_t i
         case 3:
             ((double*)out)[idx - begin] = sxo_forward64(m, b->mo + b->off[p], b->tp + b->toff[p], xs, dsq, l);
             break;
+        case 5:
+            ((int32_t*)out)[idx - begin] = sxo_msv16(m, b->msc + b->off[p], b->msv_tbm[p], dsq, l);
+            break;
         default:
             ((int32_t*)out)[idx - begin] =
                 sxo_cascade(m, b->msc + b->off[p], b->tsc + b->toff[p], b->mo + b->off[p], b->tp + b->toff[p],
@@ -687,3 +749,4 @@ int sxo_batch_msv(const sxo_batch* b, int64_t pb, int64_t pe, float* out, int nt
 int sxo_batch_viterbi(const sxo_batch* b, int64_t pb, int64_t pe, float* out, int nt) { return run_pool(b, pb, pe, out, nt, 1); }
 int sxo_batch_forward(const sxo_batch* b, int64_t pb, int64_t pe, double* out, int nt) { return run_pool(b, pb, pe, out, nt, 2); }
 int sxo_batch_forward64(const sxo_batch* b, int64_t pb, int64_t pe, double* out, int nt) { return run_pool(b, pb, pe, out, nt, 3); }
+int sxo_batch_msv16(const sxo_batch* b, int64_t pb, int64_t pe, int32_t* out, int nt) { return run_pool(b, pb, pe, out, nt, 5); }

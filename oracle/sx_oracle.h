@@ -58,6 +58,22 @@ float sxo_null_score(int64_t l);
  * a generic max-product over the explicit MSV state graph (tests). */
 float sxo_msv(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_t l);
 
+/* Quantised MSV upper bound: the product's int16 MSV prefilter, which has no
+ * counterpart in the reference (its MSV is the f32 dp.rs:92-145).  Emission scores
+ * are rounded up to int16 units of 1/1024 nat (-inf -> -32768) and the cells use
+ * saturating int16 arithmetic; N/J/B and the length-model scores are int32 in units
+ * of 2^-22 nat, rounded up.  Every rounding adds the f32 rounding allowance of a
+ * value of magnitude <= 512 nats (2^-24 * 512), so the result bounds the f32 MSV
+ * from above: 2^22 * sxo_msv() <= sxo_msv16() unless the return is INT32_MAX (a
+ * cell or the B entry saturated high).  Positions 1..M+1 are computed, M+1 with
+ * the score -32768 (the device kernel's first padding position).  L = 0 gives
+ * INT32_MIN.  Parity of the device kernel with this function is bit-exact; this
+ * function is pinned against sxo_msv by the bound and its tightness (tests). */
+int32_t sxo_msv16(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_t l);
+/* The quantisation steps of sxo_msv16. */
+int16_t sxo_q_emit(float sc);
+int32_t sxo_q_const(float v);
+
 /* dp.rs:152-189.  Parity unpinned by the reference; pinned by a generic
  * max-product over the explicit Plan7 state graph (tests). */
 float sxo_viterbi(int m, const float* msc, const float* tsc, const float* xsc16,
@@ -126,6 +142,7 @@ int sxo_batch_msv(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, floa
 int sxo_batch_viterbi(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, float* out, int nthreads);
 int sxo_batch_forward(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
 int sxo_batch_forward64(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
+int sxo_batch_msv16(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, int32_t* out, int nthreads);
 /* sxo_cascade over many pairs (the reference's search_profiles pair loop with
  * filters on, search.rs:217-224 + pipeline.rs:159-189).  stats: 6 doubles per profile;
  * compo: 20 floats per profile.  stage_out: one int per pair; sc_out: 4 floats per pair. */

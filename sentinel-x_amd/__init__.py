@@ -18,8 +18,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsxhmmer.so")
 
 STAGE_MSV, STAGE_VITERBI, STAGE_FORWARD, STAGE_BACKWARD = 0, 1, 2, 3
+# int16 MSV upper bound (no reference counterpart): int32 scores in units of 2^-22 nat,
+# at least 2^22 times the f32 MSV score; INT32_MAX when the bound saturated
+STAGE_MSV16 = 4
+MSV16_UNITS_PER_NAT = 4194304.0
+MSV16_SATURATED = 2**31 - 1
 FORM_PLAIN, FORM_BATCHED = 0, 1
-STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward"}
+STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward", 4: "msv16"}
+
+
+def _stage_dtype(stage: int):
+    return {STAGE_MSV16: np.int32, STAGE_FORWARD: np.float64, STAGE_BACKWARD: np.float64}.get(stage, np.float32)
 
 # Every entry point declared in include/sx_hmmer.h.
 EXPORTS = [
@@ -61,11 +70,14 @@ class sx_pair_scores(ctypes.Structure):
 
 class sx_cascade_stats(ctypes.Structure):
     _fields_ = [("f1", ctypes.c_double), ("f2", ctypes.c_double), ("f3", ctypes.c_double),
-                ("use_graph", ctypes.c_int32), ("pad", ctypes.c_int32), ("n_pairs", ctypes.c_uint64), ("n_msv_pass", ctypes.c_uint64), ("n_bias_pass", ctypes.c_uint64),
+                ("use_graph", ctypes.c_int32), ("msv_bound", ctypes.c_int32), ("n_pairs", ctypes.c_uint64),
+                ("n_msv_pass", ctypes.c_uint64), ("n_bias_pass", ctypes.c_uint64),
                 ("n_vit_pass", ctypes.c_uint64), ("n_fwd_pass", ctypes.c_uint64),
                 ("ms_msv", ctypes.c_double), ("ms_bias", ctypes.c_double), ("ms_vit", ctypes.c_double),
                 ("ms_fwd", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("cells_msv", ctypes.c_uint64), ("cells_vit", ctypes.c_uint64), ("cells_fwd", ctypes.c_uint64)]
+                ("cells_msv", ctypes.c_uint64), ("cells_vit", ctypes.c_uint64), ("cells_fwd", ctypes.c_uint64),
+                ("n_msv_cand", ctypes.c_uint64), ("ms_msv_bound", ctypes.c_double),
+                ("cells_msv_exact", ctypes.c_uint64)]
 
 
 class sx_search_cfg(ctypes.Structure):
@@ -334,7 +346,7 @@ class Context:
     # -- compute --
     def score_all(self, stage: int, form: int = FORM_BATCHED, want: bool = True):
         nt = len(self.lengths)
-        dt = np.float32 if stage < STAGE_FORWARD else np.float64
+        dt = _stage_dtype(stage)
         out = np.empty((self.n_profiles, nt), dt) if want else None
         ms = ctypes.c_double()
         self._chk(self._L.sx_score_all(self._h, stage, form, _ptr(out), ctypes.byref(ms)))
@@ -343,7 +355,7 @@ class Context:
     def score_pairs(self, stage: int, prof, tgt, form: int = FORM_BATCHED):
         prof = np.ascontiguousarray(prof, np.uint32)
         tgt = np.ascontiguousarray(tgt, np.uint32)
-        dt = np.float32 if stage < STAGE_FORWARD else np.float64
+        dt = _stage_dtype(stage)
         out = np.empty(len(prof), dt)
         ms = ctypes.c_double()
         self._chk(self._L.sx_score_pairs(self._h, stage, form, _ptr(prof), _ptr(tgt), len(prof), _ptr(out),
@@ -366,12 +378,14 @@ class Context:
         return out, sc
 
     def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, use_graph: bool = True,
-                    cap: Optional[int] = None):
+                    cap: Optional[int] = None, msv_bound: bool = True):
         """Returns (stats, records) for every MSV survivor (see sx_run_cascade).  With cap the
-        records are truncated to cap; without it all of them are fetched after the run."""
+        records are truncated to cap; without it all of them are fetched after the run.
+        msv_bound: run the int16 MSV bound first and the f32 MSV only on its candidates."""
         st = sx_cascade_stats()
         st.f1, st.f2, st.f3 = f1, f2, f3
         st.use_graph = 1 if use_graph else 0
+        st.msv_bound = 1 if msv_bound else 0
         n = ctypes.c_uint64()
         if cap is not None:
             out = np.empty(cap, PAIR_DTYPE)

@@ -81,6 +81,108 @@ sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__
     }
 }
 
+// Candidates of the quantised MSV bound: every pair (profile position pi, length-sorted
+// target j) whose int16 bound reaches the pair's pass threshold in the bound's units.
+// The exact test passes a pair only if its f32 score u satisfies
+// fl(fl(u - null) / ln2) >= thr, hence u >= null + ln2 * thr - delta with delta
+// covering those two roundings; nullq[j] and thrq[p] hold floor(2^22 * (null - ...)) and
+// floor(2^22 * (ln2 * thr - ...)), so a bound below their sum proves the rejection.
+// Profiles with noq set (longer than the bound's shapes, or without a threshold) make
+// every pair a candidate.  Phase 0 writes block counts, phase 1 the candidates (pi, j,
+// target id) at their scanned offsets, in (pi, j) order.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_msvq_cand(int phase, const int32_t* __restrict__ qsc, const int64_t* __restrict__ nullq,
+              const int64_t* __restrict__ thrq, const uint8_t* __restrict__ noq, const int32_t* __restrict__ perm,
+              const int32_t* __restrict__ sorted, int64_t P, int64_t T, int32_t* counts, const int64_t* offsets,
+              int64_t cap, int32_t* c_pi, int32_t* c_j, int32_t* c_t) {
+    __shared__ int s_wave[SXC_BLOCK / 64];
+    const int64_t pos = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    bool pass = false;
+    int32_t pi = 0, j = 0;
+    if (pos < P * T) {
+        pi = (int32_t)(pos / T);
+        j = (int32_t)(pos - (int64_t)pi * T);
+        const int32_t p = perm[pi];
+        pass = noq[p] != 0 || (int64_t)qsc[(int64_t)p * T + j] >= nullq[j] + thrq[p];
+    }
+    int tot;
+    const int r = sxc::block_rank(pass, s_wave, &tot);
+    if (phase == 0) {
+        if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+        return;
+    }
+    if (pass) {
+        const int64_t o = offsets[blockIdx.x] + r;
+        if (o < cap) {
+            c_pi[o] = pi;
+            c_j[o] = j;
+            c_t[o] = sorted[j];
+        }
+    }
+}
+
+// Entries per profile position of a list sorted by position: per[pi] = the length of
+// pi's run in sorted_pi[0 .. *n_dev), by two binary searches.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_count_sorted(const int64_t* __restrict__ n_dev, int64_t cap, const int32_t* __restrict__ sorted_pi, int64_t P,
+                 int32_t* per) {
+    const int64_t pi = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (pi >= P) return;
+    const int64_t n = *n_dev < cap ? *n_dev : cap;
+    auto lower = [&](int64_t v) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)sorted_pi[mid] < v)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
+    per[pi] = (int32_t)(lower(pi + 1) - lower(pi));
+}
+
+// The MSV test (pipeline.rs:166-169) over a candidate list with its f32 MSV scores usc[i]:
+// the same decision as sxc_msv_pass, and the survivors come out in list order, which is
+// the (pi, j) order sxc_msv_pass produces.  Phase 0 counts per block, phase 1 writes.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_msv_pass_list(int phase, const int64_t* __restrict__ n_dev, const int32_t* __restrict__ c_pi,
+                  const int32_t* __restrict__ c_j, const float* __restrict__ usc, const float* __restrict__ nullsc,
+                  const float* __restrict__ thr, const int32_t* __restrict__ perm, const int32_t* __restrict__ sorted,
+                  int32_t* counts, const int64_t* offsets, int64_t cap, int32_t* s_p, int32_t* s_pi, int32_t* s_t,
+                  float* s_usc) {
+    __shared__ int s_wave[SXC_BLOCK / 64];
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    bool pass = false;
+    int32_t p = 0, pi = 0, t = 0;
+    float u = 0.0f;
+    if (i < *n_dev) {
+        pi = c_pi[i];
+        const int32_t j = c_j[i];
+        p = perm[pi];
+        t = sorted[j];
+        u = usc[i];
+        const float y = (u - nullsc[j]) / SXC_LN2F;
+        pass = y >= thr[p];
+    }
+    int tot;
+    const int r = sxc::block_rank(pass, s_wave, &tot);
+    if (phase == 0) {
+        if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+        return;
+    }
+    if (pass) {
+        const int64_t o = offsets[blockIdx.x] + r;
+        if (o < cap) {
+            s_p[o] = p;
+            s_pi[o] = pi;
+            s_t[o] = t;
+            s_usc[o] = u;
+        }
+    }
+}
+
 // Exclusive scan of n int32 counts into n+1 int64 offsets (out[n] = total), one
 // workgroup of 1024 threads; n may be a device value.
 extern "C" __global__ void __launch_bounds__(1024)
@@ -244,6 +346,12 @@ sxc_res_init(const int64_t* __restrict__ n_dev, int64_t cap, float* res_vit, dou
     res_vit[i] = __builtin_nanf("");
     res_fwd[i] = __builtin_nan("");
     res_stage[i] = 1;
+}
+
+// out = min(*n, cap): the entry count downstream kernels may read from a list whose
+// true length *n may exceed its capacity (the host then grows the lists and runs again).
+extern "C" __global__ void sxc_clamp(const int64_t* __restrict__ n, int64_t cap, int64_t* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = *n < cap ? *n : cap;
 }
 
 // Zero an int32 array (graph-safe: no memset nodes).

@@ -17,13 +17,27 @@
 #define SX_NT 8
 enum { SX_TMM = 0, SX_TMI, SX_TMD, SX_TIM, SX_TII, SX_TDM, SX_TDD, SX_TBM };
 
+// Quantised MSV upper bound (the int16 prefilter of the MSV stage).  Emission scores
+// are int16 in units of 1/SX_Q_EMIT nat, rounded up; the special states N, J, B and
+// the length-model scores are int32 in units of 2^-SX_Q_SHIFT emission units
+// (2^-22 nat), rounded up.  Each rounding adds the largest f32 rounding error of a
+// value of magnitude <= 512 nats, so the integer recurrence bounds the f32 MSV of
+// dp.rs:92-145 from above.
+#define SX_Q_EMIT 1024.0
+#define SX_Q_SHIFT 12
+#define SX_Q_FINE 4194304.0      // SX_Q_EMIT * 2^SX_Q_SHIFT units per nat
+#define SX_Q_SLACK_EMIT 0.03125  // SX_Q_EMIT * 2^-24 * 512
+#define SX_Q_SLACK_FINE 128.0    // SX_Q_FINE * 2^-24 * 512
+#define SX_Q_SAT 0x7fffffff      // result of a pair whose bound saturated: may pass
+
 // Per-target length model: MSV tloop/tmove (f32 logf, dp.rs:99-100), Plan7 multihit
-// loop/move in nats (model.rs:286-289) and as probabilities (XscP).
+// loop/move in nats (model.rs:286-289) and as probabilities (XscP), and the MSV
+// tloop/tmove rounded up in the quantised bound's fine units.
 struct SxLen {
     float msv_tloop, msv_tmove;
     float lloop, lmove;
     float ploop, pmove;
-    float pad0, pad1;
+    int32_t q_tloop, q_tmove;
 };
 
 struct SxProfiles {
@@ -47,7 +61,11 @@ struct SxProfiles {
     const float* gmo;
     const int64_t* goff;
     int32_t gw;
-    int32_t pad;
+    int32_t q_tej;  // quantised MSV bound: the E->J score -ln 2 rounded up in fine units
+    // Quantised MSV bound: emission scores rounded up to int16 (same layout and offsets
+    // as msc; -inf -> -32768) and msv_tbm rounded up in fine units, per profile.
+    const int16_t* qsc;
+    const int32_t* qtbm;
 };
 
 // Models longer than this use the global-emission kernels (64 lanes per pair); the
@@ -55,6 +73,9 @@ struct SxProfiles {
 #define SX_GEM_MIN_M 1024
 #define SX_MAX_M 4096
 #define SX_GEM_WIDTH 4104  // >= SX_MAX_M + 2 + 3, a multiple of 8
+// Longest model of the quantised MSV bound (largest int16-stage capacity minus its padding
+// position); longer models skip the bound and go straight to the f32 MSV.
+#define SX_Q_MAX_M 1023
 
 struct SxTargets {
     const uint8_t* codes;

@@ -446,6 +446,168 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, int ew, 
 }
 
 // ---------------------------------------------------------------------------
+// Quantised MSV upper bound: the int16 prefilter of the MSV stage.  No counterpart in
+// the reference, whose MSV is the f32 recurrence above; this one bounds it from above.
+// ---------------------------------------------------------------------------
+// Two model positions per 32-bit register, as int16 halves: lane j's register r holds
+// position j*Q + 1 + r in the low half and j*Q + 1 + Q/2 + r in the high half, so one
+// packed instruction advances two cells and the k-1 operand of every half is the same
+// half of register r-1 (register 0 takes the previous lane's last high half and its own
+// last low half, one alignbit).  Cells use saturating adds (v_pk_add_i16 clamp): a
+// value pushed below -32768 stays at -32768, which only raises it.  Scores are rounded
+// up on the host (q_emit, q_const) with an allowance for the f32 rounding of each step,
+// so by induction every cell, the row maximum xE, xJ, xN, xB and the result are at
+// least 2^22 (fine units) or 1024 (cells) times their f32 counterparts of dp.rs:92-145.
+// A cell or B entry that reaches the top of the int16 range makes the result
+// SX_Q_SAT (the pair may pass).  Positions past M hold -32768; the first of them can
+// carry a value, which the bound's definition includes (the models use shapes with
+// capacity >= M + 1).
+typedef short sx_v2i16 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(sx_v2i16, a),
+                                                                  __builtin_bit_cast(sx_v2i16, b)));
+}
+__device__ __forceinline__ uint32_t pk_adds16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(sx_v2i16, a),
+                                                                      __builtin_bit_cast(sx_v2i16, b)));
+}
+
+// Lane j of the group receives lane j-1's v; lane 0 receives b (bitwise, as grp_from_prev).
+template <int G>
+__device__ __forceinline__ uint32_t grp_from_prev_u(uint32_t v, uint32_t b) {
+    if constexpr (G == 1) {
+        return b;
+    } else if constexpr (G <= 16) {
+        uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)v, 0x111, 0xF, 0xF, false);
+        if constexpr (G < 16) s = ((lane64() & (G - 1)) == 0) ? b : s;
+        return s;
+    } else {
+        uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)v, 0x138, 0xF, 0xF, false);
+        if constexpr (G == 32) s = ((lane64() & 31) == 0) ? b : s;
+        return s;
+    }
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dppm_i(int src) {
+    return __builtin_amdgcn_mov_dpp(src, CTRL, 0xF, 0xF, true);
+}
+
+// Integer maximum over the G lanes of a group (the butterfly of grp_reduce).
+template <int G>
+__device__ __forceinline__ int grp_max_i(int v) {
+    if constexpr (G >= 2) v = max(v, dppm_i<0xB1>(v));
+    if constexpr (G >= 4) v = max(v, dppm_i<0x4E>(v));
+    if constexpr (G >= 8) v = max(v, dppm_i<0x141>(v));
+    if constexpr (G >= 16) v = max(v, dppm_i<0x140>(v));
+    if constexpr (G == 32) {
+        const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+        const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+        v = lane64() < 32 ? max(a, b) : max(c, d);
+    } else if constexpr (G == 64) {
+        const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+        const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+        v = max(max(a, b), max(c, d));
+    }
+    return v;
+}
+
+// LDS layout of the packed int16 table (dwords).  One-lane groups: residue row x is
+// WID dwords, dword r packing positions r+1 and Q/2+r+1; WID is odd, so the 64 lanes of a
+// wave, each on its own target, read distinct banks unless they read the same address.
+// Groups of 8 or >= 16 lanes: the aligned lane blocks of GeoA over Q/2 dwords per lane.
+template <int Q>
+struct GeoQ1 {
+    static constexpr int H = Q / 2;
+    static constexpr int WID = (H & 1) ? H : H + 1;
+};
+
+template <int Q, int G, int TM>
+constexpr int q_tab_dwords() {
+    if constexpr (tm_aligned<TM>())
+        return GeoA<Q / 2, G>::TAB;
+    else
+        return SX_NCODE * GeoQ1<Q>::WID;
+}
+
+template <int Q, int G, int TM>
+__device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, int tbm, int tej, const uint8_t* seq,
+                                             int L, int tloop, int tmove) {
+    static_assert(Q % 2 == 0, "two positions per register");
+    static_assert(G == 1 || tm_aligned<TM>(), "the int16 table is laid out for one-lane or aligned groups");
+    constexpr int H = Q / 2;
+    const int j = (int)(threadIdx.x & (G - 1));
+    const uint32_t* base;
+    int width;
+    if constexpr (tm_aligned<TM>()) {
+        using GA = GeoA<H, G>;
+        const int c = GA::COPIES == 1 ? 0 : (int)((threadIdx.x / G) & 1);
+        base = tab + c * GA::COPY + j * GA::STRIDE;
+        width = GA::WID;
+    } else {
+        base = tab;
+        width = GeoQ1<Q>::WID;
+    }
+    uint32_t row[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) row[r] = 0x80008000u;
+    // as msv_pair: xC = xJ on every row, and xB = max(xN, xJ) + tmove
+    int xn = 0, xj = -(1 << 30), xb = tmove;
+    int sat = 0;
+    auto step = [&](int x) {
+        const uint32_t* __restrict__ sc = base + x * width;
+        if constexpr (tm_aligned<TM>()) sc = static_cast<const uint32_t*>(__builtin_assume_aligned(sc, 16));
+        int be = (xb + tbm + ((1 << SX_Q_SHIFT) - 1)) >> SX_Q_SHIFT;  // rounded up to cell units
+        sat |= be > 32767 ? 1 : 0;
+        be = min(max(be, -32768), 32767);
+        const uint32_t be2 = ((uint32_t)be & 0xFFFFu) * 0x10001u;
+        const uint32_t pl = grp_from_prev_u<G>(row[H - 1], 0x80000000u);
+        const uint32_t left = __builtin_amdgcn_alignbit(row[H - 1], pl, 16);
+        uint32_t e = 0x80008000u;
+#pragma unroll
+        for (int r = H - 1; r >= 1; r--) {
+            const uint32_t v = pk_adds16(pk_max16(row[r - 1], be2), sc[r]);
+            row[r] = v;
+            e = pk_max16(e, v);
+        }
+        {
+            const uint32_t v = pk_adds16(pk_max16(left, be2), sc[0]);
+            row[0] = v;
+            e = pk_max16(e, v);
+        }
+        int xe = max((int)(short)(e & 0xFFFFu), (int)e >> 16);
+        xe = grp_max_i<G>(xe);
+        sat |= xe >= 32767 ? 1 : 0;
+        xj = max(xj + tloop, xe * (1 << SX_Q_SHIFT) + tej);
+        xn = xn + tloop;
+        xb = max(xn, xj) + tmove;
+    };
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(seq);
+    if constexpr (H < 96) {
+        uint32_t nxt = wp[0];
+        for (int i = 0;;) {
+            const uint32_t w = nxt;
+            nxt = wp[1];
+            ++wp;
+            step((int)(w & 0xFFu));
+            if (++i == L) break;
+            step((int)((w >> 8) & 0xFFu));
+            if (++i == L) break;
+            step((int)((w >> 16) & 0xFFu));
+            if (++i == L) break;
+            step((int)(w >> 24));
+            if (++i == L) break;
+        }
+    } else {
+        FwdStream rs;
+        rs.init(seq);
+        for (int i = 0; i < L; i++) step(rs.next());
+    }
+    return sat ? SX_Q_SAT : xj + tmove;
+}
+
+// ---------------------------------------------------------------------------
 // Viterbi score (dp.rs:152-189)
 // ---------------------------------------------------------------------------
 // MONLY: the profile's M->D and D->D scores are all <= 0, so no D value of a row exceeds
@@ -903,7 +1065,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, int ew
 // ---------------------------------------------------------------------------
 // tile loop: one workgroup = one profile staged in LDS, one target per lane group
 // ---------------------------------------------------------------------------
-enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3 };
+enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3, ST_MSVQ = 4 };
 
 // Floats of a kernel's dynamic LDS: the staged emission rows (none for TM_GEM) and,
 // past MSV, the eight transition rows after them.
@@ -918,10 +1080,25 @@ constexpr int lds_emission_floats() {
 }
 template <int Q, int G, int ST, int TM>
 constexpr int lds_floats() {
-    if constexpr (tm_aligned<TM>())
+    if constexpr (ST == ST_MSVQ)
+        return q_tab_dwords<Q, G, TM>();
+    else if constexpr (tm_aligned<TM>())
         return GeoA<Q, G>::TAB + (ST == ST_MSV ? 0 : SX_NT * GeoA<Q, G>::TWID);
     else
         return lds_emission_floats<Q, G, TM>() + (ST == ST_MSV ? 0 : SX_NT * tm_wid<Q, G, TM>());
+}
+
+// LDS row width recorded in the kernel table (dwords for the int16 stage)
+template <int Q, int G, int ST, int TM>
+constexpr int k_wid() {
+    if constexpr (ST == ST_MSVQ) {
+        if constexpr (tm_aligned<TM>())
+            return GeoA<Q / 2, G>::WID;
+        else
+            return GeoQ1<Q>::WID;
+    } else {
+        return tm_wid<Q, G, TM>();
+    }
 }
 
 struct TileInfo {
@@ -955,7 +1132,7 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
 template <int Q, int G, int STAGE, bool SPEC, int TM>
 __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out,
                                                           SxSpecialsOut SP) {
-    static_assert(STAGE == ST_MSV || STAGE == ST_VIT || !tm_aligned<TM>(),
+    static_assert(STAGE == ST_MSV || STAGE == ST_VIT || STAGE == ST_MSVQ || !tm_aligned<TM>(),
                   "aligned lane blocks are an MSV and Viterbi layout");
     constexpr int WID = tm_wid<Q, G, TM>();
     constexpr int NTAB_E = lds_emission_floats<Q, G, TM>();  // emission rows staged in LDS
@@ -979,7 +1156,36 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
             __syncthreads();
             const int m = P.m[ti.prof];
             const int w = m + 1;
-            if constexpr (TM == TM_GEM) {
+            if constexpr (STAGE == ST_MSVQ) {
+                // packed int16 table: dword (lane j, register r) = positions
+                // j*Q + 1 + r (low half) and j*Q + 1 + Q/2 + r (high half)
+                constexpr int H = Q / 2;
+                const int16_t* src = P.qsc + P.off[ti.prof];
+                uint32_t* qt = reinterpret_cast<uint32_t*>(lds);
+                auto qv = [&](int x, int k) -> uint32_t {
+                    return k < w ? (uint32_t)(uint16_t)src[x * w + k] : 0x8000u;
+                };
+                if constexpr (tm_aligned<TM>()) {
+                    using GA = GeoA<H, G>;
+                    for (int idx = threadIdx.x; idx < GA::TAB; idx += blockDim.x) {
+                        const int r = idx - (idx / GA::COPY) * GA::COPY;
+                        const int x = r / GA::WID, pos = r - x * GA::WID;
+                        const int jj = pos / GA::STRIDE, sl = pos - jj * GA::STRIDE;
+                        uint32_t v = 0x80008000u;
+                        if (x < SX_NCODE && jj < G && sl < H) {
+                            const int k0 = jj * Q + 1 + sl;
+                            v = qv(x, k0) | (qv(x, k0 + H) << 16);
+                        }
+                        qt[idx] = v;
+                    }
+                } else {
+                    constexpr int QW = GeoQ1<Q>::WID;
+                    for (int idx = threadIdx.x; idx < SX_NCODE * QW; idx += blockDim.x) {
+                        const int x = idx / QW, r = idx - x * QW;
+                        qt[idx] = r < H ? (qv(x, r + 1) | (qv(x, r + 1 + H) << 16)) : 0x80008000u;
+                    }
+                }
+            } else if constexpr (TM == TM_GEM) {
                 // emission rows stay in global memory (padded copy)
             } else if constexpr (tm_aligned<TM>()) {
                 const float* src = P.msc + P.off[ti.prof];
@@ -1016,7 +1222,7 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                     lds[idx] = k < w ? src[x * w + k] : 0.0f;
                 }
             }
-            if (STAGE != ST_MSV && !tm_aligned<TM>()) {
+            if (STAGE != ST_MSV && STAGE != ST_MSVQ && !tm_aligned<TM>()) {
                 const float* src = (STAGE == ST_VIT ? P.tsc : P.tp) + P.toff[ti.prof];
                 const float pad = STAGE == ST_VIT ? SX_NEG_INF : 0.0f;
                 for (int idx = threadIdx.x; idx < SX_NT * WID; idx += blockDim.x) {
@@ -1039,11 +1245,17 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
             const uint8_t* seq = T.codes + T.off[tid];
             const SxLen lm = T.lm[tid];
             const int64_t o = ti.out + (ti.by_tid ? (int64_t)tid : (int64_t)gid);
-            if (STAGE == ST_MSV) {
+            if constexpr (STAGE == ST_MSVQ) {
+                int32_t r = INT32_MIN;
+                if (L > 0)
+                    r = msvq_pair<Q, G, TM>(reinterpret_cast<const uint32_t*>(lds), P.qtbm[ti.prof], P.q_tej, seq, L,
+                                            lm.q_tloop, lm.q_tmove);
+                if (jl == 0) reinterpret_cast<int32_t*>(out)[o] = r;
+            } else if constexpr (STAGE == ST_MSV) {
                 float r = SX_NEG_INF;
                 if (L > 0) r = msv_pair<Q, G, TM>(em, P.gw, P.msv_tbm[ti.prof], seq, L, lm.msv_tloop, lm.msv_tmove);
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
-            } else if (STAGE == ST_VIT) {
+            } else if constexpr (STAGE == ST_VIT) {
                 float r = SX_NEG_INF;
                 const bool monly = __builtin_amdgcn_readfirstlane(P.vit_m_only[ti.prof]) != 0;  // tile-uniform
                 if constexpr (tm_aligned<TM>()) {
@@ -1058,7 +1270,7 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                                   : vit_pair<Q, G, TM, false>(em, P.gw, tr, seq, L, lm.lloop, lm.lmove);
                 }
                 if (jl == 0) reinterpret_cast<float*>(out)[o] = r;
-            } else if (STAGE == ST_FWD) {
+            } else if constexpr (STAGE == ST_FWD) {
                 double r = -(double)__builtin_inf();
                 float* x5 = nullptr;
                 double* lsr = nullptr;
@@ -1106,7 +1318,7 @@ struct SxKernelEntry {
 };
 
 #define SXK_E(ST, Q, G, SP, H) \
-    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::tm_wid<Q, G, (int)(H)>(), sxk::tm_block<(int)(H)>(), \
+    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::k_wid<Q, G, ST, (int)(H)>(), sxk::tm_block<(int)(H)>(), \
      4 * sxk::lds_floats<Q, G, ST, (int)(H)>(), \
      reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, (int)(H)>)}
 
@@ -1130,6 +1342,7 @@ struct SxKernelEntry {
         SXK_E(ST, 48, 64, true, 2), SXK_E(ST, 64, 64, true, 2)
 
 #define SXK_ALN(G, Q) SXK_E(0, Q, G, false, sxk::TM_ALN), SXK_E(0, Q, G, false, sxk::TM_ALN1K)
+#define SXK_QALN(G, Q) SXK_E(4, Q, G, false, sxk::TM_ALN), SXK_E(4, Q, G, false, sxk::TM_ALN1K)
 
 static const SxKernelEntry k_table[] = {
     // MSV: one f32 row per lane
@@ -1169,6 +1382,14 @@ static const SxKernelEntry k_table[] = {
     // long models (M > 1024): 64 lanes per pair, emission rows from global memory
     SXK_GEM(0), SXK_GEM(1), SXK_GEM(2), SXK_GEM(3),
     SXK_GEMS(2), SXK_GEMS(3),
+    // quantised MSV bound (stage 4): one-lane groups, and aligned lane blocks of 8, 16, 32
+    SXK_E(4, 16, 1, false, 0), SXK_E(4, 24, 1, false, 0), SXK_E(4, 32, 1, false, 0), SXK_E(4, 48, 1, false, 0),
+    SXK_E(4, 64, 1, false, 0), SXK_E(4, 80, 1, false, 0), SXK_E(4, 96, 1, false, 0), SXK_E(4, 128, 1, false, 0),
+    SXK_E(4, 160, 1, false, 0), SXK_E(4, 192, 1, false, 0), SXK_E(4, 256, 1, false, 0),
+    SXK_QALN(8, 16), SXK_QALN(8, 24), SXK_QALN(8, 32), SXK_QALN(8, 40), SXK_QALN(8, 48), SXK_QALN(8, 56),
+    SXK_QALN(8, 64), SXK_QALN(8, 80), SXK_E(4, 96, 8, false, sxk::TM_ALN), SXK_E(4, 128, 8, false, sxk::TM_ALN),
+    SXK_QALN(16, 16), SXK_QALN(16, 24), SXK_QALN(16, 32), SXK_QALN(16, 40), SXK_QALN(16, 48), SXK_QALN(16, 64),
+    SXK_QALN(32, 16), SXK_QALN(32, 24), SXK_QALN(32, 32),
 };
 
 extern "C" const SxKernelEntry* sx_kernel_table(int* n) {

@@ -72,6 +72,13 @@ float logsum(float a, float b);  // dp.rs:32-40
 // MSV length model in f32 (dp.rs:98-100).
 void msv_length(size_t l, float* tloop, float* tmove);
 
+// Quantisation of the int16 MSV bound (no reference counterpart; the bound is a
+// conservative prefilter of dp.rs:92-145): an emission score rounded up to int16
+// units (-inf -> -32768, clamped to the int16 range), and a special-state score
+// rounded up to int32 fine units.
+int16_t q_emit(float sc);
+int32_t q_const(float v);
+
 // HMMER3/b..f reader (parse.rs:42-298): plain text or gzip by magic; throws
 // std::runtime_error with the reference's messages on malformed input.
 class HmmReader {

@@ -110,6 +110,14 @@ __global__ void sxc_fwd_pass(const int64_t*, const int32_t*, const double*, cons
                              const float*, double*, int32_t*);
 __global__ void sxc_res_init(const int64_t*, int64_t, float*, double*, int32_t*);
 __global__ void sxc_zero_i32(int32_t*, int64_t);
+__global__ void sxc_msvq_cand(int, const int32_t*, const int64_t*, const int64_t*, const uint8_t*, const int32_t*,
+                              const int32_t*, int64_t, int64_t, int32_t*, const int64_t*, int64_t, int32_t*, int32_t*,
+                              int32_t*);
+__global__ void sxc_count_sorted(const int64_t*, int64_t, const int32_t*, int64_t, int32_t*);
+__global__ void sxc_msv_pass_list(int, const int64_t*, const int32_t*, const int32_t*, const float*, const float*,
+                                  const float*, const int32_t*, const int32_t*, int32_t*, const int64_t*, int64_t,
+                                  int32_t*, int32_t*, int32_t*, float*);
+__global__ void sxc_clamp(const int64_t*, int64_t, int64_t*);
 __global__ void sxc_records(const int64_t*, int64_t, const int32_t*, const int32_t*, const float*, const float*,
                             const float*, const double*, const int32_t*, void*);
 }
@@ -134,6 +142,7 @@ struct CascadeState {
     size_t nprof = 0;
     uint32_t ntarg = 0;
     double f1 = -1, f2 = -1, f3 = -1;
+    int bound = -1;  // the plan runs the int16 MSV bound before the f32 MSV
     int64_t cap1 = 0;
     // records of the last run (one per MSV survivor), kept for sx_cascade_records
     std::vector<sx_pair_scores> last;
@@ -141,8 +150,13 @@ struct CascadeState {
     std::vector<float> thr_msv, thr_vit, thr_fwd, nullsc;
     std::vector<int32_t> perm;           // profile ids by increasing model length
     int threshold_violations = 0;
+    // the int16 MSV bound's thresholds in its units (per length-sorted target, per
+    // profile) and the profiles it skips (every pair a candidate)
+    std::vector<int64_t> nullq, thrq;
+    std::vector<uint8_t> noq;
     // device tables
     DeviceBuf d_thr_msv, d_thr_vit, d_thr_fwd, d_nullsc, d_perm, d_eo1, d_ng_vit, d_ng_fwd;
+    DeviceBuf d_nullq, d_thrq, d_noq, d_ng_msv;
     std::vector<DeviceBuf> msv_plists;
     struct Group {
         Shape s;
@@ -150,18 +164,25 @@ struct CascadeState {
         int n;
         uint64_t cells;  // sum of M * target length over the group's pairs
     };
-    std::vector<Group> msv_groups;
+    std::vector<Group> msv_groups;  // f32 MSV over all pairs, or the int16 bound's groups
     struct Run {
         Shape s;
         int64_t a, b;  // profile positions [a, b)
     };
     std::vector<Run> vit_runs, fwd_runs;
+    std::vector<Run> msv_runs;  // f32 MSV over the bound's candidates
     // device work buffers
     DeviceBuf usc, cnt1, off1, s_p, s_pi, s_t, s_usc, s_bias, flag, res_vit, res_fwd, res_stage;
     DeviceBuf cnt2, off2, list2, per2, tcount, toff2, eoff2, tiles2, tids2, vsc, flag3;
     DeviceBuf cnt3, off3, list3, per3, toff3, eoff3, tiles3, tids3, fsc, recs;
+    // the bound's candidates (pi, j, target id), their f32 MSV scores and tiles; clamped
+    // list counts (survivors, candidates) for the kernels that read the lists
+    DeviceBuf c_pi, c_j, c_t, cusc, cntq, offq, perq, toffq, eoffq, tilesq, nclamp;
+    const int64_t* d_n1 = nullptr;     // raw survivor count (may exceed cap1)
+    const int64_t* d_ncand = nullptr;  // raw candidate count
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t evq = nullptr;  // after the int16 bound's launches (eager runs)
     // an event before (lev0) and after (lev) every kernel launch of the three DP stages,
     // on the launch's own stream (eager runs only)
     std::vector<hipEvent_t> lev0, lev;
@@ -170,6 +191,7 @@ struct CascadeState {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (evq) (void)hipEventDestroy(evq);
         for (auto& e : lev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : lev0)
@@ -179,7 +201,7 @@ struct CascadeState {
 
 void cascade_release(Ctx& c) { c.cascade.reset(); }
 
-static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3) {
+static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3, int bound) {
     const size_t P = c.profs.size();
     const uint32_t T = c.nt;
     for (const auto& p : c.profs)
@@ -191,6 +213,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     S.f1 = f1;
     S.f2 = f2;
     S.f3 = f3;
+    S.bound = bound;
     S.thr_msv.resize(P);
     S.thr_vit.resize(P);
     S.thr_fwd.resize(P);
@@ -217,14 +240,17 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     S.perm.resize(P);
     std::iota(S.perm.begin(), S.perm.end(), 0);
     std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) { return c.profs[a].m < c.profs[b].m; });
-    std::vector<int32_t> ngv(P), ngf(P);
+    std::vector<int32_t> ngv(P), ngf(P), ngm(P);
     S.vit_runs.clear();
     S.fwd_runs.clear();
+    S.msv_runs.clear();
     for (size_t i = 0; i < P; i++) {
         const int m = c.profs[S.perm[i]].m;
         const Shape sv = c.shape_for(1, SX_FORM_BATCHED, m), sf = c.shape_for(2, SX_FORM_BATCHED, m);
+        const Shape sm = c.shape_for(0, SX_FORM_BATCHED, m);
         ngv[i] = c.kernel(1, sv.q, sv.g, false, sv.hoist).block / sv.g;
         ngf[i] = c.kernel(2, sf.q, sf.g, false, sf.hoist).block / sf.g;
+        ngm[i] = c.kernel(0, sm.q, sm.g, false, sm.hoist).block / sm.g;
         auto add = [&](std::vector<CascadeState::Run>& runs, Shape s) {
             if (!runs.empty() && runs.back().s.q == s.q && runs.back().s.g == s.g && runs.back().s.hoist == s.hoist)
                 runs.back().b = (int64_t)i + 1;
@@ -233,11 +259,39 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
         };
         add(S.vit_runs, sv);
         add(S.fwd_runs, sf);
+        if (bound) add(S.msv_runs, sm);
     }
-    // MSV groups (all pairs, implicit tiles)
+    // The int16 bound's thresholds.  The f32 test passes u only if
+    // fl(fl(u - null) / ln2) >= thr, so u >= null + ln2 * thr - delta, where delta = 2^-21
+    // (|null| + |ln2 * thr|) covers the two roundings; 1e-6 (1 + |.|) per term exceeds it.
+    // Floors of the two scaled terms sum to at most the scaled bound.
+    S.noq.assign(P, 0);
+    S.thrq.assign(P, 0);
+    S.nullq.assign(T, 0);
+    if (bound) {
+        const double F = SX_Q_FINE;
+        for (size_t p = 0; p < P; p++) {
+            const float thr = S.thr_msv[p];
+            if (c.profs[p].m > SX_Q_MAX_M || !std::isfinite(thr)) {
+                S.noq[p] = 1;  // beyond the bound's shapes, or no threshold: all candidates
+                if (thr == INFINITY) S.noq[p] = 0, S.thrq[p] = INT64_MAX / 4;  // nothing passes
+                continue;
+            }
+            const double ct = (double)LN2F * (double)thr;
+            S.thrq[p] = (int64_t)std::floor(F * ct - F * 1e-6 * (1.0 + std::fabs(ct)));
+        }
+        for (uint32_t j = 0; j < T; j++) {
+            const double n = (double)S.nullsc[c.sorted[j]];
+            S.nullq[j] = (int64_t)std::floor(F * n - F * 1e-6 * (1.0 + std::fabs(n)));
+        }
+    }
+    // MSV groups (all pairs, implicit tiles): f32 MSV, or the int16 bound (stage 4) for
+    // the profiles it covers
     std::map<std::tuple<int, int, int>, std::vector<int32_t>> g;
     for (size_t p = 0; p < P; p++) {
-        const Shape s = c.shape_for(0, SX_FORM_BATCHED, c.profs[p].m);
+        if (bound && S.noq[p]) continue;
+        if (bound && S.thrq[p] == INT64_MAX / 4) continue;
+        const Shape s = c.shape_for(bound ? SX_STAGE_MSV16 : 0, SX_FORM_BATCHED, c.profs[p].m);
         g[{s.q, s.g, s.hoist}].push_back((int32_t)p);
     }
     S.msv_plists.clear();
@@ -269,9 +323,14 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3)
     up(S.d_eo1, eo1.data(), eo1.size() * 8);
     up(S.d_ng_vit, ngv.data(), P * 4);
     up(S.d_ng_fwd, ngf.data(), P * 4);
+    up(S.d_ng_msv, ngm.data(), P * 4);
+    up(S.d_nullq, S.nullq.data(), (size_t)T * 8);
+    up(S.d_thrq, S.thrq.data(), P * 8);
+    up(S.d_noq, S.noq.data(), P);
     for (auto& e : S.ev)
         if (!e) HIPCHK(hipEventCreate(&e));
-    const size_t nl = S.msv_groups.size() + S.vit_runs.size() + S.fwd_runs.size();
+    if (!S.evq) HIPCHK(hipEventCreate(&S.evq));
+    const size_t nl = S.msv_groups.size() + S.msv_runs.size() + S.vit_runs.size() + S.fwd_runs.size();
     while (S.lev.size() < nl) {
         S.lev.push_back(nullptr);
         HIPCHK(hipEventCreate(&S.lev.back()));
@@ -285,9 +344,19 @@ static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
     const int64_t P = (int64_t)c.profs.size(), T = c.nt;
     S.cap1 = cap1;
     const int64_t nb1 = nblk(P * T), nbc = nblk(cap1);
-    dptr<float>(S.usc, (size_t)(P * T));
-    dptr<int32_t>(S.cnt1, nb1);
-    dptr<int64_t>(S.off1, nb1 + 1);
+    dptr<float>(S.usc, (size_t)(P * T));  // f32 MSV scores, or the int16 bound's int32 results
+    dptr<int32_t>(S.cnt1, std::max(nb1, nbc));
+    dptr<int64_t>(S.off1, std::max(nb1, nbc) + 1);
+    dptr<int64_t>(S.nclamp, 2);
+    if (S.bound) {
+        for (DeviceBuf* b : {&S.c_pi, &S.c_j, &S.c_t}) dptr<int32_t>(*b, cap1);
+        dptr<float>(S.cusc, cap1);
+        dptr<int32_t>(S.cntq, nb1);
+        dptr<int64_t>(S.offq, nb1 + 1);
+        dptr<int32_t>(S.perq, P);
+        for (DeviceBuf* b : {&S.toffq, &S.eoffq}) dptr<int64_t>(*b, P + 1);
+        dptr<SxTile>(S.tilesq, cap1 + P);
+    }
     dptr<int32_t>(S.s_p, cap1);
     dptr<int32_t>(S.s_pi, cap1);
     dptr<int32_t>(S.s_t, cap1);
@@ -321,60 +390,12 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
     hipStream_t st = c.stream;
     SxSpecialsOut nosp{};
     if (with_events) HIPCHK(hipEventRecord(S.ev[0], st));
-    // 1. MSV over all pairs
     int slot = 0;
-    for (auto& gr : S.msv_groups) {
-        const KernelInfo& k = c.kernel(0, gr.s.q, gr.s.g, false, gr.s.hoist);
-        SxWork w{};
-        w.profs = gr.plist;
-        w.tids = c.d_sorted;
-        w.chunk = k.block / gr.s.g;
-        w.ntids = (int32_t)T;
-        w.nchunks = (int32_t)((T + w.chunk - 1) / w.chunk);
-        w.ntiles = (int64_t)w.nchunks * gr.n;
-        w.out_stride = T;
-        w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
-        const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), w.ntiles);
-        // MSV launches run one after the other: each fills the device on its own
-        if (with_events) HIPCHK(hipEventRecord(S.lev0[slot], st));
-        c.launch(k, 0, w, S.usc.p, nosp, grid, true, slot % 32);
-        if (with_events) HIPCHK(hipEventRecord(S.lev[slot], st));
-        slot++;
-    }
-    if (with_events) HIPCHK(hipEventRecord(S.ev[1], st));
-    // 2. MSV test and ordered compaction
-    const float* usc = (const float*)S.usc.p;
+    int lslot = 0;
     int32_t *s_p = (int32_t*)S.s_p.p, *s_pi = (int32_t*)S.s_pi.p, *s_t = (int32_t*)S.s_t.p;
     float *s_usc = (float*)S.s_usc.p, *s_bias = (float*)S.s_bias.p;
-    hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 0, usc, (const float*)S.d_nullsc.p,
-                       (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
-                       (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
-    hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cnt1.p, nb1, (const int64_t*)nullptr,
-                       (int64_t*)S.off1.p);
-    hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 1, usc, (const float*)S.d_nullsc.p,
-                       (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
-                       (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
-    const int64_t* n1 = (const int64_t*)S.off1.p + nb1;
-    // 3. bias filter and its test
-    hipLaunchKernelGGL(sxc_res_init, dim3(nbc), dim3(CB), 0, st, n1, cap, (float*)S.res_vit.p, (double*)S.res_fwd.p,
-                       (int32_t*)S.res_stage.p);
-    hipLaunchKernelGGL(sxc_bias, dim3(nbc), dim3(CB), 0, st, n1, (const int32_t*)s_p, (const int32_t*)s_t,
-                       (const float*)s_usc, (const double*)S.d_eo1.p, (const int32_t*)c.d_m,
-                       (const float*)S.d_thr_msv.p, c.dtarg(), s_bias, (uint8_t*)S.flag.p);
-    // 4. survivors -> Viterbi tiles
-    auto compact = [&](const uint8_t* flag, const int64_t* n_in, const int32_t* src, DeviceBuf& cnt, DeviceBuf& off,
-                       DeviceBuf& list, DeviceBuf& per) {
-        hipLaunchKernelGGL(sxc_zero_i32, dim3(nbp), dim3(CB), 0, st, (int32_t*)per.p, P);
-        hipLaunchKernelGGL(sxc_compact, dim3(nbc), dim3(CB), 0, st, 0, flag, n_in, src, (const int32_t*)s_pi,
-                           (int32_t*)cnt.p, (const int64_t*)off.p, (int32_t*)list.p, (int32_t*)per.p);
-        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)cnt.p, nbc, (const int64_t*)nullptr,
-                           (int64_t*)off.p);
-        hipLaunchKernelGGL(sxc_compact, dim3(nbc), dim3(CB), 0, st, 1, flag, n_in, src, (const int32_t*)s_pi,
-                           (int32_t*)cnt.p, (const int64_t*)off.p, (int32_t*)list.p, (int32_t*)per.p);
-        return (const int64_t*)off.p + nbc;
-    };
     auto tiles = [&](DeviceBuf& per, DeviceBuf& ng, DeviceBuf& toff, DeviceBuf& eoff, DeviceBuf& tl,
-                     const int64_t* n_list, DeviceBuf& list, DeviceBuf& tids) {
+                     const int64_t* n_list, DeviceBuf* list, DeviceBuf* tids) {
         hipLaunchKernelGGL(sxc_tiles, dim3(nbp), dim3(CB), 0, st, 0, P, (const int32_t*)per.p,
                            (const int32_t*)ng.p, (const int32_t*)S.d_perm.p, (int32_t*)S.tcount.p,
                            (const int64_t*)toff.p, (const int64_t*)eoff.p, (SxTile*)tl.p);
@@ -385,11 +406,11 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         hipLaunchKernelGGL(sxc_tiles, dim3(nbp), dim3(CB), 0, st, 1, P, (const int32_t*)per.p,
                            (const int32_t*)ng.p, (const int32_t*)S.d_perm.p, (int32_t*)S.tcount.p,
                            (const int64_t*)toff.p, (const int64_t*)eoff.p, (SxTile*)tl.p);
-        hipLaunchKernelGGL(sxc_gather_t, dim3(nbc), dim3(CB), 0, st, n_list, (const int32_t*)list.p,
-                           (const int32_t*)s_t, (int32_t*)tids.p);
+        if (list)
+            hipLaunchKernelGGL(sxc_gather_t, dim3(nbc), dim3(CB), 0, st, n_list, (const int32_t*)list->p,
+                               (const int32_t*)s_t, (int32_t*)tids->p);
     };
-    int lslot = (int)S.msv_groups.size();
-    // A stage's runs cover disjoint survivor tiles: they go out side by side on the side
+    // A stage's runs cover disjoint tiles of a list: they go out side by side on the side
     // streams (the small runs of a stage no longer wait for the large ones), each with its
     // own head slot (slots repeat only every 32 launches, on the same side stream).
     auto stage_runs = [&](int stage, std::vector<CascadeState::Run>& runs, DeviceBuf& tl, DeviceBuf& tids,
@@ -412,8 +433,99 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         }
         fo.join();
     };
+    // 1. MSV over all pairs (the f32 scores, or the int16 bound of each pair)
+    const int msv_stage = S.bound ? SX_STAGE_MSV16 : 0;
+    for (auto& gr : S.msv_groups) {
+        const KernelInfo& k = c.kernel(msv_stage, gr.s.q, gr.s.g, false, gr.s.hoist);
+        SxWork w{};
+        w.profs = gr.plist;
+        w.tids = c.d_sorted;
+        w.chunk = k.block / gr.s.g;
+        w.ntids = (int32_t)T;
+        w.nchunks = (int32_t)((T + w.chunk - 1) / w.chunk);
+        w.ntiles = (int64_t)w.nchunks * gr.n;
+        w.out_stride = T;
+        w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
+        const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), w.ntiles);
+        // MSV launches run one after the other: each fills the device on its own
+        if (with_events) HIPCHK(hipEventRecord(S.lev0[lslot], st));
+        c.launch(k, msv_stage, w, S.usc.p, nosp, grid, true, slot % 32);
+        if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], st));
+        slot++;
+        lslot++;
+    }
+    int64_t* nclamp = (int64_t*)S.nclamp.p;
+    const float* usc = (const float*)S.usc.p;
+    if (S.bound) {
+        if (with_events) HIPCHK(hipEventRecord(S.evq, st));
+        // 1b. candidates of the bound, in (profile position, length-sorted target) order
+        const int32_t* qres = (const int32_t*)S.usc.p;
+        int32_t *c_pi = (int32_t*)S.c_pi.p, *c_j = (int32_t*)S.c_j.p, *c_t = (int32_t*)S.c_t.p;
+        for (int phase = 0; phase < 2; phase++) {
+            hipLaunchKernelGGL(sxc_msvq_cand, dim3(nb1), dim3(CB), 0, st, phase, qres,
+                               (const int64_t*)S.d_nullq.p, (const int64_t*)S.d_thrq.p, (const uint8_t*)S.d_noq.p,
+                               (const int32_t*)S.d_perm.p, c.d_sorted, P, T, (int32_t*)S.cntq.p,
+                               (const int64_t*)S.offq.p, cap, c_pi, c_j, c_t);
+            if (phase == 0)
+                hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cntq.p, nb1,
+                                   (const int64_t*)nullptr, (int64_t*)S.offq.p);
+        }
+        S.d_ncand = (const int64_t*)S.offq.p + nb1;
+        hipLaunchKernelGGL(sxc_clamp, dim3(1), dim3(64), 0, st, S.d_ncand, cap, nclamp + 1);
+        // 1c. the f32 MSV of the candidates, tiled per profile as the later stages
+        hipLaunchKernelGGL(sxc_count_sorted, dim3(nbp), dim3(CB), 0, st, (const int64_t*)(nclamp + 1), cap,
+                           (const int32_t*)c_pi, P, (int32_t*)S.perq.p);
+        tiles(S.perq, S.d_ng_msv, S.toffq, S.eoffq, S.tilesq, nclamp + 1, nullptr, nullptr);
+        stage_runs(0, S.msv_runs, S.tilesq, S.c_t, S.toffq, S.cusc.p);
+        if (with_events) HIPCHK(hipEventRecord(S.ev[1], st));
+        // 2. the MSV test over the candidates, in list order
+        for (int phase = 0; phase < 2; phase++) {
+            hipLaunchKernelGGL(sxc_msv_pass_list, dim3(nbc), dim3(CB), 0, st, phase, (const int64_t*)(nclamp + 1),
+                               (const int32_t*)c_pi, (const int32_t*)c_j, (const float*)S.cusc.p,
+                               (const float*)S.d_nullsc.p, (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p,
+                               c.d_sorted, (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
+            if (phase == 0)
+                hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cnt1.p, nbc,
+                                   (const int64_t*)nullptr, (int64_t*)S.off1.p);
+        }
+        S.d_n1 = (const int64_t*)S.off1.p + nbc;
+    } else {
+        S.d_ncand = nullptr;
+        if (with_events) HIPCHK(hipEventRecord(S.ev[1], st));
+        // 2. MSV test and ordered compaction
+        hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 0, usc, (const float*)S.d_nullsc.p,
+                           (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
+                           (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
+        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cnt1.p, nb1,
+                           (const int64_t*)nullptr, (int64_t*)S.off1.p);
+        hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 1, usc, (const float*)S.d_nullsc.p,
+                           (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
+                           (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
+        S.d_n1 = (const int64_t*)S.off1.p + nb1;
+    }
+    // the survivor count the later kernels read: at most the lists' capacity
+    hipLaunchKernelGGL(sxc_clamp, dim3(1), dim3(64), 0, st, S.d_n1, cap, nclamp);
+    const int64_t* n1 = nclamp;
+    // 3. bias filter and its test
+    hipLaunchKernelGGL(sxc_res_init, dim3(nbc), dim3(CB), 0, st, n1, cap, (float*)S.res_vit.p, (double*)S.res_fwd.p,
+                       (int32_t*)S.res_stage.p);
+    hipLaunchKernelGGL(sxc_bias, dim3(nbc), dim3(CB), 0, st, n1, (const int32_t*)s_p, (const int32_t*)s_t,
+                       (const float*)s_usc, (const double*)S.d_eo1.p, (const int32_t*)c.d_m,
+                       (const float*)S.d_thr_msv.p, c.dtarg(), s_bias, (uint8_t*)S.flag.p);
+    // 4. survivors -> Viterbi tiles
+    auto compact = [&](const uint8_t* flag, const int64_t* n_in, const int32_t* src, DeviceBuf& cnt, DeviceBuf& off,
+                       DeviceBuf& list, DeviceBuf& per) {
+        hipLaunchKernelGGL(sxc_zero_i32, dim3(nbp), dim3(CB), 0, st, (int32_t*)per.p, P);
+        hipLaunchKernelGGL(sxc_compact, dim3(nbc), dim3(CB), 0, st, 0, flag, n_in, src, (const int32_t*)s_pi,
+                           (int32_t*)cnt.p, (const int64_t*)off.p, (int32_t*)list.p, (int32_t*)per.p);
+        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)cnt.p, nbc, (const int64_t*)nullptr,
+                           (int64_t*)off.p);
+        hipLaunchKernelGGL(sxc_compact, dim3(nbc), dim3(CB), 0, st, 1, flag, n_in, src, (const int32_t*)s_pi,
+                           (int32_t*)cnt.p, (const int64_t*)off.p, (int32_t*)list.p, (int32_t*)per.p);
+        return (const int64_t*)off.p + nbc;
+    };
     const int64_t* n2 = compact((const uint8_t*)S.flag.p, n1, nullptr, S.cnt2, S.off2, S.list2, S.per2);
-    tiles(S.per2, S.d_ng_vit, S.toff2, S.eoff2, S.tiles2, n2, S.list2, S.tids2);
+    tiles(S.per2, S.d_ng_vit, S.toff2, S.eoff2, S.tiles2, n2, &S.list2, &S.tids2);
     if (with_events) HIPCHK(hipEventRecord(S.ev[2], st));
     // 5. Viterbi and its test
     stage_runs(1, S.vit_runs, S.tiles2, S.tids2, S.toff2, S.vsc.p);
@@ -423,7 +535,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
                        (uint8_t*)S.flag3.p);
     const int64_t* n3 =
         compact((const uint8_t*)S.flag3.p, n2, (const int32_t*)S.list2.p, S.cnt3, S.off3, S.list3, S.per3);
-    tiles(S.per3, S.d_ng_fwd, S.toff3, S.eoff3, S.tiles3, n3, S.list3, S.tids3);
+    tiles(S.per3, S.d_ng_fwd, S.toff3, S.eoff3, S.tiles3, n3, &S.list3, &S.tids3);
     if (with_events) HIPCHK(hipEventRecord(S.ev[3], st));
     // 6. Forward and its test
     stage_runs(2, S.fwd_runs, S.tiles3, S.tids3, S.toff3, S.fsc.p);
@@ -447,11 +559,16 @@ static float ev_ms(hipEvent_t a, hipEvent_t b) {
 static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap) {
     if (!c.cascade) c.cascade = std::make_shared<CascadeState>();
     CascadeState& S = *c.cascade;
+    const int bound = st->msv_bound ? 1 : 0;
     if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
-        c.cascade_dirty) {
-        build_plan(c, S, st->f1, st->f2, st->f3);
+        S.bound != bound || c.cascade_dirty) {
+        build_plan(c, S, st->f1, st->f2, st->f3, bound);
         const int64_t P = (int64_t)c.profs.size(), T = c.nt;
-        alloc_buffers(c, S, std::max<int64_t>(1 << 16, (int64_t)((double)P * T * std::max(st->f1, 1e-4) * 2.0)));
+        // profiles the bound skips make every pair a candidate
+        int64_t nskip = 0;
+        for (uint8_t v : S.noq) nskip += v;
+        alloc_buffers(c, S,
+                      std::max<int64_t>(1 << 16, (int64_t)((double)P * T * std::max(st->f1, 1e-4) * 2.0) + nskip * T));
         c.cascade_dirty = false;
     }
     for (int attempt = 0; attempt < 2; attempt++) {
@@ -476,11 +593,12 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
             HIPCHK(hipEventRecord(S.ev[4], c.stream));
             HIPCHK(hipEventSynchronize(S.ev[4]));
             total = ev_ms(S.ev[0], S.ev[4]);
-            st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = 0.0;
+            st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = st->ms_msv_bound = 0.0;
         } else {
             enqueue_chain(c, S, true);
             HIPCHK(hipEventSynchronize(S.ev[4]));
             st->ms_msv = ev_ms(S.ev[0], S.ev[1]);
+            st->ms_msv_bound = S.bound ? ev_ms(S.ev[0], S.evq) : 0.0;
             st->ms_bias = ev_ms(S.ev[1], S.ev[2]);
             st->ms_vit = ev_ms(S.ev[2], S.ev[3]);
             st->ms_fwd = ev_ms(S.ev[3], S.ev[4]);
@@ -488,13 +606,31 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
         }
         st->ms_total = total;
         const int64_t P = (int64_t)c.profs.size(), T = c.nt;
-        const int64_t nb1 = nblk(P * T), nbc = nblk(S.cap1);
-        int64_t n1 = 0, n2 = 0, n3 = 0;
-        HIPCHK(hipMemcpy(&n1, (int64_t*)S.off1.p + nb1, 8, hipMemcpyDeviceToHost));
-        if (n1 > S.cap1) {
-            // survivor list overflowed its capacity: grow to the exact size and run again
-            alloc_buffers(c, S, n1 + 1024);
+        const int64_t nbc = nblk(S.cap1);
+        int64_t n1 = 0, n2 = 0, n3 = 0, ncand = 0;
+        HIPCHK(hipMemcpy(&n1, S.d_n1, 8, hipMemcpyDeviceToHost));
+        if (S.d_ncand) HIPCHK(hipMemcpy(&ncand, S.d_ncand, 8, hipMemcpyDeviceToHost));
+        if (n1 > S.cap1 || ncand > S.cap1) {
+            // a list overflowed its capacity: grow to the exact size and run again
+            alloc_buffers(c, S, std::max(n1, ncand) + 1024);
             continue;
+        }
+        st->n_msv_cand = (uint64_t)ncand;
+        st->cells_msv_exact = 0;
+        std::vector<uint64_t> cand_run_cells(S.msv_runs.size(), 0);
+        if (S.bound && !st->use_graph && ncand > 0) {
+            // cells the f32 MSV rescored, per run (eager runs only: a copy of the list)
+            std::vector<int32_t> cpi((size_t)ncand), ct((size_t)ncand);
+            HIPCHK(hipMemcpy(cpi.data(), S.c_pi.p, (size_t)ncand * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ct.data(), S.c_t.p, (size_t)ncand * 4, hipMemcpyDeviceToHost));
+            size_t r = 0;
+            for (int64_t i = 0; i < ncand; i++) {
+                while (r + 1 < S.msv_runs.size() && cpi[i] >= S.msv_runs[r].b) r++;
+                const uint64_t cells =
+                    (uint64_t)c.profs[S.perm[cpi[i]]].m * (uint64_t)c.tlen[ct[i]];
+                cand_run_cells[r] += cells;
+                st->cells_msv_exact += cells;
+            }
         }
         HIPCHK(hipMemcpy(&n2, (int64_t*)S.off2.p + nbc, 8, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&n3, (int64_t*)S.off3.p + nbc, 8, hipMemcpyDeviceToHost));
@@ -547,7 +683,9 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
                 S.launches.push_back(l);
                 e++;
             };
-            for (size_t i = 0; i < S.msv_groups.size(); i++) add(0, S.msv_groups[i].s, S.msv_groups[i].cells);
+            for (size_t i = 0; i < S.msv_groups.size(); i++)
+                add(S.bound ? SX_STAGE_MSV16 : 0, S.msv_groups[i].s, S.msv_groups[i].cells);
+            for (size_t i = 0; i < S.msv_runs.size(); i++) add(0, S.msv_runs[i].s, cand_run_cells[i]);
             for (size_t i = 0; i < S.vit_runs.size(); i++) add(1, S.vit_runs[i].s, vc[i]);
             for (size_t i = 0; i < S.fwd_runs.size(); i++) add(2, S.fwd_runs[i].s, fc[i]);
         }
@@ -561,9 +699,10 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
 static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap) {
     if (!c.cascade) c.cascade = std::make_shared<CascadeState>();
     CascadeState& S = *c.cascade;
+    const int bound = st->msv_bound ? 1 : 0;
     if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
-        c.cascade_dirty) {
-        build_plan(c, S, st->f1, st->f2, st->f3);
+        S.bound != bound || c.cascade_dirty) {
+        build_plan(c, S, st->f1, st->f2, st->f3, bound);
         alloc_buffers(c, S, 1 << 16);
         c.cascade_dirty = false;
     }
@@ -576,19 +715,53 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     DeviceBuf d_pp, d_pt, d_pu, d_pb, d_fl, d_n;
     std::vector<double> eo1(25);
     double ms_total = 0.0;
-    uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0;
+    uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0, ncand = 0, cq = 0;
+    // the bound's null thresholds by target id (S.nullq is in length-sorted order)
+    std::vector<int64_t> nullq_t;
+    if (bound) {
+        nullq_t.resize(T);
+        for (uint32_t j = 0; j < T; j++) nullq_t[c.sorted[j]] = S.nullq[j];
+    }
+    std::vector<int32_t> qrow(bound ? T : 0);
     HIPCHK(hipEventRecord(S.ev[0], c.stream));
     for (size_t p = 0; p < c.profs.size(); p++) {
         const Profile& pr = c.profs[p];
-        // MSV: one launch over every target, in input order
         std::vector<int32_t> one{(int32_t)p};
-        c.run_implicit(0, SX_FORM_PLAIN, one, c.d_identity, (int)T, d_u, 0);
-        HIPCHK(hipMemcpyAsync(urow.data(), d_u, (size_t)T * 4, hipMemcpyDeviceToHost, c.stream));
-        HIPCHK(hipStreamSynchronize(c.stream));
         std::vector<uint32_t> sp_t;
-        for (uint32_t t = 0; t < T; t++) {
-            const float y = (urow[t] - S.nullsc[t]) / LN2F;
-            if (y >= S.thr_msv[p]) sp_t.push_back(t);
+        if (bound && !S.noq[p]) {
+            if (S.thrq[p] == INT64_MAX / 4) continue;  // no score passes
+            // the int16 bound: one launch over every target, then the f32 MSV of the
+            // targets whose bound reaches the threshold (one launch for this profile)
+            c.run_implicit(SX_STAGE_MSV16, SX_FORM_PLAIN, one, c.d_identity, (int)T, d_u, 0);
+            HIPCHK(hipMemcpyAsync(qrow.data(), d_u, (size_t)T * 4, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            std::vector<uint32_t> cp, ct;
+            for (uint32_t t = 0; t < T; t++)
+                if ((int64_t)qrow[t] >= nullq_t[t] + S.thrq[p]) {
+                    cp.push_back((uint32_t)p);
+                    ct.push_back(t);
+                }
+            if (ct.empty()) continue;
+            ncand += ct.size();
+            for (uint32_t t : ct) cq += (uint64_t)pr.m * (uint64_t)c.tlen[t];
+            c.run_pairs(0, SX_FORM_PLAIN, cp.data(), ct.data(), ct.size(), d_u, false, nullptr, nullptr, nullptr);
+            std::vector<float> tmp(ct.size());
+            HIPCHK(hipMemcpyAsync(tmp.data(), d_u, ct.size() * 4, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            for (size_t i = 0; i < ct.size(); i++) urow[ct[c.last_order[i]]] = tmp[i];
+            for (uint32_t t : ct) {
+                const float y = (urow[t] - S.nullsc[t]) / LN2F;
+                if (y >= S.thr_msv[p]) sp_t.push_back(t);
+            }
+        } else {
+            // MSV: one launch over every target, in input order
+            c.run_implicit(0, SX_FORM_PLAIN, one, c.d_identity, (int)T, d_u, 0);
+            HIPCHK(hipMemcpyAsync(urow.data(), d_u, (size_t)T * 4, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            for (uint32_t t = 0; t < T; t++) {
+                const float y = (urow[t] - S.nullsc[t]) / LN2F;
+                if (y >= S.thr_msv[p]) sp_t.push_back(t);
+            }
         }
         if (sp_t.empty()) continue;
         n1 += sp_t.size();
@@ -698,6 +871,9 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     st->n_fwd_pass = n4;
     st->cells_vit = cv;
     st->cells_fwd = cf;
+    st->n_msv_cand = ncand;
+    st->cells_msv_exact = cq;
+    st->ms_msv_bound = 0.0;
     if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, recs.size()) * sizeof(sx_pair_scores));
     S.last.swap(recs);
     return S.last.size();
@@ -713,6 +889,7 @@ std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1,
     st.f2 = f2;
     st.f3 = f3;
     st.use_graph = 1;
+    st.msv_bound = 1;
     c.ensure_pidx();
     c.cascade_pass_uncalibrated = true;
     c.cascade_dirty = true;

@@ -128,6 +128,7 @@ Ctx::~Ctx() {
 void Ctx::free_profiles() {
     dfree(d_msc); dfree(d_mo); dfree(d_tsc); dfree(d_tp);
     dfree(d_off); dfree(d_toff); dfree(d_m); dfree(d_tbm); dfree(d_vmo);
+    dfree(d_qsc); dfree(d_qtbm);
     dfree(d_gmsc); dfree(d_gmo); dfree(d_goff);
 }
 
@@ -173,7 +174,8 @@ void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n)
         s.lmove = x.n_move;
         s.ploop = x.pn_loop;
         s.pmove = x.pn_move;
-        s.pad0 = s.pad1 = 0.0f;
+        s.q_tloop = q_const(s.msv_tloop);
+        s.q_tmove = q_const(s.msv_tmove);
     }
     pos += 64;  // back pad: prefetch of the chunk after a target's last one
     std::vector<uint8_t> packed(pos, 0);
@@ -214,7 +216,8 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     profs = std::move(ps);
     std::vector<float> msc, mo, tsc, tp, tbm;
     std::vector<int64_t> off, toff;
-    std::vector<int32_t> ms, vmo;
+    std::vector<int32_t> ms, vmo, qtbm;
+    std::vector<int16_t> qsc;
     for (const auto& p : profs) {
         // D(k) = max(M(k-1) + tMD, D(k-1) + tDD) with D(1) = -inf: when no tMD, tDD is
         // positive, every D value is at most an earlier M value of the same row (f32
@@ -230,6 +233,8 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
         toff.push_back((int64_t)tsc.size());
         ms.push_back(p.m);
         tbm.push_back(p.msv_tbm);
+        qtbm.push_back(q_const(p.msv_tbm));
+        for (float v : p.msc) qsc.push_back(q_emit(v));
         msc.insert(msc.end(), p.msc.begin(), p.msc.end());
         mo.insert(mo.end(), p.mo.begin(), p.mo.end());
         tsc.insert(tsc.end(), p.tsc.begin(), p.tsc.end());
@@ -244,6 +249,8 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     d_m = dupload(ms);
     d_tbm = dupload(tbm);
     d_vmo = dupload(vmo);
+    d_qsc = dupload(qsc);
+    d_qtbm = dupload(qtbm);
     // padded emission rows of the long models (read from global memory by their kernels)
     std::vector<float> gmsc, gmo;
     std::vector<int64_t> goff(profs.size(), -1);
@@ -271,7 +278,8 @@ SxProfiles Ctx::dprof() const {
     SxProfiles P;
     P.msc = d_msc; P.mo = d_mo; P.tsc = d_tsc; P.tp = d_tp;
     P.off = d_off; P.toff = d_toff; P.m = d_m; P.msv_tbm = d_tbm; P.vit_m_only = d_vmo;
-    P.gmsc = d_gmsc; P.gmo = d_gmo; P.goff = d_goff; P.gw = SX_GEM_WIDTH; P.pad = 0;
+    P.gmsc = d_gmsc; P.gmo = d_gmo; P.goff = d_goff; P.gw = SX_GEM_WIDTH; P.q_tej = q_const(-LN2F);
+    P.qsc = d_qsc; P.qtbm = d_qtbm;
     return P;
 }
 
@@ -299,6 +307,20 @@ Shape Ctx::gem_shape(int m) {
 }
 
 Shape Ctx::default_shape(int stage, int form, int m) const {
+    if (stage == SX_STAGE_MSV16) {
+        // the quantised MSV bound: one lane per pair up to 127 positions, then 8- and
+        // 16-lane groups over aligned lane blocks (capacity >= M + 1: the bound
+        // includes the first padding position)
+        if (m > SX_Q_MAX_M)
+            throw SxError(SX_ERR_ARG, "model length " + std::to_string(m) + " exceeds the quantised MSV bound's " +
+                                          std::to_string(SX_Q_MAX_M));
+        for (int q : {16, 24, 32, 48, 64, 80, 96, 128})
+            if (q >= m + 1) return {q, 1, 0};
+        for (int q : {24, 32, 40, 48, 56, 64})
+            if (8 * q >= m + 1) return {q, 8, 3};
+        for (int q : {48, 64})
+            if (16 * q >= m + 1) return {q, 16, 3};
+    }
     if (m > SX_GEM_MIN_M) return gem_shape(m);
     if (form == SX_FORM_PLAIN) {
         for (int q : {4, 8, 16})
@@ -317,7 +339,7 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
         const KernelInfo& k = kv.second;
         if (k.stage != stage || k.spec || k.hoist >= 2) continue;
         const long cap = (long)k.q * k.g;
-        if (cap < m) continue;
+        if (cap < m + cap_extra(stage)) continue;
         if (lds_bytes(k) > LDS_MAX) continue;
         if (stage != 0 && k.q > 16 && cap < 1024) continue;
         const bool better = cap < best_cap ||
@@ -332,7 +354,8 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
 }
 
 Shape Ctx::shape_for(int stage, int form, int m) const {
-    if (m > SX_GEM_MIN_M) return gem_shape(m);  // long models: one fixed shape, never tuned
+    if (stage == SX_STAGE_MSV16 && m > SX_Q_MAX_M) return default_shape(stage, form, m);  // throws
+    if (stage != SX_STAGE_MSV16 && m > SX_GEM_MIN_M) return gem_shape(m);  // long models: one fixed shape
     if (form == SX_FORM_BATCHED) {
         if (forced.count(stage)) return forced.at(stage);
         auto it = tuned.find(stage);
@@ -623,15 +646,16 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             }
         lo = hi;
         if (inb.empty()) continue;
-        if (mmax > SX_GEM_MIN_M) break;  // long models have one fixed shape
+        if (mmax > (stage == SX_STAGE_MSV16 ? SX_Q_MAX_M : SX_GEM_MIN_M)) break;  // no tuned shape beyond
         // profiles spread over the bucket, repeated if the bucket is small: about 4e10
         // cells (up to 16 profiles) for the later stages; three times that (up to 64) for
         // MSV, whose one-lane shapes run a whole pair per lane, so that a tile lasts long
         // enough for the persistent grid's drain tail to swing a short sample's time
         std::vector<int32_t> sample;
-        const double cells_want = stage == 0 ? 1.2e11 : 4e10;
+        const bool msv_like = stage == 0 || stage == SX_STAGE_MSV16;
+        const double cells_want = msv_like ? 1.2e11 : 4e10;
         const int want = (int)std::clamp<int64_t>((int64_t)(cells_want / ((double)mmax * (double)sum_l)), 1,
-                                                  stage == 0 ? 64 : 16);
+                                                  msv_like ? 64 : 16);
         for (int i = 0; i < want; i++) sample.push_back(inb[(size_t)((double)i * inb.size() / want)]);
         int64_t cells = 0;
         for (int32_t p : sample) cells += (int64_t)profs[p].m * sum_l;
@@ -644,7 +668,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         for (const auto& kv : kernels) {
             const KernelInfo& k = kv.second;
             if (k.stage != stage || k.spec || k.hoist == 2) continue;
-            if ((long)k.q * k.g < mmax) continue;
+            if ((long)k.q * k.g < mmax + cap_extra(stage)) continue;
             // an MSV shape with an aligned-block twin: the twin reads LDS without bank
             // conflicts and is always the faster of the two
             if (stage == 0 && k.hoist == 0 && kernels.count(key(0, k.q, k.g, 0, 3))) continue;
@@ -693,7 +717,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
                 });
                 double ms = t1;
                 spent += t1;
-                if (stage == 0) {
+                if (msv_like) {
                     // The MSV launches of a search cover thousands of profiles x every
                     // target: rank shapes by the marginal time of a second copy of the
                     // sample, in which the launch, the stream synchronisation and the
@@ -719,7 +743,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             while (pgroups_keep.size() > 64) pgroups_keep.pop_front();
         }
         // a row never claims more than the shape covers; longer models fall through
-        table.push_back({std::min<int>(hi, best.q * best.g), best});
+        table.push_back({std::min<int>(hi, best.q * best.g - cap_extra(stage)), best});
     }
     pgroups_keep.clear();
     tuned[stage] = table;
@@ -959,10 +983,10 @@ extern "C" int sx_profile_tables(const sx_ctx* ctx, uint32_t p, int32_t* m, floa
     return SX_OK;
 }
 
-static size_t elem_size(int stage) { return stage >= SX_STAGE_FORWARD ? 8 : 4; }
+static size_t elem_size(int stage) { return stage == SX_STAGE_FORWARD || stage == SX_STAGE_BACKWARD ? 8 : 4; }
 
 extern "C" int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double* ms) {
-    if (!ctx || stage < 0 || stage > 3 || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED)) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16 || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED)) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         if (c.profs.empty() || c.nt == 0) {
@@ -980,7 +1004,7 @@ extern "C" int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double*
 
 extern "C" int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
                               void* out, double* ms) {
-    if (!ctx || stage < 0 || stage > 3 || (n && (!prof || !tgt))) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16 || (n && (!prof || !tgt))) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         if (n == 0) {
@@ -1008,7 +1032,7 @@ extern "C" int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const u
 
 extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g,
                             int32_t* hoist) {
-    if (!ctx || !q || !g || stage < 0 || stage > 3) return SX_ERR_ARG;
+    if (!ctx || !q || !g || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
     try {
         const sx::Shape s = ctx->c->shape_for(stage, form, m);
         *q = s.q;
@@ -1022,7 +1046,7 @@ extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, i
 
 extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q,
                              const int32_t* g, const int32_t* hoist) {
-    if (!ctx || stage < 0 || stage > 3 || (n && (!max_m || !q || !g))) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16 || (n && (!max_m || !q || !g))) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         std::vector<std::pair<int, sx::Shape>> tab;
@@ -1030,11 +1054,14 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
             const int h = hoist ? hoist[i] : 0;
             if (h < 0 || h > 4 || h == 2)
                 throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (long models have a fixed shape)");
-            if (max_m[i] > SX_GEM_MIN_M)
-                throw SxError(SX_ERR_ARG, "rows cover models up to " + std::to_string(SX_GEM_MIN_M) +
-                                              "; longer models use a fixed shape");
+            const int top = stage == SX_STAGE_MSV16 ? SX_Q_MAX_M : SX_GEM_MIN_M;
+            if (max_m[i] > top)
+                throw SxError(SX_ERR_ARG, "rows cover models up to " + std::to_string(top) +
+                                              (stage == SX_STAGE_MSV16 ? "; the quantised MSV bound has no longer shape"
+                                                                       : "; longer models use a fixed shape"));
             const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false, h);
-            if ((long)q[i] * g[i] < max_m[i]) throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
+            if ((long)q[i] * g[i] < max_m[i] + Ctx::cap_extra(stage))
+                throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
             if (Ctx::lds_bytes(k) > 160 * 1024) throw SxError(SX_ERR_ARG, "shape exceeds the LDS budget");
             if (i && max_m[i] <= max_m[i - 1]) throw SxError(SX_ERR_ARG, "max_m must increase");
             tab.push_back({max_m[i], sx::Shape{q[i], g[i], h}});
@@ -1049,7 +1076,7 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
 
 extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q,
                              int32_t* g, int32_t* hoist) {
-    if (!ctx || !n || stage < 0 || stage > 3) return SX_ERR_ARG;
+    if (!ctx || !n || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         auto it = c.tuned.find(stage);
@@ -1065,13 +1092,13 @@ extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, 
 }
 
 extern "C" int sx_autotune(sx_ctx* ctx, int stage, double budget_ms) {
-    if (!ctx || stage < 0 || stage > 3) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
     return guard(ctx, [&] { ctx->c->last_tune[stage] = ctx->c->autotune(stage, budget_ms); });
 }
 
 extern "C" int sx_autotune_log(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q,
                                int32_t* g, int32_t* hoist, double* gcups) {
-    if (!ctx || !n || stage < 0 || stage > 3) return SX_ERR_ARG;
+    if (!ctx || !n || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
     const auto& L = ctx->c->last_tune[stage];
     *n = (uint32_t)L.size();
     for (uint32_t i = 0; i < *n && i < cap; i++) {

@@ -13,6 +13,7 @@
 
 #include "sx_dev.h"
 #include "sx_model.h"
+#include "../../include/sx_hmmer.h"
 
 namespace sx {
 
@@ -109,6 +110,8 @@ struct Ctx {
     int64_t *d_off = nullptr, *d_toff = nullptr;
     int32_t* d_m = nullptr;
     int32_t* d_vmo = nullptr;  // SxProfiles::vit_m_only
+    int16_t* d_qsc = nullptr;  // SxProfiles::qsc, qtbm (quantised MSV bound)
+    int32_t* d_qtbm = nullptr;
     int32_t* d_pidx = nullptr;
     size_t d_pidx_n = 0;
     void set_profiles(std::vector<Profile>&& ps);
@@ -138,7 +141,10 @@ struct Ctx {
     std::map<int, std::vector<std::pair<int, Shape>>> tuned;
     std::map<int, Shape> forced;  // autotuner: every model of the stage uses this shape
     std::vector<TuneRecord> autotune(int stage, double budget_ms);
-    std::vector<TuneRecord> last_tune[4];
+    std::vector<TuneRecord> last_tune[5];
+    // positions a shape must hold beyond M: the quantised MSV bound includes the first
+    // padding position
+    static int cap_extra(int stage) { return stage == SX_STAGE_MSV16 ? 1 : 0; }
     Shape default_shape(int stage, int form, int m) const;
     Shape shape_for(int stage, int form, int m) const;
     Shape shape_for_spec(int stage, int m) const;

@@ -49,8 +49,12 @@ def _check_aligned_layout(e):
     lanes a cycle from 64 four-byte banks, so the 16 lanes of every quarter-wave must start
     on 16 distinct four-bank slots whatever residue row each group reads.  Viterbi's
     transition rows (one copy, the same addresses for every group) follow."""
-    assert e.stage in (0, 1) and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
-    stride = e.q + 4 if e.q % 8 == 0 else e.q + 8
+    assert e.stage in (0, 1, 4) and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
+    # a lane block holds Q floats, or for the int16 MSV bound (stage 4) Q/2 dwords of two
+    # int16 positions each
+    per_lane = e.q // 2 if e.stage == 4 else e.q
+    assert per_lane % 4 == 0
+    stride = per_lane + 4 if per_lane % 8 == 0 else per_lane + 8
     copies = 1 if e.g >= 16 else 2
     skew = 0 if copies == 1 else 32
     assert (stride // 4) % 2 == 1 and e.wid >= e.g * stride
@@ -96,7 +100,12 @@ def test_kernel_table_names_each_shape_once(sxa):
         elif e.hoist == 2:
             assert e.lds == (0 if e.stage == 0 else 4 * 8 * e.wid)
         else:
-            assert e.lds == 4 * (25 if e.stage == 0 else 33) * e.wid
+            assert e.lds == 4 * (25 if e.stage in (0, 4) else 33) * e.wid
+        if e.stage == 4:
+            # the int16 bound's one-lane rows are odd (distinct banks for distinct rows)
+            assert e.hoist in (0, 3, 4) and e.spec == 0
+            if e.hoist == 0:
+                assert e.g == 1 and e.wid % 2 == 1 and e.wid >= e.q // 2
     long_rows = {r for r in rows if r[4] == 2}
     for st in (0, 1, 2, 3):
         for q in (24, 32, 40, 48, 64):

@@ -110,21 +110,55 @@ def test_records_fetch_matches_truncated_run(gctx, calibrated):
     assert same(gctx.cascade_records(), plain)
 
 
-def test_launch_timing_covers_each_stage(gctx, calibrated):
+@pytest.mark.parametrize("bound", [False, True], ids=["f32", "int16-bound"])
+def test_launch_timing_covers_each_stage(gctx, calibrated, bound):
     hmms, _, seqs = calibrated
     gctx.set_profiles(hmms)
     gctx.set_targets(seqs)
-    st, _ = gctx.run_cascade(1, use_graph=False, **F)
+    st, _ = gctx.run_cascade(1, use_graph=False, msv_bound=bound, **F)
     ls = gctx.cascade_launches()
-    assert len(ls) >= 3 and set(ls["stage"]) == {0, 1, 2}
-    assert int(ls["cells"][ls["stage"] == 0].sum()) == st.cells_msv
+    assert np.all(ls["ms"] >= 0)
     assert int(ls["cells"][ls["stage"] == 1].sum()) == st.cells_vit
     assert int(ls["cells"][ls["stage"] == 2].sum()) == st.cells_fwd
-    assert np.all(ls["ms"] >= 0)
-    # the MSV launches fill the MSV stage's event window
-    assert abs(float(ls["ms"][ls["stage"] == 0].sum()) - st.ms_msv) <= 0.05 + 0.01 * st.ms_msv
-    gctx.run_cascade(1, use_graph=True, **F)
+    if not bound:
+        assert len(ls) >= 3 and set(ls["stage"]) == {0, 1, 2}
+        assert int(ls["cells"][ls["stage"] == 0].sum()) == st.cells_msv
+        # the MSV launches fill the MSV stage's event window, up to each launch's queue-head
+        # reset and launch gap (about 10 us each)
+        n0 = int((ls["stage"] == 0).sum())
+        assert abs(float(ls["ms"][ls["stage"] == 0].sum()) - st.ms_msv) <= 0.05 + 0.015 * n0 + 0.01 * st.ms_msv
+    else:
+        # the bound's launches over every pair, then the f32 MSV of its candidates
+        assert set(ls["stage"]) == {4, 0, 1, 2}
+        assert int(ls["cells"][ls["stage"] == 4].sum()) == st.cells_msv
+        assert int(ls["cells"][ls["stage"] == 0].sum()) == st.cells_msv_exact > 0
+        assert st.n_msv_pass <= st.n_msv_cand < st.n_pairs
+        n4 = int((ls["stage"] == 4).sum())
+        assert abs(float(ls["ms"][ls["stage"] == 4].sum()) - st.ms_msv_bound) <= 0.05 + 0.015 * n4 + 0.01 * st.ms_msv_bound
+        assert st.ms_msv_bound <= st.ms_msv
+    gctx.run_cascade(1, use_graph=True, msv_bound=bound, **F)
     assert len(gctx.cascade_launches()) == 0
+
+
+def test_int16_bound_leaves_every_record_unchanged(gctx, calibrated):
+    """The bound only removes pairs the f32 MSV test rejects: with and without it, every
+    form gives the same records, and the plain form counts the same candidates."""
+    hmms, _, seqs = calibrated
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    key = lambda a: np.lexsort((a["target"], a["profile"]))
+    same = lambda a, b: a[key(a)].tobytes() == b[key(b)].tobytes()
+    st0, ref = gctx.run_cascade(1, use_graph=False, msv_bound=False, **F)
+    st1, got = gctx.run_cascade(1, use_graph=False, msv_bound=True, **F)
+    assert same(ref, got) and st1.n_msv_pass == st0.n_msv_pass
+    assert st1.n_msv_cand >= st1.n_msv_pass
+    for _ in range(2):
+        _, g = gctx.run_cascade(1, use_graph=True, msv_bound=True, **F)
+        assert same(ref, g)
+    stp, plain = gctx.run_cascade(0, msv_bound=True, **F)
+    _, plain0 = gctx.run_cascade(0, msv_bound=False, **F)
+    assert same(plain, plain0)
+    assert stp.n_msv_cand == st1.n_msv_cand
 
 
 def test_cascade_requires_calibrated_profiles(orc, gctx, sxa):

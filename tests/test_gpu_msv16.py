@@ -1,0 +1,79 @@
+"""GPU parity of the int16 MSV bound (SX_STAGE_MSV16, the MSV stage's prefilter; no
+reference counterpart) with the oracle's sxo_msv16: bit-exact for every kernel shape, in
+the all-pairs and pair-list forms.  That the bound never lies below the f32 MSV is pinned
+on the CPU (test_oracle.py) and end to end by the cascade tests (identical records with
+and without the bound)."""
+import numpy as np
+import pytest
+
+from helpers import random_hmm
+from test_gpu_stages import make_targets
+
+pytestmark = pytest.mark.gpu
+
+Q_SAT, Q_EMPTY = 2**31 - 1, -2**31
+Q_MAX_M = 1023
+
+# (Q positions per lane, G lanes per pair, table mode): one-lane groups (0) and aligned
+# lane blocks of 8, 16, 32 lanes (3: 512-thread, 4: 1024-thread workgroups)
+SHAPES = [(q, 1, 0) for q in (16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256)] + \
+    [(q, 8, h) for q in (16, 24, 32, 40, 48, 56, 64, 80) for h in (3, 4)] + [(96, 8, 3), (128, 8, 3)] + \
+    [(q, 16, h) for q in (16, 24, 32, 40, 48, 64) for h in (3, 4)] + \
+    [(q, 32, h) for q in (16, 24, 32) for h in (3, 4)]
+
+
+@pytest.mark.parametrize("q,g,h", SHAPES, ids=[f"Q{q}G{g}M{h}" for q, g, h in SHAPES])
+def test_every_msv16_shape_bit_exact(orc, gctx, q, g, h):
+    """Models from half the shape's capacity to capacity - 1 (the bound computes one
+    padding position past M), targets for several tiles per profile, the last partial."""
+    mmax = min(q * g - 1, Q_MAX_M)
+    rng = np.random.default_rng(q * 11 + g * 3 + h)
+    lo = max(1, (q * g) // 2)
+    ms = sorted({lo, (lo + mmax) // 2, mmax, int(rng.integers(lo, mmax + 1))})
+    hmms = [random_hmm(orc, m, 8000 + q * 131 + g * 7 + h + i) for i, m in enumerate(ms)]
+    seqs = make_targets(orc, hmms, 300 if g == 1 else 150, 8000 + q + g + h, lmax=220, homolog_frac=0.2)
+    profs = [orc.configure(x) for x in hmms]
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(4, [(mmax, q, g, h)])
+    got, _ = gctx.score_all(4, 1)
+    pairs = (np.arange(60) % len(hmms), (np.arange(60) * 7) % len(seqs))
+    gp, _ = gctx.score_pairs(4, *pairs)
+    gctx.set_shapes(4, [])
+    want = orc.Batch(profs, seqs).msv16(nthreads=8).reshape(len(profs), len(seqs))
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+    assert np.array_equal(gp, want[pairs])
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["plain", "batched"])
+def test_msv16_default_shapes_and_saturation(orc, gctx, form):
+    """Built-in shapes over model lengths 1..1023, with empty targets (INT32_MIN) and a
+    strong hit that saturates the int16 range (INT32_MAX)."""
+    ms = [1, 2, 5, 17, 63, 64, 127, 128, 200, 255, 256, 400, 511, 700, 1023]
+    hmms = [random_hmm(orc, m, 8500 + i) for i, m in enumerate(ms)]
+    m = 200
+    mat = np.full((m + 1, 20), 0.01 / 19, np.float32)
+    mat[:, 18] = 0.99
+    t = np.tile(np.array([0.9, 0.05, 0.05, 0.7, 0.3, 0.7, 0.3], np.float32), (m + 1, 1))
+    t[m] = [1.0, 0.0, 0.0, 1.0, 0.0, 1.0, 0.0]
+    hmms.append(orc.Hmm(name="w", m=m, mat=mat, t=t, compo=mat[1:].mean(axis=0)))
+    seqs = make_targets(orc, hmms, 90, 8501, lmax=400, homolog_frac=0.2)
+    seqs += [np.zeros(0, np.uint8), np.full(180, 18, np.uint8)]
+    profs = [orc.configure(x) for x in hmms]
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(4, [])
+    got, _ = gctx.score_all(4, form)
+    want = orc.Batch(profs, seqs).msv16(nthreads=8).reshape(len(profs), len(seqs))
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+    assert np.all(got[:, -2] == Q_EMPTY)
+    assert got[-1, -1] == Q_SAT
+
+
+def test_msv16_rejects_models_beyond_its_shapes(orc, gctx, sxa):
+    gctx.set_profiles([random_hmm(orc, 1100, 8600)])
+    gctx.set_targets([np.array([1, 2, 3], np.uint8)])
+    with pytest.raises(sxa.SxError):
+        gctx.score_all(4, 1)
+    with pytest.raises(sxa.SxError):
+        gctx.set_shapes(4, [(1100, 128, 8, 3)])

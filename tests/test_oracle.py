@@ -313,3 +313,83 @@ def test_star_only_sequence(orc, tiny):
     assert orc.msv(tiny, d) == -np.inf or np.isfinite(orc.msv(tiny, d))
     want, _ = graph.msv_graph(3, tiny.msc.astype(np.float64), float(tiny.msv_tbm), 6).run(list(d), "S", "T", max)
     assert want == -math.inf and orc.msv(tiny, d) == -np.inf
+
+
+# ---------------------------------------------------------------------------
+# Quantised MSV bound (the int16 prefilter; no reference counterpart).  It is pinned
+# against the f32 MSV, itself pinned above: it must never lie below it, and it may lie
+# above it by no more than its roundings allow.
+# ---------------------------------------------------------------------------
+
+Q_SAT, Q_EMPTY = 2**31 - 1, -2**31
+
+
+def _q_slack_nats(m, L):
+    """Largest gap the bound's roundings allow: each cell of the best segment rounds its
+    emission up by at most 1 + 1/32 units of 1/1024 nat, each row rounds B up by one unit
+    and its N/J/B arithmetic by at most 129 fine units (2^-22 nat)."""
+    return (min(m + 1, L) * 1.04 + L * 129.0 / 4096 + 2.0) / 1024.0 + L * 129.0 / 4194304.0
+
+
+def test_q_quantisation_rounds_up(orc):
+    L = orc.lib()
+    assert L.sxo_q_emit(np.float32(-np.inf)) == -32768
+    for v in (-40.0, -33.0, -5.3, -1e-3, 0.0, 1e-7, 0.25, 3.9, 40.0):
+        q = L.sxo_q_emit(np.float32(v))
+        assert q >= max(-32768, min(32767, math.ceil(1024.0 * float(np.float32(v)))))
+        assert q <= max(-32768, min(32767, math.ceil(1024.0 * float(np.float32(v))) + 1))
+    for v in (-9.4, -0.6931472, -1e-4, 0.0, 2.0):
+        c = L.sxo_q_const(np.float32(v))
+        assert 4194304.0 * float(np.float32(v)) + 128 <= c < 4194304.0 * float(np.float32(v)) + 129
+
+
+@pytest.mark.parametrize("m", [1, 2, 7, 40, 150, 400, 1023])
+def test_msv16_bounds_f32_msv_from_above(orc, m):
+    from helpers import emit_homolog
+    rng = np.random.default_rng(7000 + m)
+    h = random_hmm(orc, m, 7100 + m)
+    p = orc.configure(h)
+    F = orc.Q_UNITS_PER_NAT
+    n = 0
+    for L in (1, 2, 3, 17, 64, 300, 1200):
+        for kind in range(3):
+            if kind == 2:
+                d = emit_homolog(orc, h, rng, L)
+            else:
+                d = random_seq(rng, L, star_frac=0.02 * kind, degen_frac=0.03 * kind)
+            u, q = orc.msv(p, d), orc.msv16(p, d)
+            assert q != Q_EMPTY
+            if q == Q_SAT:
+                assert u > 0.0  # only a strong hit reaches the top of the int16 range
+                continue
+            assert q >= math.floor(F * float(u)), (L, kind, q, float(u))
+            assert q / F - float(u) <= _q_slack_nats(m, len(d)), (L, kind, q / F - float(u))
+            n += 1
+    assert n >= 14
+
+
+def test_msv16_saturates_on_a_strong_hit_and_is_empty_for_l0(orc):
+    m = 200
+    mat = np.full((m + 1, 20), 0.01 / 19, np.float32)
+    mat[:, 18] = 0.99  # W at every position
+    t = np.tile(np.array([0.9, 0.05, 0.05, 0.7, 0.3, 0.7, 0.3], np.float32), (m + 1, 1))
+    t[m] = [1.0, 0.0, 0.0, 1.0, 0.0, 1.0, 0.0]
+    p = orc.configure(orc.Hmm(name="w", m=m, mat=mat, t=t, compo=mat[1:].mean(axis=0)))
+    d = np.full(200, 18, np.uint8)
+    assert float(orc.msv(p, d)) > 100.0
+    assert orc.msv16(p, d) == Q_SAT
+    assert orc.msv16(p, np.zeros(0, np.uint8)) == Q_EMPTY
+    # a star-only target: the f32 score is -inf, the bound sits at the floor of its range
+    q = orc.msv16(p, np.full(30, 24, np.uint8))
+    assert q < -30.0 * orc.Q_UNITS_PER_NAT
+
+
+def test_msv16_batch_driver_matches_single_pair(orc):
+    rng = np.random.default_rng(5)
+    hm = [random_hmm(orc, m, 40 + m) for m in (3, 60, 129)]
+    profs = [orc.configure(h) for h in hm]
+    seqs = [random_seq(rng, L, star_frac=0.01, degen_frac=0.02) for L in (0, 1, 33, 250)]
+    got = orc.Batch(profs, seqs).msv16(nthreads=4).reshape(len(profs), len(seqs))
+    for i, p in enumerate(profs):
+        for j, d in enumerate(seqs):
+            assert got[i, j] == orc.msv16(p, d)
