@@ -34,6 +34,12 @@ MSV_OPS_PER_CELL = 3
 # v_max3_f32 take about 4, so an MSV cell (add + max + half a max3) needs 8 SIMD cycles per
 # 64 cells at best: 19.7 T cells/s, 0.75 of this peak in algorithmic ops.
 VALU_PEAK_TOPS = 157.3 / 2
+# The int16 MSV bound does the same three operations per cell on packed int16 halves: one
+# v_pk_add_i16 (saturating) and two v_pk_max_i16 per two cells.  Its peak counts two
+# 16-bit lane-operations per 32-bit lane and instruction at the FP32 vector instruction
+# rate: 2 x 78.6 = 157.3 T ops/s (numerically the published FP32 rate, which counts an FMA
+# as two flops).
+VALU16_PEAK_TOPS = 157.3
 
 
 def measured_traffic(workload, kernel, mode):
@@ -101,19 +107,19 @@ def cpu_baseline(wl, recs, f, budget_s, threads=None):
             "check": {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}}
 
 
-def plain_leg(sxa, ctx, wl, n_plain, f, graph):
+def plain_leg(sxa, ctx, wl, n_plain, f, graph, bound):
     """The plain form (one launch per profile and stage) on the first n_plain profiles, and
     the batched form on exactly the same pairs, with the tuned shapes of ctx."""
     key = lambda a: np.lexsort((a["target"], a["profile"]))
     with sxa.Context(0) as cp:
         cp.set_profiles(wl.hmms[:n_plain])
         cp.set_targets(wl.seqs)
-        for st in (0, 1, 2):
+        for st in (0, 1, 2, 4):
             cp.set_shapes(st, ctx.get_shapes(st))
         sp_cells = sum(h.m for h in wl.hmms[:n_plain]) * sum(len(s) for s in wl.seqs)
-        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, **f)
-        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
-        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, msv_bound=bound, **f)
+        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
+        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
         same = bool(np.array_equal(prec[key(prec)][["profile", "target", "stage"]],
                                    brec[key(brec)][["profile", "target", "stage"]]))
     return {"profiles": n_plain, "cells": sp_cells, "ms": round(pl.ms_total, 3),
@@ -154,6 +160,21 @@ def kernel_mode(sxa, synth, tune_ms, reps=5):
         st = kc.last_search_stats
         out["search"] = {"hits": int(len(hits)), "domains": int(len(doms)), "pairs_in_tail": int(st["n_pairs"]),
                          **{k: round(st[k], 2) for k in ("ms_filters", "ms_specials", "ms_tail", "ms_report")}}
+    # the int16 MSV bound covers models up to 1024 positions: timed on those profiles
+    short = [h for h in wl.hmms if h.m <= 1024]
+    cells16 = sum(h.m for h in short) * sum(len(s) for s in wl.seqs)
+    with sxa.Context(0) as kq:
+        kq.set_profiles(short)
+        kq.set_targets(wl.seqs)
+        kq.autotune(4, tune_ms)
+        kq.score_all(4, sxa.FORM_BATCHED, want=False)
+        tb = float(np.median([kq.score_all(4, sxa.FORM_BATCHED, want=False)[1] for _ in range(reps)]))
+        kq.score_all(4, sxa.FORM_PLAIN, want=False)
+        tp = float(np.median([kq.score_all(4, sxa.FORM_PLAIN, want=False)[1] for _ in range(3)]))
+        out["msv16_bound"] = {"profiles": len(short), "cells": cells16,
+                              "batched_ms": round(tb, 3), "batched_gcups": round(cells16 / tb / 1e6, 1),
+                              "plain_ms": round(tp, 3), "plain_gcups": round(cells16 / tp / 1e6, 1),
+                              "ratio_batched_over_plain": round(tp / tb, 2)}
     return wl, out, fwd
 
 
@@ -227,6 +248,10 @@ def main():
     ap.add_argument("--tune-ms", type=float, default=3000.0,
                     help="autotuner budget per stage, three times this for MSV (0 = built-in shapes)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-bound", action="store_true",
+                    help="f32 MSV on every pair (no int16 MSV bound in front of it)")
+    ap.add_argument("--no-f32-leg", action="store_true",
+                    help="skip timing the cascade without the bound beside the headline run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--f1", type=float, default=0.02)
@@ -259,28 +284,40 @@ def main():
     ctx = sxa.Context(0)
     ctx.set_profiles(wl.hmms)
     ctx.set_targets(wl.seqs)
+    bound = not args.no_bound
     if args.tune_ms > 0:
-        for st in (0, 1, 2):
+        for st in ((4, 0, 1, 2) if bound else (0, 1, 2)):
             t0 = time.time()
-            ctx.autotune(st, args.tune_ms * (3 if st == 0 else 1))  # MSV times 3x larger samples
+            ctx.autotune(st, args.tune_ms * (3 if st in (0, 4) else 1))  # MSV times 3x larger samples
             log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
             for row in ctx.autotune_log(st):
                 log(f"[tune] stage {st} bucket<= {row[0]:5d} Q={row[1]:3d} G={row[2]:2d} reg={row[3]} "
                     f"{row[4]:9.1f} GCUPS")
     graph = not args.no_graph
     for i in range(args.warmup):
-        s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
         log(f"[bench] warmup {i}: {s.ms_total:.1f} ms")
     times = []
     for i in range(args.steps):
-        s, recs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, **f)
+        s, recs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
         times.append(s.ms_total)
         log(f"[bench] step {i}: {s.ms_total:.1f} ms = {cells / s.ms_total / 1e6:.1f} GCUPS; survivors "
-            f"MSV {s.n_msv_pass} bias {s.n_bias_pass} Viterbi {s.n_vit_pass} Forward {s.n_fwd_pass}")
+            f"MSV {s.n_msv_pass} bias {s.n_bias_pass} Viterbi {s.n_vit_pass} Forward {s.n_fwd_pass}; "
+            f"bound candidates {s.n_msv_cand}")
     ms_step = float(np.median(times))
     gcups = cells / ms_step / 1e6
+    # the same cascade with the f32 MSV on every pair (no bound): time and identical records
+    f32_leg = None
+    if bound and not args.no_f32_leg:
+        ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=False, **f)
+        s0, recs0 = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=False, **f)
+        key0 = lambda a: np.lexsort((a["target"], a["profile"]))
+        f32_leg = {"ms_per_step": round(s0.ms_total, 3), "gcups": round(cells / s0.ms_total / 1e6, 2),
+                   "speedup_of_bound": round(s0.ms_total / ms_step, 3),
+                   "identical_records": bool(recs0[key0(recs0)].tobytes() == recs[key0(recs)].tobytes())}
+        log(f"[bench] f32-MSV cascade: {f32_leg}")
     # per-stage split from one eager (event-timed) run of the same chain
-    e, erecs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, **f)
+    e, erecs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, msv_bound=bound, **f)
     stages = {
         "msv": {"ms": round(e.ms_msv, 3), "cells": int(e.cells_msv), "gcups": round(e.cells_msv / e.ms_msv / 1e6, 1)},
         "bias+compaction": {"ms": round(e.ms_bias, 3), "pairs": int(e.n_msv_pass)},
@@ -289,6 +326,14 @@ def main():
         "forward": {"ms": round(e.ms_fwd, 3), "cells": int(e.cells_fwd),
                     "gcups": round(e.cells_fwd / max(e.ms_fwd, 1e-9) / 1e6, 1)},
     }
+    if bound:
+        # the MSV stage = the bound over every pair + candidate compaction + f32 MSV of the
+        # candidates; its time includes all three
+        stages["msv"]["bound"] = {"ms": round(e.ms_msv_bound, 3),
+                                  "gcups": round(e.cells_msv / max(e.ms_msv_bound, 1e-9) / 1e6, 1)}
+        stages["msv"]["candidates"] = {"pairs": int(e.n_msv_cand), "cells": int(e.cells_msv_exact),
+                                       "ms": round(e.ms_msv - e.ms_msv_bound, 3),
+                                       "fraction_of_pairs": round(e.n_msv_cand / max(e.n_pairs, 1), 5)}
     log(f"[bench] eager split: {stages}")
     key = lambda a: np.lexsort((a["target"], a["profile"]))
     same_as_eager = bool(np.array_equal(recs[key(recs)][["profile", "target", "stage"]],
@@ -297,24 +342,30 @@ def main():
     # plain form on a profile sample, and the batched form on exactly the same pairs
     plain = None
     if args.plain_sample > 0:
-        plain = plain_leg(sxa, ctx, wl, max(1, int(round(len(ms) * args.plain_sample))), f, graph)
+        plain = plain_leg(sxa, ctx, wl, max(1, int(round(len(ms) * args.plain_sample))), f, graph, bound)
         log(f"[bench] plain form: {plain}")
 
-    # dominant kernel: the longest MSV launch of the eager run (HIP events on the launch stream)
+    # dominant kernel: the longest MSV launch of the eager run (HIP events on the launch
+    # stream): an int16 bound launch over all pairs, or an f32 MSV launch without the bound
     ls = ctx.cascade_launches()
-    msv = ls[ls["stage"] == 0]
+    mstage = 4 if bound else 0
+    msv = ls[ls["stage"] == mstage]
     d = msv[int(np.argmax(msv["ms"]))]
+    peak = VALU16_PEAK_TOPS if bound else VALU_PEAK_TOPS
     achieved = float(d["cells"]) * MSV_OPS_PER_CELL / (float(d["ms"]) * 1e-3) / 1e12
+    ms_launches = float(msv["ms"].sum())
     stage_achieved = e.cells_msv * MSV_OPS_PER_CELL / (e.ms_msv * 1e-3) / 1e12
-    kname = f"k_tiles<Q={int(d['q'])},G={int(d['g'])},MSV>"
+    kname = f"k_tiles<Q={int(d['q'])},G={int(d['g'])},{'MSV16' if bound else 'MSV'}>"
     traffic, traffic_src = measured_traffic(f"{wl.name}: {len(ms)} profiles x {len(lens)} proteins", kname,
                                             int(d["hoist"]))
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic, "traffic_source": traffic_src,
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TOP/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
             "kernel": kname, "launch_ms": round(float(d["ms"]), 3),
             "cells": int(d["cells"]), "ops_per_cell": MSV_OPS_PER_CELL,
-            "msv_stage": {"launches": int(len(msv)), "achieved": round(stage_achieved, 3),
-                          "frac": round(stage_achieved / VALU_PEAK_TOPS, 4)}}
+            "all_launches_of_kernel_stage": {"launches": int(len(msv)), "ms": round(ms_launches, 3),
+                                             "achieved": round(e.cells_msv * MSV_OPS_PER_CELL /
+                                                               (ms_launches * 1e-3) / 1e12, 3)},
+            "msv_stage": {"achieved": round(stage_achieved, 3), "frac": round(stage_achieved / peak, 4)}}
     launches = [{"stage": int(r["stage"]), "q": int(r["q"]), "g": int(r["g"]), "reg": int(r["hoist"]),
                  "ms": round(float(r["ms"]), 3), "gcups": round(float(r["cells"]) / max(float(r["ms"]), 1e-9) / 1e6, 1)}
                 for r in ls]
@@ -345,9 +396,10 @@ def main():
         "metric": "GCUPS of the MSV->bias->Viterbi->Forward cascade over all pairs, batched, 1 MI355X",
         "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
+        "dtype": "i16+f32" if bound else "f32", "data": "synthetic",
         "config": {"workload": f"{wl.name}: {len(ms)} profiles x {len(lens)} proteins", "cells_msv": cells,
-                   "f1": args.f1, "f2": args.f2, "f3": args.f3, "graph": graph},
+                   "f1": args.f1, "f2": args.f2, "f3": args.f3, "graph": graph, "msv_bound": bound},
+        "f32_msv_cascade": f32_leg,
         "stages": stages,
         "survivors": {"msv": int(e.n_msv_pass), "bias": int(e.n_bias_pass), "viterbi": int(e.n_vit_pass),
                       "forward": int(e.n_fwd_pass)},

@@ -34,8 +34,10 @@ typedef struct sx_ctx sx_ctx;
 /* DP stages (dp.rs:69 msv, :152 viterbi_score, :229 forward, :301 backward).
  * SX_STAGE_MSV16 has no reference counterpart: the int16 MSV prefilter.  It computes an
  * upper bound of the f32 MSV score with saturating packed int16 cells, as an int32 in
- * units of 2^-22 nat: score(MSV16) >= 2^22 * score(MSV) for every pair, or INT32_MAX when
- * the bound saturated (INT32_MIN for an empty target).  Models up to 1023 positions. */
+ * units of 2^-22 nat: score(MSV16) >= 2^22 * score(MSV) for every pair whose cells stay
+ * below 32 nats (INT32_MIN for an empty target); once a cell reaches 32 nats the result
+ * stays at or above 32767*4096 + q(-ln 2) + L*min(q(tloop), 0) + q(tmove), q() being
+ * the bound's rounding of a score.  Models up to 1024 positions. */
 enum { SX_STAGE_MSV = 0, SX_STAGE_VITERBI = 1, SX_STAGE_FORWARD = 2, SX_STAGE_BACKWARD = 3, SX_STAGE_MSV16 = 4 };
 /* Launch forms: PLAIN = one launch per profile over blocks of targets in input order,
  * one wavefront per pair; BATCHED = one persistent launch per stage and lane-group

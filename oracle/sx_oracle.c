@@ -236,8 +236,8 @@ float sxo_msv(int m, const float* msc, float tbmk, const uint8_t* dsq, int64_t l
  * recurrence is dp.rs:92-145 (as sxo_msv) with every score rounded up. */
 #define SXQ_S 1024.0          /* emission units per nat */
 #define SXQ_F 4194304.0       /* 2^22 special-state units per nat (SXQ_S * 4096) */
-#define SXQ_SLACK_S 0.03125   /* SXQ_S * 2^-24 * 512: f32 rounding allowance per step */
-#define SXQ_SLACK_F 128.0     /* SXQ_F * 2^-24 * 512 */
+#define SXQ_SLACK_S 0.03125   /* SXQ_S * 2^-24 * 512: f32 rounding allowance of a cell */
+#define SXQ_SLACK_F 16.0      /* SXQ_F * 2^-24 * 64: of an N/J/B step */
 
 int16_t sxo_q_emit(float sc) {
     if (sc == NEG_INF) return -32768;
@@ -262,33 +262,28 @@ int32_t sxo_msv16(int m, const float* msc, float tbmk, const uint8_t* dsq, int64
     const int32_t tm = sxo_q_const(logf(3.0f / (lf + 3.0f)));
     const int32_t tb = sxo_q_const(tbmk);
     const int32_t te = sxo_q_const(-LN2F);
-    /* positions 0..M+1; position 0 stays at the floor, M+1 has the score -32768 */
-    int32_t* prev = (int32_t*)malloc(sizeof(int32_t) * (size_t)(m + 2) * 2);
-    int32_t* cur = prev + (m + 2);
-    for (int k = 0; k < 2 * (m + 2); k++) prev[k] = -32768;
+    /* positions 0..M; position 0 stays at the floor of the range */
+    int32_t* prev = (int32_t*)malloc(sizeof(int32_t) * (size_t)w * 2);
+    int32_t* cur = prev + w;
+    for (int k = 0; k < 2 * w; k++) prev[k] = -32768;
     int32_t xn = 0, xj = -(1 << 30), xb = imax(xn, xj) + tm;
-    int sat = 0;
     for (int64_t i = 0; i < l; i++) {
         const float* rs = msc + (size_t)dsq[i] * w;
-        int32_t be = ceil4096(xb + tb);
-        if (be > 32767) sat = 1;
-        be = sat16(be);
+        const int32_t be = imax(ceil4096(xb + tb), -32768);
         int32_t xe = -32768;
-        for (int k = 1; k <= m + 1; k++) {
-            const int32_t e = k <= m ? sxo_q_emit(rs[k]) : -32768;
-            const int32_t v = sat16(imax(prev[k - 1], be) + e);
+        for (int k = 1; k <= m; k++) {
+            const int32_t v = sat16(imax(prev[k - 1], be) + sxo_q_emit(rs[k]));
             cur[k] = v;
             xe = imax(xe, v);
         }
         cur[0] = -32768;
-        if (xe >= 32767) sat = 1;
         xj = imax(xj + tl, xe * 4096 + te);
         xn += tl;
         xb = imax(xn, xj) + tm;
         int32_t* s = prev; prev = cur; cur = s;
     }
     free(prev < cur ? prev : cur);
-    return sat ? INT32_MAX : xj + tm;
+    return xj + tm;
 }
 
 /* dp.rs:152-189 */

@@ -62,13 +62,14 @@ float sxo_msv(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_
  * counterpart in the reference (its MSV is the f32 dp.rs:92-145).  Emission scores
  * are rounded up to int16 units of 1/1024 nat (-inf -> -32768) and the cells use
  * saturating int16 arithmetic; N/J/B and the length-model scores are int32 in units
- * of 2^-22 nat, rounded up.  Every rounding adds the f32 rounding allowance of a
- * value of magnitude <= 512 nats (2^-24 * 512), so the result bounds the f32 MSV
- * from above: 2^22 * sxo_msv() <= sxo_msv16() unless the return is INT32_MAX (a
- * cell or the B entry saturated high).  Positions 1..M+1 are computed, M+1 with
- * the score -32768 (the device kernel's first padding position).  L = 0 gives
- * INT32_MIN.  Parity of the device kernel with this function is bit-exact; this
- * function is pinned against sxo_msv by the bound and its tightness (tests). */
+ * of 2^-22 nat, rounded up.  Each rounding adds the f32 rounding allowance of the
+ * value it feeds (magnitude <= 512 nats for a cell, <= 64 nats for N/J/B, which holds
+ * for M <= 1024 and L <= 1e6), so the result bounds the f32 MSV from above:
+ * 2^22 * sxo_msv() <= sxo_msv16() unless a cell reached the top of the int16 range (32
+ * nats), in which case the result is at least 32767*4096 + q(-ln2) + L*min(q(tloop),0)
+ * + q(tmove).  L = 0 gives INT32_MIN.  Parity of the device kernel with this function
+ * is bit-exact; this function is pinned against sxo_msv by the bound and its
+ * tightness (tests). */
 int32_t sxo_msv16(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_t l);
 /* The quantisation steps of sxo_msv16. */
 int16_t sxo_q_emit(float sc);

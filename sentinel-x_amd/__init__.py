@@ -19,10 +19,9 @@ LIB_PATH = os.path.join(HERE, "libsxhmmer.so")
 
 STAGE_MSV, STAGE_VITERBI, STAGE_FORWARD, STAGE_BACKWARD = 0, 1, 2, 3
 # int16 MSV upper bound (no reference counterpart): int32 scores in units of 2^-22 nat,
-# at least 2^22 times the f32 MSV score; INT32_MAX when the bound saturated
+# at least 2^22 times the f32 MSV score (see sx_hmmer.h for a bound whose cells saturated)
 STAGE_MSV16 = 4
 MSV16_UNITS_PER_NAT = 4194304.0
-MSV16_SATURATED = 2**31 - 1
 FORM_PLAIN, FORM_BATCHED = 0, 1
 STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward", 4: "msv16"}
 

@@ -87,14 +87,17 @@ sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__
 // fl(fl(u - null) / ln2) >= thr, hence u >= null + ln2 * thr - delta with delta
 // covering those two roundings; nullq[j] and thrq[p] hold floor(2^22 * (null - ...)) and
 // floor(2^22 * (ln2 * thr - ...)), so a bound below their sum proves the rejection.
-// Profiles with noq set (longer than the bound's shapes, or without a threshold) make
-// every pair a candidate.  Phase 0 writes block counts, phase 1 the candidates (pi, j,
-// target id) at their scanned offsets, in (pi, j) order.
+// A bound that saturated (a cell reached 32 nats) is no longer an upper bound but stays
+// at or above capq[j], which the host derives from the target's length model; the
+// threshold is capped there, so such a pair is a candidate.  Profiles with noq set
+// (longer than the bound's shapes, or without a threshold) make every pair a candidate.
+// Phase 0 writes block counts, phase 1 the candidates (pi, j, target id) at their
+// scanned offsets, in (pi, j) order.
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_msvq_cand(int phase, const int32_t* __restrict__ qsc, const int64_t* __restrict__ nullq,
-              const int64_t* __restrict__ thrq, const uint8_t* __restrict__ noq, const int32_t* __restrict__ perm,
-              const int32_t* __restrict__ sorted, int64_t P, int64_t T, int32_t* counts, const int64_t* offsets,
-              int64_t cap, int32_t* c_pi, int32_t* c_j, int32_t* c_t) {
+              const int64_t* __restrict__ capq, const int64_t* __restrict__ thrq, const uint8_t* __restrict__ noq,
+              const int32_t* __restrict__ perm, const int32_t* __restrict__ sorted, int64_t P, int64_t T,
+              int32_t* counts, const int64_t* offsets, int64_t cap, int32_t* c_pi, int32_t* c_j, int32_t* c_t) {
     __shared__ int s_wave[SXC_BLOCK / 64];
     const int64_t pos = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
     bool pass = false;
@@ -103,7 +106,8 @@ sxc_msvq_cand(int phase, const int32_t* __restrict__ qsc, const int64_t* __restr
         pi = (int32_t)(pos / T);
         j = (int32_t)(pos - (int64_t)pi * T);
         const int32_t p = perm[pi];
-        pass = noq[p] != 0 || (int64_t)qsc[(int64_t)p * T + j] >= nullq[j] + thrq[p];
+        const int64_t th = nullq[j] + thrq[p];
+        pass = noq[p] != 0 || (int64_t)qsc[(int64_t)p * T + j] >= (th < capq[j] ? th : capq[j]);
     }
     int tot;
     const int r = sxc::block_rank(pass, s_wave, &tot);

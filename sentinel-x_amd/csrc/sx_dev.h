@@ -20,15 +20,16 @@ enum { SX_TMM = 0, SX_TMI, SX_TMD, SX_TIM, SX_TII, SX_TDM, SX_TDD, SX_TBM };
 // Quantised MSV upper bound (the int16 prefilter of the MSV stage).  Emission scores
 // are int16 in units of 1/SX_Q_EMIT nat, rounded up; the special states N, J, B and
 // the length-model scores are int32 in units of 2^-SX_Q_SHIFT emission units
-// (2^-22 nat), rounded up.  Each rounding adds the largest f32 rounding error of a
-// value of magnitude <= 512 nats, so the integer recurrence bounds the f32 MSV of
-// dp.rs:92-145 from above.
+// (2^-22 nat), rounded up.  Each rounding adds the largest f32 rounding error of the
+// value it feeds (<= 512 nats for a cell, <= 64 nats for N, J and B: true for models
+// up to SX_Q_MAX_M and targets up to SX_Q_MAX_L), so the integer recurrence bounds the
+// f32 MSV of dp.rs:92-145 from above until a cell reaches the top of the int16 range.
 #define SX_Q_EMIT 1024.0
 #define SX_Q_SHIFT 12
 #define SX_Q_FINE 4194304.0      // SX_Q_EMIT * 2^SX_Q_SHIFT units per nat
 #define SX_Q_SLACK_EMIT 0.03125  // SX_Q_EMIT * 2^-24 * 512
-#define SX_Q_SLACK_FINE 128.0    // SX_Q_FINE * 2^-24 * 512
-#define SX_Q_SAT 0x7fffffff      // result of a pair whose bound saturated: may pass
+#define SX_Q_SLACK_FINE 16.0     // SX_Q_FINE * 2^-24 * 64
+#define SX_Q_MAX_L 1000000       // longer targets are candidates of every profile
 
 // Per-target length model: MSV tloop/tmove (f32 logf, dp.rs:99-100), Plan7 multihit
 // loop/move in nats (model.rs:286-289) and as probabilities (XscP), and the MSV
@@ -73,9 +74,9 @@ struct SxProfiles {
 #define SX_GEM_MIN_M 1024
 #define SX_MAX_M 4096
 #define SX_GEM_WIDTH 4104  // >= SX_MAX_M + 2 + 3, a multiple of 8
-// Longest model of the quantised MSV bound (largest int16-stage capacity minus its padding
-// position); longer models skip the bound and go straight to the f32 MSV.
-#define SX_Q_MAX_M 1023
+// Longest model of the quantised MSV bound (its largest shape capacity); longer models
+// skip the bound and go straight to the f32 MSV.
+#define SX_Q_MAX_M 1024
 
 struct SxTargets {
     const uint8_t* codes;

@@ -450,18 +450,19 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, int ew, 
 // the reference, whose MSV is the f32 recurrence above; this one bounds it from above.
 // ---------------------------------------------------------------------------
 // Two model positions per 32-bit register, as int16 halves: lane j's register r holds
-// position j*Q + 1 + r in the low half and j*Q + 1 + Q/2 + r in the high half, so one
+// slot j*Q + 1 + r in the low half and j*Q + 1 + Q/2 + r in the high half, so one
 // packed instruction advances two cells and the k-1 operand of every half is the same
 // half of register r-1 (register 0 takes the previous lane's last high half and its own
-// last low half, one alignbit).  Cells use saturating adds (v_pk_add_i16 clamp): a
-// value pushed below -32768 stays at -32768, which only raises it.  Scores are rounded
-// up on the host (q_emit, q_const) with an allowance for the f32 rounding of each step,
-// so by induction every cell, the row maximum xE, xJ, xN, xB and the result are at
-// least 2^22 (fine units) or 1024 (cells) times their f32 counterparts of dp.rs:92-145.
-// A cell or B entry that reaches the top of the int16 range makes the result
-// SX_Q_SAT (the pair may pass).  Positions past M hold -32768; the first of them can
-// carry a value, which the bound's definition includes (the models use shapes with
-// capacity >= M + 1).
+// last low half, one alignbit).  The model sits at the right end of the Q*G slots (slot
+// s holds position s - (Q*G - M)); the slots before position 1 hold -32768 scores and
+// never exceed B - 32768 or -32768, so the cell at position 1 still sees max(-32768, B)
+// and no slot raises the row maximum above the real cells' (B moves by far less than
+// 32768 units per row).  Cells use saturating adds (v_pk_add_i16 clamp): a value pushed
+// below -32768 stays there, which only raises it.  Scores are rounded up on the host
+// (q_emit, q_const) with an allowance for the f32 rounding of each step, so by induction
+// every cell, the row maximum xE, xJ, xN, xB and the result are at least 2^22 (fine
+// units) or 1024 (cells) times their f32 counterparts of dp.rs:92-145, until a cell
+// reaches 32767; from then on the result stays above a floor the host knows.
 typedef short sx_v2i16 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b) {
@@ -552,16 +553,17 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
     uint32_t row[H];
 #pragma unroll
     for (int r = 0; r < H; r++) row[r] = 0x80008000u;
-    // as msv_pair: xC = xJ on every row, and xB = max(xN, xJ) + tmove
-    int xn = 0, xj = -(1 << 30), xb = tmove;
-    int sat = 0;
+    // As msv_pair: xC = xJ on every row and xB = max(xN, xJ) + tmove.  The B entry of a
+    // row is ceil((xB + tbm) / 4096) = max(xn', xj') >> 12 with the offsets folded in:
+    // xn' = xN + c, xj' = xJ + c, c = tmove + tbm + 4095.
+    const int c = tmove + tbm + ((1 << SX_Q_SHIFT) - 1);
+    const int ce = tej + c;
+    int xn = c, xj = -(1 << 30);
+    int be = max(c >> SX_Q_SHIFT, -32768);
     auto step = [&](int x) {
         const uint32_t* __restrict__ sc = base + x * width;
         if constexpr (tm_aligned<TM>()) sc = static_cast<const uint32_t*>(__builtin_assume_aligned(sc, 16));
-        int be = (xb + tbm + ((1 << SX_Q_SHIFT) - 1)) >> SX_Q_SHIFT;  // rounded up to cell units
-        sat |= be > 32767 ? 1 : 0;
-        be = min(max(be, -32768), 32767);
-        const uint32_t be2 = ((uint32_t)be & 0xFFFFu) * 0x10001u;
+        const uint32_t be2 = __builtin_amdgcn_perm((uint32_t)be, (uint32_t)be, 0x05040504u);
         const uint32_t pl = grp_from_prev_u<G>(row[H - 1], 0x80000000u);
         const uint32_t left = __builtin_amdgcn_alignbit(row[H - 1], pl, 16);
         uint32_t e = 0x80008000u;
@@ -576,12 +578,12 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
             row[0] = v;
             e = pk_max16(e, v);
         }
-        int xe = max((int)(short)(e & 0xFFFFu), (int)e >> 16);
+        e = pk_max16(e, __builtin_amdgcn_alignbit(e, e, 16));  // both halves: the lane's maximum
+        int xe = (int)(short)(e & 0xFFFFu);
         xe = grp_max_i<G>(xe);
-        sat |= xe >= 32767 ? 1 : 0;
-        xj = max(xj + tloop, xe * (1 << SX_Q_SHIFT) + tej);
+        xj = max(xj + tloop, xe * (1 << SX_Q_SHIFT) + ce);
         xn = xn + tloop;
-        xb = max(xn, xj) + tmove;
+        be = max(max(xn, xj) >> SX_Q_SHIFT, -32768);
     };
     const uint32_t* wp = reinterpret_cast<const uint32_t*>(seq);
     if constexpr (H < 96) {
@@ -604,7 +606,7 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
         rs.init(seq);
         for (int i = 0; i < L; i++) step(rs.next());
     }
-    return sat ? SX_Q_SAT : xj + tmove;
+    return xj - tbm - ((1 << SX_Q_SHIFT) - 1);  // xJ + tmove
 }
 
 // ---------------------------------------------------------------------------
@@ -1157,13 +1159,16 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
             const int m = P.m[ti.prof];
             const int w = m + 1;
             if constexpr (STAGE == ST_MSVQ) {
-                // packed int16 table: dword (lane j, register r) = positions
-                // j*Q + 1 + r (low half) and j*Q + 1 + Q/2 + r (high half)
+                // packed int16 table: dword (lane j, register r) = slots j*Q + 1 + r
+                // (low half) and j*Q + 1 + Q/2 + r (high half); slot s holds model
+                // position s - (Q*G - M), the slots before position 1 the score -32768
                 constexpr int H = Q / 2;
                 const int16_t* src = P.qsc + P.off[ti.prof];
                 uint32_t* qt = reinterpret_cast<uint32_t*>(lds);
-                auto qv = [&](int x, int k) -> uint32_t {
-                    return k < w ? (uint32_t)(uint16_t)src[x * w + k] : 0x8000u;
+                const int shift = Q * G - m;
+                auto qv = [&](int x, int slot) -> uint32_t {
+                    const int k = slot - shift;
+                    return k >= 1 ? (uint32_t)(uint16_t)src[x * w + k] : 0x8000u;
                 };
                 if constexpr (tm_aligned<TM>()) {
                     using GA = GeoA<H, G>;

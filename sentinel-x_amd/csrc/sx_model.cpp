@@ -166,14 +166,16 @@ void msv_length(size_t l, float* tloop, float* tmove) {
 }
 
 // The bound's units (see sx_dev.h): 1024 emission units per nat and 2^22 fine units per
-// nat; the products and the slack are exact in double, so the ceiling is the rounding.
+// nat; the slack is the f32 rounding allowance of a cell (512 nats) or of an N/J/B step
+// (64 nats).  The products and the slack are exact in double, so the ceiling is the
+// rounding.
 int16_t q_emit(float sc) {
     if (sc == NEG_INF) return -32768;
     const double v = std::ceil((double)sc * 1024.0 + 0.03125);
     return (int16_t)std::min(32767.0, std::max(-32768.0, v));
 }
 
-int32_t q_const(float v) { return (int32_t)std::ceil((double)v * 4194304.0 + 128.0); }
+int32_t q_const(float v) { return (int32_t)std::ceil((double)v * 4194304.0 + 16.0); }
 
 // ---------------------------------------------------------------------------
 // HMMER3 reader

@@ -74,6 +74,17 @@ float pass_threshold(const std::function<bool(float)>& rejects, int* violations)
     return ord_f32(hi);
 }
 
+// The least value the int16 MSV bound can end at for a target of length l once one of its
+// cells has reached 32767 (32 nats): the row's xJ is then at least 32767 * 4096 + q(-ln2)
+// and falls by at most q(tloop) per row, and the result adds q(tmove).  Targets longer
+// than SX_Q_MAX_L get the least int64 (every pair a candidate).
+static int64_t q_saturated_floor(int32_t l) {
+    if (l > SX_Q_MAX_L) return INT64_MIN / 4;
+    float tloop, tmove;
+    msv_length((size_t)l, &tloop, &tmove);
+    return 32767LL * 4096 + q_const(-LN2F) + (int64_t)l * std::min<int64_t>(q_const(tloop), 0) + q_const(tmove);
+}
+
 // Model-state odds of the two-state composition filter for every code
 // (pipeline.rs:116-131); the background state's odds are 1 for every code.
 static void bias_odds(const Profile& p, double* eo1) {
@@ -110,9 +121,9 @@ __global__ void sxc_fwd_pass(const int64_t*, const int32_t*, const double*, cons
                              const float*, double*, int32_t*);
 __global__ void sxc_res_init(const int64_t*, int64_t, float*, double*, int32_t*);
 __global__ void sxc_zero_i32(int32_t*, int64_t);
-__global__ void sxc_msvq_cand(int, const int32_t*, const int64_t*, const int64_t*, const uint8_t*, const int32_t*,
-                              const int32_t*, int64_t, int64_t, int32_t*, const int64_t*, int64_t, int32_t*, int32_t*,
-                              int32_t*);
+__global__ void sxc_msvq_cand(int, const int32_t*, const int64_t*, const int64_t*, const int64_t*, const uint8_t*,
+                              const int32_t*, const int32_t*, int64_t, int64_t, int32_t*, const int64_t*, int64_t,
+                              int32_t*, int32_t*, int32_t*);
 __global__ void sxc_count_sorted(const int64_t*, int64_t, const int32_t*, int64_t, int32_t*);
 __global__ void sxc_msv_pass_list(int, const int64_t*, const int32_t*, const int32_t*, const float*, const float*,
                                   const float*, const int32_t*, const int32_t*, int32_t*, const int64_t*, int64_t,
@@ -152,11 +163,11 @@ struct CascadeState {
     int threshold_violations = 0;
     // the int16 MSV bound's thresholds in its units (per length-sorted target, per
     // profile) and the profiles it skips (every pair a candidate)
-    std::vector<int64_t> nullq, thrq;
+    std::vector<int64_t> nullq, capq, thrq;
     std::vector<uint8_t> noq;
     // device tables
     DeviceBuf d_thr_msv, d_thr_vit, d_thr_fwd, d_nullsc, d_perm, d_eo1, d_ng_vit, d_ng_fwd;
-    DeviceBuf d_nullq, d_thrq, d_noq, d_ng_msv;
+    DeviceBuf d_nullq, d_capq, d_thrq, d_noq, d_ng_msv;
     std::vector<DeviceBuf> msv_plists;
     struct Group {
         Shape s;
@@ -268,6 +279,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
     S.noq.assign(P, 0);
     S.thrq.assign(P, 0);
     S.nullq.assign(T, 0);
+    S.capq.assign(T, 0);
     if (bound) {
         const double F = SX_Q_FINE;
         for (size_t p = 0; p < P; p++) {
@@ -283,6 +295,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
         for (uint32_t j = 0; j < T; j++) {
             const double n = (double)S.nullsc[c.sorted[j]];
             S.nullq[j] = (int64_t)std::floor(F * n - F * 1e-6 * (1.0 + std::fabs(n)));
+            S.capq[j] = q_saturated_floor(c.tlen[c.sorted[j]]);
         }
     }
     // MSV groups (all pairs, implicit tiles): f32 MSV, or the int16 bound (stage 4) for
@@ -325,6 +338,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
     up(S.d_ng_fwd, ngf.data(), P * 4);
     up(S.d_ng_msv, ngm.data(), P * 4);
     up(S.d_nullq, S.nullq.data(), (size_t)T * 8);
+    up(S.d_capq, S.capq.data(), (size_t)T * 8);
     up(S.d_thrq, S.thrq.data(), P * 8);
     up(S.d_noq, S.noq.data(), P);
     for (auto& e : S.ev)
@@ -463,7 +477,8 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         int32_t *c_pi = (int32_t*)S.c_pi.p, *c_j = (int32_t*)S.c_j.p, *c_t = (int32_t*)S.c_t.p;
         for (int phase = 0; phase < 2; phase++) {
             hipLaunchKernelGGL(sxc_msvq_cand, dim3(nb1), dim3(CB), 0, st, phase, qres,
-                               (const int64_t*)S.d_nullq.p, (const int64_t*)S.d_thrq.p, (const uint8_t*)S.d_noq.p,
+                               (const int64_t*)S.d_nullq.p, (const int64_t*)S.d_capq.p,
+                               (const int64_t*)S.d_thrq.p, (const uint8_t*)S.d_noq.p,
                                (const int32_t*)S.d_perm.p, c.d_sorted, P, T, (int32_t*)S.cntq.p,
                                (const int64_t*)S.offq.p, cap, c_pi, c_j, c_t);
             if (phase == 0)
@@ -717,10 +732,14 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     double ms_total = 0.0;
     uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0, ncand = 0, cq = 0;
     // the bound's null thresholds by target id (S.nullq is in length-sorted order)
-    std::vector<int64_t> nullq_t;
+    std::vector<int64_t> nullq_t, capq_t;
     if (bound) {
         nullq_t.resize(T);
-        for (uint32_t j = 0; j < T; j++) nullq_t[c.sorted[j]] = S.nullq[j];
+        capq_t.resize(T);
+        for (uint32_t j = 0; j < T; j++) {
+            nullq_t[c.sorted[j]] = S.nullq[j];
+            capq_t[c.sorted[j]] = S.capq[j];
+        }
     }
     std::vector<int32_t> qrow(bound ? T : 0);
     HIPCHK(hipEventRecord(S.ev[0], c.stream));
@@ -737,7 +756,7 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
             HIPCHK(hipStreamSynchronize(c.stream));
             std::vector<uint32_t> cp, ct;
             for (uint32_t t = 0; t < T; t++)
-                if ((int64_t)qrow[t] >= nullq_t[t] + S.thrq[p]) {
+                if ((int64_t)qrow[t] >= std::min(nullq_t[t] + S.thrq[p], capq_t[t])) {
                     cp.push_back((uint32_t)p);
                     ct.push_back(t);
                 }

@@ -308,18 +308,17 @@ Shape Ctx::gem_shape(int m) {
 
 Shape Ctx::default_shape(int stage, int form, int m) const {
     if (stage == SX_STAGE_MSV16) {
-        // the quantised MSV bound: one lane per pair up to 127 positions, then 8- and
-        // 16-lane groups over aligned lane blocks (capacity >= M + 1: the bound
-        // includes the first padding position)
+        // the quantised MSV bound: one lane per pair up to 192 positions, then 8- and
+        // 16-lane groups over aligned lane blocks
         if (m > SX_Q_MAX_M)
             throw SxError(SX_ERR_ARG, "model length " + std::to_string(m) + " exceeds the quantised MSV bound's " +
                                           std::to_string(SX_Q_MAX_M));
-        for (int q : {16, 24, 32, 48, 64, 80, 96, 128})
-            if (q >= m + 1) return {q, 1, 0};
-        for (int q : {24, 32, 40, 48, 56, 64})
-            if (8 * q >= m + 1) return {q, 8, 3};
+        for (int q : {16, 24, 32, 48, 64, 80, 96, 128, 160, 192})
+            if (q >= m) return {q, 1, 0};
+        for (int q : {32, 40, 48, 56, 64, 80})
+            if (8 * q >= m) return {q, 8, 3};
         for (int q : {48, 64})
-            if (16 * q >= m + 1) return {q, 16, 3};
+            if (16 * q >= m) return {q, 16, 3};
     }
     if (m > SX_GEM_MIN_M) return gem_shape(m);
     if (form == SX_FORM_PLAIN) {
@@ -339,7 +338,7 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
         const KernelInfo& k = kv.second;
         if (k.stage != stage || k.spec || k.hoist >= 2) continue;
         const long cap = (long)k.q * k.g;
-        if (cap < m + cap_extra(stage)) continue;
+        if (cap < m) continue;
         if (lds_bytes(k) > LDS_MAX) continue;
         if (stage != 0 && k.q > 16 && cap < 1024) continue;
         const bool better = cap < best_cap ||
@@ -668,7 +667,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         for (const auto& kv : kernels) {
             const KernelInfo& k = kv.second;
             if (k.stage != stage || k.spec || k.hoist == 2) continue;
-            if ((long)k.q * k.g < mmax + cap_extra(stage)) continue;
+            if ((long)k.q * k.g < mmax) continue;
             // an MSV shape with an aligned-block twin: the twin reads LDS without bank
             // conflicts and is always the faster of the two
             if (stage == 0 && k.hoist == 0 && kernels.count(key(0, k.q, k.g, 0, 3))) continue;
@@ -743,7 +742,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             while (pgroups_keep.size() > 64) pgroups_keep.pop_front();
         }
         // a row never claims more than the shape covers; longer models fall through
-        table.push_back({std::min<int>(hi, best.q * best.g - cap_extra(stage)), best});
+        table.push_back({std::min<int>(hi, best.q * best.g), best});
     }
     pgroups_keep.clear();
     tuned[stage] = table;
@@ -1060,7 +1059,7 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
                                               (stage == SX_STAGE_MSV16 ? "; the quantised MSV bound has no longer shape"
                                                                        : "; longer models use a fixed shape"));
             const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false, h);
-            if ((long)q[i] * g[i] < max_m[i] + Ctx::cap_extra(stage))
+            if ((long)q[i] * g[i] < max_m[i])
                 throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
             if (Ctx::lds_bytes(k) > 160 * 1024) throw SxError(SX_ERR_ARG, "shape exceeds the LDS budget");
             if (i && max_m[i] <= max_m[i - 1]) throw SxError(SX_ERR_ARG, "max_m must increase");

@@ -142,9 +142,6 @@ struct Ctx {
     std::map<int, Shape> forced;  // autotuner: every model of the stage uses this shape
     std::vector<TuneRecord> autotune(int stage, double budget_ms);
     std::vector<TuneRecord> last_tune[5];
-    // positions a shape must hold beyond M: the quantised MSV bound includes the first
-    // padding position
-    static int cap_extra(int stage) { return stage == SX_STAGE_MSV16 ? 1 : 0; }
     Shape default_shape(int stage, int form, int m) const;
     Shape shape_for(int stage, int form, int m) const;
     Shape shape_for_spec(int stage, int m) const;

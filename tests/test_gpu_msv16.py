@@ -8,11 +8,12 @@ import pytest
 
 from helpers import random_hmm
 from test_gpu_stages import make_targets
+from test_oracle import q_saturated_floor
 
 pytestmark = pytest.mark.gpu
 
-Q_SAT, Q_EMPTY = 2**31 - 1, -2**31
-Q_MAX_M = 1023
+Q_EMPTY = -2**31
+Q_MAX_M = 1024
 
 # (Q positions per lane, G lanes per pair, table mode): one-lane groups (0) and aligned
 # lane blocks of 8, 16, 32 lanes (3: 512-thread, 4: 1024-thread workgroups)
@@ -24,9 +25,10 @@ SHAPES = [(q, 1, 0) for q in (16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256)] +
 
 @pytest.mark.parametrize("q,g,h", SHAPES, ids=[f"Q{q}G{g}M{h}" for q, g, h in SHAPES])
 def test_every_msv16_shape_bit_exact(orc, gctx, q, g, h):
-    """Models from half the shape's capacity to capacity - 1 (the bound computes one
-    padding position past M), targets for several tiles per profile, the last partial."""
-    mmax = min(q * g - 1, Q_MAX_M)
+    """Models from half the shape's capacity to its capacity (the model sits at the right
+    end of the lane blocks, after the padding slots), targets for several tiles per
+    profile, the last partial."""
+    mmax = min(q * g, Q_MAX_M)
     rng = np.random.default_rng(q * 11 + g * 3 + h)
     lo = max(1, (q * g) // 2)
     ms = sorted({lo, (lo + mmax) // 2, mmax, int(rng.integers(lo, mmax + 1))})
@@ -47,9 +49,9 @@ def test_every_msv16_shape_bit_exact(orc, gctx, q, g, h):
 
 @pytest.mark.parametrize("form", [0, 1], ids=["plain", "batched"])
 def test_msv16_default_shapes_and_saturation(orc, gctx, form):
-    """Built-in shapes over model lengths 1..1023, with empty targets (INT32_MIN) and a
-    strong hit that saturates the int16 range (INT32_MAX)."""
-    ms = [1, 2, 5, 17, 63, 64, 127, 128, 200, 255, 256, 400, 511, 700, 1023]
+    """Built-in shapes over model lengths 1..1024, with empty targets (INT32_MIN) and a
+    strong hit whose cells saturate the int16 range."""
+    ms = [1, 2, 5, 17, 63, 64, 127, 128, 200, 255, 256, 400, 511, 700, 1023, 1024]
     hmms = [random_hmm(orc, m, 8500 + i) for i, m in enumerate(ms)]
     m = 200
     mat = np.full((m + 1, 20), 0.01 / 19, np.float32)
@@ -67,7 +69,8 @@ def test_msv16_default_shapes_and_saturation(orc, gctx, form):
     want = orc.Batch(profs, seqs).msv16(nthreads=8).reshape(len(profs), len(seqs))
     assert np.array_equal(got, want), np.argwhere(got != want)[:5]
     assert np.all(got[:, -2] == Q_EMPTY)
-    assert got[-1, -1] == Q_SAT
+    assert float(orc.msv(profs[-1], seqs[-1])) > 32.0
+    assert got[-1, -1] >= q_saturated_floor(orc, len(seqs[-1]))
 
 
 def test_msv16_rejects_models_beyond_its_shapes(orc, gctx, sxa):
@@ -77,3 +80,5 @@ def test_msv16_rejects_models_beyond_its_shapes(orc, gctx, sxa):
         gctx.score_all(4, 1)
     with pytest.raises(sxa.SxError):
         gctx.set_shapes(4, [(1100, 128, 8, 3)])
+    with pytest.raises(sxa.SxError):
+        gctx.set_shapes(4, [(300, 32, 8, 3)])  # capacity 256 below its max_m
