@@ -566,18 +566,26 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
         const uint32_t be2 = __builtin_amdgcn_perm((uint32_t)be, (uint32_t)be, 0x05040504u);
         const uint32_t pl = grp_from_prev_u<G>(row[H - 1], 0x80000000u);
         const uint32_t left = __builtin_amdgcn_alignbit(row[H - 1], pl, 16);
-        uint32_t e = 0x80008000u;
+        // the row maximum in NACC independent chains (a single chain serialises every
+        // register's max behind the previous one)
+        constexpr int NACC = H >= 16 ? 4 : H >= 8 ? 2 : 1;
+        uint32_t ea[NACC];
+#pragma unroll
+        for (int a = 0; a < NACC; a++) ea[a] = 0x80008000u;
 #pragma unroll
         for (int r = H - 1; r >= 1; r--) {
             const uint32_t v = pk_adds16(pk_max16(row[r - 1], be2), sc[r]);
             row[r] = v;
-            e = pk_max16(e, v);
+            ea[r % NACC] = pk_max16(ea[r % NACC], v);
         }
         {
             const uint32_t v = pk_adds16(pk_max16(left, be2), sc[0]);
             row[0] = v;
-            e = pk_max16(e, v);
+            ea[0] = pk_max16(ea[0], v);
         }
+        uint32_t e = ea[0];
+#pragma unroll
+        for (int a = 1; a < NACC; a++) e = pk_max16(e, ea[a]);
         e = pk_max16(e, __builtin_amdgcn_alignbit(e, e, 16));  // both halves: the lane's maximum
         int xe = (int)(short)(e & 0xFFFFu);
         xe = grp_max_i<G>(xe);
