@@ -43,18 +43,20 @@ VALU16_PEAK_TOPS = 157.3
 
 
 def measured_traffic(workload, kernel, mode):
-    """HBM bytes per launch of the dominant MSV kernel, from the rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs recorded under profiles/ (scaled by the copy-kernel calibration given
-    there); None when the record is for another workload or kernel."""
-    p = os.path.join(ROOT, "profiles", "msv_pfam_scale_traffic.json")
-    try:
-        with open(p) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None, None
-    if t.get("workload") != workload or t.get("kernel") != kernel or t.get("table_mode") != mode:
-        return None, None
-    return int(t["bytes_read"]) + int(t["bytes_written"]), "profiles/msv_pfam_scale_traffic.json"
+    """HBM bytes per launch of the dominant MSV kernel (int16 bound or f32), from the
+    rocprofv3 FETCH_SIZE and WRITE_SIZE runs recorded under profiles/ (scaled by the
+    copy-kernel calibration given there); None when no record is for this workload and
+    kernel."""
+    for name in ("msv16_pfam_scale_traffic.json", "msv_pfam_scale_traffic.json"):
+        p = os.path.join(ROOT, "profiles", name)
+        try:
+            with open(p) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if t.get("workload") == workload and t.get("kernel") == kernel and t.get("table_mode") == mode:
+            return int(t["bytes_read"]) + int(t["bytes_written"]), f"profiles/{name}"
+    return None, None
 
 
 def log(*a):
