@@ -38,7 +38,18 @@ typedef struct sx_ctx sx_ctx;
  * below 32 nats (INT32_MIN for an empty target); once a cell reaches 32 nats the result
  * stays at or above 32767*4096 + q(-ln 2) + L*min(q(tloop), 0) + q(tmove), q() being
  * the bound's rounding of a score.  Models up to 1024 positions. */
-enum { SX_STAGE_MSV = 0, SX_STAGE_VITERBI = 1, SX_STAGE_FORWARD = 2, SX_STAGE_BACKWARD = 3, SX_STAGE_MSV16 = 4 };
+/* SX_STAGE_VIT16, likewise without a reference counterpart, is the int16 Viterbi
+ * prefilter: an upper bound of the f32 Viterbi score in the same units, with the same
+ * floor once a cell reaches 32 nats (with the Viterbi length model's loop and move
+ * scores).  Profiles with a positive insert-state score are not bounded. */
+enum {
+    SX_STAGE_MSV = 0,
+    SX_STAGE_VITERBI = 1,
+    SX_STAGE_FORWARD = 2,
+    SX_STAGE_BACKWARD = 3,
+    SX_STAGE_MSV16 = 4,
+    SX_STAGE_VIT16 = 5
+};
 /* Launch forms: PLAIN = one launch per profile over blocks of targets in input order,
  * one wavefront per pair; BATCHED = one persistent launch per stage and lane-group
  * shape, length-bucketed tiles from a device-resident queue. */

@@ -59,6 +59,10 @@ def lib():
         L.sxo_msv16.argtypes = [ctypes.c_int, f32p, ctypes.c_float, u8p, i64]
         L.sxo_q_emit.restype = ctypes.c_int16
         L.sxo_q_emit.argtypes = [ctypes.c_float]
+        L.sxo_vit16.restype = ctypes.c_int32
+        L.sxo_vit16.argtypes = [ctypes.c_int, f32p, f32p, f32p, u8p, i64]
+        L.sxo_q_trans.restype = ctypes.c_int16
+        L.sxo_q_trans.argtypes = [ctypes.c_float]
         L.sxo_q_const.restype = ctypes.c_int32
         L.sxo_q_const.argtypes = [ctypes.c_float]
         L.sxo_viterbi.restype = ctypes.c_float
@@ -87,7 +91,7 @@ def lib():
                                         ctypes.c_double, ctypes.c_double, i64, i64, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int]
         for name in ("sxo_batch_msv", "sxo_batch_viterbi", "sxo_batch_forward", "sxo_batch_forward64",
-                     "sxo_batch_msv16"):
+                     "sxo_batch_msv16", "sxo_batch_vit16"):
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_void_p, i64, i64, ctypes.c_void_p, ctypes.c_int]
@@ -356,6 +360,13 @@ def msv16(p: Profile, dsq) -> int:
     return int(lib().sxo_msv16(p.m, fp(p.msc), p.msv_tbm, bp(d), len(d)))
 
 
+def vit16(p: Profile, dsq) -> int:
+    """Quantised Viterbi upper bound (no reference counterpart; see sx_oracle.h)."""
+    d = _dsq(dsq)
+    xs = xsc(len(d))
+    return int(lib().sxo_vit16(p.m, fp(p.msc), fp(p.tsc), fp(xs), bp(d), len(d)))
+
+
 def viterbi(p: Profile, dsq) -> np.float32:
     d = _dsq(dsq)
     xs = xsc(len(d))
@@ -473,6 +484,9 @@ class Batch:
 
     def msv16(self, begin=0, end=None, nthreads=1):
         return self._run(lib().sxo_batch_msv16, np.int32, begin, end, nthreads)
+
+    def vit16(self, begin=0, end=None, nthreads=1):
+        return self._run(lib().sxo_batch_vit16, np.int32, begin, end, nthreads)
 
     def viterbi(self, begin=0, end=None, nthreads=1):
         return self._run(lib().sxo_batch_viterbi, np.float32, begin, end, nthreads)

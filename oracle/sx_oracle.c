@@ -286,6 +286,59 @@ int32_t sxo_msv16(int m, const float* msc, float tbmk, const uint8_t* dsq, int64
     return xj + tm;
 }
 
+/* Quantised Viterbi upper bound (see sx_oracle.h); the recurrence of dp.rs:152-189 with
+ * every score rounded up. */
+int16_t sxo_q_trans(float t) {
+    const int16_t q = sxo_q_emit(t);
+    /* a score <= 0 cannot raise a value: fl(a + t) <= a, so 0 is a valid rounding */
+    return (t <= 0.0f && q > 0) ? 0 : q;
+}
+
+static inline int32_t q_lconst(float v) { return (int32_t)ceil((double)v * SXQ_F + SXQ_SLACK_F); }
+
+int32_t sxo_vit16(int m, const float* msc, const float* tsc, const float* xs, const uint8_t* dsq, int64_t l) {
+    if (l == 0) return INT32_MIN;
+    const int w = m + 1;
+    const int32_t tl = q_lconst(xs[0]), tm = q_lconst(xs[1]), te = sxo_q_const(-LN2F);
+    /* transitions by row MM MI MD IM II DM DD BM; out of node 0 (MM MD IM DM DD at k = 0,
+     * read by position 1) the reference's states are -inf, so those scores are -32768 */
+    int16_t* qt = (int16_t*)malloc(sizeof(int16_t) * (size_t)8 * w);
+    for (int r = 0; r < 8; r++)
+        for (int k = 0; k <= m; k++) qt[r * w + k] = (k == 0) ? -32768 : sxo_q_trans(tsc[r * w + k]);
+    const int16_t *tmm = qt, *tmi = qt + w, *tmd = qt + 2 * w, *tim = qt + 3 * w, *tii = qt + 4 * w,
+                  *tdm = qt + 5 * w, *tdd = qt + 6 * w, *tbm = qt + 7 * w;
+    int32_t* buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)w * 6);
+    for (int k = 0; k < 6 * w; k++) buf[k] = -32768;
+    int32_t *mp = buf, *ip = buf + w, *dp = buf + 2 * w, *mc = buf + 3 * w, *ic = buf + 4 * w, *dc = buf + 5 * w;
+    int32_t xn = 0, xj = -(1 << 30), xb = imax(xn, xj) + tm;
+    for (int64_t i = 0; i < l; i++) {
+        const float* rs = msc + (size_t)dsq[i] * w;
+        const int32_t bq = imax(ceil4096(xb), -32768);
+        int32_t xe = -32768;
+        mc[0] = ic[0] = dc[0] = -32768;
+        for (int k = 1; k <= m; k++) {
+            const int32_t a = sat16(mp[k - 1] + tmm[k - 1]), b = sat16(ip[k - 1] + tim[k - 1]);
+            const int32_t c = sat16(bq + tbm[k]), d = sat16(dp[k - 1] + tdm[k - 1]);
+            const int32_t mv = sat16(imax(imax(a, b), imax(c, d)) + sxo_q_emit(rs[k]));
+            mc[k] = mv;
+            ic[k] = imax(sat16(mp[k] + tmi[k]), sat16(ip[k] + tii[k]));
+            const int32_t dv = imax(sat16(mc[k - 1] + tmd[k - 1]), sat16(dc[k - 1] + tdd[k - 1]));
+            dc[k] = dv;
+            xe = imax(xe, imax(mv, dv));
+        }
+        xj = imax(xj + tl, xe * 4096 + te);
+        xn += tl;
+        xb = imax(xn, xj) + tm;
+        int32_t* t;
+        t = mp; mp = mc; mc = t;
+        t = ip; ip = ic; ic = t;
+        t = dp; dp = dc; dc = t;
+    }
+    free(buf);
+    free(qt);
+    return xj + tm;
+}
+
 /* dp.rs:152-189 */
 float sxo_viterbi(int m, const float* msc, const float* tsc, const float* xs, const uint8_t* dsq,
                   int64_t l) {
@@ -693,6 +746,9 @@ static void run_pair(const pool_job* j, const sxo_batch* b, int stage, int64_t i
         case 5:
             ((int32_t*)out)[idx - begin] = sxo_msv16(m, b->msc + b->off[p], b->msv_tbm[p], dsq, l);
             break;
+        case 6:
+            ((int32_t*)out)[idx - begin] = sxo_vit16(m, b->msc + b->off[p], b->tsc + b->toff[p], xs, dsq, l);
+            break;
         default:
             ((int32_t*)out)[idx - begin] =
                 sxo_cascade(m, b->msc + b->off[p], b->tsc + b->toff[p], b->mo + b->off[p], b->tp + b->toff[p],
@@ -745,3 +801,4 @@ int sxo_batch_viterbi(const sxo_batch* b, int64_t pb, int64_t pe, float* out, in
 int sxo_batch_forward(const sxo_batch* b, int64_t pb, int64_t pe, double* out, int nt) { return run_pool(b, pb, pe, out, nt, 2); }
 int sxo_batch_forward64(const sxo_batch* b, int64_t pb, int64_t pe, double* out, int nt) { return run_pool(b, pb, pe, out, nt, 3); }
 int sxo_batch_msv16(const sxo_batch* b, int64_t pb, int64_t pe, int32_t* out, int nt) { return run_pool(b, pb, pe, out, nt, 5); }
+int sxo_batch_vit16(const sxo_batch* b, int64_t pb, int64_t pe, int32_t* out, int nt) { return run_pool(b, pb, pe, out, nt, 6); }

@@ -75,6 +75,17 @@ int32_t sxo_msv16(int m, const float* msc, float msv_tbm, const uint8_t* dsq, in
 int16_t sxo_q_emit(float sc);
 int32_t sxo_q_const(float v);
 
+/* Quantised Viterbi upper bound: the product's int16 Viterbi prefilter, no counterpart
+ * in the reference (its Viterbi is the f32 dp.rs:152-189).  The units and roundings of
+ * sxo_msv16; transitions use sxo_q_trans (a score <= 0 stays <= 0), the transitions out
+ * of node 0 are -32768 (the reference's k = 0 states are -inf), insert states are not
+ * masked (at k = M or on '*', where the reference sets -inf), and the row maximum is over
+ * M and D.  So the result bounds the f32 Viterbi score from above, as sxo_msv16 bounds
+ * MSV, until a cell reaches 32 nats; then it stays at or above 32767*4096 + q(-ln2) +
+ * L*min(q(n_loop),0) + q(n_move).  xsc16 as sxo_viterbi (multihit: J and C coincide). */
+int32_t sxo_vit16(int m, const float* msc, const float* tsc, const float* xsc16, const uint8_t* dsq, int64_t l);
+int16_t sxo_q_trans(float t);
+
 /* dp.rs:152-189.  Parity unpinned by the reference; pinned by a generic
  * max-product over the explicit Plan7 state graph (tests). */
 float sxo_viterbi(int m, const float* msc, const float* tsc, const float* xsc16,
@@ -144,6 +155,7 @@ int sxo_batch_viterbi(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, 
 int sxo_batch_forward(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
 int sxo_batch_forward64(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
 int sxo_batch_msv16(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, int32_t* out, int nthreads);
+int sxo_batch_vit16(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, int32_t* out, int nthreads);
 /* sxo_cascade over many pairs (the reference's search_profiles pair loop with
  * filters on, search.rs:217-224 + pipeline.rs:159-189).  stats: 6 doubles per profile;
  * compo: 20 floats per profile.  stage_out: one int per pair; sc_out: 4 floats per pair. */

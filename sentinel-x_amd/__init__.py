@@ -21,13 +21,16 @@ STAGE_MSV, STAGE_VITERBI, STAGE_FORWARD, STAGE_BACKWARD = 0, 1, 2, 3
 # int16 MSV upper bound (no reference counterpart): int32 scores in units of 2^-22 nat,
 # at least 2^22 times the f32 MSV score (see sx_hmmer.h for a bound whose cells saturated)
 STAGE_MSV16 = 4
+# int16 Viterbi upper bound (no reference counterpart), in the same units
+STAGE_VIT16 = 5
 MSV16_UNITS_PER_NAT = 4194304.0
 FORM_PLAIN, FORM_BATCHED = 0, 1
-STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward", 4: "msv16"}
+STAGE_NAMES = {0: "msv", 1: "viterbi", 2: "forward", 3: "backward", 4: "msv16", 5: "vit16"}
 
 
 def _stage_dtype(stage: int):
-    return {STAGE_MSV16: np.int32, STAGE_FORWARD: np.float64, STAGE_BACKWARD: np.float64}.get(stage, np.float32)
+    return {STAGE_MSV16: np.int32, STAGE_VIT16: np.int32, STAGE_FORWARD: np.float64,
+            STAGE_BACKWARD: np.float64}.get(stage, np.float32)
 
 # Every entry point declared in include/sx_hmmer.h.
 EXPORTS = [

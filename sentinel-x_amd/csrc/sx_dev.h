@@ -67,6 +67,9 @@ struct SxProfiles {
     // as msc; -inf -> -32768) and msv_tbm rounded up in fine units, per profile.
     const int16_t* qsc;
     const int32_t* qtbm;
+    // Quantised Viterbi bound: transition scores rounded up to int16 (same layout and
+    // offsets as tsc; a score <= 0 stays <= 0; column 0, out of node 0, is -32768).
+    const int16_t* qtsc;
 };
 
 // Models longer than this use the global-emission kernels (64 lanes per pair); the

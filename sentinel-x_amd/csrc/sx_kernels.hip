@@ -618,6 +618,140 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
 }
 
 // ---------------------------------------------------------------------------
+// Quantised Viterbi upper bound: the int16 prefilter of the Viterbi stage (no
+// counterpart in the reference, whose Viterbi is vit_pair's f32 recurrence).  The units,
+// roundings, packing, padding slots and folded special states of msvq_pair, with:
+//   - transition rows in LDS after the emission rows, staged shifted so that every
+//     operand of a register sits in the same register slot (MM IM DM MD DD read at k-1,
+//     MI II BM at k); out of node 0 and in the padding slots every score is -32768, so
+//     padding cells stay exactly at -32768;
+//   - insert states not masked (the reference sets them -inf at k = M and on '*'), which
+//     only raises values;
+//   - the D chain by lazy-F: a first pass from M alone, then the value entering each half
+//     block (the previous lane's high half, or the lane's own low half) is propagated
+//     until nothing changes, at most 2G - 1 times;
+//   - the row maximum over M and D (rounding up can make a D->D score of -1e-6 nat +1
+//     unit, so M alone is no longer an upper bound of D).
+// Transition scores <= 0 are rounded to <= 0 (q_trans) and the host skips profiles with a
+// positive insert score, so an insert cell reaches 32767 only after an M cell has: the
+// saturation floor of msvq_pair holds.
+template <int Q, int G>
+struct GeoQV1 {  // one-lane groups: transition rows 16-byte aligned after the odd emission rows
+    static constexpr int H = Q / 2;
+    static constexpr int TBASE = (SX_NCODE * GeoQ1<Q>::WID + 3) & ~3;
+    static constexpr int TW = (H + 3) & ~3;
+};
+
+template <int Q, int G, int TM>
+constexpr int vq_tab_dwords() {
+    if constexpr (tm_aligned<TM>())
+        return GeoA<Q / 2, G>::TAB + SX_NT * GeoA<Q / 2, G>::TWID;
+    else
+        return GeoQV1<Q, G>::TBASE + SX_NT * GeoQV1<Q, G>::TW;
+}
+
+template <int Q, int G, int TM>
+__device__ __forceinline__ int32_t vitq_pair(const uint32_t* __restrict__ tab, int tej, const uint8_t* seq, int L,
+                                             int qloop, int qmove) {
+    static_assert(Q % 2 == 0, "two positions per register");
+    static_assert(G == 1 || tm_aligned<TM>(), "the int16 tables are laid out for one-lane or aligned groups");
+    constexpr int H = Q / 2;
+    constexpr bool HOIST = H <= 8;  // small blocks keep the lane's transitions in registers
+    const int j = (int)(threadIdx.x & (G - 1));
+    const uint32_t* base;
+    int width, tofs, twid;
+    if constexpr (tm_aligned<TM>()) {
+        using GA = GeoA<H, G>;
+        const int c = GA::COPIES == 1 ? 0 : (int)((threadIdx.x / G) & 1);
+        base = tab + c * GA::COPY + j * GA::STRIDE;
+        width = GA::WID;
+        tofs = GA::TAB + j * GA::STRIDE;
+        twid = GA::TWID;
+    } else {
+        base = tab;
+        width = GeoQ1<Q>::WID;
+        tofs = GeoQV1<Q, G>::TBASE;
+        twid = GeoQV1<Q, G>::TW;
+    }
+    uint32_t mr[H], ir[H], dr[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) mr[r] = ir[r] = dr[r] = 0x80008000u;
+    // B entry in cell units: max(xn', xj') >> 12 with xn' = xN + c, xj' = xJ + c, c = move + 4095
+    const int c0 = qmove + ((1 << SX_Q_SHIFT) - 1);
+    const int ce = tej + c0;
+    int xn = c0, xj = -(1 << 30);
+    int xbq = max(c0 >> SX_Q_SHIFT, -32768);
+    auto step = [&](int x) {
+        const uint32_t* __restrict__ sc = base + x * width;
+        // larger blocks re-read the transitions every row (hidden from the optimiser so
+        // that the loads are not hoisted into registers)
+        const uint32_t* __restrict__ tr = tab + (HOIST ? tofs : opaque(tofs));
+        if constexpr (tm_aligned<TM>()) {
+            sc = static_cast<const uint32_t*>(__builtin_assume_aligned(sc, 16));
+        }
+        tr = static_cast<const uint32_t*>(__builtin_assume_aligned(tr, 16));
+        const uint32_t* __restrict__ tmm = tr + SX_TMM * twid;
+        const uint32_t* __restrict__ tmi = tr + SX_TMI * twid;
+        const uint32_t* __restrict__ tmd = tr + SX_TMD * twid;
+        const uint32_t* __restrict__ tim = tr + SX_TIM * twid;
+        const uint32_t* __restrict__ tii = tr + SX_TII * twid;
+        const uint32_t* __restrict__ tdm = tr + SX_TDM * twid;
+        const uint32_t* __restrict__ tdd = tr + SX_TDD * twid;
+        const uint32_t* __restrict__ tbm = tr + SX_TBM * twid;
+        const uint32_t xb2 = __builtin_amdgcn_perm((uint32_t)xbq, (uint32_t)xbq, 0x05040504u);
+        // k-1 operands of register 0: the previous lane's last high half, the lane's own
+        // last low half
+        const uint32_t mL = __builtin_amdgcn_alignbit(mr[H - 1], grp_from_prev_u<G>(mr[H - 1], 0x80000000u), 16);
+        const uint32_t iL = __builtin_amdgcn_alignbit(ir[H - 1], grp_from_prev_u<G>(ir[H - 1], 0x80000000u), 16);
+        const uint32_t dL = __builtin_amdgcn_alignbit(dr[H - 1], grp_from_prev_u<G>(dr[H - 1], 0x80000000u), 16);
+        // M and I, descending so that register r-1 still holds the previous row
+#pragma unroll
+        for (int r = H - 1; r >= 0; r--) {
+            const uint32_t mp = r ? mr[r - 1] : mL;
+            const uint32_t ip = r ? ir[r - 1] : iL;
+            const uint32_t dp = r ? dr[r - 1] : dL;
+            const uint32_t s = pk_max16(pk_max16(pk_adds16(mp, tmm[r]), pk_adds16(ip, tim[r])),
+                                        pk_max16(pk_adds16(xb2, tbm[r]), pk_adds16(dp, tdm[r])));
+            const uint32_t iv = pk_max16(pk_adds16(mr[r], tmi[r]), pk_adds16(ir[r], tii[r]));
+            mr[r] = pk_adds16(s, sc[r]);
+            ir[r] = iv;
+        }
+        // D chain, ascending, from M alone; then lazy-F over the half-block boundaries
+        const uint32_t mcL = __builtin_amdgcn_alignbit(mr[H - 1], grp_from_prev_u<G>(mr[H - 1], 0x80000000u), 16);
+        dr[0] = pk_adds16(mcL, tmd[0]);
+#pragma unroll
+        for (int r = 1; r < H; r++) dr[r] = pk_max16(pk_adds16(mr[r - 1], tmd[r]), pk_adds16(dr[r - 1], tdd[r]));
+        for (int it = 0; it < 2 * G - 1; it++) {
+            const uint32_t din = __builtin_amdgcn_alignbit(dr[H - 1], grp_from_prev_u<G>(dr[H - 1], 0x80000000u), 16);
+            const uint32_t nd = pk_max16(dr[0], pk_adds16(din, tdd[0]));
+            if (!__any(nd != dr[0])) break;
+            dr[0] = nd;
+#pragma unroll
+            for (int r = 1; r < H; r++) dr[r] = pk_max16(dr[r], pk_adds16(dr[r - 1], tdd[r]));
+        }
+        uint32_t e0 = 0x80008000u, e1 = 0x80008000u;
+#pragma unroll
+        for (int r = 0; r < H; r++) {
+            if (r & 1)
+                e1 = pk_max16(e1, pk_max16(mr[r], dr[r]));
+            else
+                e0 = pk_max16(e0, pk_max16(mr[r], dr[r]));
+        }
+        uint32_t e = pk_max16(e0, e1);
+        e = pk_max16(e, __builtin_amdgcn_alignbit(e, e, 16));
+        int xe = (int)(short)(e & 0xFFFFu);
+        xe = grp_max_i<G>(xe);
+        xj = max(xj + qloop, xe * (1 << SX_Q_SHIFT) + ce);
+        xn = xn + qloop;
+        xbq = max(max(xn, xj) >> SX_Q_SHIFT, -32768);
+    };
+    FwdStream rs;
+    rs.init(seq);
+    for (int i = 0; i < L; i++) step(rs.next());
+    return xj - ((1 << SX_Q_SHIFT) - 1);  // xJ + move
+}
+
+// ---------------------------------------------------------------------------
 // Viterbi score (dp.rs:152-189)
 // ---------------------------------------------------------------------------
 // MONLY: the profile's M->D and D->D scores are all <= 0, so no D value of a row exceeds
@@ -1075,7 +1209,7 @@ __device__ __forceinline__ double bwd_pair(const float* __restrict__ eot, int ew
 // ---------------------------------------------------------------------------
 // tile loop: one workgroup = one profile staged in LDS, one target per lane group
 // ---------------------------------------------------------------------------
-enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3, ST_MSVQ = 4 };
+enum { ST_MSV = 0, ST_VIT = 1, ST_FWD = 2, ST_BWD = 3, ST_MSVQ = 4, ST_VITQ = 5 };
 
 // Floats of a kernel's dynamic LDS: the staged emission rows (none for TM_GEM) and,
 // past MSV, the eight transition rows after them.
@@ -1092,6 +1226,8 @@ template <int Q, int G, int ST, int TM>
 constexpr int lds_floats() {
     if constexpr (ST == ST_MSVQ)
         return q_tab_dwords<Q, G, TM>();
+    else if constexpr (ST == ST_VITQ)
+        return vq_tab_dwords<Q, G, TM>();
     else if constexpr (tm_aligned<TM>())
         return GeoA<Q, G>::TAB + (ST == ST_MSV ? 0 : SX_NT * GeoA<Q, G>::TWID);
     else
@@ -1101,7 +1237,7 @@ constexpr int lds_floats() {
 // LDS row width recorded in the kernel table (dwords for the int16 stage)
 template <int Q, int G, int ST, int TM>
 constexpr int k_wid() {
-    if constexpr (ST == ST_MSVQ) {
+    if constexpr (ST == ST_MSVQ || ST == ST_VITQ) {
         if constexpr (tm_aligned<TM>())
             return GeoA<Q / 2, G>::WID;
         else
@@ -1142,7 +1278,7 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
 template <int Q, int G, int STAGE, bool SPEC, int TM>
 __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out,
                                                           SxSpecialsOut SP) {
-    static_assert(STAGE == ST_MSV || STAGE == ST_VIT || STAGE == ST_MSVQ || !tm_aligned<TM>(),
+    static_assert(STAGE == ST_MSV || STAGE == ST_VIT || STAGE == ST_MSVQ || STAGE == ST_VITQ || !tm_aligned<TM>(),
                   "aligned lane blocks are an MSV and Viterbi layout");
     constexpr int WID = tm_wid<Q, G, TM>();
     constexpr int NTAB_E = lds_emission_floats<Q, G, TM>();  // emission rows staged in LDS
@@ -1166,7 +1302,7 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
             __syncthreads();
             const int m = P.m[ti.prof];
             const int w = m + 1;
-            if constexpr (STAGE == ST_MSVQ) {
+            if constexpr (STAGE == ST_MSVQ || STAGE == ST_VITQ) {
                 // packed int16 table: dword (lane j, register r) = slots j*Q + 1 + r
                 // (low half) and j*Q + 1 + Q/2 + r (high half); slot s holds model
                 // position s - (Q*G - M), the slots before position 1 the score -32768
@@ -1196,6 +1332,36 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                     for (int idx = threadIdx.x; idx < SX_NCODE * QW; idx += blockDim.x) {
                         const int x = idx / QW, r = idx - x * QW;
                         qt[idx] = r < H ? (qv(x, r + 1) | (qv(x, r + 1 + H) << 16)) : 0x80008000u;
+                    }
+                }
+                if constexpr (STAGE == ST_VITQ) {
+                    // transition rows, shifted to the cell they feed: position k-1 for
+                    // MM MD IM DM DD, k for MI II BM; -32768 in the padding slots
+                    const int16_t* tsrc = P.qtsc + P.toff[ti.prof];
+                    auto tv = [&](int r, int slot) -> uint32_t {
+                        const int k = slot - shift;
+                        const bool at_k = r == SX_TMI || r == SX_TII || r == SX_TBM;
+                        return k >= 1 ? (uint32_t)(uint16_t)tsrc[r * w + (at_k ? k : k - 1)] : 0x8000u;
+                    };
+                    int tb, twd, tst;
+                    if constexpr (tm_aligned<TM>()) {
+                        tb = GeoA<H, G>::TAB;
+                        twd = GeoA<H, G>::TWID;
+                        tst = GeoA<H, G>::STRIDE;
+                    } else {
+                        tb = GeoQV1<Q, G>::TBASE;
+                        twd = GeoQV1<Q, G>::TW;
+                        tst = GeoQV1<Q, G>::TW;
+                    }
+                    for (int idx = threadIdx.x; idx < SX_NT * twd; idx += blockDim.x) {
+                        const int r = idx / twd, pos = idx - r * twd;
+                        const int jj = pos / tst, sl = pos - jj * tst;
+                        uint32_t v = 0x80008000u;
+                        if (jj < G && sl < H) {
+                            const int s0 = jj * Q + 1 + sl;
+                            v = tv(r, s0) | (tv(r, s0 + H) << 16);
+                        }
+                        qt[tb + idx] = v;
                     }
                 }
             } else if constexpr (TM == TM_GEM) {
@@ -1258,7 +1424,16 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
             const uint8_t* seq = T.codes + T.off[tid];
             const SxLen lm = T.lm[tid];
             const int64_t o = ti.out + (ti.by_tid ? (int64_t)tid : (int64_t)gid);
-            if constexpr (STAGE == ST_MSVQ) {
+            if constexpr (STAGE == ST_VITQ) {
+                int32_t r = INT32_MIN;
+                if (L > 0) {
+                    // the Viterbi length model rounded up as the host's q_const does
+                    const int ql = (int)ceil((double)lm.lloop * SX_Q_FINE + SX_Q_SLACK_FINE);
+                    const int qm = (int)ceil((double)lm.lmove * SX_Q_FINE + SX_Q_SLACK_FINE);
+                    r = vitq_pair<Q, G, TM>(reinterpret_cast<const uint32_t*>(lds), P.q_tej, seq, L, ql, qm);
+                }
+                if (jl == 0) reinterpret_cast<int32_t*>(out)[o] = r;
+            } else if constexpr (STAGE == ST_MSVQ) {
                 int32_t r = INT32_MIN;
                 if (L > 0)
                     r = msvq_pair<Q, G, TM>(reinterpret_cast<const uint32_t*>(lds), P.qtbm[ti.prof], P.q_tej, seq, L,
@@ -1403,6 +1578,11 @@ static const SxKernelEntry k_table[] = {
     SXK_QALN(8, 64), SXK_QALN(8, 80), SXK_E(4, 96, 8, false, sxk::TM_ALN), SXK_E(4, 128, 8, false, sxk::TM_ALN),
     SXK_QALN(16, 16), SXK_QALN(16, 24), SXK_QALN(16, 32), SXK_QALN(16, 40), SXK_QALN(16, 48), SXK_QALN(16, 64),
     SXK_QALN(32, 16), SXK_QALN(32, 24), SXK_QALN(32, 32),
+    // quantised Viterbi bound (stage 5): one-lane groups, aligned lane blocks of 8, 16, 32
+    SXK_E(5, 16, 1, false, 0), SXK_E(5, 32, 1, false, 0),
+    SXK_E(5, 16, 8, false, sxk::TM_ALN), SXK_E(5, 24, 8, false, sxk::TM_ALN), SXK_E(5, 32, 8, false, sxk::TM_ALN),
+    SXK_E(5, 16, 16, false, sxk::TM_ALN), SXK_E(5, 24, 16, false, sxk::TM_ALN), SXK_E(5, 32, 16, false, sxk::TM_ALN),
+    SXK_E(5, 16, 32, false, sxk::TM_ALN), SXK_E(5, 24, 32, false, sxk::TM_ALN), SXK_E(5, 32, 32, false, sxk::TM_ALN),
 };
 
 extern "C" const SxKernelEntry* sx_kernel_table(int* n) {

@@ -177,6 +177,11 @@ int16_t q_emit(float sc) {
 
 int32_t q_const(float v) { return (int32_t)std::ceil((double)v * 4194304.0 + 16.0); }
 
+int16_t q_trans(float t) {
+    const int16_t q = q_emit(t);
+    return (t <= 0.0f && q > 0) ? (int16_t)0 : q;
+}
+
 // ---------------------------------------------------------------------------
 // HMMER3 reader
 // ---------------------------------------------------------------------------

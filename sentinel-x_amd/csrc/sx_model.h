@@ -78,6 +78,9 @@ void msv_length(size_t l, float* tloop, float* tmove);
 // rounded up to int32 fine units.
 int16_t q_emit(float sc);
 int32_t q_const(float v);
+// A transition score for the int16 Viterbi bound: q_emit, except that a score <= 0 stays
+// <= 0 (fl(a + t) <= a for t <= 0, so 0 is a valid upper rounding).
+int16_t q_trans(float t);
 
 // HMMER3/b..f reader (parse.rs:42-298): plain text or gzip by magic; throws
 // std::runtime_error with the reference's messages on malformed input.

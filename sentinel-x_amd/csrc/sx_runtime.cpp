@@ -128,7 +128,7 @@ Ctx::~Ctx() {
 void Ctx::free_profiles() {
     dfree(d_msc); dfree(d_mo); dfree(d_tsc); dfree(d_tp);
     dfree(d_off); dfree(d_toff); dfree(d_m); dfree(d_tbm); dfree(d_vmo);
-    dfree(d_qsc); dfree(d_qtbm);
+    dfree(d_qsc); dfree(d_qtbm); dfree(d_qtsc);
     dfree(d_gmsc); dfree(d_gmo); dfree(d_goff);
 }
 
@@ -217,7 +217,7 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     std::vector<float> msc, mo, tsc, tp, tbm;
     std::vector<int64_t> off, toff;
     std::vector<int32_t> ms, vmo, qtbm;
-    std::vector<int16_t> qsc;
+    std::vector<int16_t> qsc, qtsc;
     for (const auto& p : profs) {
         // D(k) = max(M(k-1) + tMD, D(k-1) + tDD) with D(1) = -inf: when no tMD, tDD is
         // positive, every D value is at most an earlier M value of the same row (f32
@@ -235,6 +235,8 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
         tbm.push_back(p.msv_tbm);
         qtbm.push_back(q_const(p.msv_tbm));
         for (float v : p.msc) qsc.push_back(q_emit(v));
+        for (int r = 0; r < SX_NT; r++)
+            for (int k = 0; k <= p.m; k++) qtsc.push_back(k == 0 ? (int16_t)-32768 : q_trans(p.t_row(r)[k]));
         msc.insert(msc.end(), p.msc.begin(), p.msc.end());
         mo.insert(mo.end(), p.mo.begin(), p.mo.end());
         tsc.insert(tsc.end(), p.tsc.begin(), p.tsc.end());
@@ -251,6 +253,7 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     d_vmo = dupload(vmo);
     d_qsc = dupload(qsc);
     d_qtbm = dupload(qtbm);
+    d_qtsc = dupload(qtsc);
     // padded emission rows of the long models (read from global memory by their kernels)
     std::vector<float> gmsc, gmo;
     std::vector<int64_t> goff(profs.size(), -1);
@@ -279,7 +282,7 @@ SxProfiles Ctx::dprof() const {
     P.msc = d_msc; P.mo = d_mo; P.tsc = d_tsc; P.tp = d_tp;
     P.off = d_off; P.toff = d_toff; P.m = d_m; P.msv_tbm = d_tbm; P.vit_m_only = d_vmo;
     P.gmsc = d_gmsc; P.gmo = d_gmo; P.goff = d_goff; P.gw = SX_GEM_WIDTH; P.q_tej = q_const(-LN2F);
-    P.qsc = d_qsc; P.qtbm = d_qtbm;
+    P.qsc = d_qsc; P.qtbm = d_qtbm; P.qtsc = d_qtsc;
     return P;
 }
 
@@ -307,6 +310,20 @@ Shape Ctx::gem_shape(int m) {
 }
 
 Shape Ctx::default_shape(int stage, int form, int m) const {
+    if (stage == SX_STAGE_VIT16) {
+        // the quantised Viterbi bound: one lane per pair up to 32 positions, then 8-, 16-
+        // and 32-lane groups over aligned lane blocks
+        if (m > SX_Q_MAX_M)
+            throw SxError(SX_ERR_ARG, "model length " + std::to_string(m) + " exceeds the quantised Viterbi bound's " +
+                                          std::to_string(SX_Q_MAX_M));
+        if (m <= 16) return {16, 1, 0};
+        if (m <= 32) return {32, 1, 0};
+        for (int q : {16, 24, 32})
+            if (8 * q >= m) return {q, 8, 3};
+        for (int q : {24, 32})
+            if (16 * q >= m) return {q, 16, 3};
+        return {32, 32, 3};
+    }
     if (stage == SX_STAGE_MSV16) {
         // the quantised MSV bound: one lane per pair up to 192 positions, then 8- and
         // 16-lane groups over aligned lane blocks
@@ -353,8 +370,9 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
 }
 
 Shape Ctx::shape_for(int stage, int form, int m) const {
-    if (stage == SX_STAGE_MSV16 && m > SX_Q_MAX_M) return default_shape(stage, form, m);  // throws
-    if (stage != SX_STAGE_MSV16 && m > SX_GEM_MIN_M) return gem_shape(m);  // long models: one fixed shape
+    const bool quant = stage == SX_STAGE_MSV16 || stage == SX_STAGE_VIT16;
+    if (quant && m > SX_Q_MAX_M) return default_shape(stage, form, m);  // throws
+    if (!quant && m > SX_GEM_MIN_M) return gem_shape(m);  // long models: one fixed shape
     if (form == SX_FORM_BATCHED) {
         if (forced.count(stage)) return forced.at(stage);
         auto it = tuned.find(stage);
@@ -645,7 +663,8 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
             }
         lo = hi;
         if (inb.empty()) continue;
-        if (mmax > (stage == SX_STAGE_MSV16 ? SX_Q_MAX_M : SX_GEM_MIN_M)) break;  // no tuned shape beyond
+        const bool quant = stage == SX_STAGE_MSV16 || stage == SX_STAGE_VIT16;
+        if (mmax > (quant ? SX_Q_MAX_M : SX_GEM_MIN_M)) break;  // no tuned shape beyond
         // profiles spread over the bucket, repeated if the bucket is small: about 4e10
         // cells (up to 16 profiles) for the later stages; three times that (up to 64) for
         // MSV, whose one-lane shapes run a whole pair per lane, so that a tile lasts long
@@ -982,10 +1001,10 @@ extern "C" int sx_profile_tables(const sx_ctx* ctx, uint32_t p, int32_t* m, floa
     return SX_OK;
 }
 
-static size_t elem_size(int stage) { return stage == SX_STAGE_FORWARD || stage == SX_STAGE_BACKWARD ? 8 : 4; }
+static size_t elem_size(int stage) { return stage == SX_STAGE_FORWARD || stage == SX_STAGE_BACKWARD ? 8 : 4; }  // MSV16 / VIT16: int32
 
 extern "C" int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double* ms) {
-    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16 || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED)) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_VIT16 || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED)) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         if (c.profs.empty() || c.nt == 0) {
@@ -1003,7 +1022,7 @@ extern "C" int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double*
 
 extern "C" int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
                               void* out, double* ms) {
-    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16 || (n && (!prof || !tgt))) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_VIT16 || (n && (!prof || !tgt))) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         if (n == 0) {
@@ -1031,7 +1050,7 @@ extern "C" int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const u
 
 extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, int32_t* q, int32_t* g,
                             int32_t* hoist) {
-    if (!ctx || !q || !g || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
+    if (!ctx || !q || !g || stage < 0 || stage > SX_STAGE_VIT16) return SX_ERR_ARG;
     try {
         const sx::Shape s = ctx->c->shape_for(stage, form, m);
         *q = s.q;
@@ -1045,7 +1064,7 @@ extern "C" int sx_shape_for(const sx_ctx* ctx, int stage, int form, int32_t m, i
 
 extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* max_m, const int32_t* q,
                              const int32_t* g, const int32_t* hoist) {
-    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16 || (n && (!max_m || !q || !g))) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_VIT16 || (n && (!max_m || !q || !g))) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         std::vector<std::pair<int, sx::Shape>> tab;
@@ -1053,11 +1072,12 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
             const int h = hoist ? hoist[i] : 0;
             if (h < 0 || h > 4 || h == 2)
                 throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (long models have a fixed shape)");
-            const int top = stage == SX_STAGE_MSV16 ? SX_Q_MAX_M : SX_GEM_MIN_M;
+            const bool quant = stage == SX_STAGE_MSV16 || stage == SX_STAGE_VIT16;
+            const int top = quant ? SX_Q_MAX_M : SX_GEM_MIN_M;
             if (max_m[i] > top)
                 throw SxError(SX_ERR_ARG, "rows cover models up to " + std::to_string(top) +
-                                              (stage == SX_STAGE_MSV16 ? "; the quantised MSV bound has no longer shape"
-                                                                       : "; longer models use a fixed shape"));
+                                              (quant ? "; the quantised bounds have no longer shape"
+                                                     : "; longer models use a fixed shape"));
             const sx::KernelInfo& k = c.kernel(stage, q[i], g[i], false, h);
             if ((long)q[i] * g[i] < max_m[i])
                 throw SxError(SX_ERR_ARG, "shape capacity below its max_m");
@@ -1075,7 +1095,7 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
 
 extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q,
                              int32_t* g, int32_t* hoist) {
-    if (!ctx || !n || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
+    if (!ctx || !n || stage < 0 || stage > SX_STAGE_VIT16) return SX_ERR_ARG;
     return guard(ctx, [&] {
         Ctx& c = *ctx->c;
         auto it = c.tuned.find(stage);
@@ -1091,13 +1111,13 @@ extern "C" int sx_get_shapes(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, 
 }
 
 extern "C" int sx_autotune(sx_ctx* ctx, int stage, double budget_ms) {
-    if (!ctx || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
+    if (!ctx || stage < 0 || stage > SX_STAGE_VIT16) return SX_ERR_ARG;
     return guard(ctx, [&] { ctx->c->last_tune[stage] = ctx->c->autotune(stage, budget_ms); });
 }
 
 extern "C" int sx_autotune_log(sx_ctx* ctx, int stage, uint32_t cap, uint32_t* n, int32_t* max_m, int32_t* q,
                                int32_t* g, int32_t* hoist, double* gcups) {
-    if (!ctx || !n || stage < 0 || stage > SX_STAGE_MSV16) return SX_ERR_ARG;
+    if (!ctx || !n || stage < 0 || stage > SX_STAGE_VIT16) return SX_ERR_ARG;
     const auto& L = ctx->c->last_tune[stage];
     *n = (uint32_t)L.size();
     for (uint32_t i = 0; i < *n && i < cap; i++) {

@@ -112,6 +112,7 @@ struct Ctx {
     int32_t* d_vmo = nullptr;  // SxProfiles::vit_m_only
     int16_t* d_qsc = nullptr;  // SxProfiles::qsc, qtbm (quantised MSV bound)
     int32_t* d_qtbm = nullptr;
+    int16_t* d_qtsc = nullptr;  // SxProfiles::qtsc (quantised Viterbi bound)
     int32_t* d_pidx = nullptr;
     size_t d_pidx_n = 0;
     void set_profiles(std::vector<Profile>&& ps);
@@ -141,7 +142,7 @@ struct Ctx {
     std::map<int, std::vector<std::pair<int, Shape>>> tuned;
     std::map<int, Shape> forced;  // autotuner: every model of the stage uses this shape
     std::vector<TuneRecord> autotune(int stage, double budget_ms);
-    std::vector<TuneRecord> last_tune[5];
+    std::vector<TuneRecord> last_tune[6];
     Shape default_shape(int stage, int form, int m) const;
     Shape shape_for(int stage, int form, int m) const;
     Shape shape_for_spec(int stage, int m) const;

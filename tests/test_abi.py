@@ -49,10 +49,10 @@ def _check_aligned_layout(e):
     lanes a cycle from 64 four-byte banks, so the 16 lanes of every quarter-wave must start
     on 16 distinct four-bank slots whatever residue row each group reads.  Viterbi's
     transition rows (one copy, the same addresses for every group) follow."""
-    assert e.stage in (0, 1, 4) and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
-    # a lane block holds Q floats, or for the int16 MSV bound (stage 4) Q/2 dwords of two
+    assert e.stage in (0, 1, 4, 5) and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
+    # a lane block holds Q floats, or for the int16 bounds (stages 4, 5) Q/2 dwords of two
     # int16 positions each
-    per_lane = e.q // 2 if e.stage == 4 else e.q
+    per_lane = e.q // 2 if e.stage in (4, 5) else e.q
     assert per_lane % 4 == 0
     stride = per_lane + 4 if per_lane % 8 == 0 else per_lane + 8
     copies = 1 if e.g >= 16 else 2
@@ -61,7 +61,7 @@ def _check_aligned_layout(e):
     if copies > 1:
         assert e.wid % 64 == 0  # the residue row moves no bank
     tab = copies * 25 * e.wid + (copies - 1) * skew
-    assert e.lds == 4 * (tab + (8 * e.g * stride if e.stage == 1 else 0))
+    assert e.lds == 4 * (tab + (8 * e.g * stride if e.stage in (1, 5) else 0))
     assert tab % 4 == 0 and (e.g * stride) % 4 == 0  # transition rows stay 16-byte aligned
     copy_stride = 25 * e.wid + skew
     for quarter in range(4):
@@ -99,9 +99,13 @@ def test_kernel_table_names_each_shape_once(sxa):
             _check_aligned_layout(e)
         elif e.hoist == 2:
             assert e.lds == (0 if e.stage == 0 else 4 * 8 * e.wid)
+        elif e.stage == 5:
+            # one-lane Viterbi bound: odd emission rows, then 16-byte-aligned transition rows
+            tw = (e.q // 2 + 3) // 4 * 4
+            assert e.lds == 4 * ((25 * e.wid + 3) // 4 * 4 + 8 * tw)
         else:
             assert e.lds == 4 * (25 if e.stage in (0, 4) else 33) * e.wid
-        if e.stage == 4:
+        if e.stage in (4, 5):
             # the int16 bound's one-lane rows are odd (distinct banks for distinct rows)
             assert e.hoist in (0, 3, 4) and e.spec == 0
             if e.hoist == 0:

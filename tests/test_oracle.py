@@ -406,3 +406,68 @@ def test_msv16_batch_driver_matches_single_pair(orc):
     for i, p in enumerate(profs):
         for j, d in enumerate(seqs):
             assert got[i, j] == orc.msv16(p, d)
+
+
+# ---------------------------------------------------------------------------
+# Quantised Viterbi bound (the int16 Viterbi prefilter; no reference counterpart),
+# pinned against the f32 Viterbi the same way as the MSV bound.
+# ---------------------------------------------------------------------------
+
+def v_saturated_floor(orc, L):
+    """The least result of a Viterbi bound one of whose cells reached 32 nats."""
+    xs = orc.xsc(L)
+    c = orc.lib().sxo_q_const
+    return 32767 * 4096 + c(np.float32(-0.6931472)) + L * min(c(xs[0]), 0) + c(xs[1])
+
+
+def _v_slack_nats(m, L):
+    """Each cell of the best path rounds its emission and transition up (at most 2 * 1.04
+    units of 1/1024 nat), each row rounds B up by one unit and N/J by 17 fine units."""
+    return (min(m, L) * 2.1 + L * 0.02 + 6.0) / 1024.0 + L * 20.0 / 4194304.0
+
+
+def test_q_trans_keeps_nonpositive_scores_nonpositive(orc):
+    L = orc.lib()
+    assert L.sxo_q_trans(np.float32(-np.inf)) == -32768
+    for v in (-1e-9, -1e-6, -1e-5, 0.0):
+        assert L.sxo_q_trans(np.float32(v)) == 0
+    for v in (-3.0, -0.105, 1e-6, 0.5):
+        q = L.sxo_q_trans(np.float32(v))
+        assert q >= math.ceil(1024.0 * float(np.float32(v)) - 1e-9) and q <= math.ceil(1024.0 * float(np.float32(v))) + 1
+
+
+@pytest.mark.parametrize("m", [1, 2, 7, 40, 150, 400, 1024])
+def test_vit16_bounds_f32_viterbi_from_above(orc, m):
+    from helpers import emit_homolog
+    rng = np.random.default_rng(9000 + m)
+    h = random_hmm(orc, m, 9100 + m)
+    p = orc.configure(h)
+    F = orc.Q_UNITS_PER_NAT
+    n = 0
+    for L in (1, 2, 3, 17, 64, 300, 1200):
+        for kind in range(3):
+            if kind == 2:
+                d = emit_homolog(orc, h, rng, L)
+            else:
+                d = random_seq(rng, L, star_frac=0.02 * kind, degen_frac=0.03 * kind)
+            v, q = orc.viterbi(p, d), orc.vit16(p, d)
+            floor_sat = v_saturated_floor(orc, len(d))
+            assert q >= min(math.floor(F * float(v)), floor_sat), (L, kind, q, float(v))
+            if q >= floor_sat:
+                continue
+            assert q >= math.floor(F * float(v)), (L, kind, q, float(v))
+            assert q / F - float(v) <= _v_slack_nats(m, len(d)), (L, kind, q / F - float(v))
+            n += 1
+    assert n >= 14
+
+
+def test_vit16_batch_driver_and_edges(orc):
+    rng = np.random.default_rng(6)
+    hm = [random_hmm(orc, m, 60 + m) for m in (1, 3, 60, 129)]
+    profs = [orc.configure(h) for h in hm]
+    seqs = [random_seq(rng, L, star_frac=0.05, degen_frac=0.02) for L in (0, 1, 33, 250)]
+    got = orc.Batch(profs, seqs).vit16(nthreads=4).reshape(len(profs), len(seqs))
+    for i, p in enumerate(profs):
+        for j, d in enumerate(seqs):
+            assert got[i, j] == orc.vit16(p, d)
+    assert np.all(got[:, 0] == -2**31)
