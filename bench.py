@@ -109,19 +109,19 @@ def cpu_baseline(wl, recs, f, budget_s, threads=None):
             "check": {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}}
 
 
-def plain_leg(sxa, ctx, wl, n_plain, f, graph, bound):
+def plain_leg(sxa, ctx, wl, n_plain, f, graph, bound, vbound):
     """The plain form (one launch per profile and stage) on the first n_plain profiles, and
     the batched form on exactly the same pairs, with the tuned shapes of ctx."""
     key = lambda a: np.lexsort((a["target"], a["profile"]))
     with sxa.Context(0) as cp:
         cp.set_profiles(wl.hmms[:n_plain])
         cp.set_targets(wl.seqs)
-        for st in (0, 1, 2, 4):
+        for st in (0, 1, 2, 4, 5):
             cp.set_shapes(st, ctx.get_shapes(st))
         sp_cells = sum(h.m for h in wl.hmms[:n_plain]) * sum(len(s) for s in wl.seqs)
-        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, msv_bound=bound, **f)
-        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
-        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
+        pl, prec = cp.run_cascade(sxa.FORM_PLAIN, msv_bound=bound, vit_bound=vbound, **f)
+        cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, vit_bound=vbound, **f)
+        bt, brec = cp.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, vit_bound=vbound, **f)
         same = bool(np.array_equal(prec[key(prec)][["profile", "target", "stage"]],
                                    brec[key(brec)][["profile", "target", "stage"]]))
     return {"profiles": n_plain, "cells": sp_cells, "ms": round(pl.ms_total, 3),
@@ -252,6 +252,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-bound", action="store_true",
                     help="f32 MSV on every pair (no int16 MSV bound in front of it)")
+    ap.add_argument("--vit-bound", action="store_true",
+                    help="run the int16 Viterbi bound in front of the f32 Viterbi in the timed cascade")
     ap.add_argument("--no-f32-leg", action="store_true",
                     help="skip timing the cascade without the bound beside the headline run")
     ap.add_argument("--no-cpu", action="store_true")
@@ -287,8 +289,9 @@ def main():
     ctx.set_profiles(wl.hmms)
     ctx.set_targets(wl.seqs)
     bound = not args.no_bound
+    vbound = bool(args.vit_bound)
     if args.tune_ms > 0:
-        for st in ((4, 0, 1, 2) if bound else (0, 1, 2)):
+        for st in ((4, 0, 5, 1, 2) if bound else (0, 5, 1, 2)):
             t0 = time.time()
             ctx.autotune(st, args.tune_ms * (3 if st in (0, 4) else 1))  # MSV times 3x larger samples
             log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
@@ -297,29 +300,36 @@ def main():
                     f"{row[4]:9.1f} GCUPS")
     graph = not args.no_graph
     for i in range(args.warmup):
-        s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
+        s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, vit_bound=vbound, **f)
         log(f"[bench] warmup {i}: {s.ms_total:.1f} ms")
     times = []
     for i in range(args.steps):
-        s, recs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, **f)
+        s, recs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, vit_bound=vbound, **f)
         times.append(s.ms_total)
         log(f"[bench] step {i}: {s.ms_total:.1f} ms = {cells / s.ms_total / 1e6:.1f} GCUPS; survivors "
             f"MSV {s.n_msv_pass} bias {s.n_bias_pass} Viterbi {s.n_vit_pass} Forward {s.n_fwd_pass}; "
-            f"bound candidates {s.n_msv_cand}")
+            f"bound candidates MSV {s.n_msv_cand} Viterbi {s.n_vit_cand}")
     ms_step = float(np.median(times))
     gcups = cells / ms_step / 1e6
-    # the same cascade with the f32 MSV on every pair (no bound): time and identical records
+    # the same cascade with the f32 MSV and Viterbi on every pair of their stages (no
+    # bounds): its time, and the same records (decisions and scores; a record the Viterbi
+    # bound rejected carries no f32 Viterbi score)
     f32_leg = None
     if bound and not args.no_f32_leg:
-        ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=False, **f)
-        s0, recs0 = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=False, **f)
+        ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=False, vit_bound=False, **f)
+        s0, recs0 = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=False, vit_bound=False, **f)
         key0 = lambda a: np.lexsort((a["target"], a["profile"]))
+        a, b = recs0[key0(recs0)], recs[key0(recs)]
+        scored = ~np.isnan(b["vit"])
+        same = bool(np.array_equal(a[["profile", "target", "stage"]], b[["profile", "target", "stage"]])
+                    and all(np.array_equal(a[k], b[k], equal_nan=True) for k in ("msv", "bias", "fwd64"))
+                    and np.array_equal(a["vit"][scored], b["vit"][scored])
+                    and bool(np.all(b["stage"][~scored & (a["stage"] >= 2)] == 2)))
         f32_leg = {"ms_per_step": round(s0.ms_total, 3), "gcups": round(cells / s0.ms_total / 1e6, 2),
-                   "speedup_of_bound": round(s0.ms_total / ms_step, 3),
-                   "identical_records": bool(recs0[key0(recs0)].tobytes() == recs[key0(recs)].tobytes())}
-        log(f"[bench] f32-MSV cascade: {f32_leg}")
+                   "speedup_of_bounds": round(s0.ms_total / ms_step, 3), "same_decisions_and_scores": same}
+        log(f"[bench] f32 cascade (no bounds): {f32_leg}")
     # per-stage split from one eager (event-timed) run of the same chain
-    e, erecs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, msv_bound=bound, **f)
+    e, erecs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, msv_bound=bound, vit_bound=vbound, **f)
     stages = {
         "msv": {"ms": round(e.ms_msv, 3), "cells": int(e.cells_msv), "gcups": round(e.cells_msv / e.ms_msv / 1e6, 1)},
         "bias+compaction": {"ms": round(e.ms_bias, 3), "pairs": int(e.n_msv_pass)},
@@ -336,6 +346,18 @@ def main():
         stages["msv"]["candidates"] = {"pairs": int(e.n_msv_cand), "cells": int(e.cells_msv_exact),
                                        "ms": round(e.ms_msv - e.ms_msv_bound, 3),
                                        "fraction_of_pairs": round(e.n_msv_cand / max(e.n_pairs, 1), 5)}
+    # the int16 Viterbi bound in front of the f32 Viterbi (one eager run, or the timed
+    # cascade's own split when it runs with the bound)
+    ev = e
+    if not vbound:
+        ev, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, msv_bound=bound, vit_bound=True, **f)
+    viterbi_bound = {"in_timed_cascade": vbound, "viterbi_stage_ms": round(ev.ms_vit, 3),
+                     "bound_ms": round(ev.ms_vit_bound, 3),
+                     "bound_gcups": round(ev.cells_vit / max(ev.ms_vit_bound, 1e-9) / 1e6, 1),
+                     "candidates": int(ev.n_vit_cand),
+                     "fraction_of_pairs": round(ev.n_vit_cand / max(ev.n_bias_pass, 1), 5),
+                     "f32_viterbi_stage_ms_without_bound": None if vbound else round(e.ms_vit, 3)}
+    log(f"[bench] Viterbi bound: {viterbi_bound}")
     log(f"[bench] eager split: {stages}")
     key = lambda a: np.lexsort((a["target"], a["profile"]))
     same_as_eager = bool(np.array_equal(recs[key(recs)][["profile", "target", "stage"]],
@@ -344,7 +366,7 @@ def main():
     # plain form on a profile sample, and the batched form on exactly the same pairs
     plain = None
     if args.plain_sample > 0:
-        plain = plain_leg(sxa, ctx, wl, max(1, int(round(len(ms) * args.plain_sample))), f, graph, bound)
+        plain = plain_leg(sxa, ctx, wl, max(1, int(round(len(ms) * args.plain_sample))), f, graph, bound, vbound)
         log(f"[bench] plain form: {plain}")
 
     # dominant kernel: the longest MSV launch of the eager run (HIP events on the launch
@@ -400,8 +422,10 @@ def main():
         "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "i16+f32" if bound else "f32", "data": "synthetic",
         "config": {"workload": f"{wl.name}: {len(ms)} profiles x {len(lens)} proteins", "cells_msv": cells,
-                   "f1": args.f1, "f2": args.f2, "f3": args.f3, "graph": graph, "msv_bound": bound},
-        "f32_msv_cascade": f32_leg,
+                   "f1": args.f1, "f2": args.f2, "f3": args.f3, "graph": graph, "msv_bound": bound,
+                   "viterbi_bound": vbound},
+        "f32_cascade_without_bounds": f32_leg,
+        "viterbi_bound": viterbi_bound,
         "stages": stages,
         "survivors": {"msv": int(e.n_msv_pass), "bias": int(e.n_bias_pass), "viterbi": int(e.n_vit_pass),
                       "forward": int(e.n_fwd_pass)},

@@ -141,20 +141,30 @@ typedef struct {
 typedef struct {
     double f1, f2, f3;          /* in: PipelineConfig thresholds */
     int32_t use_graph;          /* in: batched form replays a captured HIP graph */
-    /* in: 1 = the MSV stage first runs the int16 MSV bound (SX_STAGE_MSV16) on every
-     * pair and computes the f32 MSV only for the pairs whose bound reaches the pair's
-     * threshold; the bound never lies below the f32 score, so the survivors, their
-     * scores and every decision are the same as with 0 (f32 MSV on every pair). */
-    int32_t msv_bound;
+    /* in: integer bounds in front of the exact stages, a mask of SX_BOUND_*:
+     * SX_BOUND_MSV: the MSV stage first runs the int16 MSV bound (SX_STAGE_MSV16) on every
+     *   pair and the f32 MSV only on the pairs whose bound reaches the pair's threshold;
+     * SX_BOUND_VITERBI: the Viterbi stage first runs the int16 Viterbi bound
+     *   (SX_STAGE_VIT16) and the f32 Viterbi only on its candidates.
+     * A bound never lies below the f32 score, so every decision, the survivors and their
+     * scores are the same as with 0; the one difference is that a record rejected by
+     * the Viterbi bound carries vit = NaN (its f32 Viterbi score was not computed). */
+    int32_t bounds;
     uint64_t n_pairs, n_msv_pass, n_bias_pass, n_vit_pass, n_fwd_pass;  /* out */
     double ms_msv, ms_bias, ms_vit, ms_fwd, ms_total;                   /* out: device time */
     uint64_t cells_msv, cells_vit, cells_fwd;                           /* out: DP cells */
-    /* out (msv_bound = 1): pairs whose bound reached the threshold (rescored in f32),
-     * device time of the bound's launches alone, and the rescored cells (eager runs) */
+    /* out (SX_BOUND_MSV): pairs whose bound reached the threshold (rescored in f32),
+     * device time of the bound's launches alone (eager runs), and the rescored cells
+     * (eager runs) */
     uint64_t n_msv_cand;
     double ms_msv_bound;
     uint64_t cells_msv_exact;
+    /* out (SX_BOUND_VITERBI): the same for the Viterbi stage */
+    uint64_t n_vit_cand;
+    double ms_vit_bound;
 } sx_cascade_stats;
+#define SX_BOUND_MSV 1
+#define SX_BOUND_VITERBI 2
 
 int sx_run_cascade(sx_ctx* ctx, int form, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap,
                    uint64_t* n_out);

@@ -72,14 +72,15 @@ class sx_pair_scores(ctypes.Structure):
 
 class sx_cascade_stats(ctypes.Structure):
     _fields_ = [("f1", ctypes.c_double), ("f2", ctypes.c_double), ("f3", ctypes.c_double),
-                ("use_graph", ctypes.c_int32), ("msv_bound", ctypes.c_int32), ("n_pairs", ctypes.c_uint64),
+                ("use_graph", ctypes.c_int32), ("bounds", ctypes.c_int32), ("n_pairs", ctypes.c_uint64),
                 ("n_msv_pass", ctypes.c_uint64), ("n_bias_pass", ctypes.c_uint64),
                 ("n_vit_pass", ctypes.c_uint64), ("n_fwd_pass", ctypes.c_uint64),
                 ("ms_msv", ctypes.c_double), ("ms_bias", ctypes.c_double), ("ms_vit", ctypes.c_double),
                 ("ms_fwd", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("cells_msv", ctypes.c_uint64), ("cells_vit", ctypes.c_uint64), ("cells_fwd", ctypes.c_uint64),
                 ("n_msv_cand", ctypes.c_uint64), ("ms_msv_bound", ctypes.c_double),
-                ("cells_msv_exact", ctypes.c_uint64)]
+                ("cells_msv_exact", ctypes.c_uint64), ("n_vit_cand", ctypes.c_uint64),
+                ("ms_vit_bound", ctypes.c_double)]
 
 
 class sx_search_cfg(ctypes.Structure):
@@ -380,14 +381,16 @@ class Context:
         return out, sc
 
     def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, use_graph: bool = True,
-                    cap: Optional[int] = None, msv_bound: bool = True):
+                    cap: Optional[int] = None, msv_bound: bool = True, vit_bound: bool = False):
         """Returns (stats, records) for every MSV survivor (see sx_run_cascade).  With cap the
         records are truncated to cap; without it all of them are fetched after the run.
-        msv_bound: run the int16 MSV bound first and the f32 MSV only on its candidates."""
+        msv_bound / vit_bound: run the int16 MSV / Viterbi bound first and the f32 stage only
+        on its candidates (records the Viterbi bound rejects carry vit = NaN).  The Viterbi
+        bound is off by default: on gfx950 it is no faster than the f32 Viterbi."""
         st = sx_cascade_stats()
         st.f1, st.f2, st.f3 = f1, f2, f3
         st.use_graph = 1 if use_graph else 0
-        st.msv_bound = 1 if msv_bound else 0
+        st.bounds = (1 if msv_bound else 0) | (2 if vit_bound else 0)
         n = ctypes.c_uint64()
         if cap is not None:
             out = np.empty(cap, PAIR_DTYPE)

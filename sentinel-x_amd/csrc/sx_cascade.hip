@@ -327,6 +327,35 @@ sxc_vit_pass(const int64_t* __restrict__ n_dev, const int32_t* __restrict__ list
     flag[i] = pass ? 1 : 0;
 }
 
+// Candidates of the quantised Viterbi bound among the Viterbi stage's pairs (list entries
+// i < *n_dev, survivor indices list[i]).  The Viterbi test (pipeline.rs:174-178) passes a
+// pair only if fl(fl(v - bias) / ln2) >= thr, hence v >= bias + ln2 * thr - delta with
+// delta = 2^-21 (|bias| + |ln2 * thr|); 1e-6 (1 + |bias| + |ln2 * thr|) exceeds it.  The
+// threshold in the bound's units is capped at the target's saturation floor capv[t].
+// Every entry is marked as having reached the Viterbi stage (res_stage 2); the f32 test
+// of the candidates overwrites it.  Profiles with noqv set are candidates throughout.
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxc_vitq_cand(const int64_t* __restrict__ n_dev, const int32_t* __restrict__ list, const int32_t* __restrict__ vq,
+              const float* __restrict__ s_bias, const int32_t* __restrict__ s_p, const int32_t* __restrict__ s_t,
+              const float* __restrict__ thr, const uint8_t* __restrict__ noqv, const int64_t* __restrict__ capv,
+              int32_t* res_stage, uint8_t* flag) {
+    const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (i >= *n_dev) return;
+    const int32_t s1 = list[i];
+    const int32_t p = s_p[s1];
+    res_stage[s1] = 2;
+    bool pass = true;
+    if (!noqv[p]) {
+        const double b = (double)s_bias[s1];
+        const double ct = (double)SXC_LN2F * (double)thr[p];
+        const double F = SX_Q_FINE;
+        const int64_t th = (int64_t)floor(F * (b + ct) - F * 1e-6 * (1.0 + fabs(b) + fabs(ct)));
+        const int64_t cap = capv[s_t[s1]];
+        pass = (int64_t)vq[i] >= (th < cap ? th : cap);
+    }
+    flag[i] = pass ? 1 : 0;
+}
+
 // Forward test (pipeline.rs:179-183): exp(exp_logsurv(y)) <= F3 passes.
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_fwd_pass(const int64_t* __restrict__ n_dev, const int32_t* __restrict__ list, const double* __restrict__ fsc,

@@ -656,7 +656,9 @@ __device__ __forceinline__ int32_t vitq_pair(const uint32_t* __restrict__ tab, i
     static_assert(Q % 2 == 0, "two positions per register");
     static_assert(G == 1 || tm_aligned<TM>(), "the int16 tables are laid out for one-lane or aligned groups");
     constexpr int H = Q / 2;
-    constexpr bool HOIST = H <= 8;  // small blocks keep the lane's transitions in registers
+    // transitions are re-read from LDS every row: keeping them in registers (8 x H) cost
+    // more occupancy than the reads (129 VGPRs at Q = 16: three waves per SIMD)
+    constexpr bool HOIST = false;
     const int j = (int)(threadIdx.x & (G - 1));
     const uint32_t* base;
     int width, tofs, twid;

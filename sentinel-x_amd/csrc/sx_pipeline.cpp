@@ -129,6 +129,8 @@ __global__ void sxc_msv_pass_list(int, const int64_t*, const int32_t*, const int
                                   const float*, const int32_t*, const int32_t*, int32_t*, const int64_t*, int64_t,
                                   int32_t*, int32_t*, int32_t*, float*);
 __global__ void sxc_clamp(const int64_t*, int64_t, int64_t*);
+__global__ void sxc_vitq_cand(const int64_t*, const int32_t*, const int32_t*, const float*, const int32_t*,
+                              const int32_t*, const float*, const uint8_t*, const int64_t*, int32_t*, uint8_t*);
 __global__ void sxc_records(const int64_t*, int64_t, const int32_t*, const int32_t*, const float*, const float*,
                             const float*, const double*, const int32_t*, void*);
 }
@@ -153,7 +155,8 @@ struct CascadeState {
     size_t nprof = 0;
     uint32_t ntarg = 0;
     double f1 = -1, f2 = -1, f3 = -1;
-    int bound = -1;  // the plan runs the int16 MSV bound before the f32 MSV
+    int bound = -1;   // the plan runs the int16 MSV bound before the f32 MSV
+    int vbound = -1;  // the plan runs the int16 Viterbi bound before the f32 Viterbi
     int64_t cap1 = 0;
     // records of the last run (one per MSV survivor), kept for sx_cascade_records
     std::vector<sx_pair_scores> last;
@@ -165,9 +168,12 @@ struct CascadeState {
     // profile) and the profiles it skips (every pair a candidate)
     std::vector<int64_t> nullq, capq, thrq;
     std::vector<uint8_t> noq;
+    // the int16 Viterbi bound: profiles it skips, per-target saturation floors (by id)
+    std::vector<uint8_t> noqv;
+    std::vector<int64_t> capv;
     // device tables
     DeviceBuf d_thr_msv, d_thr_vit, d_thr_fwd, d_nullsc, d_perm, d_eo1, d_ng_vit, d_ng_fwd;
-    DeviceBuf d_nullq, d_capq, d_thrq, d_noq, d_ng_msv;
+    DeviceBuf d_nullq, d_capq, d_thrq, d_noq, d_ng_msv, d_noqv, d_capv, d_ng_vq;
     std::vector<DeviceBuf> msv_plists;
     struct Group {
         Shape s;
@@ -182,6 +188,7 @@ struct CascadeState {
     };
     std::vector<Run> vit_runs, fwd_runs;
     std::vector<Run> msv_runs;  // f32 MSV over the bound's candidates
+    std::vector<Run> vq_runs;   // the int16 Viterbi bound (profiles it covers)
     // device work buffers
     DeviceBuf usc, cnt1, off1, s_p, s_pi, s_t, s_usc, s_bias, flag, res_vit, res_fwd, res_stage;
     DeviceBuf cnt2, off2, list2, per2, tcount, toff2, eoff2, tiles2, tids2, vsc, flag3;
@@ -191,9 +198,13 @@ struct CascadeState {
     DeviceBuf c_pi, c_j, c_t, cusc, cntq, offq, perq, toffq, eoffq, tilesq, nclamp;
     const int64_t* d_n1 = nullptr;     // raw survivor count (may exceed cap1)
     const int64_t* d_ncand = nullptr;  // raw candidate count
+    // the Viterbi bound's results, candidate flags, candidate list and its tiles
+    DeviceBuf vq, flagq, cntc, offc, listc, perc, toffc, eoffc, tilesc, tidsc;
+    const int64_t* d_nvc = nullptr;    // Viterbi candidates
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t evq = nullptr;  // after the int16 bound's launches (eager runs)
+    hipEvent_t evq = nullptr;   // after the int16 MSV bound's launches (eager runs)
+    hipEvent_t evvq = nullptr;  // after the int16 Viterbi bound's launches (eager runs)
     // an event before (lev0) and after (lev) every kernel launch of the three DP stages,
     // on the launch's own stream (eager runs only)
     std::vector<hipEvent_t> lev0, lev;
@@ -203,6 +214,7 @@ struct CascadeState {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (evq) (void)hipEventDestroy(evq);
+        if (evvq) (void)hipEventDestroy(evvq);
         for (auto& e : lev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : lev0)
@@ -212,7 +224,8 @@ struct CascadeState {
 
 void cascade_release(Ctx& c) { c.cascade.reset(); }
 
-static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3, int bound) {
+static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3, int bounds) {
+    const int bound = (bounds & SX_BOUND_MSV) ? 1 : 0, vbound = (bounds & SX_BOUND_VITERBI) ? 1 : 0;
     const size_t P = c.profs.size();
     const uint32_t T = c.nt;
     for (const auto& p : c.profs)
@@ -225,6 +238,7 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
     S.f2 = f2;
     S.f3 = f3;
     S.bound = bound;
+    S.vbound = vbound;
     S.thr_msv.resize(P);
     S.thr_vit.resize(P);
     S.thr_fwd.resize(P);
@@ -251,10 +265,23 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
     S.perm.resize(P);
     std::iota(S.perm.begin(), S.perm.end(), 0);
     std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) { return c.profs[a].m < c.profs[b].m; });
-    std::vector<int32_t> ngv(P), ngf(P), ngm(P);
+    std::vector<int32_t> ngv(P), ngf(P), ngm(P), ngq(P, 64);
     S.vit_runs.clear();
     S.fwd_runs.clear();
     S.msv_runs.clear();
+    S.vq_runs.clear();
+    // the Viterbi bound skips models beyond its shapes, profiles without a finite
+    // threshold, and profiles with a positive insert score (an insert state could then
+    // climb past the M states it comes from, and the saturation floor would not hold)
+    S.noqv.assign(P, 1);
+    if (vbound)
+        for (size_t p = 0; p < P; p++) {
+            const Profile& pr = c.profs[p];
+            bool ok = pr.m <= SX_Q_MAX_M && std::isfinite(S.thr_vit[p]);
+            for (int k = 1; k <= pr.m && ok; k++)
+                if (q_trans(pr.t_row(SX_TMI)[k]) > 0 || q_trans(pr.t_row(SX_TII)[k]) > 0) ok = false;
+            S.noqv[p] = ok ? 0 : 1;
+        }
     for (size_t i = 0; i < P; i++) {
         const int m = c.profs[S.perm[i]].m;
         const Shape sv = c.shape_for(1, SX_FORM_BATCHED, m), sf = c.shape_for(2, SX_FORM_BATCHED, m);
@@ -271,7 +298,29 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
         add(S.vit_runs, sv);
         add(S.fwd_runs, sf);
         if (bound) add(S.msv_runs, sm);
+        if (vbound && !S.noqv[S.perm[i]]) {
+            const Shape sq = c.shape_for(SX_STAGE_VIT16, SX_FORM_BATCHED, m);
+            ngq[i] = c.kernel(SX_STAGE_VIT16, sq.q, sq.g, false, sq.hoist).block / sq.g;
+            // a run never spans a skipped profile: its tiles are not the bound's to score
+            if (!S.vq_runs.empty() && S.vq_runs.back().b == (int64_t)i && S.vq_runs.back().s.q == sq.q &&
+                S.vq_runs.back().s.g == sq.g && S.vq_runs.back().s.hoist == sq.hoist)
+                S.vq_runs.back().b = (int64_t)i + 1;
+            else
+                S.vq_runs.push_back({sq, (int64_t)i, (int64_t)i + 1});
+        }
     }
+    S.capv.assign(T, 0);
+    if (vbound)
+        for (uint32_t t = 0; t < T; t++) {
+            const int32_t l = c.tlen[t];
+            if (l > SX_Q_MAX_L) {
+                S.capv[t] = INT64_MIN / 4;
+                continue;
+            }
+            const Xsc x = Xsc::make((size_t)l, true);
+            S.capv[t] = 32767LL * 4096 + q_const(-LN2F) + (int64_t)l * std::min<int64_t>(q_const(x.n_loop), 0) +
+                        q_const(x.n_move);
+        }
     // The int16 bound's thresholds.  The f32 test passes u only if
     // fl(fl(u - null) / ln2) >= thr, so u >= null + ln2 * thr - delta, where delta = 2^-21
     // (|null| + |ln2 * thr|) covers the two roundings; 1e-6 (1 + |.|) per term exceeds it.
@@ -341,10 +390,15 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
     up(S.d_capq, S.capq.data(), (size_t)T * 8);
     up(S.d_thrq, S.thrq.data(), P * 8);
     up(S.d_noq, S.noq.data(), P);
+    up(S.d_noqv, S.noqv.data(), P);
+    up(S.d_capv, S.capv.data(), (size_t)T * 8);
+    up(S.d_ng_vq, ngq.data(), P * 4);
     for (auto& e : S.ev)
         if (!e) HIPCHK(hipEventCreate(&e));
     if (!S.evq) HIPCHK(hipEventCreate(&S.evq));
-    const size_t nl = S.msv_groups.size() + S.msv_runs.size() + S.vit_runs.size() + S.fwd_runs.size();
+    if (!S.evvq) HIPCHK(hipEventCreate(&S.evvq));
+    const size_t nl =
+        S.msv_groups.size() + S.msv_runs.size() + S.vq_runs.size() + S.vit_runs.size() + S.fwd_runs.size();
     while (S.lev.size() < nl) {
         S.lev.push_back(nullptr);
         HIPCHK(hipEventCreate(&S.lev.back()));
@@ -370,6 +424,16 @@ static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
         dptr<int32_t>(S.perq, P);
         for (DeviceBuf* b : {&S.toffq, &S.eoffq}) dptr<int64_t>(*b, P + 1);
         dptr<SxTile>(S.tilesq, cap1 + P);
+    }
+    if (S.vbound) {
+        dptr<int32_t>(S.vq, cap1);
+        dptr<uint8_t>(S.flagq, cap1);
+        dptr<int32_t>(S.cntc, nbc);
+        dptr<int64_t>(S.offc, nbc + 1);
+        for (DeviceBuf* b : {&S.listc, &S.tidsc}) dptr<int32_t>(*b, cap1);
+        dptr<int32_t>(S.perc, P);
+        for (DeviceBuf* b : {&S.toffc, &S.eoffc}) dptr<int64_t>(*b, P + 1);
+        dptr<SxTile>(S.tilesc, cap1 + P);
     }
     dptr<int32_t>(S.s_p, cap1);
     dptr<int32_t>(S.s_pi, cap1);
@@ -540,16 +604,34 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         return (const int64_t*)off.p + nbc;
     };
     const int64_t* n2 = compact((const uint8_t*)S.flag.p, n1, nullptr, S.cnt2, S.off2, S.list2, S.per2);
-    tiles(S.per2, S.d_ng_vit, S.toff2, S.eoff2, S.tiles2, n2, &S.list2, &S.tids2);
+    // the Viterbi stage's tiles: sized for the bound's shapes when it runs first
+    tiles(S.per2, S.vbound ? S.d_ng_vq : S.d_ng_vit, S.toff2, S.eoff2, S.tiles2, n2, &S.list2, &S.tids2);
     if (with_events) HIPCHK(hipEventRecord(S.ev[2], st));
-    // 5. Viterbi and its test
-    stage_runs(1, S.vit_runs, S.tiles2, S.tids2, S.toff2, S.vsc.p);
-    hipLaunchKernelGGL(sxc_vit_pass, dim3(nbc), dim3(CB), 0, st, n2, (const int32_t*)S.list2.p,
+    // 5. Viterbi (after the int16 Viterbi bound and its candidate test) and its test
+    const int64_t* nv = n2;
+    DeviceBuf* vlist = &S.list2;
+    if (S.vbound) {
+        stage_runs(SX_STAGE_VIT16, S.vq_runs, S.tiles2, S.tids2, S.toff2, S.vq.p);
+        if (with_events) HIPCHK(hipEventRecord(S.evvq, st));
+        hipLaunchKernelGGL(sxc_vitq_cand, dim3(nbc), dim3(CB), 0, st, n2, (const int32_t*)S.list2.p,
+                           (const int32_t*)S.vq.p, (const float*)s_bias, (const int32_t*)s_p, (const int32_t*)s_t,
+                           (const float*)S.d_thr_vit.p, (const uint8_t*)S.d_noqv.p, (const int64_t*)S.d_capv.p,
+                           (int32_t*)S.res_stage.p, (uint8_t*)S.flagq.p);
+        nv = compact((const uint8_t*)S.flagq.p, n2, (const int32_t*)S.list2.p, S.cntc, S.offc, S.listc, S.perc);
+        S.d_nvc = nv;
+        tiles(S.perc, S.d_ng_vit, S.toffc, S.eoffc, S.tilesc, nv, &S.listc, &S.tidsc);
+        stage_runs(1, S.vit_runs, S.tilesc, S.tidsc, S.toffc, S.vsc.p);
+        vlist = &S.listc;
+    } else {
+        S.d_nvc = nullptr;
+        stage_runs(1, S.vit_runs, S.tiles2, S.tids2, S.toff2, S.vsc.p);
+    }
+    hipLaunchKernelGGL(sxc_vit_pass, dim3(nbc), dim3(CB), 0, st, nv, (const int32_t*)vlist->p,
                        (const float*)S.vsc.p, (const float*)s_bias, (const int32_t*)s_p,
                        (const float*)S.d_thr_vit.p, (float*)S.res_vit.p, (int32_t*)S.res_stage.p,
                        (uint8_t*)S.flag3.p);
     const int64_t* n3 =
-        compact((const uint8_t*)S.flag3.p, n2, (const int32_t*)S.list2.p, S.cnt3, S.off3, S.list3, S.per3);
+        compact((const uint8_t*)S.flag3.p, nv, (const int32_t*)vlist->p, S.cnt3, S.off3, S.list3, S.per3);
     tiles(S.per3, S.d_ng_fwd, S.toff3, S.eoff3, S.tiles3, n3, &S.list3, &S.tids3);
     if (with_events) HIPCHK(hipEventRecord(S.ev[3], st));
     // 6. Forward and its test
@@ -574,10 +656,10 @@ static float ev_ms(hipEvent_t a, hipEvent_t b) {
 static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap) {
     if (!c.cascade) c.cascade = std::make_shared<CascadeState>();
     CascadeState& S = *c.cascade;
-    const int bound = st->msv_bound ? 1 : 0;
+    const int bound = (st->bounds & SX_BOUND_MSV) ? 1 : 0, vbound = (st->bounds & SX_BOUND_VITERBI) ? 1 : 0;
     if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
-        S.bound != bound || c.cascade_dirty) {
-        build_plan(c, S, st->f1, st->f2, st->f3, bound);
+        S.bound != bound || S.vbound != vbound || c.cascade_dirty) {
+        build_plan(c, S, st->f1, st->f2, st->f3, st->bounds);
         const int64_t P = (int64_t)c.profs.size(), T = c.nt;
         // profiles the bound skips make every pair a candidate
         int64_t nskip = 0;
@@ -608,12 +690,13 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
             HIPCHK(hipEventRecord(S.ev[4], c.stream));
             HIPCHK(hipEventSynchronize(S.ev[4]));
             total = ev_ms(S.ev[0], S.ev[4]);
-            st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = st->ms_msv_bound = 0.0;
+            st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = st->ms_msv_bound = st->ms_vit_bound = 0.0;
         } else {
             enqueue_chain(c, S, true);
             HIPCHK(hipEventSynchronize(S.ev[4]));
             st->ms_msv = ev_ms(S.ev[0], S.ev[1]);
             st->ms_msv_bound = S.bound ? ev_ms(S.ev[0], S.evq) : 0.0;
+            st->ms_vit_bound = S.vbound ? ev_ms(S.ev[2], S.evvq) : 0.0;
             st->ms_bias = ev_ms(S.ev[1], S.ev[2]);
             st->ms_vit = ev_ms(S.ev[2], S.ev[3]);
             st->ms_fwd = ev_ms(S.ev[3], S.ev[4]);
@@ -631,6 +714,9 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
             continue;
         }
         st->n_msv_cand = (uint64_t)ncand;
+        int64_t nvc = 0;
+        if (S.d_nvc) HIPCHK(hipMemcpy(&nvc, S.d_nvc, 8, hipMemcpyDeviceToHost));
+        st->n_vit_cand = (uint64_t)nvc;
         st->cells_msv_exact = 0;
         std::vector<uint64_t> cand_run_cells(S.msv_runs.size(), 0);
         if (S.bound && !st->use_graph && ncand > 0) {
@@ -675,7 +761,8 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
             auto run_cells = [&](const std::vector<CascadeState::Run>& runs, int min_stage) {
                 std::vector<uint64_t> v(runs.size(), 0);
                 for (const auto& r : recs) {
-                    if (r.stage < min_stage) continue;
+                    // min_stage 5: the records that reached Viterbi and got an f32 score
+                    if (min_stage == 5 ? (r.stage < 2 || std::isnan(r.vit)) : r.stage < min_stage) continue;
                     const int64_t pi = pos[r.profile];
                     for (size_t k = 0; k < runs.size(); k++)
                         if (pi >= runs[k].a && pi < runs[k].b) {
@@ -685,7 +772,9 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
                 }
                 return v;
             };
-            const std::vector<uint64_t> vc = run_cells(S.vit_runs, 2), fc = run_cells(S.fwd_runs, 3);
+            // the f32 Viterbi scores the bound's candidates only: the records with a score
+            const std::vector<uint64_t> vc = run_cells(S.vit_runs, S.vbound ? 5 : 2), fc = run_cells(S.fwd_runs, 3);
+            const std::vector<uint64_t> qc = run_cells(S.vq_runs, 2);
             size_t e = 0;
             auto add = [&](int stage, const Shape& sh, uint64_t cells) {
                 sx_launch_stat l{};
@@ -701,6 +790,7 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
             for (size_t i = 0; i < S.msv_groups.size(); i++)
                 add(S.bound ? SX_STAGE_MSV16 : 0, S.msv_groups[i].s, S.msv_groups[i].cells);
             for (size_t i = 0; i < S.msv_runs.size(); i++) add(0, S.msv_runs[i].s, cand_run_cells[i]);
+            for (size_t i = 0; i < S.vq_runs.size(); i++) add(SX_STAGE_VIT16, S.vq_runs[i].s, qc[i]);
             for (size_t i = 0; i < S.vit_runs.size(); i++) add(1, S.vit_runs[i].s, vc[i]);
             for (size_t i = 0; i < S.fwd_runs.size(); i++) add(2, S.fwd_runs[i].s, fc[i]);
         }
@@ -714,10 +804,10 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
 static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uint64_t cap) {
     if (!c.cascade) c.cascade = std::make_shared<CascadeState>();
     CascadeState& S = *c.cascade;
-    const int bound = st->msv_bound ? 1 : 0;
+    const int bound = (st->bounds & SX_BOUND_MSV) ? 1 : 0, vbound = (st->bounds & SX_BOUND_VITERBI) ? 1 : 0;
     if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
-        S.bound != bound || c.cascade_dirty) {
-        build_plan(c, S, st->f1, st->f2, st->f3, bound);
+        S.bound != bound || S.vbound != vbound || c.cascade_dirty) {
+        build_plan(c, S, st->f1, st->f2, st->f3, st->bounds);
         alloc_buffers(c, S, 1 << 16);
         c.cascade_dirty = false;
     }
@@ -730,7 +820,7 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     DeviceBuf d_pp, d_pt, d_pu, d_pb, d_fl, d_n;
     std::vector<double> eo1(25);
     double ms_total = 0.0;
-    uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0, ncand = 0, cq = 0;
+    uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0, ncand = 0, cq = 0, nvc = 0;
     // the bound's null thresholds by target id (S.nullq is in length-sorted order)
     std::vector<int64_t> nullq_t, capq_t;
     if (bound) {
@@ -830,6 +920,38 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
         }
         if (vp.empty()) continue;
         n2 += vp.size();
+        for (size_t i : vidx) recs[i].stage = 2;
+        for (uint32_t t : vt) cv += (uint64_t)pr.m * (uint64_t)c.tlen[t];  // every pair entering Viterbi
+        if (vbound && !S.noqv[p]) {
+            // the int16 Viterbi bound over this profile's survivors (one launch), then the
+            // f32 Viterbi on the pairs whose bound reaches the threshold
+            void* dq = c.scratch_x5.ensure(vp.size() * 8);
+            c.run_pairs(SX_STAGE_VIT16, SX_FORM_PLAIN, vp.data(), vt.data(), vp.size(), dq, false, nullptr, nullptr,
+                        nullptr);
+            std::vector<int32_t> qv(vp.size()), tmp(vp.size());
+            HIPCHK(hipMemcpyAsync(tmp.data(), dq, vp.size() * 4, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            for (size_t i = 0; i < vp.size(); i++) qv[c.last_order[i]] = tmp[i];
+            const double F = SX_Q_FINE, ct = (double)LN2F * (double)S.thr_vit[p];
+            std::vector<uint32_t> kp, kt;
+            std::vector<size_t> kidx;
+            for (size_t i = 0; i < vp.size(); i++) {
+                const double b = (double)recs[vidx[i]].bias;
+                const int64_t th = (int64_t)std::floor(F * (b + ct) - F * 1e-6 * (1.0 + std::fabs(b) + std::fabs(ct)));
+                if ((int64_t)qv[i] >= std::min(th, S.capv[vt[i]])) {
+                    kp.push_back(vp[i]);
+                    kt.push_back(vt[i]);
+                    kidx.push_back(vidx[i]);
+                }
+            }
+            nvc += kp.size();
+            vp.swap(kp);
+            vt.swap(kt);
+            vidx.swap(kidx);
+            if (vp.empty()) continue;
+        } else if (vbound) {
+            nvc += vp.size();
+        }
         // Viterbi: one launch for this profile's survivors
         void* dv = c.scratch_x5.ensure(vp.size() * 8);
         c.run_pairs(1, SX_FORM_PLAIN, vp.data(), vt.data(), vp.size(), dv, false, nullptr, nullptr, nullptr);
@@ -846,7 +968,6 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
             sx_pair_scores& r = recs[vidx[i]];
             r.vit = vrow[i];
             r.stage = 2;
-            cv += (uint64_t)pr.m * (uint64_t)c.tlen[vt[i]];
             const float y = (vrow[i] - r.bias) / LN2F;
             if (y >= S.thr_vit[p]) {
                 r.stage = 3;
@@ -893,6 +1014,8 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     st->n_msv_cand = ncand;
     st->cells_msv_exact = cq;
     st->ms_msv_bound = 0.0;
+    st->n_vit_cand = nvc;
+    st->ms_vit_bound = 0.0;
     if (out) std::memcpy(out, recs.data(), std::min<uint64_t>(cap, recs.size()) * sizeof(sx_pair_scores));
     S.last.swap(recs);
     return S.last.size();
@@ -908,7 +1031,7 @@ std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1,
     st.f2 = f2;
     st.f3 = f3;
     st.use_graph = 1;
-    st.msv_bound = 1;
+    st.bounds = SX_BOUND_MSV;  // the int16 Viterbi bound is no faster than the f32 Viterbi
     c.ensure_pidx();
     c.cascade_pass_uncalibrated = true;
     c.cascade_dirty = true;

@@ -61,7 +61,10 @@ def test_batched_cascade_matches_oracle(orc, gctx, calibrated):
         if np.float32(r["bias"]) == out[1]:
             same_bias += 1
             assert r["stage"] == stage
-            if stage >= 2:
+            # a record rejected by the int16 Viterbi bound (stage 2) has no f32 score
+            if stage >= 3:
+                assert not np.isnan(r["vit"])
+            if stage >= 2 and not np.isnan(r["vit"]):
                 assert np.float32(r["vit"]) == out[2]
         else:
             assert abs(float(r["bias"]) - float(out[1])) <= 1e-4 * max(1.0, abs(float(out[1])))
@@ -110,16 +113,25 @@ def test_records_fetch_matches_truncated_run(gctx, calibrated):
     assert same(gctx.cascade_records(), plain)
 
 
-@pytest.mark.parametrize("bound", [False, True], ids=["f32", "int16-bound"])
-def test_launch_timing_covers_each_stage(gctx, calibrated, bound):
+@pytest.mark.parametrize("bound,vbound", [(False, False), (True, False), (True, True)],
+                         ids=["f32", "msv-bound", "msv-and-viterbi-bounds"])
+def test_launch_timing_covers_each_stage(gctx, calibrated, bound, vbound):
     hmms, _, seqs = calibrated
     gctx.set_profiles(hmms)
     gctx.set_targets(seqs)
-    st, _ = gctx.run_cascade(1, use_graph=False, msv_bound=bound, **F)
+    st, rec = gctx.run_cascade(1, use_graph=False, msv_bound=bound, vit_bound=vbound, **F)
     ls = gctx.cascade_launches()
     assert np.all(ls["ms"] >= 0)
-    assert int(ls["cells"][ls["stage"] == 1].sum()) == st.cells_vit
     assert int(ls["cells"][ls["stage"] == 2].sum()) == st.cells_fwd
+    if vbound:
+        # the bound scores every pair entering Viterbi, the f32 Viterbi its candidates
+        assert 5 in set(ls["stage"])
+        assert int(ls["cells"][ls["stage"] == 5].sum()) == st.cells_vit
+        scored = rec[(rec["stage"] >= 2) & ~np.isnan(rec["vit"])]
+        assert len(scored) == st.n_vit_cand >= st.n_vit_pass
+        assert st.ms_vit_bound <= st.ms_vit
+    else:
+        assert int(ls["cells"][ls["stage"] == 1].sum()) == st.cells_vit
     if not bound:
         assert len(ls) >= 3 and set(ls["stage"]) == {0, 1, 2}
         assert int(ls["cells"][ls["stage"] == 0].sum()) == st.cells_msv
@@ -129,14 +141,14 @@ def test_launch_timing_covers_each_stage(gctx, calibrated, bound):
         assert abs(float(ls["ms"][ls["stage"] == 0].sum()) - st.ms_msv) <= 0.05 + 0.015 * n0 + 0.01 * st.ms_msv
     else:
         # the bound's launches over every pair, then the f32 MSV of its candidates
-        assert set(ls["stage"]) == {4, 0, 1, 2}
+        assert set(ls["stage"]) - {5} == {4, 0, 1, 2}
         assert int(ls["cells"][ls["stage"] == 4].sum()) == st.cells_msv
         assert int(ls["cells"][ls["stage"] == 0].sum()) == st.cells_msv_exact > 0
         assert st.n_msv_pass <= st.n_msv_cand < st.n_pairs
         n4 = int((ls["stage"] == 4).sum())
         assert abs(float(ls["ms"][ls["stage"] == 4].sum()) - st.ms_msv_bound) <= 0.05 + 0.015 * n4 + 0.01 * st.ms_msv_bound
         assert st.ms_msv_bound <= st.ms_msv
-    gctx.run_cascade(1, use_graph=True, msv_bound=bound, **F)
+    gctx.run_cascade(1, use_graph=True, msv_bound=bound, vit_bound=vbound, **F)
     assert len(gctx.cascade_launches()) == 0
 
 
@@ -148,17 +160,51 @@ def test_int16_bound_leaves_every_record_unchanged(gctx, calibrated):
     gctx.set_targets(seqs)
     key = lambda a: np.lexsort((a["target"], a["profile"]))
     same = lambda a, b: a[key(a)].tobytes() == b[key(b)].tobytes()
-    st0, ref = gctx.run_cascade(1, use_graph=False, msv_bound=False, **F)
-    st1, got = gctx.run_cascade(1, use_graph=False, msv_bound=True, **F)
+    st0, ref = gctx.run_cascade(1, use_graph=False, msv_bound=False, vit_bound=False, **F)
+    st1, got = gctx.run_cascade(1, use_graph=False, msv_bound=True, vit_bound=False, **F)
     assert same(ref, got) and st1.n_msv_pass == st0.n_msv_pass
     assert st1.n_msv_cand >= st1.n_msv_pass
     for _ in range(2):
-        _, g = gctx.run_cascade(1, use_graph=True, msv_bound=True, **F)
+        _, g = gctx.run_cascade(1, use_graph=True, msv_bound=True, vit_bound=False, **F)
         assert same(ref, g)
-    stp, plain = gctx.run_cascade(0, msv_bound=True, **F)
-    _, plain0 = gctx.run_cascade(0, msv_bound=False, **F)
+    stp, plain = gctx.run_cascade(0, msv_bound=True, vit_bound=False, **F)
+    _, plain0 = gctx.run_cascade(0, msv_bound=False, vit_bound=False, **F)
     assert same(plain, plain0)
     assert stp.n_msv_cand == st1.n_msv_cand
+
+
+def test_int16_viterbi_bound_changes_only_unscored_records(gctx, calibrated):
+    """With the Viterbi bound every decision and score stays the same; the records the
+    bound rejects (stage 2) carry no f32 Viterbi score.  Batched, graph and plain forms
+    reject the same pairs."""
+    hmms, _, seqs = calibrated
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    key = lambda a: np.lexsort((a["target"], a["profile"]))
+    st0, ref = gctx.run_cascade(1, use_graph=False, vit_bound=False, **F)
+    st1, got = gctx.run_cascade(1, use_graph=False, vit_bound=True, **F)
+    assert st1.n_vit_cand > 0 and st0.n_vit_cand == 0
+    ref, got = ref[key(ref)], got[key(got)]
+    assert np.array_equal(ref[["profile", "target", "stage"]], got[["profile", "target", "stage"]])
+    for f in ("msv", "bias", "fwd64"):
+        assert np.array_equal(ref[f], got[f], equal_nan=True)
+    nan = np.isnan(got["vit"])
+    assert np.all(got["stage"][nan & (ref["stage"] >= 2)] == 2)
+    assert np.array_equal(got["vit"][~nan], ref["vit"][~nan])
+    assert np.all(np.isnan(ref["vit"][nan]) | (ref["stage"][nan] == 2))
+    assert st1.n_vit_pass == st0.n_vit_pass and st1.n_vit_cand >= st1.n_vit_pass
+    assert st1.n_vit_cand < st1.n_bias_pass  # the bound rejected some pairs
+    for _ in range(2):
+        _, g = gctx.run_cascade(1, use_graph=True, vit_bound=True, **F)
+        assert g[key(g)].tobytes() == got.tobytes()
+    stp, plain = gctx.run_cascade(0, vit_bound=True, **F)
+    plain = plain[key(plain)]
+    # the plain form's Forward runs other lane shapes (another f32 summation order)
+    assert np.array_equal(plain[["profile", "target", "stage"]], got[["profile", "target", "stage"]])
+    for f in ("msv", "bias", "vit"):
+        assert np.array_equal(plain[f], got[f], equal_nan=True)
+    assert np.allclose(plain["fwd64"], got["fwd64"], atol=1e-3, equal_nan=True)
+    assert stp.n_vit_cand == st1.n_vit_cand
 
 
 def test_cascade_requires_calibrated_profiles(orc, gctx, sxa):
