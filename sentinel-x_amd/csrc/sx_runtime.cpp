@@ -325,16 +325,16 @@ Shape Ctx::default_shape(int stage, int form, int m) const {
         return {32, 32, 3};
     }
     if (stage == SX_STAGE_MSV16) {
-        // the quantised MSV bound: one lane per pair up to 192 positions, then 8- and
+        // the quantised MSV bound: one lane per pair up to 256 positions, then 8- and
         // 16-lane groups over aligned lane blocks
         if (m > SX_Q_MAX_M)
             throw SxError(SX_ERR_ARG, "model length " + std::to_string(m) + " exceeds the quantised MSV bound's " +
                                           std::to_string(SX_Q_MAX_M));
-        for (int q : {16, 24, 32, 48, 64, 80, 96, 128, 160, 192})
+        for (int q : {16, 24, 32, 48, 64, 80, 96, 112, 128, 144, 160, 176, 192, 208, 224, 240, 256})
             if (q >= m) return {q, 1, 0};
-        for (int q : {32, 40, 48, 56, 64, 80})
+        for (int q : {40, 48, 56, 64, 72, 80, 88})
             if (8 * q >= m) return {q, 8, 3};
-        for (int q : {48, 64})
+        for (int q : {48, 56, 64})
             if (16 * q >= m) return {q, 16, 3};
     }
     if (m > SX_GEM_MIN_M) return gem_shape(m);
@@ -617,8 +617,12 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
 // profiles against a length-stratified sample of the targets, and keep the fastest.
 // ---------------------------------------------------------------------------
 std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
-    static const int edges[] = {8, 16, 24, 32, 48, 64, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768,
-                                1024, 1280, 1536, 2048, 2560, 3072, 4096};
+    static const std::vector<int> coarse = {8,   16,  24,  32,  48,   64,   96,   128,  160,  192,  256, 320,
+                                            384, 512, 640, 768, 1024, 1280, 1536, 2048, 2560, 3072, 4096};
+    // the MSV bound has a finer menu of capacities (its time follows the capacity)
+    static const std::vector<int> fine = {16,  24,  32,  48,  64,  80,  96,  112, 128, 144, 160, 176, 192, 208, 224,
+                                          240, 256, 288, 320, 384, 448, 512, 576, 640, 704, 768, 896, 1024};
+    const std::vector<int>& edges = stage == SX_STAGE_MSV16 ? fine : coarse;
     std::vector<TuneRecord> log;
     if (profs.empty() || nt == 0) return log;
     ensure_pidx();
