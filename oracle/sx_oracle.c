@@ -254,6 +254,17 @@ static inline int32_t sat16(int32_t v) { return v < -32768 ? -32768 : v > 32767 
 /* ceil(v / 4096) for any sign */
 static inline int32_t ceil4096(int32_t v) { return (int32_t)floor(((double)v + 4095.0) / 4096.0); }
 
+/* Cells hold t = s - F - 32768 for a frame floor F >= the row's B entry, so
+ * max(s[k-1], B) + e[k] is the saturating sum t[k-1] + e[k]; F moves to B on rows where
+ * B exceeds it and on every 16th row (i % 16 == 0) where B is below it, shifting every
+ * cell and position 0 (the entry to position 1) by the move. */
+static inline void q_shift(int32_t* v, int n, int32_t d) {
+    const int32_t d1 = d > 32767 ? 32767 : d, d2 = d - d1;
+    for (int k = 0; k < n; k++) v[k] = sat16(v[k] - d1);
+    if (d2 > 0)
+        for (int k = 0; k < n; k++) v[k] = sat16(v[k] - d2);
+}
+
 int32_t sxo_msv16(int m, const float* msc, float tbmk, const uint8_t* dsq, int64_t l) {
     if (l == 0) return INT32_MIN;
     const int w = m + 1;
@@ -262,27 +273,37 @@ int32_t sxo_msv16(int m, const float* msc, float tbmk, const uint8_t* dsq, int64
     const int32_t tm = sxo_q_const(logf(3.0f / (lf + 3.0f)));
     const int32_t tb = sxo_q_const(tbmk);
     const int32_t te = sxo_q_const(-LN2F);
-    /* positions 0..M; position 0 stays at the floor of the range */
+    int16_t* qe = (int16_t*)malloc(sizeof(int16_t) * (size_t)w * SXO_NCODE);
+    for (int k = 0; k < w * SXO_NCODE; k++) qe[k] = sxo_q_emit(msc[k]);
+    /* positions 0..M in frame units; position 0 is the floor, shifted with the rest */
     int32_t* prev = (int32_t*)malloc(sizeof(int32_t) * (size_t)w * 2);
     int32_t* cur = prev + w;
     for (int k = 0; k < 2 * w; k++) prev[k] = -32768;
     int32_t xn = 0, xj = -(1 << 30), xb = imax(xn, xj) + tm;
+    int32_t fl = ceil4096(xb + tb);
     for (int64_t i = 0; i < l; i++) {
-        const float* rs = msc + (size_t)dsq[i] * w;
-        const int32_t be = imax(ceil4096(xb + tb), -32768);
+        const int16_t* rs = qe + (size_t)dsq[i] * w;
+        const int32_t be = ceil4096(xb + tb);
+        if (be > fl || (i % 16 == 0 && be < fl)) {
+            q_shift(prev, w, be - fl);
+            fl = be;
+        }
         int32_t xe = -32768;
         for (int k = 1; k <= m; k++) {
-            const int32_t v = sat16(imax(prev[k - 1], be) + sxo_q_emit(rs[k]));
+            const int32_t v = sat16(prev[k - 1] + rs[k]);
             cur[k] = v;
             xe = imax(xe, v);
         }
         cur[0] = -32768;
+        xe = xe + fl + 32768;
+        if (xe > 32767) xe = 32767;
         xj = imax(xj + tl, xe * 4096 + te);
         xn += tl;
         xb = imax(xn, xj) + tm;
         int32_t* s = prev; prev = cur; cur = s;
     }
     free(prev < cur ? prev : cur);
+    free(qe);
     return xj + tm;
 }
 

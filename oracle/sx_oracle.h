@@ -65,9 +65,12 @@ float sxo_msv(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_
  * of 2^-22 nat, rounded up.  Each rounding adds the f32 rounding allowance of the
  * value it feeds (magnitude <= 512 nats for a cell, <= 64 nats for N/J/B, which holds
  * for M <= 1024 and L <= 1e6), so the result bounds the f32 MSV from above:
- * 2^22 * sxo_msv() <= sxo_msv16() unless a cell reached the top of the int16 range (32
- * nats), in which case the result is at least 32767*4096 + q(-ln2) + L*min(q(tloop),0)
- * + q(tmove).  L = 0 gives INT32_MIN.  Parity of the device kernel with this function
+ * 2^22 * sxo_msv() <= sxo_msv16() unless a row maximum reached 32767 (32 nats), in
+ * which case the result is at least 32767*4096 + q(-ln2) + L*min(q(tloop),0) + q(tmove).
+ * Cells are kept relative to a frame floor F >= the row's B entry, so that the max with
+ * B is implied by the saturating add; F moves to B when B rises above it and on every
+ * 16th row (B falls with xN), so a segment starts at most 16 rows of that fall
+ * (16 * ceil(|q(tloop)| / 4096) units of 1/1024 nat) above B.  L = 0 gives INT32_MIN.  Parity of the device kernel with this function
  * is bit-exact; this function is pinned against sxo_msv by the bound and its
  * tightness (tests). */
 int32_t sxo_msv16(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_t l);

@@ -453,16 +453,24 @@ __device__ __forceinline__ float msv_pair(const float* __restrict__ em, int ew, 
 // slot j*Q + 1 + r in the low half and j*Q + 1 + Q/2 + r in the high half, so one
 // packed instruction advances two cells and the k-1 operand of every half is the same
 // half of register r-1 (register 0 takes the previous lane's last high half and its own
-// last low half, one alignbit).  The model sits at the right end of the Q*G slots (slot
-// s holds position s - (Q*G - M)); the slots before position 1 hold -32768 scores and
-// never exceed B - 32768 or -32768, so the cell at position 1 still sees max(-32768, B)
-// and no slot raises the row maximum above the real cells' (B moves by far less than
-// 32768 units per row).  Cells use saturating adds (v_pk_add_i16 clamp): a value pushed
-// below -32768 stays there, which only raises it.  Scores are rounded up on the host
-// (q_emit, q_const) with an allowance for the f32 rounding of each step, so by induction
-// every cell, the row maximum xE, xJ, xN, xB and the result are at least 2^22 (fine
-// units) or 1024 (cells) times their f32 counterparts of dp.rs:92-145, until a cell
-// reaches 32767; from then on the result stays above a floor the host knows.
+// last low half, one alignbit).
+//
+// Frame.  A cell holds t = s - F - 32768, s its score in 1/1024 nat and F the pair's
+// frame floor, which is never below the row's B entry: t = -32768 stands for F.  The
+// recurrence max(s[k-1], B) + e[k] then needs no max with B: every t is >= -32768, so
+// s[k-1] >= F >= B, and the saturating add t[k-1] + e[k] (v_pk_add_i16 clamp) only
+// raises a sum below -32768 to F.  One add and one max (the row maximum) per two
+// cells.  F follows B: it moves up to B on any row whose B exceeds it (rare: xJ rising),
+// and down to B on every 16th row (B falls with xN by |tloop| a row), so F - B stays
+// below 16 rows of that drift.  A move by d shifts every t by -d (saturating; above
+// 32767 in two steps).  Slots before position 1 hold -32768 scores: after each row they
+// hold exactly -32768, so position 1 sees F, or after a move the shifted floor, which
+// the value injected on the left of the group's first slot reproduces: the result does
+// not depend on the shape.  Scores are rounded up on the host (q_emit, q_const) with an
+// allowance for the f32 rounding of each step, so by induction every cell, the row
+// maximum xE, xJ, xN, xB and the result are at least 2^22 (fine units) or 1024 (cells)
+// times their f32 counterparts of dp.rs:92-145, until a cell reaches 32767; from then
+// on the result stays above a floor the host knows.
 typedef short sx_v2i16 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b) {
@@ -559,12 +567,31 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
     const int c = tmove + tbm + ((1 << SX_Q_SHIFT) - 1);
     const int ce = tej + c;
     int xn = c, xj = -(1 << 30);
-    int be = max(c >> SX_Q_SHIFT, -32768);
-    auto step = [&](int x) {
+    int be = c >> SX_Q_SHIFT;
+    int fl = be;  // the frame floor F
+    // sched: a row on which the frame may move down (every 16th)
+    auto step = [&](int x, bool sched) {
         const uint32_t* __restrict__ sc = base + x * width;
         if constexpr (tm_aligned<TM>()) sc = static_cast<const uint32_t*>(__builtin_assume_aligned(sc, 16));
-        const uint32_t be2 = __builtin_amdgcn_perm((uint32_t)be, (uint32_t)be, 0x05040504u);
-        const uint32_t pl = grp_from_prev_u<G>(row[H - 1], 0x80000000u);
+        uint32_t inj = 0x80000000u;  // the floor, in the high half
+        const bool mv = be > fl || (sched && be < fl);
+        if (__any(mv)) {
+            // d < 65535: be < 32767 (xE is capped there) and F > -32768
+            const int d = mv ? be - fl : 0;
+            fl += d;
+            const int d1 = min(d, 32767);
+            const uint32_t n1 = __builtin_amdgcn_perm((uint32_t)(-d1), (uint32_t)(-d1), 0x05040504u);
+#pragma unroll
+            for (int r = 0; r < H; r++) row[r] = pk_adds16(row[r], n1);
+            if (__any(d > 32767)) {
+                const int d2 = d - d1;
+                const uint32_t n2 = __builtin_amdgcn_perm((uint32_t)(-d2), (uint32_t)(-d2), 0x05040504u);
+#pragma unroll
+                for (int r = 0; r < H; r++) row[r] = pk_adds16(row[r], n2);
+            }
+            inj = (uint32_t)max(-32768 - d, -32768) << 16;  // -d < 32768: no upper clamp
+        }
+        const uint32_t pl = grp_from_prev_u<G>(row[H - 1], inj);
         const uint32_t left = __builtin_amdgcn_alignbit(row[H - 1], pl, 16);
         // the row maximum in NACC independent chains (a single chain serialises every
         // register's max behind the previous one)
@@ -574,12 +601,12 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
         for (int a = 0; a < NACC; a++) ea[a] = 0x80008000u;
 #pragma unroll
         for (int r = H - 1; r >= 1; r--) {
-            const uint32_t v = pk_adds16(pk_max16(row[r - 1], be2), sc[r]);
+            const uint32_t v = pk_adds16(row[r - 1], sc[r]);
             row[r] = v;
             ea[r % NACC] = pk_max16(ea[r % NACC], v);
         }
         {
-            const uint32_t v = pk_adds16(pk_max16(left, be2), sc[0]);
+            const uint32_t v = pk_adds16(left, sc[0]);
             row[0] = v;
             ea[0] = pk_max16(ea[0], v);
         }
@@ -588,10 +615,13 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
         for (int a = 1; a < NACC; a++) e = pk_max16(e, ea[a]);
         e = pk_max16(e, __builtin_amdgcn_alignbit(e, e, 16));  // both halves: the lane's maximum
         int xe = (int)(short)(e & 0xFFFFu);
-        xe = grp_max_i<G>(xe);
+        // capped at 32767 as the absolute cells of a frameless bound are: the floor of a
+        // saturated bound is then the same (F >= -29.7 nats > -32 for L <= 10^6, M <= 1024,
+        // so a cell at 32767 reaches the cap)
+        xe = min(grp_max_i<G>(xe) + fl + 32768, 32767);
         xj = max(xj + tloop, xe * (1 << SX_Q_SHIFT) + ce);
         xn = xn + tloop;
-        be = max(max(xn, xj) >> SX_Q_SHIFT, -32768);
+        be = max(xn, xj) >> SX_Q_SHIFT;
     };
     // Four rows per loaded word: the compiler then schedules a row's LDS reads ahead into
     // the previous row, which pays on the aligned lane blocks and short one-lane rows.  On
@@ -604,19 +634,19 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
             const uint32_t w = nxt;
             nxt = wp[1];
             ++wp;
-            step((int)(w & 0xFFu));
+            step((int)(w & 0xFFu), (i & 15) == 0);
             if (++i == L) break;
-            step((int)((w >> 8) & 0xFFu));
+            step((int)((w >> 8) & 0xFFu), false);
             if (++i == L) break;
-            step((int)((w >> 16) & 0xFFu));
+            step((int)((w >> 16) & 0xFFu), false);
             if (++i == L) break;
-            step((int)(w >> 24));
+            step((int)(w >> 24), false);
             if (++i == L) break;
         }
     } else {
         FwdStream rs;
         rs.init(seq);
-        for (int i = 0; i < L; i++) step(rs.next());
+        for (int i = 0; i < L; i++) step(rs.next(), (i & 15) == 0);
     }
     return xj - tbm - ((1 << SX_Q_SHIFT) - 1);  // xJ + tmove
 }
@@ -624,7 +654,7 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
 // ---------------------------------------------------------------------------
 // Quantised Viterbi upper bound: the int16 prefilter of the Viterbi stage (no
 // counterpart in the reference, whose Viterbi is vit_pair's f32 recurrence).  The units,
-// roundings, packing, padding slots and folded special states of msvq_pair, with:
+// roundings, packing and folded special states of msvq_pair, with:
 //   - transition rows in LDS after the emission rows, staged shifted so that every
 //     operand of a register sits in the same register slot (MM IM DM MD DD read at k-1,
 //     MI II BM at k); out of node 0 and in the padding slots every score is -32768, so
@@ -636,9 +666,10 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
 //     until nothing changes, at most 2G - 1 times;
 //   - the row maximum over M and D (rounding up can make a D->D score of -1e-6 nat +1
 //     unit, so M alone is no longer an upper bound of D).
-// Transition scores <= 0 are rounded to <= 0 (q_trans) and the host skips profiles with a
-// positive insert score, so an insert cell reaches 32767 only after an M cell has: the
-// saturation floor of msvq_pair holds.
+// Cells hold absolute scores (no frame: B enters through the BM max).  Transition scores
+// <= 0 are rounded to <= 0 (q_trans) and the host skips profiles with a positive insert
+// score, so an insert cell reaches 32767 only after an M cell has: the result then stays
+// above the saturation floor of msvq_pair.
 template <int Q, int G>
 struct GeoQV1 {  // one-lane groups: transition rows 16-byte aligned after the odd emission rows
     static constexpr int H = Q / 2;
