@@ -54,8 +54,11 @@ def measured_traffic(workload, kernel, mode):
                 t = json.load(f)
         except (OSError, ValueError):
             continue
-        if t.get("workload") == workload and t.get("kernel") == kernel and t.get("table_mode") == mode:
-            return int(t["bytes_read"]) + int(t["bytes_written"]), f"profiles/{name}"
+        if t.get("workload") != workload:
+            continue
+        for k in t.get("kernels", [t]):
+            if k.get("kernel") == kernel and k.get("table_mode") == mode:
+                return int(k["bytes_read"]) + int(k["bytes_written"]), f"profiles/{name}"
     return None, None
 
 
@@ -384,7 +387,7 @@ def main():
                                             int(d["hoist"]))
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": kname, "launch_ms": round(float(d["ms"]), 3),
+            "kernel": kname, "table_mode": int(d["hoist"]), "launch_ms": round(float(d["ms"]), 3),
             "cells": int(d["cells"]), "ops_per_cell": MSV_OPS_PER_CELL,
             "all_launches_of_kernel_stage": {"launches": int(len(msv)), "ms": round(ms_launches, 3),
                                              "achieved": round(e.cells_msv * MSV_OPS_PER_CELL /
