@@ -34,10 +34,13 @@ typedef struct sx_ctx sx_ctx;
 /* DP stages (dp.rs:69 msv, :152 viterbi_score, :229 forward, :301 backward).
  * SX_STAGE_MSV16 has no reference counterpart: the int16 MSV prefilter.  It computes an
  * upper bound of the f32 MSV score with saturating packed int16 cells, as an int32 in
- * units of 2^-22 nat: score(MSV16) >= 2^22 * score(MSV) for every pair whose cells stay
- * below 32 nats (INT32_MIN for an empty target); once a cell reaches 32 nats the result
+ * units of 2^-22 nat: score(MSV16) >= 2^22 * score(MSV) for every pair whose row maxima
+ * stay below 32 nats (INT32_MIN for an empty target); once one reaches 32 nats the result
  * stays at or above 32767*4096 + q(-ln 2) + L*min(q(tloop), 0) + q(tmove), q() being
- * the bound's rounding of a score.  Models up to 1024 positions. */
+ * the bound's rounding of a score.  Cells are kept relative to a floor at or above the
+ * row's B entry (moved to B when B rises above it and every 16th row), so a segment may
+ * start up to 16 rows of B's fall (|tloop| a row) above B.  Models up to 1024
+ * positions. */
 /* SX_STAGE_VIT16, likewise without a reference counterpart, is the int16 Viterbi
  * prefilter: an upper bound of the f32 Viterbi score in the same units, with the same
  * floor once a cell reaches 32 nats (with the Viterbi length model's loop and move
