@@ -813,9 +813,17 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, 
     float xn = 0.0f, xb = lmove, xj = SX_NEG_INF, xc = SX_NEG_INF;
     FwdStream rs;
     rs.init(seq);
+    // the insert states of a row that read '*' are -inf (dp.rs:165,175): a rare residue,
+    // so they are cleared at the top of the next row, when any lane of the wave read it,
+    // instead of by a select in every cell
+    bool star = false;
     for (int i = 0; i < L; i++) {
         const int x = rs.next();
-        const bool ins_ok = x != SX_CODE_STAR;
+        if (__any(star)) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) ir[q] = star ? SX_NEG_INF : ir[q];
+        }
+        star = x == SX_CODE_STAR;
         const int kbo = HOIST ? kb : opaque(kb);
         const float* __restrict__ tmm = tr + SX_TMM * WID + kbo;
         const float* __restrict__ tmi = tr + SX_TMI * WID + kbo;
@@ -838,7 +846,7 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, 
             const float s = fmaxf(fmaxf(mp + tmm[q - 1], ip + tim[q - 1]), fmaxf(xb + tbm[q], dp + tdm[q - 1]));
             const float iv = fmaxf(mr[q] + tmi[q], ir[q] + tii[q]);
             mr[q] = s + sc[q];
-            ir[q] = ins_ok ? iv : SX_NEG_INF;
+            ir[q] = iv;
         }
         // D chain, ascending; the value entering from the left lane is resolved below
         const float mcL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
@@ -895,9 +903,14 @@ __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, cons
     float xn = 0.0f, xb = lmove, xj = SX_NEG_INF, xc = SX_NEG_INF;
     FwdStream rs;
     rs.init(seq);
+    bool star = false;  // as in vit_pair
     for (int i = 0; i < L; i++) {
         const int x = rs.next();
-        const bool ins_ok = x != SX_CODE_STAR;
+        if (__any(star)) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) ir[q] = star ? SX_NEG_INF : ir[q];
+        }
+        star = x == SX_CODE_STAR;
         // re-read every row (an opaque offset keeps the compiler from hoisting the rows
         // into 8*Q registers, which is what costs TM_REG its occupancy)
         const float* tb = tr + opaque(tj >> 2) * 4;  // a multiple of four floats: stays aligned
@@ -925,7 +938,7 @@ __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, cons
             const float s = fmaxf(fmaxf(mp + tmm[q], ip + tim[q]), fmaxf(xb + tbm[q], dp + tdm[q]));
             const float iv = fmaxf(mr[q] + tmi[q], ir[q] + tii[q]);
             mr[q] = s + sc[q];
-            ir[q] = ins_ok ? iv : SX_NEG_INF;
+            ir[q] = iv;
         }
         // D chain, ascending; the value entering from the left lane is resolved below
         const float mcL = grp_from_prev<G>(mr[Q - 1], SX_NEG_INF);
