@@ -217,7 +217,12 @@ sxc_scan(const int32_t* __restrict__ in, int64_t n_host, const int64_t* n_dev, i
 // Two-state composition filter (pipeline.rs:107-150) on each MSV survivor, then the
 // bias-corrected MSV test (pipeline.rs:170-173).  eo1[p*25 + x] holds the model-state
 // emission odds computed on the host exactly as the reference (the background state's
-// odds are 1 for every code).
+// odds are 1 for every code).  The reference normalises the two forward values by their
+// maximum on every residue and sums the logarithms of the maxima; here they are scaled
+// by a power of two instead (exact), the exponents summed as an integer and one
+// logarithm taken at the end: the same sum, without two divisions and a log a residue.
+// The f64 result differs from the reference's in its last bits only (both round each
+// step); its f32 score and the filter decision agree (tested against the oracle).
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_bias(const int64_t* __restrict__ n1, const int32_t* __restrict__ s_p, const int32_t* __restrict__ s_t,
          const float* __restrict__ s_usc, const double* __restrict__ eo1, const int32_t* __restrict__ pm,
@@ -225,30 +230,29 @@ sxc_bias(const int64_t* __restrict__ n1, const int32_t* __restrict__ s_p, const 
     const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
     if (i >= *n1) return;
     const int32_t p = s_p[i], t = s_t[i];
-    const uint8_t* seq = T.codes + T.off[t];
+    // targets start 16-byte aligned with 64 bytes of padding after the last: whole dwords
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(T.codes + T.off[t]);
     const int L = T.len[t];
     const double* e1 = eo1 + (int64_t)p * 25;
     const double l0 = 400.0;
     const double l1 = (double)pm[p] / 8.0;
     const double t00 = l0 / (l0 + 1.0), t01 = 1.0 / (l0 + 1.0);
     const double t10 = 1.0 / (l1 + 1.0), t11 = l1 / (l1 + 1.0);
-    const double tiny = 2.2250738585072014e-308;
-    double logsc = 0.0;
-    double d0 = 1.0 * 0.999, d1 = e1[seq[0]] * 0.001;
-    double mx = fmax(fmax(d0, d1), tiny);
-    d0 = d0 / mx;
-    d1 = d1 / mx;
-    logsc += log(mx);
+    uint32_t w = wp[0];
+    double d0 = 1.0 * 0.999, d1 = e1[w & 0xFFu] * 0.001;
+    int ex = 0;  // the values are d * 2^ex
     for (int k = 1; k < L; k++) {
-        const int x = seq[k];
-        const double n0 = (d0 * t00 + d1 * t10) * 1.0;
+        if ((k & 3) == 0) w = wp[k >> 2];
+        const int x = (int)((w >> ((k & 3) * 8)) & 0xFFu);
+        const double n0 = d0 * t00 + d1 * t10;
         const double nn1 = (d0 * t01 + d1 * t11) * e1[x];
-        mx = fmax(fmax(n0, nn1), tiny);
-        d0 = n0 / mx;
-        d1 = nn1 / mx;
-        logsc += log(mx);
+        // d0 > 0 on every residue (t00 > 0), so the exponent is defined
+        const int e = __builtin_amdgcn_frexp_exp(fmax(n0, nn1));
+        d0 = __builtin_ldexp(n0, -e);
+        d1 = __builtin_ldexp(nn1, -e);
+        ex += e;
     }
-    logsc += log(d0 + d1);
+    const double logsc = (double)ex * 0.6931471805599453 + log(d0 + d1);
     const double p1 = (double)L / ((double)L + 1.0);
     const float fs = (float)(logsc + (double)L * log(p1) + log(1.0 - p1));
     s_bias[i] = fs;
