@@ -319,14 +319,18 @@ constexpr int tm_block() {
 // is then kept twice, the second copy 32 banks (SKEW words) further round, rows are
 // padded to a multiple of 64 floats so the row index moves no bank, and the odd group
 // of each pair reads the second copy.  The 8 slots {j*STRIDE mod 64 : j < 8} and the
-// same set turned 32 banks are disjoint (j - j' would have to be 8 mod 16).
+// same set turned 32 banks are disjoint (j - j' would have to be 8 mod 16).  Groups of 4
+// lanes share a quarter-wave in fours: the table is kept four times, each copy 16 banks
+// further round, with STRIDE/4 = 1 (mod 4), so lane j of copy c starts on slot
+// 4c + j*STRIDE/4 (mod 16): j is that slot mod 4 and then c the rest, 16 distinct slots.
 template <int Q, int G>
 struct GeoA {
-    static_assert(Q % 4 == 0 && (G == 8 || G >= 16), "aligned lane blocks need Q % 4 == 0 and G = 8 or G >= 16");
-    static constexpr int STRIDE = (Q % 8 == 0) ? Q + 4 : Q + 8;
-    static constexpr int COPIES = G >= 16 ? 1 : 2;
+    static_assert(Q % 4 == 0 && (G == 4 || G == 8 || G >= 16), "aligned lane blocks need Q % 4 == 0 and G = 4, 8 or >= 16");
+    static constexpr int STRIDE = G == 4 ? Q + ((4 - Q % 16) + 16) % 16 : (Q % 8 == 0) ? Q + 4 : Q + 8;
+    static constexpr int COPIES = G >= 16 ? 1 : G == 8 ? 2 : 4;
     static constexpr int WID = COPIES == 1 ? G * STRIDE : (G * STRIDE + 63) / 64 * 64;
-    static constexpr int SKEW = COPIES == 1 ? 0 : 32;
+    static constexpr int SKEW = COPIES == 1 ? 0 : 64 / COPIES;
+    static_assert(G != 4 || STRIDE % 16 == 4, "four-lane stride");
     static constexpr int COPY = SX_NCODE * WID + SKEW;        // floats from one copy to the next
     static constexpr int TAB = COPIES * SX_NCODE * WID + (COPIES - 1) * SKEW;  // emission floats staged
     // Viterbi's eight transition rows follow the emission rows, one copy (every group of
@@ -360,8 +364,8 @@ struct EmRows {
             base = em + j * Q + 1 + GEM_SHIFT;
             width = ew;
         } else if constexpr (tm_aligned<TM>()) {
-            // the odd group of each pair of 8-lane groups reads the second copy
-            const int c = GeoA<Q, G>::COPIES == 1 ? 0 : (int)((threadIdx.x / G) & 1);
+            // the groups sharing a quarter-wave read different copies
+            const int c = (int)((threadIdx.x / G) & (GeoA<Q, G>::COPIES - 1));
             base = em + c * GeoA<Q, G>::COPY + j * GeoA<Q, G>::STRIDE;
             width = GeoA<Q, G>::WID;
         } else {
@@ -551,7 +555,7 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
     int width;
     if constexpr (tm_aligned<TM>()) {
         using GA = GeoA<H, G>;
-        const int c = GA::COPIES == 1 ? 0 : (int)((threadIdx.x / G) & 1);
+        const int c = (int)((threadIdx.x / G) & (GA::COPIES - 1));
         base = tab + c * GA::COPY + j * GA::STRIDE;
         width = GA::WID;
     } else {
@@ -699,7 +703,7 @@ __device__ __forceinline__ int32_t vitq_pair(const uint32_t* __restrict__ tab, i
     int width, tofs, twid;
     if constexpr (tm_aligned<TM>()) {
         using GA = GeoA<H, G>;
-        const int c = GA::COPIES == 1 ? 0 : (int)((threadIdx.x / G) & 1);
+        const int c = (int)((threadIdx.x / G) & (GA::COPIES - 1));
         base = tab + c * GA::COPY + j * GA::STRIDE;
         width = GA::WID;
         tofs = GA::TAB + j * GA::STRIDE;
@@ -1633,6 +1637,8 @@ static const SxKernelEntry k_table[] = {
     SXK_QALN(16, 16), SXK_QALN(16, 24), SXK_QALN(16, 32), SXK_QALN(16, 40), SXK_QALN(16, 48), SXK_QALN(16, 56),
     SXK_QALN(16, 64),
     SXK_QALN(32, 16), SXK_QALN(32, 24), SXK_QALN(32, 32),
+    // four-lane groups (four table copies) for 257-512 positions
+    SXK_QALN(4, 72), SXK_QALN(4, 80), SXK_QALN(4, 88), SXK_QALN(4, 96), SXK_QALN(4, 112), SXK_QALN(4, 128),
     // quantised Viterbi bound (stage 5): one-lane groups, aligned lane blocks of 8, 16, 32
     SXK_E(5, 16, 1, false, 0), SXK_E(5, 32, 1, false, 0),
     SXK_E(5, 16, 8, false, sxk::TM_ALN), SXK_E(5, 24, 8, false, sxk::TM_ALN), SXK_E(5, 32, 8, false, sxk::TM_ALN),

@@ -45,19 +45,25 @@ class _KernelEntry(ctypes.Structure):
 
 
 def _check_aligned_layout(e):
-    """Aligned lane blocks (MSV and Viterbi, G = 8 or G >= 16): a 128-bit LDS read serves 16
-    lanes a cycle from 64 four-byte banks, so the 16 lanes of every quarter-wave must start
-    on 16 distinct four-bank slots whatever residue row each group reads.  Viterbi's
-    transition rows (one copy, the same addresses for every group) follow."""
-    assert e.stage in (0, 1, 4, 5) and e.spec == 0 and (e.g == 8 or e.g >= 16) and e.q % 4 == 0
+    """Aligned lane blocks (MSV and Viterbi, G = 8 or G >= 16; the MSV bound also G = 4): a
+    128-bit LDS read serves 16 lanes a cycle from 64 four-byte banks, so the 16 lanes of
+    every quarter-wave must start on 16 distinct four-bank slots whatever residue row each
+    group reads.  Viterbi's transition rows (one copy, the same addresses for every group)
+    follow."""
+    assert e.stage in (0, 1, 4, 5) and e.spec == 0 and (e.g == 8 or e.g >= 16 or (e.g, e.stage) == (4, 4))
+    assert e.q % 4 == 0
     # a lane block holds Q floats, or for the int16 bounds (stages 4, 5) Q/2 dwords of two
     # int16 positions each
     per_lane = e.q // 2 if e.stage in (4, 5) else e.q
     assert per_lane % 4 == 0
-    stride = per_lane + 4 if per_lane % 8 == 0 else per_lane + 8
-    copies = 1 if e.g >= 16 else 2
-    skew = 0 if copies == 1 else 32
-    assert (stride // 4) % 2 == 1 and e.wid >= e.g * stride
+    if e.g == 4:
+        stride = next(v for v in range(per_lane, per_lane + 16) if v % 16 == 4)
+    else:
+        stride = per_lane + 4 if per_lane % 8 == 0 else per_lane + 8
+        assert (stride // 4) % 2 == 1
+    copies = {4: 4, 8: 2}.get(e.g, 1)
+    skew = 0 if copies == 1 else 64 // copies
+    assert e.wid >= e.g * stride
     if copies > 1:
         assert e.wid % 64 == 0  # the residue row moves no bank
     tab = copies * 25 * e.wid + (copies - 1) * skew

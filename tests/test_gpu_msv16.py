@@ -16,11 +16,12 @@ Q_EMPTY = -2**31
 Q_MAX_M = 1024
 
 # (Q positions per lane, G lanes per pair, table mode): one-lane groups (0) and aligned
-# lane blocks of 8, 16, 32 lanes (3: 512-thread, 4: 1024-thread workgroups)
+# lane blocks of 4, 8, 16, 32 lanes (3: 512-thread, 4: 1024-thread workgroups)
 SHAPES = [(q, 1, 0) for q in (16, 24, 32, 48, 64, 80, 96, 112, 128, 144, 160, 176, 192, 208, 224, 240, 256,
                               288, 320)] + \
     [(q, 8, h) for q in (16, 24, 32, 40, 48, 56, 64, 72, 80) for h in (3, 4)] + \
     [(88, 8, 3), (96, 8, 3), (128, 8, 3)] + \
+    [(q, 4, h) for q in (72, 80, 88, 96, 112, 128) for h in (3, 4)] + \
     [(q, 16, h) for q in (16, 24, 32, 40, 48, 56, 64) for h in (3, 4)] + \
     [(q, 32, h) for q in (16, 24, 32) for h in (3, 4)]
 
