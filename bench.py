@@ -333,6 +333,7 @@ def main():
         log(f"[bench] f32 cascade (no bounds): {f32_leg}")
     # per-stage split from one eager (event-timed) run of the same chain
     e, erecs = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=False, msv_bound=bound, vit_bound=vbound, **f)
+    ls = ctx.cascade_launches()  # this run's launches (the legs below run the cascade again)
     stages = {
         "msv": {"ms": round(e.ms_msv, 3), "cells": int(e.cells_msv), "gcups": round(e.cells_msv / e.ms_msv / 1e6, 1)},
         "bias+compaction": {"ms": round(e.ms_bias, 3), "pairs": int(e.n_msv_pass)},
@@ -374,7 +375,6 @@ def main():
 
     # dominant kernel: the longest MSV launch of the eager run (HIP events on the launch
     # stream): an int16 bound launch over all pairs, or an f32 MSV launch without the bound
-    ls = ctx.cascade_launches()
     mstage = 4 if bound else 0
     msv = ls[ls["stage"] == mstage]
     d = msv[int(np.argmax(msv["ms"]))]
