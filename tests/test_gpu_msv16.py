@@ -3,6 +3,8 @@ reference counterpart) with the oracle's sxo_msv16: bit-exact for every kernel s
 the all-pairs and pair-list forms.  That the bound never lies below the f32 MSV is pinned
 on the CPU (test_oracle.py) and end to end by the cascade tests (identical records with
 and without the bound)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -85,3 +87,36 @@ def test_msv16_rejects_models_beyond_its_shapes(orc, gctx, sxa):
         gctx.set_shapes(4, [(1100, 128, 8, 3)])
     with pytest.raises(sxa.SxError):
         gctx.set_shapes(4, [(300, 32, 8, 3)])  # capacity 256 below its max_m
+
+
+def test_msv16_long_multi_hit_targets(orc, gctx):
+    """Targets of 2,000-40,000 residues, some with hits spread along them (xJ lifts B
+    several times, so the frame floor moves up as well as down on its 16-row schedule),
+    on the default shapes and on a one-lane, a four-lane and an eight-lane shape: the
+    bound is the oracle's, bit for bit, and it never lies below the f32 MSV."""
+    from helpers import emit_homolog, random_seq
+    rng = np.random.default_rng(8700)
+    ms = [60, 150, 300, 450]
+    hmms = [random_hmm(orc, m, 8700 + i) for i, m in enumerate(ms)]
+    seqs = []
+    for L in (2000, 9000, 40000):
+        seqs.append(random_seq(rng, L, star_frac=0.001, degen_frac=0.01))
+        parts = []
+        while sum(len(p) for p in parts) < L:
+            parts.append(random_seq(rng, int(rng.integers(200, 2000)), 0.0, 0.0))
+            parts.append(emit_homolog(orc, hmms[int(rng.integers(len(hmms)))], rng, int(rng.integers(60, 300))))
+        seqs.append(np.concatenate(parts)[:L].astype(np.uint8))
+    profs = [orc.configure(h) for h in hmms]
+    want = orc.Batch(profs, seqs).msv16(nthreads=8).reshape(len(profs), len(seqs))
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    F = orc.Q_UNITS_PER_NAT
+    for table in ([], [(64, 64, 1, 0), (512, 128, 4, 3)], [(512, 64, 8, 3)]):
+        gctx.set_shapes(4, table)
+        got, _ = gctx.score_all(4, 1)
+        assert np.array_equal(got, want), (table, np.argwhere(got != want)[:5])
+    gctx.set_shapes(4, [])
+    for i, p in enumerate(profs):
+        for j, d in enumerate(seqs):
+            u = float(orc.msv(p, d))
+            assert want[i, j] >= min(math.floor(F * u), q_saturated_floor(orc, len(d))), (i, j)
