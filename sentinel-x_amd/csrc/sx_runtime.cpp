@@ -753,7 +753,10 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
                     ms = (t2 - t1 > 0.25 * t1) ? t2 - t1 : t2 / 2;
                 }
                 log.push_back({hi, s, ms, cells / ms / 1e6});
-                if (ms < best_ms) {
+                // a shape with more slots than the best so far must beat it by more than
+                // the timing noise (3%): it does more work on every model of the bucket
+                const bool looser = (long)s.q * s.g > (long)best.q * best.g;
+                if (ms < best_ms * (looser && best_ms < 1e300 ? 0.97 : 1.0)) {
                     best_ms = ms;
                     best = s;
                 }
