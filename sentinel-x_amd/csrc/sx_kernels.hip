@@ -1612,6 +1612,9 @@ static const SxKernelEntry k_table[] = {
     SXK_ALN(8, 16), SXK_ALN(8, 20), SXK_ALN(8, 24), SXK_ALN(8, 32), SXK_ALN(8, 40), SXK_ALN(8, 48),
     // Viterbi, Forward, Backward: three rows per lane
     SXK_DP3(1), SXK_DP3(2), SXK_DP3(3),
+    // Viterbi, finer capacities between the Q = 8 / 12 / 16 ones (transitions in registers)
+    SXK_E(1, 10, 4, false, 1), SXK_E(1, 10, 8, false, 1), SXK_E(1, 10, 16, false, 1), SXK_E(1, 10, 32, false, 1),
+    SXK_E(1, 14, 4, false, 1), SXK_E(1, 14, 8, false, 1), SXK_E(1, 14, 16, false, 1), SXK_E(1, 14, 32, false, 1),
     // Viterbi over aligned lane blocks, 512-thread workgroups
     SXK_E(1, 8, 8, false, sxk::TM_ALN), SXK_E(1, 12, 8, false, sxk::TM_ALN), SXK_E(1, 16, 8, false, sxk::TM_ALN),
     SXK_E(1, 8, 16, false, sxk::TM_ALN), SXK_E(1, 12, 16, false, sxk::TM_ALN), SXK_E(1, 16, 16, false, sxk::TM_ALN),

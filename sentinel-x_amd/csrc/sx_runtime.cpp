@@ -619,10 +619,11 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
 std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
     static const std::vector<int> coarse = {8,   16,  24,  32,  48,   64,   96,   128,  160,  192,  256, 320,
                                             384, 512, 640, 768, 1024, 1280, 1536, 2048, 2560, 3072, 4096};
-    // the MSV bound has a finer menu of capacities (its time follows the capacity)
+    // the MSV bound and Viterbi have finer menus of capacities (their time follows the
+    // capacity)
     static const std::vector<int> fine = {16,  24,  32,  48,  64,  80,  96,  112, 128, 144, 160, 176, 192, 208, 224,
                                           240, 256, 288, 320, 384, 448, 512, 576, 640, 704, 768, 896, 1024};
-    const std::vector<int>& edges = stage == SX_STAGE_MSV16 ? fine : coarse;
+    const std::vector<int>& edges = stage == SX_STAGE_MSV16 || stage == SX_STAGE_VITERBI ? fine : coarse;
     std::vector<TuneRecord> log;
     if (profs.empty() || nt == 0) return log;
     ensure_pidx();

@@ -285,6 +285,24 @@ def test_every_dp3_shape(orc, gctx, q, g, h, stage):
         assert np.abs(got[fin] - want[fin]).max() <= FWD_TOL
 
 
+VIT_FINE_SHAPES = [(q, g) for q in (10, 14) for g in (4, 8, 16, 32)]
+
+
+@pytest.mark.parametrize("q,g", VIT_FINE_SHAPES, ids=[f"Q{q}G{g}" for q, g in VIT_FINE_SHAPES])
+def test_fine_viterbi_shapes_bit_exact(orc, gctx, q, g):
+    """The Viterbi-only shapes with Q = 10 and 14 positions a lane (transitions in
+    registers): bit-exact with the oracle, models up to their capacity."""
+    cap = q * g
+    hmms, profs, seqs = _shape_case(orc, cap, 2600 + q * 131 + g)
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(1, [(cap, q, g, 1)])
+    got, _ = gctx.score_all(1, 1)
+    gctx.set_shapes(1, [])
+    want = orc.Batch(profs, seqs).viterbi(nthreads=8).reshape(len(profs), len(seqs))
+    assert np.array_equal(got, want)
+
+
 def test_empty_target_scores_neg_inf(orc, gctx):
     hmms = [random_hmm(orc, 20, 77)]
     gctx.set_profiles(hmms)
