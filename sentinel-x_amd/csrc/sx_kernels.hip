@@ -1642,7 +1642,8 @@ static const SxKernelEntry k_table[] = {
     SXK_QALN(16, 64),
     SXK_QALN(32, 16), SXK_QALN(32, 24), SXK_QALN(32, 32),
     // four-lane groups (four table copies) for 257-512 positions
-    SXK_QALN(4, 72), SXK_QALN(4, 80), SXK_QALN(4, 88), SXK_QALN(4, 96), SXK_QALN(4, 112), SXK_QALN(4, 128),
+    SXK_QALN(4, 72), SXK_QALN(4, 80), SXK_QALN(4, 88), SXK_QALN(4, 96), SXK_QALN(4, 104), SXK_QALN(4, 112),
+    SXK_QALN(4, 128),
     // quantised Viterbi bound (stage 5): one-lane groups, aligned lane blocks of 8, 16, 32
     SXK_E(5, 16, 1, false, 0), SXK_E(5, 32, 1, false, 0),
     SXK_E(5, 16, 8, false, sxk::TM_ALN), SXK_E(5, 24, 8, false, sxk::TM_ALN), SXK_E(5, 32, 8, false, sxk::TM_ALN),

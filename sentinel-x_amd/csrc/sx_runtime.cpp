@@ -621,8 +621,8 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
                                             384, 512, 640, 768, 1024, 1280, 1536, 2048, 2560, 3072, 4096};
     // the MSV bound and Viterbi have finer menus of capacities (their time follows the
     // capacity)
-    static const std::vector<int> fine = {16,  24,  32,  48,  64,  80,  96,  112, 128, 144, 160, 176, 192, 208, 224,
-                                          240, 256, 288, 320, 384, 448, 512, 576, 640, 704, 768, 896, 1024};
+    static const std::vector<int> fine = {16,  24,  32,  48,  64,  80,  96,  112, 128, 144, 160, 176, 192, 208, 224, 240,
+                                          256, 288, 320, 352, 384, 416, 448, 512, 576, 640, 704, 768, 896, 1024};
     const std::vector<int>& edges = stage == SX_STAGE_MSV16 || stage == SX_STAGE_VITERBI ? fine : coarse;
     std::vector<TuneRecord> log;
     if (profs.empty() || nt == 0) return log;

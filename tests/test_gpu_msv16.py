@@ -23,7 +23,7 @@ SHAPES = [(q, 1, 0) for q in (16, 24, 32, 48, 64, 80, 96, 112, 128, 144, 160, 17
                               288, 320)] + \
     [(q, 8, h) for q in (16, 24, 32, 40, 48, 56, 64, 72, 80) for h in (3, 4)] + \
     [(88, 8, 3), (96, 8, 3), (128, 8, 3)] + \
-    [(q, 4, h) for q in (72, 80, 88, 96, 112, 128) for h in (3, 4)] + \
+    [(q, 4, h) for q in (72, 80, 88, 96, 104, 112, 128) for h in (3, 4)] + \
     [(q, 16, h) for q in (16, 24, 32, 40, 48, 56, 64) for h in (3, 4)] + \
     [(q, 32, h) for q in (16, 24, 32) for h in (3, 4)]
 
