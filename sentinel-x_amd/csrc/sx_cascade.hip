@@ -92,37 +92,44 @@ sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__
 // threshold is capped there, so such a pair is a candidate.  Profiles with noq set
 // (longer than the bound's shapes, or without a threshold) make every pair a candidate.
 // Phase 0 writes block counts, phase 1 the candidates (pi, j, target id) at their
-// scanned offsets, in (pi, j) order.
+// scanned offsets, in (pi, j) order.  A block tests SX_QCAND_ITEMS x SXC_BLOCK
+// consecutive pairs, a block's width at a time, so that the scan between the phases runs
+// over that many times fewer block counts.
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_msvq_cand(int phase, const int32_t* __restrict__ qsc, const int64_t* __restrict__ nullq,
               const int64_t* __restrict__ capq, const int64_t* __restrict__ thrq, const uint8_t* __restrict__ noq,
               const int32_t* __restrict__ perm, const int32_t* __restrict__ sorted, int64_t P, int64_t T,
               int32_t* counts, const int64_t* offsets, int64_t cap, int32_t* c_pi, int32_t* c_j, int32_t* c_t) {
     __shared__ int s_wave[SXC_BLOCK / 64];
-    const int64_t pos = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
-    bool pass = false;
-    int32_t pi = 0, j = 0;
-    if (pos < P * T) {
-        pi = (int32_t)(pos / T);
-        j = (int32_t)(pos - (int64_t)pi * T);
-        const int32_t p = perm[pi];
-        const int64_t th = nullq[j] + thrq[p];
-        pass = noq[p] != 0 || (int64_t)qsc[(int64_t)p * T + j] >= (th < capq[j] ? th : capq[j]);
-    }
-    int tot;
-    const int r = sxc::block_rank(pass, s_wave, &tot);
-    if (phase == 0) {
-        if (threadIdx.x == 0) counts[blockIdx.x] = tot;
-        return;
-    }
-    if (pass) {
-        const int64_t o = offsets[blockIdx.x] + r;
-        if (o < cap) {
-            c_pi[o] = pi;
-            c_j[o] = j;
-            c_t[o] = sorted[j];
+    const int64_t base = (int64_t)blockIdx.x * SXC_BLOCK * SX_QCAND_ITEMS;
+    int64_t run = phase ? offsets[blockIdx.x] : 0;
+    int32_t total = 0;
+    for (int k = 0; k < SX_QCAND_ITEMS; k++) {
+        const int64_t pos = base + (int64_t)k * SXC_BLOCK + threadIdx.x;
+        bool pass = false;
+        int32_t pi = 0, j = 0;
+        if (pos < P * T) {
+            pi = (int32_t)(pos / T);
+            j = (int32_t)(pos - (int64_t)pi * T);
+            const int32_t p = perm[pi];
+            const int64_t th = nullq[j] + thrq[p];
+            pass = noq[p] != 0 || (int64_t)qsc[(int64_t)p * T + j] >= (th < capq[j] ? th : capq[j]);
         }
+        int tot;
+        const int r = sxc::block_rank(pass, s_wave, &tot);
+        if (phase == 1 && pass) {
+            const int64_t o = run + r;
+            if (o < cap) {
+                c_pi[o] = pi;
+                c_j[o] = j;
+                c_t[o] = sorted[j];
+            }
+        }
+        run += tot;
+        total += tot;
+        __syncthreads();  // every lane has read s_wave before the next item's rank rewrites it
     }
+    if (phase == 0 && threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
 // Entries per profile position of a list sorted by position: per[pi] = the length of

@@ -419,8 +419,9 @@ static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
     if (S.bound) {
         for (DeviceBuf* b : {&S.c_pi, &S.c_j, &S.c_t}) dptr<int32_t>(*b, cap1);
         dptr<float>(S.cusc, cap1);
-        dptr<int32_t>(S.cntq, nb1);
-        dptr<int64_t>(S.offq, nb1 + 1);
+        const int64_t nbq = (P * T + (int64_t)CB * SX_QCAND_ITEMS - 1) / ((int64_t)CB * SX_QCAND_ITEMS);
+        dptr<int32_t>(S.cntq, nbq);
+        dptr<int64_t>(S.offq, nbq + 1);
         dptr<int32_t>(S.perq, P);
         for (DeviceBuf* b : {&S.toffq, &S.eoffq}) dptr<int64_t>(*b, P + 1);
         dptr<SxTile>(S.tilesq, cap1 + P);
@@ -539,17 +540,18 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         // 1b. candidates of the bound, in (profile position, length-sorted target) order
         const int32_t* qres = (const int32_t*)S.usc.p;
         int32_t *c_pi = (int32_t*)S.c_pi.p, *c_j = (int32_t*)S.c_j.p, *c_t = (int32_t*)S.c_t.p;
+        const int64_t nbq = (P * T + (int64_t)CB * SX_QCAND_ITEMS - 1) / ((int64_t)CB * SX_QCAND_ITEMS);
         for (int phase = 0; phase < 2; phase++) {
-            hipLaunchKernelGGL(sxc_msvq_cand, dim3(nb1), dim3(CB), 0, st, phase, qres,
+            hipLaunchKernelGGL(sxc_msvq_cand, dim3(nbq), dim3(CB), 0, st, phase, qres,
                                (const int64_t*)S.d_nullq.p, (const int64_t*)S.d_capq.p,
                                (const int64_t*)S.d_thrq.p, (const uint8_t*)S.d_noq.p,
                                (const int32_t*)S.d_perm.p, c.d_sorted, P, T, (int32_t*)S.cntq.p,
                                (const int64_t*)S.offq.p, cap, c_pi, c_j, c_t);
             if (phase == 0)
-                hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cntq.p, nb1,
+                hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cntq.p, nbq,
                                    (const int64_t*)nullptr, (int64_t*)S.offq.p);
         }
-        S.d_ncand = (const int64_t*)S.offq.p + nb1;
+        S.d_ncand = (const int64_t*)S.offq.p + nbq;
         hipLaunchKernelGGL(sxc_clamp, dim3(1), dim3(64), 0, st, S.d_ncand, cap, nclamp + 1);
         // 1c. the f32 MSV of the candidates, tiled per profile as the later stages
         hipLaunchKernelGGL(sxc_count_sorted, dim3(nbp), dim3(CB), 0, st, (const int64_t*)(nclamp + 1), cap,
