@@ -44,41 +44,47 @@ __device__ __forceinline__ int block_rank(bool pass, int* s_wave, int* total) {
 
 // MSV test over every pair, enumerated as (profile position pi, length-sorted target j).
 // usc is indexed [profile][j] and nullsc [j], so the reads are coalesced.
-// Phase 0 writes block counts, phase 1 writes the survivors at their scanned offsets.
+// Phase 0 writes block counts, phase 1 writes the survivors at their scanned offsets.  A
+// block tests SX_QCAND_ITEMS x SXC_BLOCK consecutive pairs, as sxc_msvq_cand does.
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__ nullsc,
              const float* __restrict__ thr, const int32_t* __restrict__ perm, const int32_t* __restrict__ sorted,
              int64_t P, int64_t T, int32_t* counts, const int64_t* offsets, int64_t cap, int32_t* s_p, int32_t* s_pi,
              int32_t* s_t, float* s_usc) {
     __shared__ int s_wave[SXC_BLOCK / 64];
-    const int64_t pos = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
-    bool pass = false;
-    int32_t p = 0, pi = 0, t = 0;
-    float u = 0.0f;
-    if (pos < P * T) {
-        pi = (int32_t)(pos / T);
-        const int64_t j = pos - (int64_t)pi * T;
-        p = perm[pi];
-        t = sorted[j];
-        u = usc[(int64_t)p * T + j];
-        const float y = (u - nullsc[j]) / SXC_LN2F;
-        pass = y >= thr[p];
-    }
-    int tot;
-    const int r = sxc::block_rank(pass, s_wave, &tot);
-    if (phase == 0) {
-        if (threadIdx.x == 0) counts[blockIdx.x] = tot;
-        return;
-    }
-    if (pass) {
-        const int64_t o = offsets[blockIdx.x] + r;
-        if (o < cap) {
-            s_p[o] = p;
-            s_pi[o] = pi;
-            s_t[o] = t;
-            s_usc[o] = u;
+    const int64_t base = (int64_t)blockIdx.x * SXC_BLOCK * SX_QCAND_ITEMS;
+    int64_t run = phase ? offsets[blockIdx.x] : 0;
+    int32_t total = 0;
+    for (int k = 0; k < SX_QCAND_ITEMS; k++) {
+        const int64_t pos = base + (int64_t)k * SXC_BLOCK + threadIdx.x;
+        bool pass = false;
+        int32_t p = 0, pi = 0, t = 0;
+        float u = 0.0f;
+        if (pos < P * T) {
+            pi = (int32_t)(pos / T);
+            const int64_t j = pos - (int64_t)pi * T;
+            p = perm[pi];
+            t = sorted[j];
+            u = usc[(int64_t)p * T + j];
+            const float y = (u - nullsc[j]) / SXC_LN2F;
+            pass = y >= thr[p];
         }
+        int tot;
+        const int r = sxc::block_rank(pass, s_wave, &tot);
+        if (phase == 1 && pass) {
+            const int64_t o = run + r;
+            if (o < cap) {
+                s_p[o] = p;
+                s_pi[o] = pi;
+                s_t[o] = t;
+                s_usc[o] = u;
+            }
+        }
+        run += tot;
+        total += tot;
+        __syncthreads();  // every lane has read s_wave before the next item's rank rewrites it
     }
+    if (phase == 0 && threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
 // Candidates of the quantised MSV bound: every pair (profile position pi, length-sorted

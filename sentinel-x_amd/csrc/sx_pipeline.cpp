@@ -574,15 +574,16 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         S.d_ncand = nullptr;
         if (with_events) HIPCHK(hipEventRecord(S.ev[1], st));
         // 2. MSV test and ordered compaction
-        hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 0, usc, (const float*)S.d_nullsc.p,
+        const int64_t nbm = (P * T + (int64_t)CB * SX_QCAND_ITEMS - 1) / ((int64_t)CB * SX_QCAND_ITEMS);
+        hipLaunchKernelGGL(sxc_msv_pass, dim3(nbm), dim3(CB), 0, st, 0, usc, (const float*)S.d_nullsc.p,
                            (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
                            (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
-        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cnt1.p, nb1,
+        hipLaunchKernelGGL(sxc_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)S.cnt1.p, nbm,
                            (const int64_t*)nullptr, (int64_t*)S.off1.p);
-        hipLaunchKernelGGL(sxc_msv_pass, dim3(nb1), dim3(CB), 0, st, 1, usc, (const float*)S.d_nullsc.p,
+        hipLaunchKernelGGL(sxc_msv_pass, dim3(nbm), dim3(CB), 0, st, 1, usc, (const float*)S.d_nullsc.p,
                            (const float*)S.d_thr_msv.p, (const int32_t*)S.d_perm.p, c.d_sorted, P, T,
                            (int32_t*)S.cnt1.p, (const int64_t*)S.off1.p, cap, s_p, s_pi, s_t, s_usc);
-        S.d_n1 = (const int64_t*)S.off1.p + nb1;
+        S.d_n1 = (const int64_t*)S.off1.p + nbm;
     }
     // the survivor count the later kernels read: at most the lists' capacity
     hipLaunchKernelGGL(sxc_clamp, dim3(1), dim3(64), 0, st, S.d_n1, cap, nclamp);
