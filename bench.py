@@ -79,11 +79,33 @@ def calibrate(sxa, synth, hmms, n):
         synth.calibrate(hmms, scorer, n=n)
 
 
+def host_cpus():
+    """The host's logical CPUs (nproc), the CPUs this process may run on, the CPU model,
+    and the thread count the CPU legs use: the CPUs this process may use, at most the 16
+    host CPUs a one-GPU box allots to its GPU's jobs."""
+    n_all = os.cpu_count() or 1
+    try:
+        n_aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n_aff = n_all
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": n_all, "affinity": n_aff, "model": model, "threads": max(1, min(16, n_aff))}
+
+
 def cpu_baseline(wl, recs, f, budget_s, threads=None):
     """The oracle's threaded cascade on a bounded sample (first k profiles x all proteins)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the checker, timed here as the CPU baseline
-    threads = threads or min(16, os.cpu_count() or 1)
+    hc = host_cpus()
+    threads = threads or hc["threads"]
     lens = np.array([len(s) for s in wl.seqs], np.int64)
     ms = np.array([h.m for h in wl.hmms], np.int64)
     tot_l = int(lens.sum())
@@ -109,6 +131,7 @@ def cpu_baseline(wl, recs, f, budget_s, threads=None):
     return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "port",
             "sample": f"cascade on {k} of {len(ms)} profiles x all {nt} proteins ({cells:.3g} MSV cells, "
                       f"{dt:.1f} s)",
+            "host": hc, "per_core_gcups": round(cells / dt / 1e9 / threads, 3),
             "check": {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}}
 
 
@@ -336,7 +359,8 @@ def main():
     ls = ctx.cascade_launches()  # this run's launches (the legs below run the cascade again)
     stages = {
         "msv": {"ms": round(e.ms_msv, 3), "cells": int(e.cells_msv), "gcups": round(e.cells_msv / e.ms_msv / 1e6, 1)},
-        "bias+compaction": {"ms": round(e.ms_bias, 3), "pairs": int(e.n_msv_pass)},
+        "bias+compaction": {"ms": round(e.ms_bias, 3), "pairs": int(e.n_msv_pass),
+                            "host_recheck": {"pairs": int(e.n_bias_recheck), "ms": round(e.ms_recheck, 3)}},
         "viterbi": {"ms": round(e.ms_vit, 3), "cells": int(e.cells_vit),
                     "gcups": round(e.cells_vit / max(e.ms_vit, 1e-9) / 1e6, 1)},
         "forward": {"ms": round(e.ms_fwd, 3), "cells": int(e.cells_fwd),
@@ -411,7 +435,7 @@ def main():
         cpu = cpu_baseline(wl, recs, f, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu}")
         if kmode is not None:
-            threads = min(16, os.cpu_count() or 1)
+            threads = host_cpus()["threads"]
             k1, k3 = min(args.fwd_err_profiles, len(wl1.hmms)), min(20, len(wl3.hmms))
             fwd_err = {"marker_set": forward_error(wl1, fwd1, k1, threads),
                        "precision": forward_error(wl3, fwd3, k3, threads)}

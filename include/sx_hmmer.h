@@ -84,6 +84,13 @@ const char* sx_last_error(const sx_ctx* ctx);
  * resident on the device until the next call. */
 int sx_set_targets(sx_ctx* ctx, const uint8_t* codes, const uint64_t* offsets, uint32_t n_targets);
 
+/* alphabet::encode (alphabet.rs:58-93), the digitisation Targets::new applies to each
+ * sequence (search.rs:56-67): ASCII residues seq[0 .. n) to codes 0..24 (case-insensitive;
+ * U -> C, O -> K, B J Z X degenerate, other letters X, gap and stop characters and any
+ * other byte '*'), ASCII whitespace dropped.  out holds at least n bytes; *n_out receives
+ * the number of codes written.  Needs no device. */
+int sx_encode(const uint8_t* seq, size_t n, uint8_t* out, size_t* n_out);
+
 /* Hmm::configure (model.rs:146-245) for each model, then upload.  Replaces the
  * per-batch `configure()` of search_hmm_file_models (search.rs:257-263). */
 int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n);
@@ -91,6 +98,13 @@ int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n);
 int sx_load_hmm_file(sx_ctx* ctx, const char* path, uint32_t* n_loaded);
 uint32_t sx_num_profiles(const sx_ctx* ctx);
 uint32_t sx_num_targets(const sx_ctx* ctx);
+/* ModelInfo (search.rs:245-249) of profile p, as search_hmm_file_models returns it for
+ * every model in file order (search.rs:264), and the profile / acc fields of Hit
+ * (search.rs:101-116): name, accession and description (NULL when the model has none)
+ * and the model length.  Any output pointer may be NULL.  The strings stay valid until
+ * the context's profiles change. */
+int sx_profile_info(const sx_ctx* ctx, uint32_t p, const char** name, const char** acc, const char** desc,
+                    int32_t* m);
 /* Copy out the configured tables of profile p (for checking against the oracle):
  * msc, mo: 25*(M+1); tsc, tp: 8*(M+1) in the order MM MI MD IM II DM DD BM. */
 int sx_profile_tables(const sx_ctx* ctx, uint32_t p, int32_t* m, float* msc, float* tsc, float* mo,
@@ -131,14 +145,22 @@ int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tg
  * cap records, in no particular order (sort on (profile, target) for a canonical one);
  * *n_out is the total number, which may exceed cap (then only cap are written).  stage = 1 rejected by the bias-corrected MSV
  * test, 2 by Viterbi, 3 by Forward, 4 passed every filter; vit / fwd are NaN for
- * stages not reached. */
+ * stages not reached.
+ *
+ * Every msv, bias and vit value, and every decision, is the reference's bit for bit.
+ * The device's bias filter keeps the reference's f64 recurrence but takes its logarithm
+ * differently, under an error bound: where the bound cannot certify the f32 rounding of
+ * the score (a few pairs in a million), the host recomputes that score in the
+ * reference's order with glibc and re-decides the pair, and the record's flags carry
+ * SX_REC_BIAS_HOST. */
+#define SX_REC_BIAS_HOST 1
 typedef struct {
     uint32_t profile, target;
     float msv, bias, vit;
     float fwd;       /* f32 score as the reference returns it */
     double fwd64;    /* the same before the final f32 rounding */
     int32_t stage;
-    int32_t pad;
+    int32_t flags;   /* SX_REC_BIAS_HOST: bias score recomputed on the host */
 } sx_pair_scores;
 
 typedef struct {
@@ -165,6 +187,10 @@ typedef struct {
     /* out (SX_BOUND_VITERBI): the same for the Viterbi stage */
     uint64_t n_vit_cand;
     double ms_vit_bound;
+    /* out: bias scores the host recomputed (SX_REC_BIAS_HOST records) and the host time
+     * that took, in ms; ms_total includes it */
+    uint64_t n_bias_recheck;
+    double ms_recheck;
 } sx_cascade_stats;
 #define SX_BOUND_MSV 1
 #define SX_BOUND_VITERBI 2

@@ -91,7 +91,7 @@ def lib():
                                         ctypes.c_double, ctypes.c_double, i64, i64, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int]
         for name in ("sxo_batch_msv", "sxo_batch_viterbi", "sxo_batch_forward", "sxo_batch_forward64",
-                     "sxo_batch_msv16", "sxo_batch_vit16"):
+                     "sxo_batch_msv16", "sxo_batch_vit16", "sxo_batch_backward64"):
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_void_p, i64, i64, ctypes.c_void_p, ctypes.c_int]
@@ -496,6 +496,9 @@ class Batch:
 
     def forward64(self, begin=0, end=None, nthreads=1):
         return self._run(lib().sxo_batch_forward64, np.float64, begin, end, nthreads)
+
+    def backward64(self, begin=0, end=None, nthreads=1):
+        return self._run(lib().sxo_batch_backward64, np.float64, begin, end, nthreads)
 
     def cascade(self, f1=0.02, f2=1e-3, f3=1e-5, begin=0, end=None, nthreads=1):
         """Per pair: (stage reached 0..4, [msv, filtersc, vit, fwd]) of pipeline.rs:159-189."""

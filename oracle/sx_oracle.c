@@ -704,27 +704,47 @@ double sxo_exp_logsurv(double x, double tau, double lambda) {
     return -lambda * (x - tau);
 }
 
+/* SXO_PLANT selects a deliberate mistake in the cascade's decision logic, for the
+ * mutation tests only (tests/test_oracle_cascade.py builds each into a scratch library
+ * and checks that the closed-form boundary tests catch it): 1 swaps F2 and F3, 2 uses the
+ * null score where the bias filter score belongs, 3 rejects on >= instead of >.  Every
+ * real build has 0. */
+#ifndef SXO_PLANT
+#define SXO_PLANT 0
+#endif
+#if SXO_PLANT == 3
+#define SXO_REJECTS(p, f) ((p) >= (f))
+#else
+#define SXO_REJECTS(p, f) ((p) > (f))
+#endif
+
 /* pipeline.rs:159-189 (the cfg.filters && calibrated branch). */
 int sxo_cascade(int m, const float* msc, const float* tsc, const float* mo, const float* tp,
                 float msv_tbm, const float* compo, const double* st, double f1, double f2, double f3,
                 const uint8_t* dsq, int64_t l, float* out) {
     out[0] = out[1] = out[2] = out[3] = NAN;
     if (l == 0 || m == 0) return 0;
+    if (SXO_PLANT == 1) {
+        const double f = f2;
+        f2 = f3;
+        f3 = f;
+    }
     float nullsc = sxo_null_score(l);
     float xs[16];
     sxo_xsc(l, 1, xs);
     float usc = sxo_msv(m, msc, msv_tbm, dsq, l);
     out[0] = usc;
-    if (sxo_gumbel_surv((double)((usc - nullsc) / LN2F), st[0], st[1]) > f1) return 0;
+    if (SXO_REJECTS(sxo_gumbel_surv((double)((usc - nullsc) / LN2F), st[0], st[1]), f1)) return 0;
     float filtersc = sxo_bias_filter(m, compo, dsq, l);
     out[1] = filtersc;
-    if (sxo_gumbel_surv((double)((usc - filtersc) / LN2F), st[0], st[1]) > f1) return 1;
+    if (SXO_PLANT == 2) filtersc = nullsc;
+    if (SXO_REJECTS(sxo_gumbel_surv((double)((usc - filtersc) / LN2F), st[0], st[1]), f1)) return 1;
     float vsc = sxo_viterbi(m, msc, tsc, xs, dsq, l);
     out[2] = vsc;
-    if (sxo_gumbel_surv((double)((vsc - filtersc) / LN2F), st[2], st[3]) > f2) return 2;
+    if (SXO_REJECTS(sxo_gumbel_surv((double)((vsc - filtersc) / LN2F), st[2], st[3]), f2)) return 2;
     float fwd = (float)sxo_forward(m, mo, tp, xs, dsq, l, NULL);
     out[3] = fwd;
-    if (exp(sxo_exp_logsurv((double)((fwd - filtersc) / LN2F), st[4], st[5])) > f3) return 3;
+    if (SXO_REJECTS(exp(sxo_exp_logsurv((double)((fwd - filtersc) / LN2F), st[4], st[5])), f3)) return 3;
     return 4;
 }
 
@@ -769,6 +789,9 @@ static void run_pair(const pool_job* j, const sxo_batch* b, int stage, int64_t i
             break;
         case 6:
             ((int32_t*)out)[idx - begin] = sxo_vit16(m, b->msc + b->off[p], b->tsc + b->toff[p], xs, dsq, l);
+            break;
+        case 7:
+            ((double*)out)[idx - begin] = sxo_backward64(m, b->mo + b->off[p], b->tp + b->toff[p], xs, dsq, l);
             break;
         default:
             ((int32_t*)out)[idx - begin] =
@@ -823,3 +846,4 @@ int sxo_batch_forward(const sxo_batch* b, int64_t pb, int64_t pe, double* out, i
 int sxo_batch_forward64(const sxo_batch* b, int64_t pb, int64_t pe, double* out, int nt) { return run_pool(b, pb, pe, out, nt, 3); }
 int sxo_batch_msv16(const sxo_batch* b, int64_t pb, int64_t pe, int32_t* out, int nt) { return run_pool(b, pb, pe, out, nt, 5); }
 int sxo_batch_vit16(const sxo_batch* b, int64_t pb, int64_t pe, int32_t* out, int nt) { return run_pool(b, pb, pe, out, nt, 6); }
+int sxo_batch_backward64(const sxo_batch* b, int64_t pb, int64_t pe, double* out, int nt) { return run_pool(b, pb, pe, out, nt, 7); }

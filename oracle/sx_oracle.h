@@ -159,6 +159,7 @@ int sxo_batch_forward(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, 
 int sxo_batch_forward64(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
 int sxo_batch_msv16(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, int32_t* out, int nthreads);
 int sxo_batch_vit16(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, int32_t* out, int nthreads);
+int sxo_batch_backward64(const sxo_batch* b, int64_t pair_begin, int64_t pair_end, double* out, int nthreads);
 /* sxo_cascade over many pairs (the reference's search_profiles pair loop with
  * filters on, search.rs:217-224 + pipeline.rs:159-189).  stats: 6 doubles per profile;
  * compo: 20 floats per profile.  stage_out: one int per pair; sc_out: 4 floats per pair. */

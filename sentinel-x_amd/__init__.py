@@ -38,7 +38,7 @@ EXPORTS = [
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
     "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
     "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_stats", "sx_results_copy",
-    "sx_results_free",
+    "sx_results_free", "sx_encode", "sx_profile_info",
 ]
 
 
@@ -67,7 +67,12 @@ class sx_hmm(ctypes.Structure):
 class sx_pair_scores(ctypes.Structure):
     _fields_ = [("profile", ctypes.c_uint32), ("target", ctypes.c_uint32), ("msv", ctypes.c_float),
                 ("bias", ctypes.c_float), ("vit", ctypes.c_float), ("fwd", ctypes.c_float),
-                ("fwd64", ctypes.c_double), ("stage", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("fwd64", ctypes.c_double), ("stage", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+# sx_pair_scores.flags: the bias score was recomputed on the host (the device's error
+# bound could not certify its f32 rounding)
+REC_BIAS_HOST = 1
 
 
 class sx_cascade_stats(ctypes.Structure):
@@ -80,7 +85,8 @@ class sx_cascade_stats(ctypes.Structure):
                 ("cells_msv", ctypes.c_uint64), ("cells_vit", ctypes.c_uint64), ("cells_fwd", ctypes.c_uint64),
                 ("n_msv_cand", ctypes.c_uint64), ("ms_msv_bound", ctypes.c_double),
                 ("cells_msv_exact", ctypes.c_uint64), ("n_vit_cand", ctypes.c_uint64),
-                ("ms_vit_bound", ctypes.c_double)]
+                ("ms_vit_bound", ctypes.c_double), ("n_bias_recheck", ctypes.c_uint64),
+                ("ms_recheck", ctypes.c_double)]
 
 
 class sx_search_cfg(ctypes.Structure):
@@ -98,7 +104,7 @@ DOMAIN_DTYPE = np.dtype([("score", "<f4"), ("bias", "<f4"), ("c_evalue", "<f8"),
                          ("env_from", "<u4"), ("env_to", "<u4"), ("reported", "<i4"), ("pad", "<i4")])
 
 PAIR_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("msv", "<f4"), ("bias", "<f4"), ("vit", "<f4"),
-                       ("fwd", "<f4"), ("fwd64", "<f8"), ("stage", "<i4"), ("pad", "<i4")])
+                       ("fwd", "<f4"), ("fwd64", "<f8"), ("stage", "<i4"), ("flags", "<i4")])
 
 LAUNCH_DTYPE = np.dtype([("stage", "<i4"), ("q", "<i4"), ("g", "<i4"), ("hoist", "<i4"), ("ms", "<f8"),
                          ("cells", "<u8")])
@@ -127,6 +133,9 @@ def lib():
         L.sx_num_targets.argtypes = [vp]
         L.sx_num_targets.restype = u32
         L.sx_profile_tables.argtypes = [vp, u32, ctypes.POINTER(i32), vp, vp, vp, vp, vp]
+        L.sx_encode.argtypes = [vp, ctypes.c_size_t, vp, ctypes.POINTER(ctypes.c_size_t)]
+        cpp = ctypes.POINTER(ctypes.c_char_p)
+        L.sx_profile_info.argtypes = [vp, u32, cpp, cpp, cpp, ctypes.POINTER(i32)]
         L.sx_score_all.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
         L.sx_score_pairs.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, u64, vp, ctypes.POINTER(ctypes.c_double)]
         L.sx_specials.argtypes = [vp, ctypes.c_int, vp, vp, u64, vp, vp]
@@ -232,6 +241,20 @@ def _take_results(L, r, stats=None):
     return hits, doms
 
 
+def encode(seq) -> np.ndarray:
+    """alphabet::encode (alphabet.rs:91-93) through sx_encode: ASCII residues (bytes or
+    str) to codes 0..24, whitespace dropped.  Needs no device."""
+    if isinstance(seq, str):
+        seq = seq.encode()
+    a = np.frombuffer(bytes(seq), np.uint8)
+    out = np.empty(max(1, len(a)), np.uint8)
+    n = ctypes.c_size_t()
+    rc = lib().sx_encode(_ptr(a) if len(a) else None, len(a), _ptr(out), ctypes.byref(n))
+    if rc != 0:
+        raise SxError(f"sx_encode failed ({rc})")
+    return out[:n.value].copy()
+
+
 def search_host(hmms, seqs, prof, tgt, fwd, bwd, fsp, bsp, **cfg):
     """The host part of the search alone (sx_search_host): pairs (prof[i], tgt[i]) with their
     whole-sequence Forward / Backward scores and special rows (fsp / bsp: for each pair a
@@ -332,6 +355,16 @@ class Context:
         self._chk(self._L.sx_load_hmm_file(self._h, path.encode(), ctypes.byref(n)))
         self.n_profiles = self._L.sx_num_profiles(self._h)
         return n.value
+
+    def profile_info(self, p: int):
+        """ModelInfo (search.rs:245-249) of profile p: (name, acc, desc, m); acc and desc
+        are None when the model has none."""
+        nm, ac, de = ctypes.c_char_p(), ctypes.c_char_p(), ctypes.c_char_p()
+        m = ctypes.c_int32()
+        self._chk(self._L.sx_profile_info(self._h, p, ctypes.byref(nm), ctypes.byref(ac), ctypes.byref(de),
+                                          ctypes.byref(m)))
+        dec = lambda v: v.value.decode() if v.value is not None else None
+        return dec(nm), dec(ac), dec(de), m.value
 
     def profile_tables(self, p: int):
         m = ctypes.c_int32()

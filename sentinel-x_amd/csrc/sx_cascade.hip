@@ -13,10 +13,12 @@
 // Survivor lists keep the order (profile position, target length): every list is built
 // by a count / exclusive-scan / write sequence, never by atomic appends, so a stage's
 // tiles are contiguous runs of one profile.  Built without -ffp-contract, so the bias
-// filter's f64 sums round as the reference's; only its log() may differ in the last
-// bit.
+// filter's f64 products, sums and quotients round as the reference's; its logarithm is
+// taken differently, under an error bound that certifies the f32 score or hands the pair
+// to the host (sxc_bias).
 #include <hip/hip_runtime.h>
 
+#include "../../include/sx_hmmer.h"
 #include "sx_dev.h"
 
 #define SXC_LN2F 0.693147182464599609375f
@@ -96,7 +98,9 @@ sxc_msv_pass(int phase, const float* __restrict__ usc, const float* __restrict__
 // A bound that saturated (a cell reached 32 nats) is no longer an upper bound but stays
 // at or above capq[j], which the host derives from the target's length model; the
 // threshold is capped there, so such a pair is a candidate.  Profiles with noq set
-// (longer than the bound's shapes, or without a threshold) make every pair a candidate.
+// (longer than the bound's shapes, or without a finite threshold) make every pair a
+// candidate; a profile whose threshold no score passes (thrq at or above SX_Q_NOPASS) has
+// no bound launch, so its rows of qsc hold stale values and none of its pairs is one.
 // Phase 0 writes block counts, phase 1 the candidates (pi, j, target id) at their
 // scanned offsets, in (pi, j) order.  A block tests SX_QCAND_ITEMS x SXC_BLOCK
 // consecutive pairs, a block's width at a time, so that the scan between the phases runs
@@ -118,8 +122,10 @@ sxc_msvq_cand(int phase, const int32_t* __restrict__ qsc, const int64_t* __restr
             pi = (int32_t)(pos / T);
             j = (int32_t)(pos - (int64_t)pi * T);
             const int32_t p = perm[pi];
-            const int64_t th = nullq[j] + thrq[p];
-            pass = noq[p] != 0 || (int64_t)qsc[(int64_t)p * T + j] >= (th < capq[j] ? th : capq[j]);
+            const int64_t tq = thrq[p];
+            const int64_t th = nullq[j] + tq;
+            pass = noq[p] != 0 ||
+                   (tq < SX_Q_NOPASS && (int64_t)qsc[(int64_t)p * T + j] >= (th < capq[j] ? th : capq[j]));
         }
         int tot;
         const int r = sxc::block_rank(pass, s_wave, &tot);
@@ -230,16 +236,37 @@ sxc_scan(const int32_t* __restrict__ in, int64_t n_host, const int64_t* n_dev, i
 // Two-state composition filter (pipeline.rs:107-150) on each MSV survivor, then the
 // bias-corrected MSV test (pipeline.rs:170-173).  eo1[p*25 + x] holds the model-state
 // emission odds computed on the host exactly as the reference (the background state's
-// odds are 1 for every code).  The reference normalises the two forward values by their
-// maximum on every residue and sums the logarithms of the maxima; here they are scaled
-// by a power of two instead (exact), the exponents summed as an integer and one
-// logarithm taken at the end: the same sum, without two divisions and a log a residue.
-// The f64 result differs from the reference's in its last bits only (both round each
-// step); its f32 score and the filter decision agree (tested against the oracle).
+// odds are 1 for every code, so the reference's product with them is exact).
+//
+// The two forward values d0, d1 and each row maximum m_i follow the reference's f64
+// operations in its order (products, sums, max, division by the maximum; one of the
+// two quotients is m_i / m_i = 1 exactly), so they are bit-identical to the host's.
+// The reference then sums ln(m_i) with glibc's log, one call a residue; here the m_i are
+// multiplied into a running product kept as a mantissa in [0.5, 1) and an integer
+// exponent (exact renormalisation), and one device log is taken at the end.  With
+// u = 2^-53, the two f64 results differ by at most
+//   beta = u (1 + 1/16) [ (L+1) + 2|ex| ln2 + 2k |ln mant| + 2g A + sum_i A_i
+//                         + (2(k+g+1) + 1) L |ln p1| + 3 |s1| + 2(k+g) |ln(1-p1)| + 6 |v| ]
+// where the device log is within k = 4 ulps and glibc's within g = 2 ulps of the true
+// logarithm (both claim 1 ulp or less), A_i = sum_{j<=i} |ln m_j| (each term bounded
+// above by |m_j - 1| / min(m_j, 1), evaluated in f32 with a 1% margin and summed in
+// f64), A = A_{L+1}, s1 the partial sum before ln(1-p1) and v the result.  The terms are
+// the product's roundings (one per factor), the exponent's ln2, the device log and the
+// rounding of its sum, glibc's per-residue errors and the host sum's roundings
+// (u |partial sum| each), the tail's logs and roundings, and those of v -/+ beta.
+// If v - beta and v + beta round to the same f32, that f32 is the reference's score
+// (rounding is monotone).  Otherwise the pair is flagged for the host, which recomputes it
+// in the reference's order with glibc, and the lower f32 is kept meanwhile: each later
+// test passes more pairs with a lower bias score, so every pair the reference passes
+// still reaches the next stage and the host can re-decide it from the scores computed.
+// With recheck_all every pair is flagged (a test of that host path).
+// The flagged survivor indices are appended to fl_list (fl_n[0] counts them; entries
+// past fl_cap are dropped, and the host then scans every record's flag instead).
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_bias(const int64_t* __restrict__ n1, const int32_t* __restrict__ s_p, const int32_t* __restrict__ s_t,
          const float* __restrict__ s_usc, const double* __restrict__ eo1, const int32_t* __restrict__ pm,
-         const float* __restrict__ thr, SxTargets T, float* s_bias, uint8_t* flag) {
+         const float* __restrict__ thr, SxTargets T, float* s_bias, uint8_t* flag, uint8_t* bflag,
+         int32_t* fl_n, int32_t* fl_list, int32_t fl_cap, int32_t recheck_all) {
     const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
     if (i >= *n1) return;
     const int32_t p = s_p[i], t = s_t[i];
@@ -251,25 +278,70 @@ sxc_bias(const int64_t* __restrict__ n1, const int32_t* __restrict__ s_p, const 
     const double l1 = (double)pm[p] / 8.0;
     const double t00 = l0 / (l0 + 1.0), t01 = 1.0 / (l0 + 1.0);
     const double t10 = 1.0 / (l1 + 1.0), t11 = l1 / (l1 + 1.0);
+    const double MINPOS = 2.2250738585072014e-308;
+    double mant = 1.0;  // running product of the row maxima = mant * 2^ex
+    int ex = 0;
+    double a_run = 0.0, a_sum = 0.0;  // A_i and sum_i A_i (upper bounds)
+    double d0, d1;
+    // one row maximum: normalise (d0, d1) by it as the reference does and fold it into
+    // the product and the bound sums
+    auto take = [&](double n0, double nn1) {
+        const double mx = fmax(fmax(n0, nn1), MINPOS);
+        if (mx == n0 || mx == nn1) {
+            // x / x = 1 exactly: one division for the smaller value
+            const double q = (mx == n0 ? nn1 : n0) / mx;
+            d0 = mx == n0 ? 1.0 : q;
+            d1 = mx == n0 ? q : 1.0;
+        } else {
+            d0 = n0 / mx;
+            d1 = nn1 / mx;
+        }
+        const double pr = mant * mx;
+        ex += __builtin_amdgcn_frexp_exp(pr);
+        mant = __builtin_amdgcn_frexp_mant(pr);
+        const float mf = (float)mx;
+        a_run += (double)((fabsf(mf - 1.0f) + 1e-7f) * __builtin_amdgcn_rcpf(fminf(mf, 1.0f)) * 1.01f);
+        a_sum += a_run;
+    };
     uint32_t w = wp[0];
-    double d0 = 1.0 * 0.999, d1 = e1[w & 0xFFu] * 0.001;
-    int ex = 0;  // the values are d * 2^ex
+    take(1.0 * 0.999, e1[w & 0xFFu] * 0.001);
     for (int k = 1; k < L; k++) {
         if ((k & 3) == 0) w = wp[k >> 2];
         const int x = (int)((w >> ((k & 3) * 8)) & 0xFFu);
         const double n0 = d0 * t00 + d1 * t10;
         const double nn1 = (d0 * t01 + d1 * t11) * e1[x];
-        // d0 > 0 on every residue (t00 > 0), so the exponent is defined
-        const int e = __builtin_amdgcn_frexp_exp(fmax(n0, nn1));
-        d0 = __builtin_ldexp(n0, -e);
-        d1 = __builtin_ldexp(nn1, -e);
-        ex += e;
+        take(n0, nn1);
     }
-    const double logsc = (double)ex * 0.6931471805599453 + log(d0 + d1);
+    // the reference's final ln(d0 + d1) is one more factor of the product
+    const double s = d0 + d1;
+    {
+        const double pr = mant * s;
+        ex += __builtin_amdgcn_frexp_exp(pr);
+        mant = __builtin_amdgcn_frexp_mant(pr);
+        const float sf = (float)s;
+        a_run += (double)((fabsf(sf - 1.0f) + 1e-7f) * __builtin_amdgcn_rcpf(fminf(sf, 1.0f)) * 1.01f);
+        a_sum += a_run;
+    }
+    const double lmant = log(mant);
+    const double logsc = (double)ex * 0.6931471805599453 + lmant;
     const double p1 = (double)L / ((double)L + 1.0);
-    const float fs = (float)(logsc + (double)L * log(p1) + log(1.0 - p1));
-    s_bias[i] = fs;
-    const float y = (s_usc[i] - fs) / SXC_LN2F;
+    const double lp1 = log(p1), lq1 = log(1.0 - p1);
+    const double s1 = logsc + (double)L * lp1;
+    const double v = s1 + lq1;
+    const double kd = 4.0, kg = 2.0;
+    const double beta = 0x1p-53 * (17.0 / 16.0) *
+                        ((double)(L + 1) + 2.0 * fabs((double)ex) * 0.6931471805599453 + 2.0 * kd * fabs(lmant) +
+                         2.0 * kg * a_run + a_sum + (2.0 * (kd + kg + 1.0) + 1.0) * (double)L * fabs(lp1) +
+                         3.0 * fabs(s1) + 2.0 * (kd + kg) * fabs(lq1) + 6.0 * fabs(v));
+    const float lo = (float)(v - beta), hi = (float)(v + beta);
+    const bool amb = recheck_all != 0 || lo != hi;
+    s_bias[i] = lo;
+    bflag[i] = amb ? 1 : 0;
+    if (amb) {
+        const int slot = atomicAdd(fl_n, 1);
+        if (slot < fl_cap) fl_list[slot] = (int32_t)i;
+    }
+    const float y = (s_usc[i] - lo) / SXC_LN2F;
     flag[i] = y >= thr[p] ? 1 : 0;
 }
 
@@ -414,15 +486,16 @@ struct SxPairRec {
     uint32_t profile, target;
     float msv, bias, vit, fwd;
     double fwd64;
-    int32_t stage, pad;
+    int32_t stage, flags;
 };
 
-// One record per MSV survivor, in list order.
+// One record per MSV survivor, in list order; flags marks the bias scores the host
+// must recompute (sxc_bias).
 extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
 sxc_records(const int64_t* __restrict__ n_dev, int64_t cap, const int32_t* __restrict__ s_p,
             const int32_t* __restrict__ s_t, const float* __restrict__ s_usc, const float* __restrict__ s_bias,
             const float* __restrict__ res_vit, const double* __restrict__ res_fwd,
-            const int32_t* __restrict__ res_stage, SxPairRec* out) {
+            const int32_t* __restrict__ res_stage, const uint8_t* __restrict__ bflag, SxPairRec* out) {
     const int64_t i = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
     if (i >= *n_dev || i >= cap) return;
     SxPairRec r;
@@ -434,6 +507,6 @@ sxc_records(const int64_t* __restrict__ n_dev, int64_t cap, const int32_t* __res
     r.fwd64 = res_fwd[i];
     r.fwd = (float)res_fwd[i];
     r.stage = res_stage[i];
-    r.pad = 0;
+    r.flags = bflag[i] ? SX_REC_BIAS_HOST : 0;
     out[i] = r;
 }

@@ -31,6 +31,7 @@ enum { SX_TMM = 0, SX_TMI, SX_TMD, SX_TIM, SX_TII, SX_TDM, SX_TDD, SX_TBM };
 #define SX_Q_SLACK_FINE 16.0     // SX_Q_FINE * 2^-24 * 64
 #define SX_Q_MAX_L 1000000       // longer targets are candidates of every profile
 #define SX_QCAND_ITEMS 16        // pairs a thread tests in the candidate compaction
+#define SX_Q_NOPASS (INT64_MAX / 8)  // bound thresholds at or above this: no score passes
 
 // Per-target length model: MSV tloop/tmove (f32 logf, dp.rs:99-100), Plan7 multihit
 // loop/move in nats (model.rs:286-289) and as probabilities (XscP), and the MSV

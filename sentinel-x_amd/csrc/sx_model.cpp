@@ -51,17 +51,6 @@ uint8_t encode_residue(uint8_t b) {
     return CODE_STAR;
 }
 
-std::vector<uint8_t> encode(const uint8_t* s, size_t n) {
-    std::vector<uint8_t> out;
-    out.reserve(n);
-    for (size_t i = 0; i < n; i++) {
-        const uint8_t b = s[i];
-        if (b == ' ' || b == '\t' || b == '\n' || b == '\r' || b == '\f') continue;
-        out.push_back(encode_residue(b));
-    }
-    return out;
-}
-
 static inline float lnp32(float p) { return p > 0.0f ? (float)std::log((double)p) : NEG_INF; }
 
 Profile configure(const Hmm& h) {

@@ -23,8 +23,7 @@ constexpr float LN2F = 0.693147182464599609375f;
 
 extern const float BG[K];                            // alphabet.rs:36-39
 int degenerate_members(int code, const int** mem);   // alphabet.rs:42-54
-uint8_t encode_residue(uint8_t b);                   // alphabet.rs:58-88
-std::vector<uint8_t> encode(const uint8_t* s, size_t n);  // alphabet.rs:91-93
+uint8_t encode_residue(uint8_t b);                   // alphabet.rs:58-88 (sx_encode: alphabet.rs:91-93)
 
 struct Stats {  // model.rs:27-40
     bool has_msv = false, has_vit = false, has_fwd = false;

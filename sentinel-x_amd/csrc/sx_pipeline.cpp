@@ -5,9 +5,13 @@
 #include "sx_pipeline.h"
 
 #include <algorithm>
+#include <cfloat>
+#include <chrono>
 #include <cmath>
+#include <atomic>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "../../include/sx_hmmer.h"
 
@@ -102,6 +106,37 @@ static void bias_odds(const Profile& p, double* eo1) {
     eo1[CODE_STAR] = 1.0;
 }
 
+// bias_filter_score (pipeline.rs:107-150): the reference's f64 operations in its order,
+// with glibc's log.  The host recomputes here the scores whose f32 rounding the device's
+// error bound cannot certify (sxc_bias).
+float bias_filter_score(const Profile& p, const uint8_t* dsq, int l) {
+    double eo1[NCODE];
+    bias_odds(p, eo1);
+    const double l0 = 400.0;
+    const double l1 = (double)p.m / 8.0;
+    const double t00 = l0 / (l0 + 1.0), t01 = 1.0 / (l0 + 1.0);
+    const double t10 = 1.0 / (l1 + 1.0), t11 = l1 / (l1 + 1.0);
+    double logsc = 0.0;
+    double d0 = 1.0 * 0.999, d1 = eo1[dsq[0]] * 0.001;
+    double mx = std::max(std::max(d0, d1), DBL_MIN);
+    d0 /= mx;
+    d1 /= mx;
+    logsc += std::log(mx);
+    for (int i = 1; i < l; i++) {
+        const int x = dsq[i];
+        // the background state's odds are 1 for every code
+        const double n0 = d0 * t00 + d1 * t10;
+        const double n1 = (d0 * t01 + d1 * t11) * eo1[x];
+        mx = std::max(std::max(n0, n1), DBL_MIN);
+        d0 = n0 / mx;
+        d1 = n1 / mx;
+        logsc += std::log(mx);
+    }
+    logsc += std::log(d0 + d1);
+    const double p1 = (double)l / ((double)l + 1.0);
+    return (float)(logsc + (double)l * std::log(p1) + std::log(1.0 - p1));
+}
+
 }  // namespace sx
 
 extern "C" {
@@ -109,7 +144,7 @@ __global__ void sxc_msv_pass(int, const float*, const float*, const float*, cons
                              int64_t, int32_t*, const int64_t*, int64_t, int32_t*, int32_t*, int32_t*, float*);
 __global__ void sxc_scan(const int32_t*, int64_t, const int64_t*, int64_t*);
 __global__ void sxc_bias(const int64_t*, const int32_t*, const int32_t*, const float*, const double*, const int32_t*,
-                         const float*, SxTargets, float*, uint8_t*);
+                         const float*, SxTargets, float*, uint8_t*, uint8_t*, int32_t*, int32_t*, int32_t, int32_t);
 __global__ void sxc_compact(int, const uint8_t*, const int64_t*, const int32_t*, const int32_t*, int32_t*,
                             const int64_t*, int32_t*, int32_t*);
 __global__ void sxc_tiles(int, int64_t, const int32_t*, const int32_t*, const int32_t*, int32_t*, const int64_t*,
@@ -132,7 +167,7 @@ __global__ void sxc_clamp(const int64_t*, int64_t, int64_t*);
 __global__ void sxc_vitq_cand(const int64_t*, const int32_t*, const int32_t*, const float*, const int32_t*,
                               const int32_t*, const float*, const uint8_t*, const int64_t*, int32_t*, uint8_t*);
 __global__ void sxc_records(const int64_t*, int64_t, const int32_t*, const int32_t*, const float*, const float*,
-                            const float*, const double*, const int32_t*, void*);
+                            const float*, const double*, const int32_t*, const uint8_t*, void*);
 }
 
 namespace sx {
@@ -140,6 +175,9 @@ namespace sx {
 namespace {
 
 const int CB = 256;  // cascade kernels' block size
+// capacity of the device list of bias scores handed to the host; past it the host scans
+// every record's flag
+const int32_t BIAS_LIST_CAP = 1 << 16;
 
 int64_t nblk(int64_t n) { return (n + CB - 1) / CB; }
 
@@ -191,6 +229,8 @@ struct CascadeState {
     std::vector<Run> vq_runs;   // the int16 Viterbi bound (profiles it covers)
     // device work buffers
     DeviceBuf usc, cnt1, off1, s_p, s_pi, s_t, s_usc, s_bias, flag, res_vit, res_fwd, res_stage;
+    // bias scores for the host to recompute: per-survivor flag, count, index list
+    DeviceBuf bflag, bl_n, bl_list;
     DeviceBuf cnt2, off2, list2, per2, tcount, toff2, eoff2, tiles2, tids2, vsc, flag3;
     DeviceBuf cnt3, off3, list3, per3, toff3, eoff3, tiles3, tids3, fsc, recs;
     // the bound's candidates (pi, j, target id), their f32 MSV scores and tiles; clamped
@@ -223,6 +263,58 @@ struct CascadeState {
 };
 
 void cascade_release(Ctx& c) { c.cascade.reset(); }
+
+// A record whose bias score the device could not certify (sxc_bias): the score is
+// recomputed in the reference's order and the pair re-decided (pipeline.rs:170-183) from
+// the scores the device computed with the lower f32 of the bound.  With that lower score
+// every test passes at least the pairs the reference passes, so each score the reference
+// computes is there; scores it would not compute are cleared.  Returns the stage the
+// device gave the record.
+static int recheck_bias(const Ctx& c, const CascadeState& S, sx_pair_scores& r) {
+    const int before = r.stage;
+    const float fs = bias_filter_score(c.profs[r.profile], c.h_codes.data() + c.h_toff[r.target], c.tlen[r.target]);
+    r.bias = fs;
+    r.flags |= SX_REC_BIAS_HOST;
+    if (before < 2) return before;
+    if (!((r.msv - fs) / LN2F >= S.thr_msv[r.profile])) {
+        r.stage = 1;
+        r.vit = r.fwd = NAN;
+        r.fwd64 = NAN;
+        return before;
+    }
+    r.stage = 2;
+    if (before < 3 || !((r.vit - fs) / LN2F >= S.thr_vit[r.profile])) {
+        r.fwd = NAN;
+        r.fwd64 = NAN;
+        return before;
+    }
+    r.stage = 3;
+    if (before == 4 && (r.fwd - fs) / LN2F >= S.thr_fwd[r.profile]) r.stage = 4;
+    return before;
+}
+
+// recheck_bias over a list of record indices, on host threads when there are many;
+// adjusts the bias / Viterbi pass counts for the records whose decision changed.
+static void recheck_records(const Ctx& c, const CascadeState& S, std::vector<sx_pair_scores>& recs,
+                            const std::vector<int64_t>& idx, int64_t* n_bias_pass, int64_t* n_vit_pass) {
+    std::vector<int8_t> before(idx.size());
+    const size_t nth = idx.size() < 4096 ? 1 : std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t k; (k = next.fetch_add(256)) < idx.size();)
+            for (size_t j = k; j < std::min(idx.size(), k + 256); j++)
+                before[j] = (int8_t)recheck_bias(c, S, recs[(size_t)idx[j]]);
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nth; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (size_t j = 0; j < idx.size(); j++) {
+        const int after = recs[(size_t)idx[j]].stage;
+        if (before[j] >= 2 && after < 2) --*n_bias_pass;
+        if (before[j] >= 3 && after < 3) --*n_vit_pass;
+    }
+}
 
 static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3, int bounds) {
     const int bound = (bounds & SX_BOUND_MSV) ? 1 : 0, vbound = (bounds & SX_BOUND_VITERBI) ? 1 : 0;
@@ -442,6 +534,9 @@ static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
     dptr<float>(S.s_usc, cap1);
     dptr<float>(S.s_bias, cap1);
     dptr<uint8_t>(S.flag, cap1);
+    dptr<uint8_t>(S.bflag, cap1);
+    dptr<int32_t>(S.bl_n, 1);
+    dptr<int32_t>(S.bl_list, BIAS_LIST_CAP);
     dptr<float>(S.res_vit, cap1);
     dptr<double>(S.res_fwd, cap1);
     dptr<int32_t>(S.res_stage, cap1);
@@ -465,7 +560,7 @@ static void alloc_buffers(Ctx& c, CascadeState& S, int64_t cap1) {
 // so it can be captured.  with_events records stage boundaries for the timing split.
 static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
     const int64_t P = (int64_t)c.profs.size(), T = c.nt, cap = S.cap1;
-    const int64_t nb1 = nblk(P * T), nbc = nblk(cap), nbp = nblk(P);
+    const int64_t nbc = nblk(cap), nbp = nblk(P);
     hipStream_t st = c.stream;
     SxSpecialsOut nosp{};
     if (with_events) HIPCHK(hipEventRecord(S.ev[0], st));
@@ -591,9 +686,11 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
     // 3. bias filter and its test
     hipLaunchKernelGGL(sxc_res_init, dim3(nbc), dim3(CB), 0, st, n1, cap, (float*)S.res_vit.p, (double*)S.res_fwd.p,
                        (int32_t*)S.res_stage.p);
+    hipLaunchKernelGGL(sxc_zero_i32, dim3(1), dim3(CB), 0, st, (int32_t*)S.bl_n.p, (int64_t)1);
     hipLaunchKernelGGL(sxc_bias, dim3(nbc), dim3(CB), 0, st, n1, (const int32_t*)s_p, (const int32_t*)s_t,
                        (const float*)s_usc, (const double*)S.d_eo1.p, (const int32_t*)c.d_m,
-                       (const float*)S.d_thr_msv.p, c.dtarg(), s_bias, (uint8_t*)S.flag.p);
+                       (const float*)S.d_thr_msv.p, c.dtarg(), s_bias, (uint8_t*)S.flag.p, (uint8_t*)S.bflag.p,
+                       (int32_t*)S.bl_n.p, (int32_t*)S.bl_list.p, BIAS_LIST_CAP, (int32_t)c.bias_recheck_all);
     // 4. survivors -> Viterbi tiles
     auto compact = [&](const uint8_t* flag, const int64_t* n_in, const int32_t* src, DeviceBuf& cnt, DeviceBuf& off,
                        DeviceBuf& list, DeviceBuf& per) {
@@ -644,7 +741,8 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
                        (const float*)S.d_thr_fwd.p, (double*)S.res_fwd.p, (int32_t*)S.res_stage.p);
     hipLaunchKernelGGL(sxc_records, dim3(nbc), dim3(CB), 0, st, n1, cap, (const int32_t*)s_p, (const int32_t*)s_t,
                        (const float*)s_usc, (const float*)s_bias, (const float*)S.res_vit.p,
-                       (const double*)S.res_fwd.p, (const int32_t*)S.res_stage.p, S.recs.p);
+                       (const double*)S.res_fwd.p, (const int32_t*)S.res_stage.p, (const uint8_t*)S.bflag.p,
+                       S.recs.p);
     if (with_events) HIPCHK(hipEventRecord(S.ev[4], st));
     HIPCHK(hipGetLastError());
 }
@@ -738,13 +836,32 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
         }
         HIPCHK(hipMemcpy(&n2, (int64_t*)S.off2.p + nbc, 8, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&n3, (int64_t*)S.off3.p + nbc, 8, hipMemcpyDeviceToHost));
+        std::vector<sx_pair_scores>& recs = S.last;
+        recs.resize((size_t)n1);
+        if (n1) HIPCHK(hipMemcpy(recs.data(), S.recs.p, (size_t)n1 * sizeof(sx_pair_scores), hipMemcpyDeviceToHost));
+        {
+            // the bias scores the device could not certify: recomputed on the host
+            const auto tr = std::chrono::steady_clock::now();
+            int32_t nbl = 0;
+            HIPCHK(hipMemcpy(&nbl, S.bl_n.p, 4, hipMemcpyDeviceToHost));
+            std::vector<int64_t> bl;
+            if (nbl <= BIAS_LIST_CAP) {
+                std::vector<int32_t> b32((size_t)nbl);
+                if (nbl) HIPCHK(hipMemcpy(b32.data(), S.bl_list.p, (size_t)nbl * 4, hipMemcpyDeviceToHost));
+                bl.assign(b32.begin(), b32.end());
+            } else {
+                for (int64_t i = 0; i < n1; i++)
+                    if (recs[(size_t)i].flags & SX_REC_BIAS_HOST) bl.push_back(i);
+            }
+            recheck_records(c, S, recs, bl, &n2, &n3);
+            st->n_bias_recheck = (uint64_t)bl.size();
+            st->ms_recheck = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr).count();
+            st->ms_total += st->ms_recheck;
+        }
         st->n_pairs = (uint64_t)(P * T);
         st->n_msv_pass = (uint64_t)n1;
         st->n_bias_pass = (uint64_t)n2;
         st->n_vit_pass = (uint64_t)n3;
-        std::vector<sx_pair_scores>& recs = S.last;
-        recs.resize((size_t)n1);
-        if (n1) HIPCHK(hipMemcpy(recs.data(), S.recs.p, (size_t)n1 * sizeof(sx_pair_scores), hipMemcpyDeviceToHost));
         uint64_t npass = 0, cv = 0, cf = 0;
         for (const auto& r : recs) {
             const uint64_t cells = (uint64_t)c.profs[r.profile].m * (uint64_t)c.tlen[r.target];
@@ -820,7 +937,9 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     std::vector<float> urow(T), vrow;
     std::vector<double> frow;
     float* d_u = (float*)c.scratch_out.ensure((size_t)T * 8);
-    DeviceBuf d_pp, d_pt, d_pu, d_pb, d_fl, d_n;
+    DeviceBuf d_pp, d_pt, d_pu, d_pb, d_fl, d_n, d_bf;
+    uint64_t nrecheck = 0;
+    double ms_recheck = 0.0;
     std::vector<double> eo1(25);
     double ms_total = 0.0;
     uint64_t n1 = 0, n2 = 0, n3 = 0, n4 = 0, cv = 0, cf = 0, ncand = 0, cq = 0, nvc = 0;
@@ -893,14 +1012,30 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
         HIPCHK(hipMemcpyAsync(dpt, pt.data(), k * 4, hipMemcpyHostToDevice, c.stream));
         HIPCHK(hipMemcpyAsync(dpu, pu.data(), k * 4, hipMemcpyHostToDevice, c.stream));
         HIPCHK(hipMemcpyAsync(dn, &kk, 8, hipMemcpyHostToDevice, c.stream));
+        uint8_t* dbf = dptr<uint8_t>(d_bf, k);
+        hipLaunchKernelGGL(sxc_zero_i32, dim3(1), dim3(CB), 0, c.stream, (int32_t*)S.bl_n.p, (int64_t)1);
         hipLaunchKernelGGL(sxc_bias, dim3(nblk(k)), dim3(CB), 0, c.stream, (const int64_t*)dn, (const int32_t*)dpp,
                            (const int32_t*)dpt, (const float*)dpu, (const double*)S.d_eo1.p, (const int32_t*)c.d_m,
-                           (const float*)S.d_thr_msv.p, c.dtarg(), dpb, dfl);
+                           (const float*)S.d_thr_msv.p, c.dtarg(), dpb, dfl, dbf, (int32_t*)S.bl_n.p,
+                           (int32_t*)S.bl_list.p, BIAS_LIST_CAP, (int32_t)c.bias_recheck_all);
         std::vector<float> pb(k);
-        std::vector<uint8_t> fl(k);
+        std::vector<uint8_t> fl(k), bf(k);
         HIPCHK(hipMemcpyAsync(pb.data(), dpb, k * 4, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipMemcpyAsync(fl.data(), dfl, k, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipMemcpyAsync(bf.data(), dbf, k, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
+        {
+            // the bias scores the device could not certify, recomputed on the host before
+            // any decision uses them
+            const auto tr = std::chrono::steady_clock::now();
+            for (int64_t i = 0; i < k; i++) {
+                if (!bf[i]) continue;
+                pb[i] = bias_filter_score(pr, c.h_codes.data() + c.h_toff[sp_t[i]], c.tlen[sp_t[i]]);
+                fl[i] = (pu[i] - pb[i]) / LN2F >= S.thr_msv[p] ? 1 : 0;
+                nrecheck++;
+            }
+            ms_recheck += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr).count();
+        }
         std::vector<uint32_t> vp, vt;
         std::vector<size_t> vidx;
         const size_t base = recs.size();
@@ -914,6 +1049,7 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
             r.fwd = NAN;
             r.fwd64 = NAN;
             r.stage = 1;
+            r.flags = bf[i] ? SX_REC_BIAS_HOST : 0;
             recs.push_back(r);
             if (fl[i]) {
                 vp.push_back((uint32_t)p);
@@ -1006,6 +1142,8 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     HIPCHK(hipEventSynchronize(S.ev[4]));
     ms_total = ev_ms(S.ev[0], S.ev[4]);
     st->ms_total = ms_total;
+    st->n_bias_recheck = nrecheck;
+    st->ms_recheck = ms_recheck;
     st->ms_msv = st->ms_bias = st->ms_vit = st->ms_fwd = 0.0;
     st->n_pairs = (uint64_t)c.profs.size() * c.nt;
     st->n_msv_pass = n1;
@@ -1028,7 +1166,7 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
 
 namespace sx {
 
-std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3) {
+std::vector<PassPair> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3) {
     sx_cascade_stats st{};
     st.f1 = f1;
     st.f2 = f2;
@@ -1047,10 +1185,12 @@ std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1,
     }
     c.cascade_pass_uncalibrated = false;
     c.cascade_dirty = true;
-    std::vector<std::pair<uint32_t, uint32_t>> out;
+    std::vector<PassPair> out;
     for (const auto& r : c.cascade->last)
-        if (r.stage == 4) out.push_back({r.profile, r.target});
-    std::sort(out.begin(), out.end());
+        if (r.stage == 4) out.push_back({r.profile, r.target, r.fwd});
+    std::sort(out.begin(), out.end(), [](const PassPair& a, const PassPair& b) {
+        return a.profile != b.profile ? a.profile < b.profile : a.target < b.target;
+    });
     return out;
 }
 

@@ -15,9 +15,20 @@ double exp_logsurv(double x, double tau, double lambda);  // pipeline.rs:98-104
 // Smallest f32 value that a monotone reject-test lets through (see sx_pipeline.cpp).
 float pass_threshold(const std::function<bool(float)>& rejects, int* violations);
 
+// bias_filter_score (pipeline.rs:107-150) on the host, in the reference's operation order
+// with glibc: the recomputation of the scores the device's bound cannot certify.
+float bias_filter_score(const Profile& p, const uint8_t* dsq, int l);
+
+// A pair that passed every filter, with the Forward score that decided its F3 test
+// (pipeline.rs:179-183), the score score_target goes on with.
+struct PassPair {
+    uint32_t profile, target;
+    float fwd;
+};
+
 // The batched cascade over every pair, with no filter for profiles lacking STATS lines
 // (as score_target, pipeline.rs:164); returns the pairs that pass every filter, sorted.
-std::vector<std::pair<uint32_t, uint32_t>> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3);
+std::vector<PassPair> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3);
 
 }  // namespace sx
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <limits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -89,6 +90,8 @@ Ctx::Ctx(int dev) : device(dev) {
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     n_cu = prop.multiProcessorCount;
     HIPCHK(hipMalloc((void**)&d_heads, sizeof(unsigned long long) * 64));
+    // test setting: every bias score of the cascade goes through the host recomputation
+    if (const char* e = std::getenv("SX_BIAS_RECHECK_ALL")) bias_recheck_all = e[0] == '1';
     int nk = 0;
     const SxKernelEntry* tab = sx_kernel_table(&nk);
     for (int i = 0; i < nk; i++) {
@@ -991,6 +994,29 @@ extern "C" int sx_load_hmm_file(sx_ctx* ctx, const char* path, uint32_t* n_loade
         ctx->c->set_profiles(std::move(ps));
         if (n_loaded) *n_loaded = k;
     });
+}
+
+extern "C" int sx_encode(const uint8_t* seq, size_t n, uint8_t* out, size_t* n_out) {
+    if ((n && (!seq || !out)) || !n_out) return SX_ERR_ARG;
+    size_t k = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint8_t b = seq[i];
+        if (b == ' ' || b == '\t' || b == '\n' || b == '\r' || b == '\f') continue;  // is_ascii_whitespace
+        out[k++] = sx::encode_residue(b);
+    }
+    *n_out = k;
+    return SX_OK;
+}
+
+extern "C" int sx_profile_info(const sx_ctx* ctx, uint32_t p, const char** name, const char** acc, const char** desc,
+                               int32_t* m) {
+    if (!ctx || p >= ctx->c->profs.size()) return SX_ERR_ARG;
+    const sx::Profile& pr = ctx->c->profs[p];
+    if (name) *name = pr.name.c_str();
+    if (acc) *acc = pr.acc ? pr.acc->c_str() : nullptr;
+    if (desc) *desc = pr.desc ? pr.desc->c_str() : nullptr;
+    if (m) *m = pr.m;
+    return SX_OK;
 }
 
 extern "C" uint32_t sx_num_profiles(const sx_ctx* ctx) { return ctx ? (uint32_t)ctx->c->profs.size() : 0; }

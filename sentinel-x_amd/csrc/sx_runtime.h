@@ -101,6 +101,9 @@ struct Ctx {
 
     // the search's cascade lets profiles without STATS lines through every filter
     bool cascade_pass_uncalibrated = false;
+    // every bias score of the cascade is recomputed on the host (set from the environment
+    // variable SX_BIAS_RECHECK_ALL=1 at creation; a test of the host recomputation)
+    bool bias_recheck_all = false;
 
     // profiles
     std::vector<Profile> profs;

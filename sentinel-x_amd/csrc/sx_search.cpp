@@ -180,7 +180,10 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     const std::vector<Threshold> ths = resolve_thresholds(c.profs, cfg);
     const double z = std::max(cfg.z > 0.0 ? cfg.z : (double)c.nt, 1.0);
     if (c.profs.empty() || c.nt == 0) return;
-    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    // with the filters on, the tail goes on with the Forward score that decided the F3
+    // test (score_target computes Forward once, pipeline.rs:180); with them off, with the
+    // score of the special-row Forward
+    std::vector<PassPair> pairs;
     auto t0 = clk::now();
     if (cfg.filters) {
         pairs = cascade_pass_pairs(c, cfg.f1, cfg.f2, cfg.f3);
@@ -188,7 +191,7 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     } else {
         pairs.reserve((size_t)c.profs.size() * c.nt);
         for (uint32_t p = 0; p < c.profs.size(); p++)
-            for (uint32_t t = 0; t < c.nt; t++) pairs.push_back({p, t});
+            for (uint32_t t = 0; t < c.nt; t++) pairs.push_back({p, t, NAN});
     }
     std::vector<const uint8_t*> seq(c.nt);
     for (uint32_t t = 0; t < c.nt; t++) seq[t] = c.h_codes.data() + c.h_toff[t];
@@ -212,10 +215,10 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
         int64_t rows = 0;
         size_t b = a;
         for (; b < pairs.size(); b++) {
-            const int64_t r = (int64_t)len[pairs[b].second] + 1;
+            const int64_t r = (int64_t)len[pairs[b].target] + 1;
             if (b > a && rows + r > max_rows) break;
-            cp.push_back(pairs[b].first);
-            ct.push_back(pairs[b].second);
+            cp.push_back(pairs[b].profile);
+            ct.push_back(pairs[b].target);
             roff.push_back(rows);
             rows += r;
         }
@@ -227,6 +230,8 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
         t0 = clk::now();
         c.special_rows(SX_STAGE_FORWARD, cp.data(), ct.data(), n, fsp.data(), fwd.data());
         c.special_rows(SX_STAGE_BACKWARD, cp.data(), ct.data(), n, bsp.data(), bwd.data());
+        if (cfg.filters)
+            for (uint64_t i = 0; i < n; i++) fwd[i] = (double)pairs[a + i].fwd;
         res.stats.ms_specials += ms_since(t0);
         t0 = clk::now();
         run_tails(c.profs, seq, len, cp.data(), ct.data(), n, fwd.data(), bwd.data(), fsp.data(), bsp.data(), roff,

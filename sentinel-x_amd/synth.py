@@ -163,8 +163,11 @@ def _low_complexity(rng, L):
 WORKLOAD_NAMES = {0: "single-M200", 1: "marker-set", 2: "pfam-scale", 3: "precision", 4: "mag-sweep"}
 
 
-def make_workload(k: int, profile_scale: float = 1.0, target_scale: float = 1.0) -> Workload:
-    """Build workload k; the scale factors shrink the profile / protein counts (for tests)."""
+def make_workload(k: int, profile_scale: float = 1.0, target_scale: float = 1.0,
+                  n_genomes: Optional[int] = None) -> Workload:
+    """Build workload k; the scale factors shrink the profile / protein counts (for tests).
+    For workload 4, n_genomes (if given) sets the number of genomes, each of full size
+    (target_scale then shrinks only the genome count when n_genomes is None)."""
     ss = np.random.SeedSequence(1000 + k)
     r_prof, r_len, r_res, r_hom = [np.random.default_rng(s) for s in ss.spawn(4)]
 
@@ -199,9 +202,10 @@ def make_workload(k: int, profile_scale: float = 1.0, target_scale: float = 1.0)
     elif k == 4:
         ms = _lognormal_int(r_len, 300, 0.5, 25, 2500, npr(5000))
         hmms = [make_profile(r_prof, int(m), f"synth4_{i}") for i, m in enumerate(ms)]
-        ngen = ntg(200)
+        ngen = n_genomes if n_genomes is not None else ntg(200)
         sizes_kb = r_len.uniform(1500, 6000, ngen)
-        counts = np.maximum(1, np.rint(sizes_kb * target_scale).astype(np.int64))
+        per_kb = 1.0 if n_genomes is not None else target_scale
+        counts = np.maximum(1, np.rint(sizes_kb * per_kb).astype(np.int64))
         genome_of = np.repeat(np.arange(ngen), counts)
         lens = _lognormal_int(r_len, 280, 0.5, 30, 5000, int(counts.sum()))
         seqs, hom = _proteins(r_res, lens, hmms, 0.02)

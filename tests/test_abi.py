@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -122,3 +123,22 @@ def test_kernel_table_names_each_shape_once(sxa):
             assert (st, q, 64, 0, 2) in long_rows
             if st >= 2:
                 assert (st, q, 64, 1, 2) in long_rows
+
+
+def test_encode_matches_the_reference_vectors_and_the_oracle(sxa, orc):
+    """sx_encode restates alphabet::encode (alphabet.rs:58-93); the expected codes are the
+    reference's own test vectors (alphabet.rs:104-116) and every byte value against the
+    oracle's restatement."""
+    if not os.path.exists(sxa.LIB_PATH):
+        pytest.skip("libsxhmmer.so not built")
+    assert list(sxa.encode(b"ACDEFGHIKLMNPQRSTVWY")) == list(range(20))
+    assert list(sxa.encode(b"uo")) == [1, 8]
+    assert list(sxa.encode(b"BJZX*")) == [20, 21, 22, 23, 24]
+    assert list(sxa.encode(b"m k\n")) == [10, 8]
+    assert list(sxa.encode(b"-._~!1")) == [24] * 6
+    assert list(sxa.encode(b"zq")) == [22, 13]
+    assert len(sxa.encode(b"")) == 0 and len(sxa.encode(b" \t\r\n\f")) == 0
+    every = bytes(range(256)) * 3
+    assert np.array_equal(sxa.encode(every), orc.encode(every))
+    # \v is not ASCII whitespace in Rust's is_ascii_whitespace: it is encoded, as '*'
+    assert list(sxa.encode(b"\x0bA")) == [24, 0]
