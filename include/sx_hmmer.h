@@ -120,6 +120,18 @@ int sx_score_all(sx_ctx* ctx, int stage, int form, void* out, double* ms);
 int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
                    void* out, double* ms);
 
+/* sx_score_pairs without allocating memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * sx_score_pairs, for at most SX_NOALLOC_MAX_PAIRS pairs (SX_ERR_ARG beyond).  The pair
+ * list, its tiles, the device results and the queue heads live in a static device array
+ * of about 9 MiB that each device holds from library load until the process ends; the host
+ * staging is reserved when the context is created.  Every call on one device shares that
+ * array, so calls that overlap on one device (from two contexts or two threads) race. */
+#define SX_NOALLOC_MAX_PAIRS (1u << 18)
+int sx_score_pairs_noalloc(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
+                           void* out, double* ms);
+
 /* Forward (dp.rs:229-298) or Backward (dp.rs:301-392) special-state rows for a pair
  * list, as score_target consumes them (pipeline.rs:180-206).  sp receives 5*(L+1)
  * floats per pair (rows N, B, E, J, C; pairs back to back in list order), score one

@@ -38,8 +38,9 @@ EXPORTS = [
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
     "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
     "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_stats", "sx_results_copy",
-    "sx_results_free", "sx_encode", "sx_profile_info",
+    "sx_results_free", "sx_encode", "sx_profile_info", "sx_score_pairs_noalloc",
 ]
+NOALLOC_MAX_PAIRS = 1 << 18  # SX_NOALLOC_MAX_PAIRS
 
 
 class SxError(RuntimeError):
@@ -138,6 +139,8 @@ def lib():
         L.sx_profile_info.argtypes = [vp, u32, cpp, cpp, cpp, ctypes.POINTER(i32)]
         L.sx_score_all.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
         L.sx_score_pairs.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, u64, vp, ctypes.POINTER(ctypes.c_double)]
+        L.sx_score_pairs_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, u64, vp,
+                                             ctypes.POINTER(ctypes.c_double)]
         L.sx_specials.argtypes = [vp, ctypes.c_int, vp, vp, u64, vp, vp]
         L.sx_run_cascade.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sx_cascade_stats), vp, u64,
                                      ctypes.POINTER(u64)]
@@ -396,6 +399,18 @@ class Context:
         ms = ctypes.c_double()
         self._chk(self._L.sx_score_pairs(self._h, stage, form, _ptr(prof), _ptr(tgt), len(prof), _ptr(out),
                                          ctypes.byref(ms)))
+        return out, ms.value
+
+    def score_pairs_noalloc(self, stage: int, prof, tgt, form: int = FORM_BATCHED, out=None):
+        """score_pairs through sx_score_pairs_noalloc (the library allocates nothing during
+        the call; at most NOALLOC_MAX_PAIRS pairs).  out, if given, is the result array."""
+        prof = np.ascontiguousarray(prof, np.uint32)
+        tgt = np.ascontiguousarray(tgt, np.uint32)
+        if out is None:
+            out = np.empty(len(prof), _stage_dtype(stage))
+        ms = ctypes.c_double()
+        self._chk(self._L.sx_score_pairs_noalloc(self._h, stage, form, _ptr(prof), _ptr(tgt), len(prof), _ptr(out),
+                                                 ctypes.byref(ms)))
         return out, ms.value
 
     def specials(self, stage: int, prof, tgt):

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -73,6 +74,33 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// The device memory of sx_score_pairs_noalloc: queue heads, the pair list's target ids,
+// its tiles and the results, for at most SX_NOALLOC_MAX_PAIRS pairs.  Each device holds
+// its own copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives for the device (noalloc_scratch), never through the name.
+constexpr size_t NA_HEADS = 0, NA_TIDS = 1024;
+constexpr size_t NA_TILES = NA_TIDS + (size_t)SX_NOALLOC_MAX_PAIRS * sizeof(int32_t);
+constexpr size_t NA_OUT = NA_TILES + (size_t)SX_NOALLOC_MAX_PAIRS * sizeof(SxTile);
+constexpr size_t NA_BYTES = NA_OUT + (size_t)SX_NOALLOC_MAX_PAIRS * sizeof(double);
+static_assert(NA_TILES % 256 == 0 && NA_OUT % 256 == 0, "scratch regions stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// The scratch array's address on device dev (which must be current), looked up once per
+// device.
+static void* noalloc_scratch(int dev) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw SxError(SX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    addr[dev] = ptr;
+    return ptr;
+}
+
 Ctx::Ctx(int dev) : device(dev) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw SxError(SX_ERR_NODEVICE, "no HIP device available");
@@ -105,11 +133,21 @@ Ctx::Ctx(int dev) : device(dev) {
         ki.block = tab[i].block;
         ki.lds = tab[i].lds;
         ki.fn = tab[i].fn;
+        ki.index = i;
         // every (stage, Q, G, special rows, table mode) names exactly one kernel
         if (!kernels.emplace(key(ki.stage, ki.q, ki.g, ki.spec, ki.hoist), ki).second)
             throw SxError(SX_ERR_STATE, "kernel table has two entries for stage " + std::to_string(ki.stage) +
                                             " Q=" + std::to_string(ki.q) + " G=" + std::to_string(ki.g));
     }
+    // the allocation-free pair scoring: the device's scratch array and the host staging
+    na_dev = noalloc_scratch(dev);
+    na_order.resize(SX_NOALLOC_MAX_PAIRS);
+    na_tids.resize(SX_NOALLOC_MAX_PAIRS);
+    na_tiles.resize(SX_NOALLOC_MAX_PAIRS);
+    na_res.resize(SX_NOALLOC_MAX_PAIRS);
+    na_cnt.assign((size_t)nk, 0);
+    na_kern.assign((size_t)nk, nullptr);
+    for (const auto& kv : kernels) na_kern[(size_t)kv.second.index] = &kv.second;
 }
 
 Ctx::~Ctx() {
@@ -444,13 +482,13 @@ void Fanout::join() {
 }
 
 void Ctx::launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
-                 int head_slot, hipStream_t s) {
+                 int head_slot, hipStream_t s, unsigned long long* heads) {
     if (!s) s = stream;
     const size_t lds = lds_bytes(k);
     if (lds > LDS_MAX) throw SxError(SX_ERR_ARG, "profile tables exceed the LDS budget");
     if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (persistent) {
-        w.head = d_heads + head_slot;
+        w.head = (heads ? heads : d_heads) + head_slot;
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, w.head, 1);
         HIPCHK(hipGetLastError());
     } else {
@@ -612,6 +650,110 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         }
     }
     fo.join();
+}
+
+// run_pairs (scores only) with no allocation: the pairs are ordered, tiled and grouped by
+// shape in host staging reserved at creation (std::sort on indices with the index as the
+// last key gives the stable order of run_pairs, and a counting pass over the kernel table
+// groups the tiles), and every device buffer is a region of the scratch array.  The tiles
+// and results therefore match run_pairs's, pair for pair.
+void Ctx::score_pairs_noalloc(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* out,
+                              double* ms) {
+    if (n > SX_NOALLOC_MAX_PAIRS) throw SxError(SX_ERR_ARG, "pair count exceeds SX_NOALLOC_MAX_PAIRS");
+    for (uint64_t i = 0; i < n; i++)
+        if (prof[i] >= profs.size() || tgt[i] >= nt) throw SxError(SX_ERR_ARG, "pair index out of range");
+    if (n == 0) {
+        if (ms) *ms = 0.0;
+        return;
+    }
+    uint32_t* order = na_order.data();
+    for (uint32_t i = 0; i < (uint32_t)n; i++) order[i] = i;
+    if (form == SX_FORM_BATCHED)
+        std::sort(order, order + n, [&](uint32_t a, uint32_t b) {
+            if (prof[a] != prof[b]) return prof[a] < prof[b];
+            if (tlen[tgt[a]] != tlen[tgt[b]]) return tlen[tgt[a]] > tlen[tgt[b]];
+            return a < b;
+        });
+    else
+        std::sort(order, order + n, [&](uint32_t a, uint32_t b) { return prof[a] != prof[b] ? prof[a] < prof[b] : a < b; });
+    for (uint64_t i = 0; i < n; i++) na_tids[i] = (int32_t)tgt[order[i]];
+    // tiles per kernel, then their offsets, then the tiles grouped by kernel
+    std::fill(na_cnt.begin(), na_cnt.end(), 0);
+    auto each_run = [&](auto&& f) {
+        for (uint64_t i = 0, e; i < n; i = e) {
+            const uint32_t p = prof[order[i]];
+            for (e = i; e < n && prof[order[e]] == p; e++) {
+            }
+            const Shape s = shape_for(stage, form, profs[p].m);
+            f(kernel(stage, s.q, s.g, false, s.hoist), p, i, e);
+        }
+    };
+    each_run([&](const KernelInfo& k, uint32_t, uint64_t i, uint64_t e) {
+        const uint64_t ng = (uint64_t)(k.block / k.g);
+        na_cnt[(size_t)k.index] += (int64_t)((e - i + ng - 1) / ng);
+    });
+    int64_t tot = 0;
+    for (int64_t& v : na_cnt) {
+        const int64_t c = v;
+        v = tot;
+        tot += c;
+    }
+    each_run([&](const KernelInfo& k, uint32_t p, uint64_t i, uint64_t e) {
+        const uint64_t ng = (uint64_t)(k.block / k.g);
+        for (uint64_t a = i; a < e; a += ng) {
+            SxTile& t = na_tiles[(size_t)na_cnt[(size_t)k.index]++];
+            t.prof = (int32_t)p;
+            t.count = (int32_t)std::min<uint64_t>(ng, e - a);
+            t.first = (int64_t)a;
+            t.out = (int64_t)a;
+        }
+    });
+    // na_cnt[k] now ends kernel k's tiles
+    uint8_t* base = (uint8_t*)na_dev;
+    unsigned long long* heads = (unsigned long long*)(base + NA_HEADS);
+    int32_t* d_tids = (int32_t*)(base + NA_TIDS);
+    SxTile* d_tiles = (SxTile*)(base + NA_TILES);
+    void* d_out = base + NA_OUT;
+    HIPCHK(hipMemcpyAsync(d_tids, na_tids.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_tiles, na_tiles.data(), (size_t)tot * sizeof(SxTile), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipEventRecord(ev0, stream));
+    int slot = 0;
+    {
+        Fanout fo(*this, stream);  // batched: the shape groups side by side
+        int64_t start = 0;
+        for (size_t ki = 0; ki < na_cnt.size(); ki++) {
+            const int64_t end = na_cnt[ki], cnt = end - start;
+            if (cnt > 0) {
+                const KernelInfo& k = *na_kern[ki];
+                SxWork w{};
+                w.tiles = d_tiles + start;
+                w.tids = d_tids;
+                w.ntiles = cnt;
+                SxSpecialsOut sp{};
+                if (form == SX_FORM_BATCHED) {
+                    const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
+                    launch(k, stage, w, d_out, sp, grid, true, slot++ % 64, fo.next(), heads);
+                } else {
+                    launch(k, stage, w, d_out, sp, (int)w.ntiles, false, 0);
+                }
+            }
+            start = end;
+        }
+        fo.join();
+    }
+    HIPCHK(hipEventRecord(ev1, stream));
+    const size_t es = stage == SX_STAGE_FORWARD || stage == SX_STAGE_BACKWARD ? 8 : 4;
+    HIPCHK(hipMemcpyAsync(na_res.data(), d_out, n * es, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (ms) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
+        *ms = t;
+    }
+    if (out) {
+        const uint8_t* r = (const uint8_t*)na_res.data();
+        for (uint64_t i = 0; i < n; i++) std::memcpy((uint8_t*)out + (size_t)order[i] * es, r + i * es, es);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,6 +1216,14 @@ extern "C" int sx_score_pairs(sx_ctx* ctx, int stage, int form, const uint32_t* 
                 std::memcpy((uint8_t*)out + c.last_order[i] * es, tmp.data() + i * es, es);
         }
     });
+}
+
+extern "C" int sx_score_pairs_noalloc(sx_ctx* ctx, int stage, int form, const uint32_t* prof, const uint32_t* tgt,
+                                      uint64_t n, void* out, double* ms) {
+    if (!ctx || stage < 0 || stage > SX_STAGE_VIT16 || (form != SX_FORM_PLAIN && form != SX_FORM_BATCHED) ||
+        (n && (!prof || !tgt)))
+        return SX_ERR_ARG;
+    return guard(ctx, [&] { ctx->c->score_pairs_noalloc(stage, form, prof, tgt, n, out, ms); });
 }
 
 extern "C" int sx_specials(sx_ctx* ctx, int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp,

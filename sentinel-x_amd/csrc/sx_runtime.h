@@ -40,6 +40,7 @@ struct KernelInfo {
     int stage = 0, q = 0, g = 0, spec = 0, hoist = 0, wid = 0;
     int block = 256;  // threads per workgroup; a tile holds block / g pairs
     int lds = 0;      // bytes of dynamic LDS per workgroup (the staged profile tables)
+    int index = 0;    // position in the kernel table
     const void* fn = nullptr;
 };
 
@@ -161,13 +162,25 @@ struct Ctx {
     // s = nullptr: the context's stream.  Two launches that may run at the same time
     // need different head slots.
     void launch(const KernelInfo& k, int stage, SxWork w, void* out, SxSpecialsOut sp, int grid, bool persistent,
-                int head_slot, hipStream_t s = nullptr);
+                int head_slot, hipStream_t s = nullptr, unsigned long long* heads = nullptr);
     void run_all(int stage, int form, void* dout);
     void run_implicit(int stage, int form, const std::vector<int32_t>& plist, const int32_t* dtids, int ntids,
                       void* dout, int64_t out_stride);
     void run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* dout, bool spec,
                    const std::vector<int64_t>* row_off, float* dx5, double* dls);
     double timed(const std::function<void()>& f);
+    // sx_score_pairs_noalloc: run_pairs with every device buffer in the device's static
+    // scratch array (na_dev, from hipGetSymbolAddress at creation) and host staging
+    // reserved at creation, so that a call allocates nothing
+    void* na_dev = nullptr;
+    std::vector<uint32_t> na_order;
+    std::vector<int32_t> na_tids;
+    std::vector<SxTile> na_tiles;
+    std::vector<double> na_res;
+    std::vector<int64_t> na_cnt;             // tiles per kernel-table index
+    std::vector<const KernelInfo*> na_kern;  // kernel-table index -> kernel
+    void score_pairs_noalloc(int stage, int form, const uint32_t* prof, const uint32_t* tgt, uint64_t n, void* out,
+                             double* ms);
     // Forward (stage 2) or Backward (stage 3) special rows of n pairs: sp receives
     // 5 x (L+1) floats per pair (rows N B E J C, pairs back to back), score one value each
     void special_rows(int stage, const uint32_t* prof, const uint32_t* tgt, uint64_t n, float* sp, double* score);
