@@ -208,9 +208,10 @@ def kernel_mode(sxa, synth, tune_ms, reps=5):
 
 def precision_gpu(sxa, synth):
     """Forward on all 200 profiles of the precision-stress workload (M 1500-2500) against
-    20 of its proteins (L 8,000-35,000 with low-complexity runs): the device scores and
-    the rate (4,000 pairs, enough to fill the device)."""
-    wl = synth.make_workload(3, profile_scale=1.0, target_scale=0.01)
+    100 of its proteins (L 8,000-35,000 with low-complexity runs): the device scores and
+    the rate.  20,000 pairs of one 64-lane wavefront each fill the device about ten times
+    over, so the longest pairs do not set the time alone."""
+    wl = synth.make_workload(3, profile_scale=1.0, target_scale=0.05)
     ms = np.array([h.m for h in wl.hmms], np.int64)
     cells = int(ms.sum()) * sum(len(s) for s in wl.seqs)
     with sxa.Context(0) as pc:
@@ -436,7 +437,7 @@ def main():
         log(f"[bench] cpu baseline: {cpu}")
         if kmode is not None:
             threads = host_cpus()["threads"]
-            k1, k3 = min(args.fwd_err_profiles, len(wl1.hmms)), min(20, len(wl3.hmms))
+            k1, k3 = min(args.fwd_err_profiles, len(wl1.hmms)), min(4, len(wl3.hmms))
             fwd_err = {"marker_set": forward_error(wl1, fwd1, k1, threads),
                        "precision": forward_error(wl3, fwd3, k3, threads)}
             fwd_err["marker_set"]["sample"] = f"first {k1} profiles x all {len(wl1.seqs)} proteins"

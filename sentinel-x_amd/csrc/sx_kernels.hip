@@ -383,6 +383,26 @@ struct EmRows {
     }
 };
 
+// Forward's transition records: for cell q of lane j (model position k = j*Q + 1 + q) the
+// eight transition probabilities it reads, in one 32-byte record
+//   {MM(k-1), IM(k-1), DM(k-1), BM(k)} {MI(k), II(k), MD(k), DD(k)},
+// so a cell's operands are two ds_read_b128 at immediate offsets from the lane's base
+// (positions 0 and past M hold 0).  Lane blocks are 8Q + 4 floats apart: an odd number of
+// 16-byte slots, so the 16 lanes of a quarter-wave that read different positions start on
+// 16 distinct four-bank slots; groups of a workgroup read the same addresses (the records
+// do not depend on the residue), which the LDS broadcasts.  After the records, each lane
+// block's D-chain products: PC[q] = DD(k0) ... DD(k0+q-1) for q = 0..Q (PC[0] = 1, PC[Q]
+// the block's slope) and their sum PC[0] + ... + PC[Q-1], in blocks of PSTRIDE floats (an
+// odd number of 16-byte slots).
+template <int Q, int G>
+struct FwdRec {
+    static constexpr int RSTRIDE = 8 * Q + 4;
+    static constexpr int PSTRIDE = ((Q + 4) / 4) % 2 ? Q + 4 : Q + 8;
+    static constexpr int REC = G * RSTRIDE;
+    static constexpr int TAB = REC + G * PSTRIDE;
+};
+constexpr int round4(int n) { return (n + 3) & ~3; }
+
 // ---------------------------------------------------------------------------
 // MSV (dp.rs:92-145)
 // ---------------------------------------------------------------------------
@@ -981,28 +1001,36 @@ __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, cons
 // ---------------------------------------------------------------------------
 // Forward (dp.rs:229-298), multihit probabilities e_move = e_loop = 1/2
 // ---------------------------------------------------------------------------
+// One forward pass over a lane's cells per row.  The D chain is affine across lanes:
+// with c the D value at the lane's first position k0 (set by the lanes before it), the
+// lane's values are D[k0+q] = Dl[q] + c * PC[q], where Dl is the chain started from
+// Dl[0] = 0, and the value it hands on is out(c) = Dl_out + c * PC[Q].  The pass computes
+// M, I and Dl cell by cell (each cell's record read once), an affine scan of the lanes'
+// maps gives every lane its c, and the row sum adds c * sum(PC) to the sum of M and Dl.
+// The stored D values stay in the (Dl, c) form: the next row corrects each one as it
+// reads it (one FMA per cell), so no extra pass is needed.
 template <int Q, int G, bool SPEC, int TM>
-__device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew, const float* __restrict__ tr,
+__device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew, const float* __restrict__ trec,
                                            const uint8_t* seq, int L, float ploop, float pmove, float* x5,
                                            double* lsrow, bool writer) {
-    constexpr int WID = Geo<Q, G>::WID;
     constexpr bool HOIST = TM == TM_REG;
+    // long models: the next emission row is prefetched into registers up to Q = 32 (beyond,
+    // the second copy of the row would leave one wave per SIMD)
+    constexpr bool GEM = TM == TM_GEM && Q <= 32;
+    using FR = FwdRec<Q, G>;
     const int j = (int)(threadIdx.x & (G - 1));
-    const int kb = j * Geo<Q, G>::STRIDE + 1;  // LDS slot of the lane's first position
     const EmRows<Q, G, TM> er(eot, ew, j);
-    float blk = 1.0f;  // product of p_dd over the lane's block: the D-chain map's slope
-    if constexpr (G > 1) {
-        const float* __restrict__ pdd0 = tr + SX_TDD * WID + kb;
-#pragma unroll
-        for (int q = 0; q < Q; q++) blk *= pdd0[q - 1];
-    }
-    float mr[Q], ir[Q], dr[Q];
+    const float* __restrict__ pcb = trec + FR::REC + j * FR::PSTRIDE;
+    const float blk = pcb[Q];       // the block's D-chain slope
+    const float spc = pcb[Q + 1];   // sum of PC[0..Q-1]
+    float mr[Q], ir[Q], dl[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         mr[q] = 0.0f;
         ir[q] = 0.0f;
-        dr[q] = 0.0f;
+        dl[q] = 0.0f;
     }
+    float c = 0.0f;  // this lane's incoming D value of the stored row
     float xn = 1.0f, xb = pmove, xj = 0.0f, xc = 0.0f;
     double lnscale = 0.0;
     if constexpr (SPEC) {
@@ -1013,55 +1041,92 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
     }
     FwdStream rs;
     rs.init(seq);
+    int x = rs.next();
+    // long models read their emission rows from global memory: the next row's values are
+    // loaded while this row is computed (the residue stream is known ahead)
+    float eon[GEM ? Q : 1];
+    if constexpr (GEM) {
+        const float* __restrict__ e0 = er.row(x);
+#pragma unroll
+        for (int q = 0; q < Q; q++) eon[q] = e0[q];
+    }
+    bool star_prev = false;
     for (int i = 0; i < L; i++) {
-        const int x = rs.next();
-        const float ins = x != SX_CODE_STAR ? 1.0f : 0.0f;
-        const int kbo = HOIST ? kb : opaque(kb);
-        const float* __restrict__ pmm = tr + SX_TMM * WID + kbo;
-        const float* __restrict__ pmi = tr + SX_TMI * WID + kbo;
-        const float* __restrict__ pmd = tr + SX_TMD * WID + kbo;
-        const float* __restrict__ pim = tr + SX_TIM * WID + kbo;
-        const float* __restrict__ pii = tr + SX_TII * WID + kbo;
-        const float* __restrict__ pdm = tr + SX_TDM * WID + kbo;
-        const float* __restrict__ pdd = tr + SX_TDD * WID + kbo;
-        const float* __restrict__ pbm = tr + SX_TBM * WID + kbo;
-        const float* __restrict__ eo = er.row(x);
-        const float mL = grp_from_prev<G>(mr[Q - 1], 0.0f);
-        const float iL = grp_from_prev<G>(ir[Q - 1], 0.0f);
-        const float dL = grp_from_prev<G>(dr[Q - 1], 0.0f);
+        // a '*' residue emits no insert state (dp.rs:256,260): the inserts of a '*' row are
+        // cleared at the top of the next row, under a wave vote, instead of a multiply in
+        // every cell (the row itself reads no insert state of its own)
+        if constexpr (HOIST) {
+            // (a branch here would keep the transitions from being hoisted into registers)
 #pragma unroll
-        for (int q = Q - 1; q >= 0; q--) {
-            const float mp = q ? mr[q - 1] : mL;
-            const float ip = q ? ir[q - 1] : iL;
-            const float dp = q ? dr[q - 1] : dL;
-            float s = xb * pbm[q];
-            s = fmaf(mp, pmm[q - 1], s);
-            s = fmaf(ip, pim[q - 1], s);
-            s = fmaf(dp, pdm[q - 1], s);
-            const float iv = fmaf(mr[q], pmi[q], ir[q] * pii[q]) * ins;
-            mr[q] = s * eo[q];
-            ir[q] = iv;
-        }
-        const float mcL = grp_from_prev<G>(mr[Q - 1], 0.0f);
-        dr[0] = mcL * pmd[-1];
+            for (int q = 0; q < Q; q++) ir[q] = star_prev ? 0.0f : ir[q];
+        } else if (__builtin_amdgcn_ballot_w64(star_prev) != 0ull) [[unlikely]] {
+            __asm__ volatile("s_nop 0");  // a side effect: this rare path stays a branch, not selects
+            if (star_prev) {
 #pragma unroll
-        for (int q = 1; q < Q; q++) dr[q] = fmaf(dr[q - 1], pdd[q - 1], mr[q - 1] * pmd[q - 1]);
-        if constexpr (G > 1) {
-            float A = dr[Q - 1];
-            grp_affine_scan<G, true>(A, blk);
-            const float din = grp_from_prev<G>(A, 0.0f);
-            float P = pdd[-1];
-            dr[0] = fmaf(din, P, dr[0]);
-#pragma unroll
-            for (int q = 1; q < Q; q++) {
-                P *= pdd[q - 1];
-                dr[q] = fmaf(din, P, dr[q]);
+                for (int q = 0; q < Q; q++) ir[q] = 0.0f;
             }
         }
-        float e = 0.0f;
+        star_prev = x == SX_CODE_STAR;
+        const float* __restrict__ eo = er.row(x);
+        float eoc[GEM ? Q : 1];
+        // the next residue, read without a branch: past the last row the stream reads the
+        // target's padding, and row 0 is then loaded instead (its values go unused)
+        const int xs = rs.next();
+        x = i + 1 < L ? xs : 0;
+        if constexpr (GEM) {
+            const float* __restrict__ en = er.row(x);
 #pragma unroll
-        for (int q = 0; q < Q; q++) e += mr[q] + dr[q];
-        float xe = grp_reduce<G>(e, OpAdd());
+            for (int q = 0; q < Q; q++) {
+                eoc[q] = eon[q];
+                eon[q] = en[q];
+            }
+        }
+        const int ro = HOIST ? 0 : opaque(0);
+        const float4* __restrict__ rec = reinterpret_cast<const float4*>(trec + j * FR::RSTRIDE + ro);
+        // the products PC[q]: read from LDS every row where the tables are (held in
+        // registers they would cost Q more VGPRs), and with the transitions in registers
+        // formed as a running product of their DD values (an LDS read through an opaque
+        // address would keep the transitions from being hoisted)
+        const float* __restrict__ pc = pcb + ro;
+        // the previous lane's last M, I and (corrected) D of the stored row
+        const float dlast = fmaf(c, HOIST ? pcb[Q - 1] : pc[Q - 1], dl[Q - 1]);
+        float mpo = grp_from_prev<G>(mr[Q - 1], 0.0f);
+        float ipo = grp_from_prev<G>(ir[Q - 1], 0.0f);
+        float dpo = grp_from_prev<G>(dlast, 0.0f);
+        float dnext = 0.0f;  // Dl of the next cell
+        float em = 0.0f, ed = 0.0f;
+        float pcq = 1.0f;  // PC[q], with the transitions in registers
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const float4 ta = rec[2 * q], tb = rec[2 * q + 1];  // {MM IM DM BM}, {MI II MD DD}
+            const float mo = mr[q], io = ir[q];
+            const float pq = HOIST ? pcq : pc[q];
+            if constexpr (HOIST) pcq *= tb.w;
+            const float dold = q + 1 < Q ? fmaf(c, pq, dl[q]) : dlast;  // stored D at this cell
+            float s = xb * ta.w;
+            s = fmaf(mpo, ta.x, s);
+            s = fmaf(ipo, ta.y, s);
+            s = fmaf(dpo, ta.z, s);
+            const float mv = s * (GEM ? eoc[q] : eo[q]);
+            ir[q] = fmaf(mo, tb.x, io * tb.y);
+            mr[q] = mv;
+            dl[q] = dnext;
+            em += mv;
+            ed += dnext;
+            dnext = fmaf(dnext, tb.w, mv * tb.z);
+            mpo = mo;
+            ipo = io;
+            dpo = dold;
+        }
+        float cn = dnext;  // the lane's map out(c) = dnext + c * blk, scanned over the group
+        if constexpr (G > 1) {
+            grp_affine_scan<G, true>(cn, blk);
+            cn = grp_from_prev<G>(cn, 0.0f);
+        } else {
+            cn = 0.0f;
+        }
+        c = cn;
+        float xe = grp_reduce<G>(fmaf(c, spc, em + ed), OpAdd());
         xj = fmaf(xj, ploop, xe * 0.5f);
         xc = fmaf(xc, ploop, xe * 0.5f);
         xn = xn * ploop;
@@ -1072,8 +1137,9 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
             for (int q = 0; q < Q; q++) {
                 mr[q] *= inv;
                 ir[q] *= inv;
-                dr[q] *= inv;
+                dl[q] *= inv;
             }
+            c *= inv;
             xn *= inv;
             xb *= inv;
             xj *= inv;
@@ -1285,6 +1351,8 @@ constexpr int lds_floats() {
         return vq_tab_dwords<Q, G, TM>();
     else if constexpr (tm_aligned<TM>())
         return GeoA<Q, G>::TAB + (ST == ST_MSV ? 0 : SX_NT * GeoA<Q, G>::TWID);
+    else if constexpr (ST == ST_FWD)
+        return round4(lds_emission_floats<Q, G, TM>()) + FwdRec<Q, G>::TAB;
     else
         return lds_emission_floats<Q, G, TM>() + (ST == ST_MSV ? 0 : SX_NT * tm_wid<Q, G, TM>());
 }
@@ -1330,9 +1398,16 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
     return ti;
 }
 
+// Waves per SIMD the compiler must leave room for: the Forward kernels with transitions in
+// registers need two (at one they leave the SIMD idle through every dependent step).
+template <int STAGE, int TM>
+constexpr int k_min_waves() {
+    return STAGE == ST_FWD && TM == TM_REG ? 2 : 1;
+}
+
 template <int Q, int G, int STAGE, bool SPEC, int TM>
-__global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out,
-                                                          SxSpecialsOut SP) {
+__global__ void __launch_bounds__(tm_block<TM>(), (k_min_waves<STAGE, TM>()))
+    k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out, SxSpecialsOut SP) {
     static_assert(STAGE == ST_MSV || STAGE == ST_VIT || STAGE == ST_MSVQ || STAGE == ST_VITQ || !tm_aligned<TM>(),
                   "aligned lane blocks are an MSV and Viterbi layout");
     constexpr int WID = tm_wid<Q, G, TM>();
@@ -1456,7 +1531,37 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                     lds[idx] = k < w ? src[x * w + k] : 0.0f;
                 }
             }
-            if (STAGE != ST_MSV && STAGE != ST_MSVQ && !tm_aligned<TM>()) {
+            if constexpr (STAGE == ST_FWD) {
+                // transition records (FwdRec): lane jj's cell q is model position
+                // k = jj*Q + 1 + q; fields read at k-1 (MM IM DM MD DD) or k (BM MI II)
+                const float* src = P.tp + P.toff[ti.prof];
+                float* rec = lds + round4(NTAB_E);
+                using FR = FwdRec<Q, G>;
+                constexpr int RS = FR::RSTRIDE;
+                for (int idx = threadIdx.x; idx < FR::REC; idx += blockDim.x) {
+                    const int jj = idx / RS, r = idx - jj * RS;
+                    const int q = r >> 3, c = r & 7;
+                    const int k = jj * Q + 1 + q;
+                    // c: 0 MM, 1 IM, 2 DM (at k-1), 3 BM, 4 MI, 5 II, 6 MD, 7 DD (at k)
+                    const int row = c == 0 ? SX_TMM : c == 1 ? SX_TIM : c == 2 ? SX_TDM : c == 3 ? SX_TBM
+                                  : c == 4 ? SX_TMI : c == 5 ? SX_TII : c == 6 ? SX_TMD : SX_TDD;
+                    const int kk = c <= 2 ? k - 1 : k;
+                    rec[idx] = q < Q && kk >= 0 && kk < w ? src[row * w + kk] : 0.0f;
+                }
+                // the D-chain products of each lane block, one thread a block
+                for (int jj = threadIdx.x; jj < G; jj += blockDim.x) {
+                    float* pcb = rec + FR::REC + jj * FR::PSTRIDE;
+                    float prod = 1.0f, sum = 0.0f;
+                    for (int q = 0; q < Q; q++) {
+                        const int k = jj * Q + 1 + q;
+                        pcb[q] = prod;
+                        sum += prod;
+                        prod *= k < w ? src[SX_TDD * w + k] : 0.0f;
+                    }
+                    pcb[Q] = prod;
+                    pcb[Q + 1] = sum;
+                }
+            } else if (STAGE != ST_MSV && STAGE != ST_MSVQ && !tm_aligned<TM>()) {
                 const float* src = (STAGE == ST_VIT ? P.tsc : P.tp) + P.toff[ti.prof];
                 const float pad = STAGE == ST_VIT ? SX_NEG_INF : 0.0f;
                 for (int idx = threadIdx.x; idx < SX_NT * WID; idx += blockDim.x) {
@@ -1522,8 +1627,8 @@ __global__ void __launch_bounds__(tm_block<TM>()) k_tiles(SxProfiles P, SxTarget
                     lsr = SP.ls + SP.row_off[o];
                 }
                 if (L > 0)
-                    r = fwd_pair<Q, G, SPEC, TM>(em, P.gw, lds + NTAB_E, seq, L, lm.ploop, lm.pmove, x5, lsr,
-                                                 jl == 0);
+                    r = fwd_pair<Q, G, SPEC, TM>(em, P.gw, lds + round4(NTAB_E), seq, L, lm.ploop, lm.pmove, x5,
+                                                 lsr, jl == 0);
                 if (jl == 0) reinterpret_cast<double*>(out)[o] = r;
             } else {
                 double r = -(double)__builtin_inf();

@@ -4,7 +4,8 @@
         --shapes 16,16,0 16,16,3 32,8,0 32,8,3
 
 Each shape is q,g,mode (mode = the table mode of sx_set_shapes: 0 LDS, 1 transitions in
-registers, 3 / 4 MSV aligned lane blocks with 512 / 1024-thread workgroups).  For each
+registers, 3 / 4 MSV aligned lane blocks with 512 / 1024-thread workgroups), or "auto" for
+the built-in choice (the only one for models over 1024 positions).  For each
 shape the batched form scores every profile against the targets (log-normal lengths,
 median 300) twice; the faster run's device time and rate are printed as one JSON line,
 with whether the scores equal those of the first shape (they must: every shape computes
@@ -46,8 +47,12 @@ def main():
         ctx.set_profiles(hmms)
         ctx.set_targets(seqs)
         for s in a.shapes:
-            q, g, mode = map(int, s.split(","))
-            ctx.set_shapes(a.stage, [(min(q * g, 1024), q, g, mode)])
+            if s == "auto":
+                ctx.set_shapes(a.stage, [])
+                q, g, mode = ctx.shape_for(a.stage, a.m)
+            else:
+                q, g, mode = map(int, s.split(","))
+                ctx.set_shapes(a.stage, [(min(q * g, 1024), q, g, mode)])
             out, t1 = ctx.score_all(a.stage, sxa.FORM_BATCHED)
             t2 = ctx.score_all(a.stage, sxa.FORM_BATCHED, want=False)[1]
             if ref is None:
