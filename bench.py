@@ -157,7 +157,35 @@ def plain_leg(sxa, ctx, wl, n_plain, f, graph, bound, vbound):
             "identical_decisions": same}
 
 
-def kernel_mode(sxa, synth, tune_ms, reps=5):
+SHAPES_DIR = os.path.join(ROOT, "sentinel-x_amd", "shapes")
+
+
+def install_shapes(ctx, wl, choice, tune_ms, stages):
+    """The shape tables of ctx: the committed table for this workload (or the file named
+    by choice), else the autotuner's (choice 'tune', or no table for the workload).
+    Returns where the tables came from."""
+    sys.path.insert(0, os.path.join(ROOT, "sentinel-x_amd"))
+    import tune_shapes
+    key = tune_shapes.workload_key(wl)
+    path = choice if choice not in (None, "tune") else os.path.join(SHAPES_DIR, f"{wl.name}.json")
+    if choice != "tune" and os.path.exists(path):
+        got = tune_shapes.load(ctx, path)
+        if got == key:
+            return os.path.relpath(path, ROOT)
+        log(f"[bench] {path} is for '{got}', not '{key}': autotuning instead")
+    if tune_ms <= 0:
+        return "built-in"
+    for st in stages:
+        t0 = time.time()
+        ctx.autotune(st, tune_ms * (3 if st in (0, 4) else 1))  # MSV times 3x larger samples
+        log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
+        for row in ctx.autotune_log(st):
+            log(f"[tune] stage {st} bucket<= {row[0]:5d} Q={row[1]:3d} G={row[2]:2d} reg={row[3]} "
+                f"{row[4]:9.1f} GCUPS")
+    return "autotuned"
+
+
+def kernel_mode(sxa, synth, tune_ms, reps=5, shapes=None):
     """Every pair through every stage (no filters) on the marker-set workload (400 profiles,
     M 80-1500, x 4,000 proteins): batched and plain GCUPS per stage, on identical pairs;
     then the full search (filters on) with its phase times.  Returns the workload, the
@@ -171,8 +199,7 @@ def kernel_mode(sxa, synth, tune_ms, reps=5):
     with sxa.Context(0) as kc:
         kc.set_profiles(wl.hmms)
         kc.set_targets(wl.seqs)
-        for st in (0, 1, 2):
-            kc.autotune(st, tune_ms)
+        out["shape_table"] = install_shapes(kc, wl, shapes, tune_ms, (0, 1, 2, 4))
         for st, name in ((0, "msv"), (1, "viterbi"), (2, "forward")):
             kc.score_all(st, sxa.FORM_BATCHED, want=False)
             tb = float(np.median([kc.score_all(st, sxa.FORM_BATCHED, want=False)[1] for _ in range(reps)]))
@@ -182,6 +209,7 @@ def kernel_mode(sxa, synth, tune_ms, reps=5):
                          "plain_ms": round(tp, 3), "plain_gcups": round(cells / tp / 1e6, 1),
                          "ratio_batched_over_plain": round(tp / tb, 2)}
         fwd, _ = kc.score_all(2, sxa.FORM_BATCHED)
+        shapes16 = kc.get_shapes(4)
         # the whole hmmsearch-equivalent search (filters on, E <= 10): where its time goes
         for _ in range(2):
             hits, doms = kc.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True)
@@ -194,7 +222,7 @@ def kernel_mode(sxa, synth, tune_ms, reps=5):
     with sxa.Context(0) as kq:
         kq.set_profiles(short)
         kq.set_targets(wl.seqs)
-        kq.autotune(4, tune_ms)
+        kq.set_shapes(4, shapes16)
         kq.score_all(4, sxa.FORM_BATCHED, want=False)
         tb = float(np.median([kq.score_all(4, sxa.FORM_BATCHED, want=False)[1] for _ in range(reps)]))
         kq.score_all(4, sxa.FORM_PLAIN, want=False)
@@ -223,6 +251,78 @@ def precision_gpu(sxa, synth):
                                  f"{len(wl.seqs)} proteins (L {min(len(x) for x in wl.seqs)}-"
                                  f"{max(len(x) for x in wl.seqs)})",
                      "cells": cells, "forward_ms": round(ms_, 3), "forward_gcups": round(cells / ms_ / 1e6, 1)}
+
+
+def single_m200_leg(sxa, synth, f, reps=5):
+    """The single-M200 workload (1 profile of 200 positions x 2,000 proteins, 5% homologues):
+    the batched and plain cascades and every stage on every pair, GCUPS per stage.  At
+    1.4e8 cells a stage it does not fill the device; it is the reference's CPU-sized case."""
+    wl = synth.make_workload(0)
+    calibrate(sxa, synth, wl.hmms, 200)
+    cells = wl.cells
+    out = {"workload": f"{wl.name}: 1 profile (M 200) x {len(wl.seqs)} proteins", "cells_per_stage": cells}
+    with sxa.Context(0) as c0:
+        c0.set_profiles(wl.hmms)
+        c0.set_targets(wl.seqs)
+        for form, name in ((sxa.FORM_BATCHED, "batched"), (sxa.FORM_PLAIN, "plain")):
+            c0.run_cascade(form, **f)
+            t = []
+            for _ in range(reps):
+                st, _ = c0.run_cascade(form, **f)
+                t.append(st.ms_total)
+            ms = float(np.median(t))
+            out[f"cascade_{name}"] = {"ms": round(ms, 3), "gcups": round(cells / ms / 1e6, 1),
+                                      "survivors": [int(st.n_msv_pass), int(st.n_bias_pass), int(st.n_vit_pass),
+                                                    int(st.n_fwd_pass)]}
+        for stage, name in ((0, "msv"), (1, "viterbi"), (2, "forward")):
+            c0.score_all(stage, sxa.FORM_BATCHED, want=False)
+            ms = float(np.median([c0.score_all(stage, sxa.FORM_BATCHED, want=False)[1] for _ in range(reps)]))
+            out[name] = {"ms": round(ms, 3), "gcups": round(cells / ms / 1e6, 1)}
+    return out
+
+
+def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=1000.0):
+    """The MAG-sweep workload: marker profiles against whole genomes of uneven size.
+    packed: every genome's proteins in one target set, so each stage is one batched
+    persistent launch per shape over all genomes (graph replay; all genomes complete
+    together).  per_genome: the genomes one after another, each its own cascade; the
+    latency of a genome is the host wall time from setting its proteins to holding its
+    records (p50 / p99 over genomes), and the throughput is the sweep's total cells over
+    the summed latencies."""
+    wl = synth.make_workload(4, profile_scale=profile_scale, n_genomes=n_genomes)
+    calibrate(sxa, synth, wl.hmms, 200)
+    ms = np.array([h.m for h in wl.hmms], np.int64)
+    lens = np.array([len(x) for x in wl.seqs], np.int64)
+    cells = int(ms.sum()) * int(lens.sum())
+    out = {"workload": f"{wl.name}: {len(ms)} profiles (M {ms.min()}-{ms.max()}) x {n_genomes} genomes "
+                       f"({len(lens)} proteins)", "cells": cells}
+    with sxa.Context(0) as cm:
+        cm.set_profiles(wl.hmms)
+        cm.set_targets(wl.seqs)
+        out["shape_table"] = install_shapes(cm, wl, shapes, tune_ms, (4, 0, 1, 2))
+        cm.run_cascade(sxa.FORM_BATCHED, use_graph=True, **f)
+        t = []
+        for _ in range(2):
+            st, _ = cm.run_cascade(sxa.FORM_BATCHED, use_graph=True, cap=0, **f)
+            t.append(st.ms_total)
+        pk = float(min(t))
+        out["packed"] = {"ms": round(pk, 3), "gcups": round(cells / pk / 1e6, 1), "hits": int(st.n_fwd_pass)}
+        lat = []
+        gcells = 0
+        for g in range(n_genomes):
+            idx = np.flatnonzero(wl.genome_of == g)
+            seqs = [wl.seqs[i] for i in idx]
+            t0 = time.perf_counter()
+            cm.set_targets(seqs)
+            st, rec = cm.run_cascade(sxa.FORM_BATCHED, use_graph=False, **f)
+            lat.append((time.perf_counter() - t0) * 1e3)
+            gcells += int(ms.sum()) * int(sum(len(x) for x in seqs))
+        lat = np.array(lat)
+        out["per_genome"] = {"latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+                             "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+                             "latency_ms_max": round(float(lat.max()), 2),
+                             "gcups": round(gcells / lat.sum() / 1e6, 1)}
+    return out
 
 
 def forward_error(wl, fwd_gpu, k, threads, z=None):
@@ -274,8 +374,11 @@ def main():
     ap.add_argument("--profile-scale", type=float, default=1.0)
     ap.add_argument("--target-scale", type=float, default=1.0)
     ap.add_argument("--plain-sample", type=float, default=0.02, help="fraction of profiles run in the plain form (0 = skip)")
+    ap.add_argument("--shapes", default=None,
+                    help="shape table file (default sentinel-x_amd/shapes/<workload>.json when it exists and "
+                         "matches the workload); 'tune' to autotune instead")
     ap.add_argument("--tune-ms", type=float, default=3000.0,
-                    help="autotuner budget per stage, three times this for MSV (0 = built-in shapes)")
+                    help="autotuner budget per stage when tuning, three times this for MSV (0 = built-in shapes)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-bound", action="store_true",
                     help="f32 MSV on every pair (no int16 MSV bound in front of it)")
@@ -292,6 +395,9 @@ def main():
                     help="skip the every-pair-every-stage runs and the Forward-error report")
     ap.add_argument("--fwd-err-profiles", type=int, default=40,
                     help="marker-set profiles (x all proteins) checked against the FP64 Forward")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-M200 leg")
+    ap.add_argument("--mag-sweep", default="slice", choices=("slice", "full", "none"),
+                    help="MAG-sweep leg: 1,000 markers x 40 genomes (slice), 5,000 x 200 (full), or none")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -317,14 +423,8 @@ def main():
     ctx.set_targets(wl.seqs)
     bound = not args.no_bound
     vbound = bool(args.vit_bound)
-    if args.tune_ms > 0:
-        for st in ((4, 0, 5, 1, 2) if bound else (0, 5, 1, 2)):
-            t0 = time.time()
-            ctx.autotune(st, args.tune_ms * (3 if st in (0, 4) else 1))  # MSV times 3x larger samples
-            log(f"[bench] stage {st} shapes (max_m, Q, G, reg) in {time.time() - t0:.1f} s: {ctx.get_shapes(st)}")
-            for row in ctx.autotune_log(st):
-                log(f"[tune] stage {st} bucket<= {row[0]:5d} Q={row[1]:3d} G={row[2]:2d} reg={row[3]} "
-                    f"{row[4]:9.1f} GCUPS")
+    shape_src = install_shapes(ctx, wl, args.shapes, args.tune_ms, (4, 0, 5, 1, 2) if bound else (0, 5, 1, 2))
+    log(f"[bench] shape tables: {shape_src}")
     graph = not args.no_graph
     for i in range(args.warmup):
         s, _ = ctx.run_cascade(sxa.FORM_BATCHED, use_graph=graph, msv_bound=bound, vit_bound=vbound, **f)
@@ -426,10 +526,23 @@ def main():
     kmode = prec = None
     if not args.no_kernel_mode:
         t0 = time.time()
-        wl1, kmode, fwd1 = kernel_mode(sxa, synth, min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0)
+        wl1, kmode, fwd1 = kernel_mode(sxa, synth, min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0,
+                                       shapes="tune" if args.shapes == "tune" else None)
         log(f"[bench] kernel mode ({time.time() - t0:.1f} s): {kmode}")
         wl3, fwd3, prec = precision_gpu(sxa, synth)
         log(f"[bench] precision slice: {prec}")
+
+    single = mag = None
+    if not args.no_single:
+        t0 = time.time()
+        single = single_m200_leg(sxa, synth, f)
+        log(f"[bench] single-M200 ({time.time() - t0:.1f} s): {single}")
+    if args.mag_sweep != "none":
+        t0 = time.time()
+        ps, ng = (1.0, 200) if args.mag_sweep == "full" else (0.2, 40)
+        mag = mag_sweep_leg(sxa, synth, f, ps, ng, shapes="tune" if args.shapes == "tune" else None,
+                            tune_ms=min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0)
+        log(f"[bench] mag-sweep ({time.time() - t0:.1f} s): {mag}")
 
     cpu = fwd_err = None
     if not args.no_cpu:
@@ -451,7 +564,7 @@ def main():
         "dtype": "i16+f32" if bound else "f32", "data": "synthetic",
         "config": {"workload": f"{wl.name}: {len(ms)} profiles x {len(lens)} proteins", "cells_msv": cells,
                    "f1": args.f1, "f2": args.f2, "f3": args.f3, "graph": graph, "msv_bound": bound,
-                   "viterbi_bound": vbound},
+                   "viterbi_bound": vbound, "shape_table": shape_src},
         "f32_cascade_without_bounds": f32_leg,
         "viterbi_bound": viterbi_bound,
         "stages": stages,
@@ -465,6 +578,8 @@ def main():
         "kernel_mode": kmode,
         "precision_forward": prec,
         "forward_error_vs_fp64": fwd_err,
+        "single_m200": single,
+        "mag_sweep": mag,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -200,6 +200,11 @@ struct CascadeState {
     std::vector<sx_pair_scores> last;
     // host-side tables
     std::vector<float> thr_msv, thr_vit, thr_fwd, nullsc;
+    // what the thresholds were computed for (profile set, F1-F3, uncalibrated profiles let
+    // through): a new target set keeps them
+    uint64_t thr_version = ~0ull;
+    double thr_f[3] = {-1, -1, -1};
+    bool thr_uncal = false;
     std::vector<int32_t> perm;           // profile ids by increasing model length
     int threshold_violations = 0;
     // the int16 MSV bound's thresholds in its units (per length-sorted target, per
@@ -331,11 +336,20 @@ static void build_plan(Ctx& c, CascadeState& S, double f1, double f2, double f3,
     S.f3 = f3;
     S.bound = bound;
     S.vbound = vbound;
+    const bool thr_ok = S.thr_version == c.profiles_version && S.thr_f[0] == f1 && S.thr_f[1] == f2 &&
+                        S.thr_f[2] == f3 && S.thr_uncal == c.cascade_pass_uncalibrated && S.thr_msv.size() == P;
+    if (!thr_ok) {
+        S.thr_version = c.profiles_version;
+        S.thr_f[0] = f1;
+        S.thr_f[1] = f2;
+        S.thr_f[2] = f3;
+        S.thr_uncal = c.cascade_pass_uncalibrated;
+    }
     S.thr_msv.resize(P);
     S.thr_vit.resize(P);
     S.thr_fwd.resize(P);
-    S.threshold_violations = 0;
-    for (size_t p = 0; p < P; p++) {
+    if (!thr_ok) S.threshold_violations = 0;
+    for (size_t p = 0; p < P && !thr_ok; p++) {
         const Stats& st = c.profs[p].stats;
         if (!st.calibrated()) {
             // the reference runs no filter for an uncalibrated model (pipeline.rs:164)

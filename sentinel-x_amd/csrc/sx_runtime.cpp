@@ -253,6 +253,7 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
                                           "; the longest supported model has " + std::to_string(SX_MAX_M) +
                                           " positions");
     cascade_dirty = true;
+    profiles_version++;
     free_profiles();
     profs = std::move(ps);
     std::vector<float> msc, mo, tsc, tp, tbm;

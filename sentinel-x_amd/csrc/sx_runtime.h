@@ -188,6 +188,9 @@ struct Ctx {
     // device filter cascade (sx_pipeline.cpp); rebuilt when profiles, targets or shapes change
     std::shared_ptr<CascadeState> cascade;
     bool cascade_dirty = true;
+    // counts set_profiles calls: the cascade keeps the per-profile pass thresholds, which
+    // depend on the profiles and F1-F3 only, across changes of targets and shapes
+    uint64_t profiles_version = 0;
 };
 
 }  // namespace sx
