@@ -519,7 +519,8 @@ def main():
                                                                (ms_launches * 1e-3) / 1e12, 3)},
             "msv_stage": {"achieved": round(stage_achieved, 3), "frac": round(stage_achieved / peak, 4)}}
     launches = [{"stage": int(r["stage"]), "q": int(r["q"]), "g": int(r["g"]), "reg": int(r["hoist"]),
-                 "ms": round(float(r["ms"]), 3), "gcups": round(float(r["cells"]) / max(float(r["ms"]), 1e-9) / 1e6, 1)}
+                 "ms": round(float(r["ms"]), 3), "cells": int(r["cells"]),
+                 "gcups": round(float(r["cells"]) / max(float(r["ms"]), 1e-9) / 1e6, 1)}
                 for r in ls]
     log(f"[bench] launches: {launches}")
 
