@@ -214,7 +214,12 @@ def kernel_mode(sxa, synth, tune_ms, reps=5, shapes=None):
         for _ in range(2):
             hits, doms = kc.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True)
         st = kc.last_search_stats
+        # the tail's pairs get Forward, Backward and region finding on the device; what
+        # reaches the host is their scores and region lists (bytes_to_host), not the
+        # special rows (28 bytes a row a pass: five f32 values and the f64 scale)
         out["search"] = {"hits": int(len(hits)), "domains": int(len(doms)), "pairs_in_tail": int(st["n_pairs"]),
+                         "rows_in_tail": int(st["n_rows"]), "regions": int(st["n_regions"]),
+                         "bytes_to_host": int(st["bytes_to_host"]), "special_row_bytes": int(st["n_rows"]) * 56,
                          **{k: round(st[k], 2) for k in ("ms_filters", "ms_specials", "ms_tail", "ms_report")}}
     # the int16 MSV bound covers models up to 1024 positions: timed on those profiles
     short = [h for h in wl.hmms if h.m <= 1024]

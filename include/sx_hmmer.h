@@ -274,14 +274,18 @@ typedef struct {           /* DomainHit (search.rs:84-97) */
 typedef struct sx_results sx_results;
 
 /* Where an sx_search call spent its time (host wall clock, ms) and how much reached the
- * host part: n_pairs pairs got special rows and domain definition, n_rows = their
- * summed L+1.  No counterpart in the reference (it does not time its phases). */
+ * host part: n_pairs pairs got Forward, Backward and domain definition, n_rows = their
+ * summed L+1 (special rows computed and decoded on the device), n_regions = the regions
+ * that posterior decoding found on the device and handed to the host tail, bytes_to_host
+ * = what those steps copied back (per-pair scores and the region list; no special rows).
+ * No counterpart in the reference (it does not time its phases). */
 typedef struct {
     uint64_t n_pairs, n_rows;
     double ms_filters;   /* the device cascade (0 with filters off) */
-    double ms_specials;  /* device Forward + Backward special rows, with their copies */
-    double ms_tail;      /* the rest of score_target on the host threads */
+    double ms_specials;  /* device Forward + Backward and region finding, with their copies */
+    double ms_tail;      /* the rest of score_target on the host threads, from the regions on */
     double ms_report;    /* report: thresholds, E-values, domain flags */
+    uint64_t n_regions, bytes_to_host;
 } sx_search_stats;
 
 /* Search the context's profiles against its targets. */

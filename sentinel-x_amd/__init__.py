@@ -224,7 +224,8 @@ def search_config(cutoff="evalue", seq_e=10.0, dom_e=10.0, seq_t=0.0, dom_t=0.0,
 class SearchStats(ctypes.Structure):
     """sx_search_stats: where a search spent its time (ms) and how much reached the host."""
     _fields_ = [("n_pairs", ctypes.c_uint64), ("n_rows", ctypes.c_uint64), ("ms_filters", ctypes.c_double),
-                ("ms_specials", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_report", ctypes.c_double)]
+                ("ms_specials", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_report", ctypes.c_double),
+                ("n_regions", ctypes.c_uint64), ("bytes_to_host", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

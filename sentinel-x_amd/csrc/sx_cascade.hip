@@ -510,3 +510,123 @@ sxc_records(const int64_t* __restrict__ n_dev, int64_t cap, const int32_t* __res
     r.flags = bflag[i] ? SX_REC_BIAS_HOST : 0;
     out[i] = r;
 }
+
+// ---------------------------------------------------------------------------
+// Posterior decoding and region finding next to Forward / Backward
+// (pipeline.rs:186-242), one thread per pair.
+// ---------------------------------------------------------------------------
+
+// dp.rs:32-40 in f32
+__device__ __forceinline__ float sxr_logsum(float a, float b) {
+    const float hi = a > b ? a : b, lo = a > b ? b : a;
+    const float d = hi - lo;
+    if (!(d < 30.0f)) return hi;
+    return hi + log1pf(expf(-d));
+}
+
+__device__ __forceinline__ float sxr_lnf(float x) { return x > 0.0f ? logf(x) : -__builtin_inff(); }
+
+// For pair p: the scaled special values x5 (N B E J C, 5 floats a row) and row log scales
+// ls of its Forward (f*) and Backward (b*) passes start at row in[p].roff (rows 0..L).
+// The log special rows are lnf(x) + (f32)ls as dp.rs:284-292 and :372-390 store them
+// (Forward row 0: N = 0, E = J = C = -inf; Backward row 0: E = J = C = -inf; row L:
+// N = B = J = -inf).  With overall = the Backward score (the given Forward score if that
+// is not finite, pipeline.rs:190), the posterior sums of pipeline.rs:194-206 and the
+// region scan of :208-242 run in one pass over the rows; btot / etot are kept in bt / et
+// (rows 0..L at roff) for the multi-domain test (is_multidomain, :527-536) of each region
+// as it closes.  res[p] gets the Forward score of the rows, the Backward score, nexpected
+// = btot[L] and the region count; each region goes to the list regs as (pair, i, j,
+// multi) at a slot taken from *nreg_all (a pair's regions keep their order; slots past
+// cap are counted, not written).
+extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+sxr_regions(int64_t n, const SxRegionIn* __restrict__ in, const float* __restrict__ fx5,
+            const double* __restrict__ fls, const float* __restrict__ bx5, const double* __restrict__ bls,
+            float* bt, float* et, SxRegionRes* __restrict__ res, SxRegion* __restrict__ regs, int32_t* nreg_all,
+            int32_t cap) {
+    const int64_t p = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    const SxRegionIn a = in[p];
+    const int L = a.len;
+    const float NI = -__builtin_inff();
+    SxRegionRes o;
+    if (L == 0) {
+        o.fsc = o.bsc = NI;
+        o.nexp = 0.0f;
+        o.nreg = 0;
+        res[p] = o;
+        return;
+    }
+    const float* F = fx5 + a.roff * 5;
+    const float* B = bx5 + a.roff * 5;
+    const double* FL = fls + a.roff;
+    const double* BL = bls + a.roff;
+    // the scores: Forward lnf(xC(L) * c_move) + ls(L) (dp.rs:293-297); Backward N(0)
+    const float fs = (float)((double)sxr_lnf(F[(int64_t)L * 5 + 4] * a.pmove) + FL[L]);
+    const float bn1 = L > 1 ? sxr_lnf(B[5 + 0]) + (float)BL[1] : NI;
+    const float bb0 = sxr_lnf(B[1]) + (float)BL[0];
+    const float bs = sxr_logsum(bn1 + a.lloop, bb0 + a.lmove);
+    const float overall = isfinite(bs) ? bs : (a.has_fwd ? a.fwd : fs);
+    // row 0 of each pass: Forward N B J C, Backward B (the rest is not read)
+    float fn = 0.0f, fb = sxr_lnf(F[1]) + (float)FL[0], fj = NI, fc = NI;
+    float bb = bb0;
+    float btot = 0.0f, etot = 0.0f;
+    float* BT = bt + a.roff;
+    float* ET = et + a.roff;
+    BT[0] = 0.0f;
+    ET[0] = 0.0f;
+    const float rt1 = 0.25f, rt2 = 0.10f, rt3 = 0.20f;
+    int i = -1, nr = 0;
+    bool triggered = false;
+    for (int r = 1; r <= L; r++) {
+        const float fl = (float)FL[r], bl = (float)BL[r];
+        const float* fr = F + (int64_t)r * 5;
+        const float* br = B + (int64_t)r * 5;
+        const float bnr = r < L ? sxr_lnf(br[0]) + bl : NI;
+        const float ber = sxr_lnf(br[2]) + bl;
+        const float bjr = r < L ? sxr_lnf(br[3]) + bl : NI;
+        const float bcr = sxr_lnf(br[4]) + bl;
+        const float fer = sxr_lnf(fr[2]) + fl;
+        const float bt_prev = btot, et_prev = etot;
+        btot = btot + expf(fb + bb - overall);
+        etot = etot + expf(fer + ber - overall);
+        const float njc = expf(fn + bnr + a.lloop - overall) + expf(fj + bjr + a.lloop - overall) +
+                          expf(fc + bcr + a.lloop - overall);
+        const float mocc = 1.0f - njc;
+        BT[r] = btot;
+        ET[r] = etot;
+        if (!triggered) {
+            if (mocc - (btot - bt_prev) < rt2 || i == -1) i = r;
+            if (mocc >= rt1) triggered = true;
+        } else if (mocc - (etot - et_prev) < rt2) {
+            const int ri = i > 1 ? i : 1;
+            float mx = -1.0f;
+            for (int jj = ri; jj <= r; jj++) {
+                const float v = fminf(ET[jj] - ET[ri - 1], btot - BT[jj - 1]);
+                if (v > mx) mx = v;
+            }
+            const int32_t slot = atomicAdd(nreg_all, 1);
+            if (slot < cap) {
+                SxRegion g;
+                g.pair = (int32_t)p;
+                g.i = ri;
+                g.j = r;
+                g.multi = mx >= rt3 ? 1 : 0;
+                regs[slot] = g;
+            }
+            nr++;
+            i = -1;
+            triggered = false;
+        }
+        // this row's Forward N B J C and Backward B are the next row's i-1 values
+        fn = sxr_lnf(fr[0]) + fl;
+        fb = sxr_lnf(fr[1]) + fl;
+        fj = sxr_lnf(fr[3]) + fl;
+        fc = sxr_lnf(fr[4]) + fl;
+        bb = r < L ? sxr_lnf(br[1]) + bl : NI;
+    }
+    o.fsc = fs;
+    o.bsc = bs;
+    o.nexp = btot;
+    o.nreg = nr;
+    res[p] = o;
+}

@@ -130,3 +130,23 @@ struct SxSpecialsOut {
     double* ls;       // 1 double per row
     const int64_t* row_off;  // per pair (pair = out index) first row offset
 };
+
+// Per-pair input of the region scan (sxr_regions): first row, length, the multihit
+// length-model loop / move scores and move probability of Xsc(L) (model.rs:284-312), and
+// the Forward score that stands in for a non-finite Backward score when has_fwd is set.
+struct SxRegionIn {
+    int64_t roff;
+    int32_t len;
+    float lloop, lmove, pmove, fwd;
+    int32_t has_fwd;
+};
+// Per-pair result of the region scan.
+struct SxRegionRes {
+    float fsc, bsc, nexp;
+    int32_t nreg;
+};
+// One region: rows i..j (1-based, inclusive) of pair `pair`, multi = 1 when the
+// multi-domain test holds.
+struct SxRegion {
+    int32_t pair, i, j, multi;
+};

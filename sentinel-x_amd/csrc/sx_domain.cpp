@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 
 namespace sx {
 
@@ -975,13 +976,8 @@ bool rescore(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool nul
 // pipeline.rs:186-294
 // ---------------------------------------------------------------------------
 
-bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, float bcksc,
-                const SpecRows& f, const SpecRows& b, TailWork& ws, RawHit& out) {
-    if (l == 0 || p.m == 0 || !std::isfinite(fwdsc)) return false;
-    const float overall = std::isfinite(bcksc) ? bcksc : fwdsc;
+void find_regions(int l, float overall, const SpecRows& f, const SpecRows& b, TailRegions& out) {
     const Xsc xs = Xsc::make((size_t)l, true);
-    const float nullsc = null_score((size_t)l);
-
     // posterior decoding of begin, end and core occupancy per residue
     std::vector<float> btot((size_t)l + 1, 0.0f), etot(btot), mocc(btot);
     for (int i = 1; i <= l; i++) {
@@ -992,13 +988,12 @@ bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float f
                           std::exp(f.c[i - 1] + b.c[i] + xs.c_loop - overall);
         mocc[i] = 1.0f - njc;
     }
-    const float nexpected = btot[l];
+    out.nexpected = btot[l];
+    out.spans.clear();
 
     // regions by posterior heuristics
     const float rt1 = 0.25f, rt2 = 0.10f, rt3 = 0.20f;
-    std::vector<float> n2sc((size_t)l + 1, 0.0f);
-    std::vector<Envelope> envs;
-    int nregions = 0, i = -1;
+    int i = -1;
     bool triggered = false;
     for (int j = 1; j <= l; j++) {
         if (!triggered) {
@@ -1006,16 +1001,37 @@ bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float f
             if (mocc[j] >= rt1) triggered = true;
         } else if (mocc[j] - (etot[j] - etot[j - 1]) < rt2) {
             const int ri = std::max(i, 1);
-            nregions++;
-            Envelope env;
-            if (is_multidomain(btot, etot, ri, j, rt3)) {
-                for (const auto& ij : region_ensemble(p, dsq, ri, j, l, ws, n2sc))
-                    if (rescore(p, dsq, ij.first, ij.second, l, null2, ws, n2sc, true, env)) envs.push_back(env);
-            } else if (rescore(p, dsq, ri, j, l, null2, ws, n2sc, false, env)) {
-                envs.push_back(env);
-            }
+            out.spans.push_back({ri, j, is_multidomain(btot, etot, ri, j, rt3)});
             i = -1;
             triggered = false;
+        }
+    }
+}
+
+bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, float bcksc,
+                const SpecRows& f, const SpecRows& b, TailWork& ws, RawHit& out) {
+    if (l == 0 || p.m == 0 || !std::isfinite(fwdsc)) return false;
+    TailRegions r;
+    find_regions(l, std::isfinite(bcksc) ? bcksc : fwdsc, f, b, r);
+    return score_regions(p, dsq, l, null2, fwdsc, r, ws, out);
+}
+
+bool score_regions(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, const TailRegions& r,
+                   TailWork& ws, RawHit& out) {
+    if (l == 0 || p.m == 0 || !std::isfinite(fwdsc)) return false;
+    const float nullsc = null_score((size_t)l);
+    const float nexpected = r.nexpected;
+    std::vector<float> n2sc((size_t)l + 1, 0.0f);
+    std::vector<Envelope> envs;
+    const int nregions = (int)r.spans.size();
+    for (const RegionSpan& g : r.spans) {
+        Envelope env;
+        if (g.i < 1 || g.j < g.i || g.j > l) throw std::runtime_error("region outside the target");
+        if (g.multi) {
+            for (const auto& ij : region_ensemble(p, dsq, g.i, g.j, l, ws, n2sc))
+                if (rescore(p, dsq, ij.first, ij.second, l, null2, ws, n2sc, true, env)) envs.push_back(env);
+        } else if (rescore(p, dsq, g.i, g.j, l, null2, ws, n2sc, false, env)) {
+            envs.push_back(env);
         }
     }
     if (nregions == 0 || envs.empty()) return false;

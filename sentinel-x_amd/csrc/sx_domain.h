@@ -2,8 +2,9 @@
 // posterior domain decoding from the Forward/Backward special rows, region finding,
 // multi-domain resolution by a stochastic-trace ensemble, envelope rescoring with
 // full-matrix Forward/Backward, null2 and optimal-accuracy coordinates, and the
-// sequence / domain scores.  This part runs on the host, as in the reference; the
-// special rows and whole-sequence scores it starts from come from the device.
+// sequence / domain scores.  This part runs on the host, as in the reference.  The device
+// search hands it the regions (posterior decoding and region finding run on the device,
+// next to Backward); sx_search_host starts from special rows and finds the regions here.
 //
 // Arithmetic follows the reference's f32/f64 operation order (built without FP
 // contraction, glibc logf/expf/log/exp) so that scores agree to f32 rounding and
@@ -88,8 +89,29 @@ struct TailWork {
     Posterior pp;
 };
 
+// The regions that posterior decoding finds (pipeline.rs:194-242): rows i..j (1-based,
+// inclusive) in order, each with the outcome of the multi-domain test, and the expected
+// number of domains btot[L].
+struct RegionSpan {
+    int i, j;
+    bool multi;
+};
+struct TailRegions {
+    float nexpected = 0.0f;
+    std::vector<RegionSpan> spans;
+};
+
+// pipeline.rs:194-242 on the log special rows, overall = the Backward score (the Forward
+// score if that is not finite).
+void find_regions(int l, float overall, const SpecRows& f, const SpecRows& b, TailRegions& out);
+
+// pipeline.rs:208-294 from the regions on: multi-domain resolution, envelope rescoring and
+// the scores.  Returns false where score_target returns None.
+bool score_regions(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, const TailRegions& r,
+                   TailWork& ws, RawHit& out);
+
 // pipeline.rs:186-294, given the whole-sequence Forward and Backward scores and their
-// special rows.  Returns false where score_target returns None.
+// special rows: find_regions, then score_regions.
 bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, float bcksc,
                 const SpecRows& f, const SpecRows& b, TailWork& ws, RawHit& out);
 

@@ -17,13 +17,6 @@
 
 namespace sx {
 
-#define HIPCHK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            throw SxError(SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
 // pipeline.rs:87-95
 double gumbel_surv(double x, double mu, double lambda) {
     const double y = lambda * (x - mu);

@@ -30,13 +30,6 @@ extern "C" const SxKernelEntry* sx_kernel_table(int* n);
 
 namespace sx {
 
-#define HIPCHK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            throw SxError(SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
 static const size_t LDS_MAX = 160 * 1024;
 
 template <class T>

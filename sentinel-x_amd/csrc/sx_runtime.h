@@ -24,6 +24,15 @@ struct SxError : std::runtime_error {
     SxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// throws SxError(SX_ERR_HIP) with the failing call's text when a HIP call fails
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw SxError(SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+
 struct DeviceBuf {
     void* p = nullptr;
     size_t bytes = 0;

@@ -1,7 +1,8 @@
 // sx_search.cpp -- hmmsearch-equivalent search over the device cascade (sentinel-x
-// src/hmmer/search.rs:131-271): pairs that pass the filters get Forward and Backward
-// special rows from the device, the rest of score_target runs on host threads
-// (sx_domain.cpp), and `report` applies the E-value / bit / GA-NC-TC thresholds.
+// src/hmmer/search.rs:131-271): pairs that pass the filters get Forward, Backward,
+// posterior decoding and region finding on the device (sxr_regions), the rest of
+// score_target runs on host threads from the regions on (sx_domain.cpp), and `report`
+// applies the E-value / bit / GA-NC-TC thresholds.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +18,12 @@
 
 #include "sx_domain.h"
 #include "sx_pipeline.h"
+
+extern "C" {
+__global__ void sxr_regions(int64_t, const SxRegionIn*, const float*, const double*, const float*, const double*,
+                            float*, float*, SxRegionRes*, SxRegion*, int32_t*, int32_t);
+__global__ void sxc_zero_i32(int32_t*, int64_t);
+}
 
 namespace sx {
 
@@ -57,12 +64,9 @@ std::vector<Threshold> resolve_thresholds(const std::vector<Profile>& profs, con
     return out;
 }
 
-// The host part of score_target for a list of pairs on host worker threads.
-// fsp / bsp hold 5 x (L+1) rows per pair starting at row_off[i].
-void run_tails(const std::vector<Profile>& profs, const std::vector<const uint8_t*>& seq, const std::vector<int>& len,
-               const uint32_t* prof, const uint32_t* tgt, uint64_t n, const double* fwd, const double* bwd,
-               const float* fsp, const float* bsp, const std::vector<int64_t>& row_off, bool null2, int threads,
-               std::vector<RawHit>& hit, std::vector<char>& ok) {
+// body(i, ws, out) for pairs 0..n-1 on host worker threads; ok[i] = its result.
+template <class Body>
+void parallel_pairs(uint64_t n, int threads, std::vector<RawHit>& hit, std::vector<char>& ok, Body&& body) {
     hit.assign(n, RawHit{});
     ok.assign(n, 0);
     if (n == 0) return;
@@ -72,22 +76,8 @@ void run_tails(const std::vector<Profile>& profs, const std::vector<const uint8_
     std::vector<std::string> err((size_t)nth);
     auto worker = [&](int w) {
         TailWork ws;
-        SpecRows f, b;
         try {
-            for (uint64_t i; (i = next.fetch_add(1)) < n;) {
-                const int L = len[tgt[i]];
-                const size_t r0 = (size_t)row_off[i] * 5, W = (size_t)L + 1;
-                auto load = [&](const float* src, SpecRows& d) {
-                    d.n.assign(src + r0, src + r0 + W);
-                    d.b.assign(src + r0 + W, src + r0 + 2 * W);
-                    d.e.assign(src + r0 + 2 * W, src + r0 + 3 * W);
-                    d.j.assign(src + r0 + 3 * W, src + r0 + 4 * W);
-                    d.c.assign(src + r0 + 4 * W, src + r0 + 5 * W);
-                };
-                load(fsp, f);
-                load(bsp, b);
-                ok[i] = score_tail(profs[prof[i]], seq[tgt[i]], L, null2, (float)fwd[i], (float)bwd[i], f, b, ws, hit[i]);
-            }
+            for (uint64_t i; (i = next.fetch_add(1)) < n;) ok[i] = body(i, ws, hit[i]);
         } catch (const std::exception& e) {
             err[(size_t)w] = e.what();
             next.store(n);
@@ -99,6 +89,111 @@ void run_tails(const std::vector<Profile>& profs, const std::vector<const uint8_
     for (auto& t : pool) t.join();
     for (const auto& e : err)
         if (!e.empty()) throw SxError(SX_ERR_STATE, "domain definition: " + e);
+}
+
+// The host part of score_target for pairs whose special rows are given: fsp / bsp hold
+// 5 x (L+1) rows per pair starting at row_off[i].
+void run_tails(const std::vector<Profile>& profs, const std::vector<const uint8_t*>& seq, const std::vector<int>& len,
+               const uint32_t* prof, const uint32_t* tgt, uint64_t n, const double* fwd, const double* bwd,
+               const float* fsp, const float* bsp, const std::vector<int64_t>& row_off, bool null2, int threads,
+               std::vector<RawHit>& hit, std::vector<char>& ok) {
+    parallel_pairs(n, threads, hit, ok, [&](uint64_t i, TailWork& ws, RawHit& out) {
+        const int L = len[tgt[i]];
+        const size_t r0 = (size_t)row_off[i] * 5, W = (size_t)L + 1;
+        SpecRows f, b;
+        auto load = [&](const float* src, SpecRows& d) {
+            d.n.assign(src + r0, src + r0 + W);
+            d.b.assign(src + r0 + W, src + r0 + 2 * W);
+            d.e.assign(src + r0 + 2 * W, src + r0 + 3 * W);
+            d.j.assign(src + r0 + 3 * W, src + r0 + 4 * W);
+            d.c.assign(src + r0 + 4 * W, src + r0 + 5 * W);
+        };
+        load(fsp, f);
+        load(bsp, b);
+        return score_tail(profs[prof[i]], seq[tgt[i]], L, null2, (float)fwd[i], (float)bwd[i], f, b, ws, out);
+    });
+}
+
+// The host part of score_target from the regions on, for pairs whose regions the device
+// found.
+void run_region_tails(const std::vector<Profile>& profs, const std::vector<const uint8_t*>& seq,
+                      const std::vector<int>& len, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
+                      const float* fwd, const std::vector<TailRegions>& regs, bool null2, int threads,
+                      std::vector<RawHit>& hit, std::vector<char>& ok) {
+    parallel_pairs(n, threads, hit, ok, [&](uint64_t i, TailWork& ws, RawHit& out) {
+        return score_regions(profs[prof[i]], seq[tgt[i]], len[tgt[i]], null2, fwd[i], regs[i], ws, out);
+    });
+}
+
+// Device buffers of the region scan, kept across the chunks of one search.
+struct RegionBufs {
+    DeviceBuf bx5, bls, in, bt, et, res, regs, cnt;
+};
+
+// Forward and Backward of n pairs into device buffers (rows at roff[i], rows in all),
+// then posterior decoding and region finding on the device (sxr_regions).  fwd_in (may be
+// null) is the Forward score the tail goes on with.  Fills regs[i] and, from the device's
+// special rows, fsc[i] = the Forward score; returns the bytes copied to the host.
+uint64_t device_regions(Ctx& c, RegionBufs& B, const uint32_t* prof, const uint32_t* tgt, uint64_t n,
+                        const std::vector<int64_t>& roff, int64_t rows, const float* fwd_in,
+                        std::vector<TailRegions>& regs, std::vector<float>& fsc) {
+    regs.assign(n, TailRegions{});
+    fsc.assign(n, NEG_INF);
+    if (n == 0) return 0;
+    float* fx5 = (float*)c.scratch_x5.ensure((size_t)rows * 5 * sizeof(float));
+    double* fls = (double*)c.scratch_ls.ensure((size_t)rows * sizeof(double));
+    float* bx5 = (float*)B.bx5.ensure((size_t)rows * 5 * sizeof(float));
+    double* bls = (double*)B.bls.ensure((size_t)rows * sizeof(double));
+    void* d = c.scratch_out.ensure(n * 8);
+    c.run_pairs(SX_STAGE_FORWARD, SX_FORM_BATCHED, prof, tgt, n, d, true, &roff, fx5, fls);
+    c.run_pairs(SX_STAGE_BACKWARD, SX_FORM_BATCHED, prof, tgt, n, d, true, &roff, bx5, bls);
+    std::vector<SxRegionIn> in(n);
+    for (uint64_t i = 0; i < n; i++) {
+        const int L = c.tlen[tgt[i]];
+        const Xsc xs = Xsc::make((size_t)L, true);
+        in[i] = {roff[i], L, xs.n_loop, xs.n_move, xs.pc_move, fwd_in ? fwd_in[i] : 0.0f, fwd_in ? 1 : 0};
+    }
+    // a region takes at least two rows (the row that triggers it and the row that closes
+    // it), so a pair of length L has at most L/2
+    const int64_t cap = rows / 2 + (int64_t)n;
+    if (cap > INT32_MAX) throw SxError(SX_ERR_ARG, "too many special rows in one region scan");
+    auto* din = (SxRegionIn*)B.in.ensure(n * sizeof(SxRegionIn));
+    auto* dres = (SxRegionRes*)B.res.ensure(n * sizeof(SxRegionRes));
+    auto* dreg = (SxRegion*)B.regs.ensure((size_t)cap * sizeof(SxRegion));
+    auto* dcnt = (int32_t*)B.cnt.ensure(sizeof(int32_t));
+    float* dbt = (float*)B.bt.ensure((size_t)rows * sizeof(float));
+    float* det = (float*)B.et.ensure((size_t)rows * sizeof(float));
+    HIPCHK(hipMemcpyAsync(din, in.data(), n * sizeof(SxRegionIn), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(sxc_zero_i32, dim3(1), dim3(256), 0, c.stream, dcnt, (int64_t)1);
+    const int64_t nb = ((int64_t)n + 255) / 256;
+    hipLaunchKernelGGL(sxr_regions, dim3((unsigned)nb), dim3(256), 0, c.stream, (int64_t)n, (const SxRegionIn*)din,
+                       (const float*)fx5, (const double*)fls, (const float*)bx5, (const double*)bls, dbt, det, dres,
+                       dreg, dcnt, (int32_t)cap);
+    HIPCHK(hipGetLastError());
+    std::vector<SxRegionRes> res(n);
+    int32_t cnt = 0;
+    HIPCHK(hipMemcpyAsync(res.data(), dres, n * sizeof(SxRegionRes), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (cnt < 0 || cnt > cap) throw SxError(SX_ERR_STATE, "region list overflow");
+    std::vector<SxRegion> list((size_t)cnt);
+    if (cnt > 0) {
+        HIPCHK(hipMemcpyAsync(list.data(), dreg, (size_t)cnt * sizeof(SxRegion), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+    }
+    for (const SxRegion& g : list) {
+        if (g.pair < 0 || (uint64_t)g.pair >= n) throw SxError(SX_ERR_STATE, "region of an unknown pair");
+        regs[(size_t)g.pair].spans.push_back({g.i, g.j, g.multi != 0});
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        auto& sp = regs[i].spans;
+        // one thread writes a pair's regions in order; the list keeps them by slot
+        std::sort(sp.begin(), sp.end(), [](const RegionSpan& a, const RegionSpan& b) { return a.j < b.j; });
+        if ((int32_t)sp.size() != res[i].nreg) throw SxError(SX_ERR_STATE, "region count mismatch");
+        regs[i].nexpected = res[i].nexp;
+        fsc[i] = res[i].fsc;
+    }
+    return n * sizeof(SxRegionRes) + sizeof(int32_t) + (uint64_t)cnt * sizeof(SxRegion);
 }
 
 // report (search.rs:160-210) for one profile; raws in target order
@@ -170,7 +265,8 @@ void check_search_cfg(const sx_search_cfg* cfg) {
     if (cfg->cutoff < SX_CUT_EVALUE || cfg->cutoff > SX_CUT_TC) throw SxError(SX_ERR_ARG, "unknown cutoff kind");
 }
 
-// The device search: cascade, special rows in chunks, host tails, report.
+// The device search: cascade, Forward / Backward and region finding on the device in
+// chunks, host tails from the regions on, report.
 void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point t0) {
@@ -196,22 +292,25 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     std::vector<const uint8_t*> seq(c.nt);
     for (uint32_t t = 0; t < c.nt; t++) seq[t] = c.h_codes.data() + c.h_toff[t];
     const std::vector<int>& len = c.tlen;
-    // chunks of at most 16M special rows per direction (320 MB of floats each)
+    // chunks of at most 16M special rows per direction (320 MB of floats each, on the
+    // device)
     const int64_t max_rows = 16LL << 20;
     size_t a = 0;
     std::vector<uint32_t> cp, ct;
-    std::vector<double> fwd, bwd;
-    std::vector<float> fsp, bsp;
+    std::vector<float> fin, fsc;
     std::vector<int64_t> roff;
+    std::vector<TailRegions> regs;
     std::vector<RawHit> hit;
     std::vector<char> ok;
     std::vector<uint32_t> all_p, all_t;
     std::vector<RawHit> all_hit;
     std::vector<char> all_ok;
+    RegionBufs bufs;
     while (a < pairs.size()) {
         cp.clear();
         ct.clear();
         roff.clear();
+        fin.clear();
         int64_t rows = 0;
         size_t b = a;
         for (; b < pairs.size(); b++) {
@@ -219,26 +318,22 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
             if (b > a && rows + r > max_rows) break;
             cp.push_back(pairs[b].profile);
             ct.push_back(pairs[b].target);
+            fin.push_back(pairs[b].fwd);
             roff.push_back(rows);
             rows += r;
         }
         const uint64_t n = cp.size();
-        fwd.assign(n, 0.0);
-        bwd.assign(n, 0.0);
-        fsp.assign((size_t)rows * 5, 0.0f);
-        bsp.assign((size_t)rows * 5, 0.0f);
         t0 = clk::now();
-        c.special_rows(SX_STAGE_FORWARD, cp.data(), ct.data(), n, fsp.data(), fwd.data());
-        c.special_rows(SX_STAGE_BACKWARD, cp.data(), ct.data(), n, bsp.data(), bwd.data());
-        if (cfg.filters)
-            for (uint64_t i = 0; i < n; i++) fwd[i] = (double)pairs[a + i].fwd;
+        res.stats.bytes_to_host += device_regions(c, bufs, cp.data(), ct.data(), n, roff, rows,
+                                                  cfg.filters ? fin.data() : nullptr, regs, fsc);
         res.stats.ms_specials += ms_since(t0);
         t0 = clk::now();
-        run_tails(c.profs, seq, len, cp.data(), ct.data(), n, fwd.data(), bwd.data(), fsp.data(), bsp.data(), roff,
-                  cfg.null2 != 0, cfg.threads, hit, ok);
+        run_region_tails(c.profs, seq, len, cp.data(), ct.data(), n, cfg.filters ? fin.data() : fsc.data(), regs,
+                         cfg.null2 != 0, cfg.threads, hit, ok);
         res.stats.ms_tail += ms_since(t0);
         res.stats.n_pairs += n;
         res.stats.n_rows += (uint64_t)rows;
+        for (const TailRegions& r : regs) res.stats.n_regions += r.spans.size();
         all_p.insert(all_p.end(), cp.begin(), cp.end());
         all_t.insert(all_t.end(), ct.begin(), ct.end());
         all_hit.insert(all_hit.end(), std::make_move_iterator(hit.begin()), std::make_move_iterator(hit.end()));

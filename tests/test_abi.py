@@ -84,6 +84,17 @@ def _check_aligned_layout(e):
         assert len(slots) == 16, (e.q, e.g, quarter, sorted(slots))
 
 
+def _round4(n):
+    return (n + 3) // 4 * 4
+
+
+def _fwd_rec_floats(q, g):
+    """Forward's LDS records per lane group: 8 Q + 4 floats of transitions, then Q + 4 or
+    Q + 8 floats of D-chain products (the stride whose dword count over 4 is odd)."""
+    pstride = q + 4 if ((q + 4) // 4) % 2 else q + 8
+    return g * (8 * q + 4) + g * pstride
+
+
 def test_kernel_table_names_each_shape_once(sxa):
     """Each (stage, Q, G, special rows, table mode) is one distinct kernel; the special-row
     writers are separate kernels from the score-only ones (a launch must never pick one for
@@ -104,6 +115,10 @@ def test_kernel_table_names_each_shape_once(sxa):
         assert e.block == {3: 512, 4: 1024}.get(e.hoist, 256)
         if e.hoist in (3, 4):
             _check_aligned_layout(e)
+        elif e.stage == 2:
+            # Forward: emission rows (none for long models), then per lane group the
+            # 32-byte transition records of Q positions and the D-chain products
+            assert e.lds == 4 * (_round4(0 if e.hoist == 2 else 25 * e.wid) + _fwd_rec_floats(e.q, e.g))
         elif e.hoist == 2:
             assert e.lds == (0 if e.stage == 0 else 4 * 8 * e.wid)
         elif e.stage == 5:

@@ -135,10 +135,11 @@ def test_cfg1_cascade_on_every_pair(orc, gctx, cfg1):
 
 @pytest.mark.parametrize("cutoff", ["evalue", "ga", "nc", "tc"])
 def test_cfg1_search_matches_the_host_tail_on_oracle_rows(orc, sxa, gctx, cfg1, cutoff):
-    """sx_search (device cascade, device special rows, host tail) against sx_search_host
-    fed with the oracle's Forward / Backward special rows for the pairs the oracle's
-    cascade passes: the same hits, domain coordinates and reporting flags, scores within
-    0.01 bits (the device sums Forward in another order)."""
+    """sx_search (device cascade, device Forward / Backward with posterior decoding and
+    region finding, host tail from the regions on) against sx_search_host fed with the
+    oracle's Forward / Backward special rows for the pairs the oracle's cascade passes
+    (regions found on the host): the same hits, domain coordinates and reporting flags,
+    scores within 0.01 bits (the device sums Forward in another order)."""
     hmms, profs, seqs = cfg1
     _load(gctx, hmms, seqs)
     kw = dict(cutoff=cutoff, seq_e=10.0, dom_e=10.0, filters=True, **F)
@@ -162,6 +163,7 @@ def test_cfg1_search_matches_the_host_tail_on_oracle_rows(orc, sxa, gctx, cfg1, 
     assert np.max(np.abs(hits["score"] - hh["score"])) <= 0.01
     assert np.max(np.abs(hits["bias"] - hh["bias"])) <= 0.01
     assert np.allclose(hits["evalue"], hh["evalue"], rtol=0.01, atol=0)
+    assert np.allclose(hits["nexpected"], hh["nexpected"], rtol=1e-3, atol=1e-4)
     coords = ["hmm_from", "hmm_to", "ali_from", "ali_to", "env_from", "env_to"]
     assert np.array_equal(doms[coords], hd[coords])
     assert np.max(np.abs(doms["score"] - hd["score"])) <= 0.01

@@ -1,5 +1,6 @@
-"""GPU end-to-end search (sx_search): device cascade and device Forward / Backward special
-rows, host domain definition and report, against the HMMER 3.4 fixture as the reference
+"""GPU end-to-end search (sx_search): device cascade, device Forward / Backward with
+posterior decoding and region finding, host domain definition from the regions on and
+report, against the HMMER 3.4 fixture as the reference
 checks it (search.rs:316-355), plus the filtered form and models without STATS lines."""
 import os
 
@@ -82,6 +83,33 @@ def test_search_device_and_host_paths_agree(orc, sxa, gctx, fixture34):
     assert np.abs(hg["score"] - hh["score"]).max() <= 0.01
     for k in ("hmm_from", "hmm_to", "ali_from", "ali_to", "env_from", "env_to"):
         assert np.array_equal(dg[k], dh[k])
+
+
+def test_device_regions_match_the_host_scan_of_the_same_rows(sxa, gctx, fixture34):
+    """Posterior decoding and region finding on the device (sx_search) against the host scan
+    (find_regions, through sx_search_host) of the device's own special rows: the same
+    regions, so the same hits, domains and coordinates; scores and nexpected differ at
+    most by the device's f32 exp / log rounding.  The search copies back only scores and
+    the region list, never the special rows."""
+    hmms, tnames, seqs = fixture34
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    hg, dg = gctx.search(filters=False)
+    st = dict(gctx.last_search_stats)
+    pp = np.repeat(np.arange(len(hmms), dtype=np.uint32), len(seqs))
+    tt = np.tile(np.arange(len(seqs), dtype=np.uint32), len(hmms))
+    fsp, fwd = gctx.specials(sxa.STAGE_FORWARD, pp, tt)
+    bsp, bwd = gctx.specials(sxa.STAGE_BACKWARD, pp, tt)
+    hh, dh = sxa.search_host(hmms, seqs, pp, tt, fwd, bwd, fsp, bsp, filters=False)
+    assert len(hg) >= 6
+    assert np.array_equal(hg[["profile", "target", "ndom", "qlen", "tlen"]], hh[["profile", "target", "ndom", "qlen", "tlen"]])
+    assert np.abs(hg["score"] - hh["score"]).max() <= 1e-3
+    assert np.allclose(hg["nexpected"], hh["nexpected"], rtol=1e-4, atol=1e-5)
+    for k in ("hmm_from", "hmm_to", "ali_from", "ali_to", "env_from", "env_to", "reported"):
+        assert np.array_equal(dg[k], dh[k]), k
+    assert np.abs(dg["score"] - dh["score"]).max() <= 1e-3
+    assert st["n_regions"] >= len(hg)
+    assert 0 < st["bytes_to_host"] < 4 * st["n_rows"]
 
 
 def test_uncalibrated_models_skip_the_filters(orc, gctx):
