@@ -228,11 +228,12 @@ typedef struct {
 } sx_launch_stat;
 int sx_cascade_launches(sx_ctx* ctx, uint32_t cap, uint32_t* n, sx_launch_stat* out);
 
-/* hmmsearch-equivalent search (search.rs:131-271): the filter cascade on the device,
- * Forward and Backward special rows on the device for the pairs that pass it, then the
- * rest of score_target on the host (pipeline.rs:186-294: posterior domain decoding,
- * region finding, stochastic-trace clustering of multi-domain regions, envelope
- * rescoring, null2, optimal-accuracy coordinates, sequence and domain scores) and
+/* hmmsearch-equivalent search (search.rs:131-271): the filter cascade on the device;
+ * for the pairs that pass it, Forward, Backward, posterior decoding and region finding
+ * on the device (pipeline.rs:186-242); then the rest of score_target on the host
+ * (pipeline.rs:208-294 from the regions on: stochastic-trace clustering of multi-domain
+ * regions, envelope rescoring, null2, optimal-accuracy coordinates, sequence and domain
+ * scores) and
  * `report` (search.rs:160-210: sequence E-value = P x Z, domain c-Evalue = P x the
  * profile's number of reported hits, inclusion by E-value, bits or GA/NC/TC).
  *
