@@ -330,6 +330,56 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
     return out
 
 
+def ingest_leg(sxa, synth, f, n_profiles=1000, batch=256):
+    """Model-file ingest: n_profiles Pfam-scale profiles (calibrated) written as an HMMER3
+    text file and searched against the 50,000 Pfam-scale proteins (filters on, E <= 10).
+    parse_configure: the file parsed and configured into a packed profile cache on one
+    host thread and on all of them (sx_convert_hmm_file).  search: the whole job three
+    ways, host wall time: the file loaded whole, then searched (sx_load_hmm_file +
+    sx_search); streamed `batch` models at a time with parsing overlapped
+    (sx_search_hmm_file); and streamed from the cache.  Built-in launch shapes."""
+    import shutil
+    import tempfile
+    wl = synth.make_workload(2, profile_scale=n_profiles / 20000.0)
+    calibrate(sxa, synth, wl.hmms, 200)
+    d = tempfile.mkdtemp(prefix="sx_ingest_")
+    try:
+        path, cache = os.path.join(d, "lib.hmm"), os.path.join(d, "lib.sxpc")
+        synth.write_hmm(path, wl.hmms)
+        threads = host_cpus()["threads"]
+        ms = np.array([h.m for h in wl.hmms])
+        out = {"workload": f"{len(wl.hmms)} pfam-scale profiles (M {ms.min()}-{ms.max()}) x {len(wl.seqs)} proteins",
+               "file_mb": round(os.path.getsize(path) / 2**20, 1), "batch": batch, "threads": threads}
+        for th in (1, threads):
+            t0 = time.perf_counter()
+            sxa.convert_hmm_file(path, cache, th)
+            dt = time.perf_counter() - t0
+            out[f"parse_configure_{th}t"] = {"s": round(dt, 3), "profiles_per_s": round(len(wl.hmms) / dt, 1)}
+        out["cache_mb"] = round(os.path.getsize(cache) / 2**20, 1)
+        kw = dict(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True, **f)
+        with sxa.Context(0) as c:
+            c.set_targets(wl.seqs)
+            c.load_hmm_file(cache)
+            c.search(**kw)  # warm-up: kernels loaded, buffers sized
+            c.set_profiles([])
+            t0 = time.perf_counter()
+            c.load_hmm_file(path)
+            t1 = time.perf_counter()
+            h0, d0 = c.search(**kw)
+            t2 = time.perf_counter()
+            out["load_then_search"] = {"s": round(t2 - t0, 3), "load_s": round(t1 - t0, 3), "hits": int(len(h0))}
+            for name, src in (("streamed", path), ("streamed_from_cache", cache)):
+                t0 = time.perf_counter()
+                h1, d1, _ = c.search_hmm_file(src, batch=batch, **kw)
+                dt = time.perf_counter() - t0
+                st = c.last_search_stats
+                out[name] = {"s": round(dt, 3), "ingest_ms_not_overlapped": round(st["ms_ingest"], 1),
+                             "same_hits": bool(np.array_equal(h0, h1) and np.array_equal(d0, d1))}
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def forward_error(wl, fwd_gpu, k, threads, z=None):
     """Device Forward (f32 score, as the reference returns it) against the oracle's FP64
     Forward on the first k profiles x all proteins: nats error, and the inclusion decisions
@@ -401,6 +451,8 @@ def main():
     ap.add_argument("--fwd-err-profiles", type=int, default=40,
                     help="marker-set profiles (x all proteins) checked against the FP64 Forward")
     ap.add_argument("--no-single", action="store_true", help="skip the single-M200 leg")
+    ap.add_argument("--ingest", type=int, default=1000,
+                    help="profiles of the model-file ingest leg (0 = skip)")
     ap.add_argument("--mag-sweep", default="slice", choices=("slice", "full", "none"),
                     help="MAG-sweep leg: 1,000 markers x 40 genomes (slice), 5,000 x 200 (full), or none")
     args = ap.parse_args()
@@ -550,6 +602,12 @@ def main():
                             tune_ms=min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0)
         log(f"[bench] mag-sweep ({time.time() - t0:.1f} s): {mag}")
 
+    ingest = None
+    if args.ingest > 0:
+        t0 = time.time()
+        ingest = ingest_leg(sxa, synth, f, n_profiles=args.ingest)
+        log(f"[bench] ingest ({time.time() - t0:.1f} s): {ingest}")
+
     cpu = fwd_err = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, recs, f, args.cpu_budget)
@@ -586,6 +644,7 @@ def main():
         "forward_error_vs_fp64": fwd_err,
         "single_m200": single,
         "mag_sweep": mag,
+        "ingest": ingest,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

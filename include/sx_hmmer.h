@@ -94,7 +94,9 @@ int sx_encode(const uint8_t* seq, size_t n, uint8_t* out, size_t* n_out);
 /* Hmm::configure (model.rs:146-245) for each model, then upload.  Replaces the
  * per-batch `configure()` of search_hmm_file_models (search.rs:257-263). */
 int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n);
-/* Append the models of a HMMER3 file (parse.rs:42-54 open_hmm + HmmReader). */
+/* Append the models of a HMMER3 file, text or gzip (parse.rs:42-54 open_hmm + HmmReader;
+ * records parsed and configured on host threads, in file order, with the reference's
+ * messages on malformed input), or of a packed profile cache (sx_write_profile_cache). */
 int sx_load_hmm_file(sx_ctx* ctx, const char* path, uint32_t* n_loaded);
 uint32_t sx_num_profiles(const sx_ctx* ctx);
 uint32_t sx_num_targets(const sx_ctx* ctx);
@@ -287,6 +289,11 @@ typedef struct {
     double ms_tail;      /* the rest of score_target on the host threads, from the regions on */
     double ms_report;    /* report: thresholds, E-values, domain flags */
     uint64_t n_regions, bytes_to_host;
+    /* sx_search_hmm_file: models read, time spent obtaining each batch (parsing and
+     * configuring, or reading a cache, past what overlapped the previous batch's search)
+     * and the part of it spent waiting on the parser threads */
+    uint64_t n_models;
+    double ms_ingest, ms_ingest_wait;
 } sx_search_stats;
 
 /* Search the context's profiles against its targets. */
@@ -301,6 +308,29 @@ int sx_search(sx_ctx* ctx, const sx_search_cfg* cfg, sx_results** out);
 int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,
                    const sx_search_cfg* cfg, uint64_t npairs, const uint32_t* prof, const uint32_t* tgt,
                    const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results** out);
+
+/* search_hmm_file_models (search.rs:252-271): every model of an HMMER3 file (text or
+ * gzip, or a packed profile cache written by sx_write_profile_cache / sx_convert_hmm_file)
+ * searched against the context's targets, `batch` models at a time (0: 256, as the
+ * reference; the results do not depend on it).  While the device searches a batch, host
+ * threads parse and configure the next ones.  Hits refer to models by their index in the
+ * file (sx_results_model); the context's profile set is the last batch afterwards.
+ * Parse errors are the reference's messages (parse.rs), SX_ERR_PARSE. */
+int sx_search_hmm_file(sx_ctx* ctx, const char* path, const sx_search_cfg* cfg, uint32_t batch, sx_results** out);
+
+/* The models a search covered, by profile index: the ModelInfo list (name, acc, m) of
+ * search_hmm_file_models (search.rs:245-271).  acc is NULL when absent; the strings live
+ * as long as the results. */
+int sx_results_models(const sx_results* r, uint32_t* n);
+int sx_results_model(const sx_results* r, uint32_t i, const char** name, const char** acc, int32_t* m);
+
+/* Packed profile cache: the context's configured profiles (what configure builds from the
+ * parsed models, model.rs:146-245), written so that sx_load_hmm_file and
+ * sx_search_hmm_file read them back without parsing or configuring.  No counterpart in
+ * the reference.  sx_convert_hmm_file does the same from a model file without a device
+ * (threads: parser threads, 0 = all cores; errors with sx_last_error(NULL)). */
+int sx_write_profile_cache(const sx_ctx* ctx, const char* path);
+int sx_convert_hmm_file(const char* hmm_path, const char* cache_path, int threads, uint32_t* n_models);
 
 int sx_results_size(const sx_results* r, uint64_t* nhits, uint64_t* ndomains);
 int sx_results_stats(const sx_results* r, sx_search_stats* st);

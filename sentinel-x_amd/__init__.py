@@ -38,7 +38,8 @@ EXPORTS = [
     "sx_num_profiles", "sx_num_targets", "sx_profile_tables", "sx_score_all", "sx_score_pairs", "sx_specials",
     "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
     "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_stats", "sx_results_copy",
-    "sx_results_free", "sx_encode", "sx_profile_info", "sx_score_pairs_noalloc",
+    "sx_results_free", "sx_encode", "sx_profile_info", "sx_score_pairs_noalloc", "sx_search_hmm_file",
+    "sx_results_models", "sx_results_model", "sx_write_profile_cache", "sx_convert_hmm_file",
 ]
 NOALLOC_MAX_PAIRS = 1 << 18  # SX_NOALLOC_MAX_PAIRS
 
@@ -155,6 +156,12 @@ def lib():
         L.sx_results_stats.argtypes = [vp, ctypes.POINTER(SearchStats)]
         L.sx_results_free.argtypes = [vp]
         L.sx_results_free.restype = None
+        L.sx_search_hmm_file.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(sx_search_cfg), u32, ctypes.POINTER(vp)]
+        L.sx_results_models.argtypes = [vp, ctypes.POINTER(u32)]
+        L.sx_results_model.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                       ctypes.POINTER(i32)]
+        L.sx_write_profile_cache.argtypes = [vp, ctypes.c_char_p]
+        L.sx_convert_hmm_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(u32)]
         L.sx_shape_for.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)]
         L.sx_set_shapes.argtypes = [vp, ctypes.c_int, u32, vp, vp, vp, vp]
@@ -225,13 +232,16 @@ class SearchStats(ctypes.Structure):
     """sx_search_stats: where a search spent its time (ms) and how much reached the host."""
     _fields_ = [("n_pairs", ctypes.c_uint64), ("n_rows", ctypes.c_uint64), ("ms_filters", ctypes.c_double),
                 ("ms_specials", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_report", ctypes.c_double),
-                ("n_regions", ctypes.c_uint64), ("bytes_to_host", ctypes.c_uint64)]
+                ("n_regions", ctypes.c_uint64), ("bytes_to_host", ctypes.c_uint64), ("n_models", ctypes.c_uint64),
+                ("ms_ingest", ctypes.c_double), ("ms_ingest_wait", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-def _take_results(L, r, stats=None):
+def _take_results(L, r, stats=None, models=None):
+    """(hits, domains) of an sx_results, which it frees; the stats and the model list
+    ((name, acc, m) per profile index) go into `stats` / `models` when given."""
     nh, nd = ctypes.c_uint64(), ctypes.c_uint64()
     try:
         L.sx_results_size(r, ctypes.byref(nh), ctypes.byref(nd))
@@ -240,9 +250,28 @@ def _take_results(L, r, stats=None):
         L.sx_results_copy(r, _ptr(hits), _ptr(doms))
         if stats is not None:
             L.sx_results_stats(r, ctypes.byref(stats))
+        if models is not None:
+            n = ctypes.c_uint32()
+            L.sx_results_models(r, ctypes.byref(n))
+            nm, ac, m = ctypes.c_char_p(), ctypes.c_char_p(), ctypes.c_int32()
+            for i in range(n.value):
+                L.sx_results_model(r, i, ctypes.byref(nm), ctypes.byref(ac), ctypes.byref(m))
+                models.append((nm.value.decode(), ac.value.decode() if ac.value is not None else None, m.value))
     finally:
         L.sx_results_free(r)
     return hits, doms
+
+
+def convert_hmm_file(hmm_path: str, cache_path: str, threads: int = 0) -> int:
+    """sx_convert_hmm_file: parse and configure every model of an HMMER3 file (text or
+    gzip) and write the packed profile cache; returns the number of models.  Needs no
+    device."""
+    n = ctypes.c_uint32()
+    L = lib()
+    rc = L.sx_convert_hmm_file(hmm_path.encode(), cache_path.encode(), int(threads), ctypes.byref(n))
+    if rc != 0:
+        raise SxError(f"sx_convert_hmm_file failed ({rc}): {L.sx_last_error(None).decode()}")
+    return n.value
 
 
 def encode(seq) -> np.ndarray:
@@ -355,10 +384,29 @@ class Context:
         return out
 
     def load_hmm_file(self, path: str) -> int:
+        """Append the models of an HMMER3 file (text, gzip or a packed profile cache)."""
         n = ctypes.c_uint32()
         self._chk(self._L.sx_load_hmm_file(self._h, path.encode(), ctypes.byref(n)))
         self.n_profiles = self._L.sx_num_profiles(self._h)
         return n.value
+
+    def write_profile_cache(self, path: str):
+        """sx_write_profile_cache: the loaded profiles as a packed profile cache."""
+        self._chk(self._L.sx_write_profile_cache(self._h, path.encode()))
+
+    def search_hmm_file(self, path: str, batch: int = 0, **cfg):
+        """search_hmm_file_models (search.rs:252-271) through sx_search_hmm_file: every model
+        of the file against the loaded targets, `batch` at a time (0: 256).  Returns (hits,
+        domains, models) with models[i] = (name, acc, m) of profile index i; the phase
+        times go to self.last_search_stats."""
+        c = search_config(**cfg)
+        r = ctypes.c_void_p()
+        self._chk(self._L.sx_search_hmm_file(self._h, path.encode(), ctypes.byref(c), int(batch), ctypes.byref(r)))
+        st, models = SearchStats(), []
+        hits, doms = _take_results(self._L, r, st, models)
+        self.last_search_stats = st.as_dict()
+        self.n_profiles = self._L.sx_num_profiles(self._h)
+        return hits, doms, models
 
     def profile_info(self, p: int):
         """ModelInfo (search.rs:245-249) of profile p: (name, acc, desc, m); acc and desc

@@ -456,6 +456,15 @@ std::unique_ptr<HmmReader> HmmReader::from_string(const std::string& text) {
     return r;
 }
 
+std::unique_ptr<HmmReader> HmmReader::from_string(std::string text, const std::string& source, size_t line0) {
+    std::unique_ptr<HmmReader> r(new HmmReader);
+    r->impl_->from_text = true;
+    r->impl_->text = std::move(text);
+    r->impl_->source = source;
+    r->impl_->lineno = line0;
+    return r;
+}
+
 bool HmmReader::next(Hmm& out) {
     if (impl_->done) return false;
     try {

@@ -87,6 +87,8 @@ class HmmReader {
 public:
     static std::unique_ptr<HmmReader> open(const std::string& path);
     static std::unique_ptr<HmmReader> from_string(const std::string& text);
+    // A piece of a file: messages name `source` and count lines from line0 + 1.
+    static std::unique_ptr<HmmReader> from_string(std::string text, const std::string& source, size_t line0);
     // Returns false at end of input.
     bool next(Hmm& out);
     virtual ~HmmReader();

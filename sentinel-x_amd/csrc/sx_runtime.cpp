@@ -13,6 +13,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sx_hmmer.h"
@@ -20,6 +21,7 @@
 #include "sx_model.h"
 #include "sx_runtime.h"
 #include "sx_pipeline.h"
+#include "sx_ingest.h"
 #include "sx_search.h"
 
 struct SxKernelEntry {
@@ -1116,20 +1118,81 @@ extern "C" int sx_set_profiles(sx_ctx* ctx, const sx_hmm* hmms, uint32_t n) {
     });
 }
 
+static int host_threads() { return (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
 extern "C" int sx_load_hmm_file(sx_ctx* ctx, const char* path, uint32_t* n_loaded) {
     if (!ctx || !path) return SX_ERR_ARG;
     return guard(ctx, [&] {
         std::vector<sx::Profile> ps = ctx->c->profs;
-        auto r = sx::HmmReader::open(path);
-        sx::Hmm h;
-        uint32_t k = 0;
-        while (r->next(h)) {
-            ps.push_back(sx::configure(h));
-            k++;
-        }
+        std::vector<sx::Profile> add = sx::load_profiles(path, host_threads());
+        const uint32_t k = (uint32_t)add.size();
+        for (auto& p : add) ps.push_back(std::move(p));
         ctx->c->set_profiles(std::move(ps));
         if (n_loaded) *n_loaded = k;
     });
+}
+
+extern "C" int sx_write_profile_cache(const sx_ctx* ctx, const char* path) {
+    if (!ctx || !path) return SX_ERR_ARG;
+    return guard(const_cast<sx_ctx*>(ctx), [&] {
+        sx::CacheWriter w(path);
+        for (const auto& p : ctx->c->profs) w.add(p);
+        w.close();
+    });
+}
+
+extern "C" int sx_convert_hmm_file(const char* hmm_path, const char* cache_path, int threads, uint32_t* n_models) {
+    if (!hmm_path || !cache_path) return SX_ERR_ARG;
+    try {
+        auto s = sx::ProfileStream::open(hmm_path, threads > 0 ? threads : host_threads(), 1024);
+        sx::CacheWriter w(cache_path);
+        std::vector<sx::Profile> b;
+        uint32_t n = 0;
+        for (;;) {
+            b.clear();
+            if (s->next(256, b) == 0) break;
+            for (const auto& p : b) w.add(p);
+            n += (uint32_t)b.size();
+        }
+        w.close();
+        if (n_models) *n_models = n;
+        return SX_OK;
+    } catch (const SxError& e) {
+        g_create_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_create_err = "out of host memory";
+        return SX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return SX_ERR_PARSE;
+    }
+}
+
+extern "C" int sx_search_hmm_file(sx_ctx* ctx, const char* path, const sx_search_cfg* cfg, uint32_t batch,
+                                  sx_results** out) {
+    if (!ctx || !path || !cfg || !out) return SX_ERR_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        auto r = std::make_unique<sx_results>();
+        sx::search_hmm_file(*ctx->c, path, *cfg, batch, *r);
+        *out = r.release();
+    });
+}
+
+extern "C" int sx_results_models(const sx_results* r, uint32_t* n) {
+    if (!r || !n) return SX_ERR_ARG;
+    *n = (uint32_t)r->models.size();
+    return SX_OK;
+}
+
+extern "C" int sx_results_model(const sx_results* r, uint32_t i, const char** name, const char** acc, int32_t* m) {
+    if (!r || i >= r->models.size()) return SX_ERR_ARG;
+    const auto& x = r->models[i];
+    if (name) *name = x.name.c_str();
+    if (acc) *acc = x.acc ? x.acc->c_str() : nullptr;
+    if (m) *m = x.m;
+    return SX_OK;
 }
 
 extern "C" int sx_encode(const uint8_t* seq, size_t n, uint8_t* out, size_t* n_out) {

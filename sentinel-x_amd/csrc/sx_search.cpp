@@ -17,6 +17,7 @@
 #include "sx_search.h"
 
 #include "sx_domain.h"
+#include "sx_ingest.h"
 #include "sx_pipeline.h"
 
 extern "C" {
@@ -275,6 +276,8 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     check_search_cfg(&cfg);
     const std::vector<Threshold> ths = resolve_thresholds(c.profs, cfg);
     const double z = std::max(cfg.z > 0.0 ? cfg.z : (double)c.nt, 1.0);
+    res.models.clear();
+    for (const Profile& p : c.profs) res.models.push_back({p.name, p.acc, p.m});
     if (c.profs.empty() || c.nt == 0) return;
     // with the filters on, the tail goes on with the Forward score that decided the F3
     // test (score_target computes Forward once, pipeline.rs:180); with them off, with the
@@ -343,6 +346,48 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     t0 = clk::now();
     tails_and_report(c.profs, len, ths, z, all_p, all_t, all_hit, all_ok, res);
     res.stats.ms_report = ms_since(t0);
+}
+
+void search_hmm_file(Ctx& c, const std::string& path, const sx_search_cfg& cfg, uint32_t batch, sx_results& res) {
+    using clk = std::chrono::steady_clock;
+    check_search_cfg(&cfg);
+    if (batch == 0) batch = 256;
+    const int threads = cfg.threads > 0 ? cfg.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    // two batches parsed ahead of the one on the device
+    auto stream = ProfileStream::open(path, threads, (size_t)batch * 2);
+    res = sx_results{};
+    std::vector<Profile> b;
+    uint32_t base = 0;
+    for (;;) {
+        b.clear();
+        const auto t0 = clk::now();
+        if (stream->next(batch, b) == 0) break;
+        res.stats.ms_ingest += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        const uint32_t nb = (uint32_t)b.size();
+        c.set_profiles(std::move(b));
+        sx_results r;
+        search(c, cfg, r);
+        const uint64_t dbase = res.doms.size();
+        for (sx_hit h : r.hits) {
+            h.profile += base;
+            h.dom_first += dbase;
+            res.hits.push_back(h);
+        }
+        res.doms.insert(res.doms.end(), r.doms.begin(), r.doms.end());
+        res.models.insert(res.models.end(), r.models.begin(), r.models.end());
+        sx_search_stats& s = res.stats;
+        s.n_pairs += r.stats.n_pairs;
+        s.n_rows += r.stats.n_rows;
+        s.ms_filters += r.stats.ms_filters;
+        s.ms_specials += r.stats.ms_specials;
+        s.ms_tail += r.stats.ms_tail;
+        s.ms_report += r.stats.ms_report;
+        s.n_regions += r.stats.n_regions;
+        s.bytes_to_host += r.stats.bytes_to_host;
+        base += nb;
+    }
+    res.stats.n_models = base;
+    res.stats.ms_ingest_wait = stream->ms_wait();
 }
 
 void search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,

@@ -248,3 +248,50 @@ def calibrate(hmms: List[SynthHmm], scorer: Callable, n: int = 200, seed: int = 
         tau = fmu - math.log(-math.log(1.0 - tailp)) / LAMBDA + math.log(tailp) / LAMBDA
         h.stats = {"MSV": (mmu, LAMBDA), "VITERBI": (vmu, LAMBDA), "FORWARD": (tau, LAMBDA)}
     return hmms
+
+
+_AA = "ACDEFGHIKLMNPQRSTVWY"
+
+
+def _nlp(p) -> str:
+    """HMMER3 text value of a probability: -ln(p) with five decimals, '*' for 0."""
+    return " ".join("*" if v <= 0.0 else f"{-math.log(v):.5f}" for v in np.asarray(p, np.float64))
+
+
+def hmm_text(h: SynthHmm) -> str:
+    """One model in HMMER3/f text (the format parse.rs reads): probabilities written as
+    -ln(p) to five decimals, insert emissions the background, cutoffs and STATS as held."""
+    out = ["HMMER3/f [3.4 | Aug 2023]", f"NAME  {h.name}"]
+    if h.acc:
+        out.append(f"ACC   {h.acc}")
+    if h.desc:
+        out.append(f"DESC  {h.desc}")
+    out += [f"LENG  {h.m}", "ALPH  amino"]
+    for tag, c in (("GA", h.ga), ("TC", h.tc), ("NC", h.nc)):
+        if c is not None:
+            out.append(f"{tag}    {c[0]:.2f} {c[1]:.2f};")
+    for key in ("MSV", "VITERBI", "FORWARD"):
+        if key in (h.stats or {}):
+            a, b = h.stats[key]
+            out.append(f"STATS LOCAL {key:<9} {a:.4f}  {b:.5f}")
+    out.append("HMM          " + "        ".join(_AA))
+    out.append("            m->m     m->i     m->d     i->m     i->i     d->m     d->d")
+    ins = _nlp(BGN)
+    out.append("  COMPO   " + _nlp(h.compo))
+    out.append("          " + ins)
+    out.append("          " + _nlp(h.t[0]))
+    for k in range(1, h.m + 1):
+        out.append(f"{k:>7}   " + _nlp(h.mat[k]) + f" {k} - - -")
+        out.append("          " + ins)
+        out.append("          " + _nlp(h.t[k]))
+    out.append("//")
+    return "\n".join(out) + "\n"
+
+
+def write_hmm(path: str, hmms: List[SynthHmm]) -> None:
+    """Write models as an HMMER3/f file (gzip-compressed when path ends in .gz)."""
+    import gzip
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "wt") as fh:
+        for h in hmms:
+            fh.write(hmm_text(h))
