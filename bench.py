@@ -330,14 +330,15 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
     return out
 
 
-def ingest_leg(sxa, synth, f, n_profiles=1000, batch=256):
+def ingest_leg(sxa, synth, f, n_profiles=2000, batches=(256, 512, 0)):
     """Model-file ingest: n_profiles Pfam-scale profiles (calibrated) written as an HMMER3
     text file and searched against the 50,000 Pfam-scale proteins (filters on, E <= 10).
     parse_configure: the file parsed and configured into a packed profile cache on one
     host thread and on all of them (sx_convert_hmm_file).  search: the whole job three
     ways, host wall time: the file loaded whole, then searched (sx_load_hmm_file +
-    sx_search); streamed `batch` models at a time with parsing overlapped
-    (sx_search_hmm_file); and streamed from the cache.  Built-in launch shapes."""
+    sx_search); streamed a batch of models at a time with parsing overlapped
+    (sx_search_hmm_file, for each of `batches`; 0 = the library's default); and streamed
+    from the cache at the default batch.  Built-in launch shapes."""
     import shutil
     import tempfile
     wl = synth.make_workload(2, profile_scale=n_profiles / 20000.0)
@@ -349,7 +350,7 @@ def ingest_leg(sxa, synth, f, n_profiles=1000, batch=256):
         threads = host_cpus()["threads"]
         ms = np.array([h.m for h in wl.hmms])
         out = {"workload": f"{len(wl.hmms)} pfam-scale profiles (M {ms.min()}-{ms.max()}) x {len(wl.seqs)} proteins",
-               "file_mb": round(os.path.getsize(path) / 2**20, 1), "batch": batch, "threads": threads}
+               "file_mb": round(os.path.getsize(path) / 2**20, 1), "threads": threads}
         for th in (1, threads):
             t0 = time.perf_counter()
             sxa.convert_hmm_file(path, cache, th)
@@ -368,12 +369,15 @@ def ingest_leg(sxa, synth, f, n_profiles=1000, batch=256):
             h0, d0 = c.search(**kw)
             t2 = time.perf_counter()
             out["load_then_search"] = {"s": round(t2 - t0, 3), "load_s": round(t1 - t0, 3), "hits": int(len(h0))}
-            for name, src in (("streamed", path), ("streamed_from_cache", cache)):
+            runs = [(f"streamed_batch_{b or 'default'}", path, b) for b in batches]
+            runs.append(("streamed_from_cache", cache, 0))
+            for name, src, b in runs:
                 t0 = time.perf_counter()
-                h1, d1, _ = c.search_hmm_file(src, batch=batch, **kw)
+                h1, d1, _ = c.search_hmm_file(src, batch=b, **kw)
                 dt = time.perf_counter() - t0
                 st = c.last_search_stats
                 out[name] = {"s": round(dt, 3), "ingest_ms_not_overlapped": round(st["ms_ingest"], 1),
+                             "filters_ms": round(st["ms_filters"], 1),
                              "same_hits": bool(np.array_equal(h0, h1) and np.array_equal(d0, d1))}
         return out
     finally:
@@ -451,7 +455,7 @@ def main():
     ap.add_argument("--fwd-err-profiles", type=int, default=40,
                     help="marker-set profiles (x all proteins) checked against the FP64 Forward")
     ap.add_argument("--no-single", action="store_true", help="skip the single-M200 leg")
-    ap.add_argument("--ingest", type=int, default=1000,
+    ap.add_argument("--ingest", type=int, default=2000,
                     help="profiles of the model-file ingest leg (0 = skip)")
     ap.add_argument("--mag-sweep", default="slice", choices=("slice", "full", "none"),
                     help="MAG-sweep leg: 1,000 markers x 40 genomes (slice), 5,000 x 200 (full), or none")

@@ -311,8 +311,9 @@ int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const 
 
 /* search_hmm_file_models (search.rs:252-271): every model of an HMMER3 file (text or
  * gzip, or a packed profile cache written by sx_write_profile_cache / sx_convert_hmm_file)
- * searched against the context's targets, `batch` models at a time (0: 256, as the
- * reference; the results do not depend on it).  While the device searches a batch, host
+ * searched against the context's targets, `batch` models at a time (the reference takes
+ * 256; 0 here means 2048, since a device launch over a few hundred profiles leaves most
+ * of the chip idle; the results do not depend on it).  While the device searches a batch, host
  * threads parse and configure the next ones.  Hits refer to models by their index in the
  * file (sx_results_model); the context's profile set is the last batch afterwards.
  * Parse errors are the reference's messages (parse.rs), SX_ERR_PARSE. */

@@ -396,7 +396,8 @@ class Context:
 
     def search_hmm_file(self, path: str, batch: int = 0, **cfg):
         """search_hmm_file_models (search.rs:252-271) through sx_search_hmm_file: every model
-        of the file against the loaded targets, `batch` at a time (0: 256).  Returns (hits,
+        of the file against the loaded targets, `batch` at a time (0: 2048; the reference
+        takes 256, and the results do not depend on it).  Returns (hits,
         domains, models) with models[i] = (name, acc, m) of profile index i; the phase
         times go to self.last_search_stats."""
         c = search_config(**cfg)

@@ -351,7 +351,10 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
 void search_hmm_file(Ctx& c, const std::string& path, const sx_search_cfg& cfg, uint32_t batch, sx_results& res) {
     using clk = std::chrono::steady_clock;
     check_search_cfg(&cfg);
-    if (batch == 0) batch = 256;
+    // the reference's batch is 256 models (search.rs:257); a device launch over so few
+    // profiles leaves most of the chip idle, so the default batch is larger (the results
+    // do not depend on it)
+    if (batch == 0) batch = 2048;
     const int threads = cfg.threads > 0 ? cfg.threads : (int)std::max(1u, std::thread::hardware_concurrency());
     // two batches parsed ahead of the one on the device
     auto stream = ProfileStream::open(path, threads, (size_t)batch * 2);
