@@ -88,6 +88,13 @@ struct SxTargets {
     const int64_t* off;    // start of each target in codes (16-byte aligned)
     const int32_t* len;
     const SxLen* lm;
+    // The same residues interleaved for one-lane groups over the length-sorted order:
+    // sorted positions 64c .. 64c+63 form chunk c, and word k (residues 4k .. 4k+3) of
+    // position 64c + j sits at ilv[ilv_off[c] + 64 k + j], so a wave's 64 lanes read 256
+    // contiguous bytes per four rows.  A chunk holds 4 more words a lane than its longest
+    // target needs (the stream's look-ahead).
+    const uint32_t* ilv;
+    const int64_t* ilv_off;
 };
 
 // A tile: one profile against `count` targets taken from tids[first .. first+count).
@@ -112,6 +119,8 @@ struct SxWork {
     int32_t ntids;
     int32_t out_by_pos;       // implicit form: result index = profile * out_stride + position
                               // in tids (0: + target id)
+    int32_t ilv;              // implicit form over the length-sorted order with chunks a
+                              // multiple of 64: one-lane groups may read SxTargets::ilv
     int64_t out_stride;
     unsigned long long* head; // persistent-queue head; nullptr -> tile = blockIdx.x (+ base)
     int64_t tile_base;

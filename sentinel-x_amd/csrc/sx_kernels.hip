@@ -564,9 +564,37 @@ constexpr int q_tab_dwords() {
         return SX_NCODE * GeoQ1<Q>::WID;
 }
 
+// Residue words of a target, word k at wp[k * ws]: ws = 1 for the packed codes, 64 for the
+// interleaved copy (SxTargets::ilv).
+struct WordStream {
+    const uint32_t* p;
+    int ws;
+    uint32_t cur, n1, n2;
+    int r;
+    __device__ __forceinline__ void init(const uint32_t* base, int stride) {
+        p = base;
+        ws = stride;
+        cur = p[0];
+        n1 = p[ws];
+        n2 = p[2 * ws];
+        r = 0;
+    }
+    __device__ __forceinline__ int next() {
+        const int x = (int)((cur >> (r * 8)) & 0xFFu);
+        if (++r == 4) {
+            r = 0;
+            cur = n1;
+            n1 = n2;
+            n2 = p[3 * ws];
+            p += ws;
+        }
+        return x;
+    }
+};
+
 template <int Q, int G, int TM>
-__device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, int tbm, int tej, const uint8_t* seq,
-                                             int L, int tloop, int tmove) {
+__device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, int tbm, int tej,
+                                             const uint32_t* __restrict__ words, int ws, int L, int tloop, int tmove) {
     static_assert(Q % 2 == 0, "two positions per register");
     static_assert(G == 1 || tm_aligned<TM>(), "the int16 table is laid out for one-lane or aligned groups");
     constexpr int H = Q / 2;
@@ -652,13 +680,13 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
     // the previous row, which pays on the aligned lane blocks and short one-lane rows.  On
     // one-lane rows of 96 positions or more it costs occupancy (190 VGPRs against 99 at
     // Q = 128) and the residue stream is faster (measured 3-15%).
-    const uint32_t* wp = reinterpret_cast<const uint32_t*>(seq);
+    const uint32_t* wp = words;
     if constexpr (G > 1 || H < 48) {
         uint32_t nxt = wp[0];
         for (int i = 0;;) {
             const uint32_t w = nxt;
-            nxt = wp[1];
-            ++wp;
+            nxt = wp[ws];
+            wp += ws;
             step((int)(w & 0xFFu), (i & 15) == 0);
             if (++i == L) break;
             step((int)((w >> 8) & 0xFFu), false);
@@ -669,8 +697,8 @@ __device__ __forceinline__ int32_t msvq_pair(const uint32_t* __restrict__ tab, i
             if (++i == L) break;
         }
     } else {
-        FwdStream rs;
-        rs.init(seq);
+        WordStream rs;
+        rs.init(words, ws);
         for (int i = 0; i < L; i++) step(rs.next(), (i & 15) == 0);
     }
     return xj - tbm - ((1 << SX_Q_SHIFT) - 1);  // xJ + tmove
@@ -1595,9 +1623,21 @@ __global__ void __launch_bounds__(tm_block<TM>(), (k_min_waves<STAGE, TM>()))
                 if (jl == 0) reinterpret_cast<int32_t*>(out)[o] = r;
             } else if constexpr (STAGE == ST_MSVQ) {
                 int32_t r = INT32_MIN;
+                // one-lane groups over the sorted order read the interleaved copy: the
+                // wave's 64 targets are positions 64c .. 64c+63 (first is a multiple of
+                // the chunk, itself a multiple of 64), one per lane
+                const uint32_t* words = reinterpret_cast<const uint32_t*>(seq);
+                int ws = 1;
+                if constexpr (G == 1) {
+                    if (W.ilv && !W.tiles) {
+                        const int64_t s = ti.first + gid;
+                        words = T.ilv + T.ilv_off[s >> 6] + (s & 63);
+                        ws = 64;
+                    }
+                }
                 if (L > 0)
-                    r = msvq_pair<Q, G, TM>(reinterpret_cast<const uint32_t*>(lds), P.qtbm[ti.prof], P.q_tej, seq, L,
-                                            lm.q_tloop, lm.q_tmove);
+                    r = msvq_pair<Q, G, TM>(reinterpret_cast<const uint32_t*>(lds), P.qtbm[ti.prof], P.q_tej, words, ws,
+                                            L, lm.q_tloop, lm.q_tmove);
                 if (jl == 0) reinterpret_cast<int32_t*>(out)[o] = r;
             } else if constexpr (STAGE == ST_MSV) {
                 float r = SX_NEG_INF;

@@ -622,6 +622,7 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         w.profs = gr.plist;
         w.tids = c.d_sorted;
         w.chunk = k.block / gr.s.g;
+        w.ilv = w.chunk % 64 == 0 ? 1 : 0;
         w.ntids = (int32_t)T;
         w.nchunks = (int32_t)((T + w.chunk - 1) / w.chunk);
         w.ntiles = (int64_t)w.nchunks * gr.n;

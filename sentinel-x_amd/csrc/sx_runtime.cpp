@@ -170,6 +170,7 @@ void Ctx::free_profiles() {
 
 void Ctx::free_targets() {
     dfree(d_codes); dfree(d_soff); dfree(d_len); dfree(d_lm); dfree(d_sorted); dfree(d_identity);
+    dfree(d_ilv); dfree(d_ilvoff);
 }
 
 const KernelInfo& Ctx::kernel(int stage, int q, int g, bool spec, int hoist) const {
@@ -234,6 +235,29 @@ void Ctx::set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n)
     std::iota(sorted.begin(), sorted.end(), 0);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return tlen[a] > tlen[b]; });
     d_sorted = dupload(sorted);
+    // the interleaved copy of the residues over the sorted order (SxTargets::ilv)
+    {
+        const size_t nc = ((size_t)n + 63) / 64;
+        std::vector<int64_t> coff(nc + 1, 0);
+        for (size_t c = 0; c < nc; c++) {
+            const int64_t lc = tlen[sorted[c * 64]];  // the chunk's longest target
+            coff[c + 1] = coff[c] + ((lc + 3) / 4 + 4) * 64;
+        }
+        std::vector<uint32_t> ilv((size_t)coff[nc] + 64, 0u);
+        for (size_t c = 0; c < nc; c++) {
+            for (int j = 0; j < 64 && c * 64 + j < n; j++) {
+                const int t = sorted[c * 64 + j];
+                const uint8_t* src = h_codes.data() + h_toff[t];
+                for (int32_t k = 0; 4 * k < tlen[t]; k++) {
+                    uint32_t wv = 0;
+                    for (int b = 0; b < 4 && 4 * k + b < tlen[t]; b++) wv |= (uint32_t)src[4 * k + b] << (8 * b);
+                    ilv[(size_t)coff[c] + (size_t)k * 64 + j] = wv;
+                }
+            }
+        }
+        d_ilv = dupload(ilv);
+        d_ilvoff = dupload(coff);
+    }
     std::vector<int32_t> ident(n);
     std::iota(ident.begin(), ident.end(), 0);
     d_identity = dupload(ident);
@@ -326,6 +350,7 @@ SxProfiles Ctx::dprof() const {
 SxTargets Ctx::dtarg() const {
     SxTargets T;
     T.codes = d_codes; T.off = d_soff; T.len = d_len; T.lm = d_lm;
+    T.ilv = d_ilv; T.ilv_off = d_ilvoff;
     return T;
 }
 
@@ -548,6 +573,7 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         w.profs = (const int32_t*)pb.p + goff[gi++];
         w.tids = dtids;
         w.chunk = k.block / g;
+        w.ilv = dtids == d_sorted && ntids == (int)nt && w.chunk % 64 == 0 ? 1 : 0;
         w.ntids = ntids;
         w.nchunks = (ntids + w.chunk - 1) / w.chunk;
         w.ntiles = (int64_t)w.nchunks * (int64_t)kv.second.size();

@@ -145,6 +145,8 @@ struct Ctx {
     SxLen* d_lm = nullptr;
     int32_t* d_sorted = nullptr;
     int32_t* d_identity = nullptr;
+    uint32_t* d_ilv = nullptr;    // SxTargets::ilv
+    int64_t* d_ilvoff = nullptr;  // SxTargets::ilv_off
     void set_targets(const uint8_t* codes, const uint64_t* offsets, uint32_t n);
     void free_targets();
 
