@@ -254,6 +254,8 @@ typedef struct {
     float seq_t, dom_t;    /* SX_CUT_BITS thresholds */
     double f1, f2, f3;     /* filter thresholds */
     double z;              /* database size for E-values (hmmsearch -Z); <= 0: number of targets */
+    int32_t host_rescore;  /* 1: envelope rescoring on host threads instead of the device */
+    int32_t pad;
 } sx_search_cfg;
 
 typedef struct {           /* Hit (search.rs:101-116), by index */
@@ -294,6 +296,10 @@ typedef struct {
      * and the part of it spent waiting on the parser threads */
     uint64_t n_models;
     double ms_ingest, ms_ingest_wait;
+    /* envelope rescoring on the device: its time (ms, with copies; not part of ms_tail)
+     * and the envelopes rescored */
+    double ms_rescore;
+    uint64_t n_envelopes;
 } sx_search_stats;
 
 /* Search the context's profiles against its targets. */
@@ -308,6 +314,34 @@ int sx_search(sx_ctx* ctx, const sx_search_cfg* cfg, sx_results** out);
 int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets, uint32_t nt,
                    const sx_search_cfg* cfg, uint64_t npairs, const uint32_t* prof, const uint32_t* tgt,
                    const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results** out);
+
+/* Envelope rescoring (rescore_isolated_domain, pipeline.rs:545-591) of target rows
+ * i..j (1-based, inclusive) against a profile, with the unihit length model of the whole
+ * target: full-matrix Forward and Backward over the envelope (dp.rs:442-601), posterior
+ * decoding (dp.rs:1274-1306), the null2 odds (dp.rs:1310-1345, asked for with null2 = 1)
+ * and the optimal-accuracy coordinates (dp.rs:1355-1564).  sx_search runs it for every
+ * envelope of the pairs it scores; these entry points expose it on its own.  sx_rescore
+ * runs on the device, one wave per envelope; sx_rescore_host runs the same steps on the
+ * host (forward_matrix ... optimal_accuracy_coords in the reference's operation order). */
+typedef struct {
+    uint32_t profile, target;
+    int32_t i, j;
+    int32_t null2;
+} sx_envelope;
+
+typedef struct {
+    float envsc;        /* the envelope's Forward score (nats), pipeline.rs:561 */
+    float bcksc;        /* its Backward score */
+    int32_t status;     /* 0 rescored; 1 Forward not finite; 2 Backward not finite (rescore returns None) */
+    int32_t oa;         /* 1: the optimal-accuracy trace has an M state and gives the coordinates below;
+                           0: the reference falls back to a Viterbi trace (pipeline.rs:580-588) */
+    int32_t iali, jali; /* target rows (absolute) */
+    int32_t kali, mali; /* model positions */
+    float null2[20];    /* null2 odds of the 20 residues before the degenerate codes and the log */
+} sx_envelope_score;
+
+int sx_rescore(sx_ctx* ctx, const sx_envelope* env, uint64_t n, sx_envelope_score* out);
+int sx_rescore_host(sx_ctx* ctx, const sx_envelope* env, uint64_t n, sx_envelope_score* out);
 
 /* search_hmm_file_models (search.rs:252-271): every model of an HMMER3 file (text or
  * gzip, or a packed profile cache written by sx_write_profile_cache / sx_convert_hmm_file)

@@ -373,33 +373,35 @@ def viterbi(p: Profile, dsq) -> np.float32:
     return np.float32(lib().sxo_viterbi(p.m, fp(p.msc), fp(p.tsc), fp(xs), bp(d), len(d)))
 
 
-def forward(p: Profile, dsq, specials: bool = False, multihit: bool = True):
-    """Returns the double lnf(xC*c_move)+lnscale; float32() of it is the reference score."""
+def forward(p: Profile, dsq, specials: bool = False, multihit: bool = True, xs=None):
+    """Returns the double lnf(xC*c_move)+lnscale; float32() of it is the reference score.
+    xs: the 16 length-model values (default: xsc(len(dsq), multihit))."""
     d = _dsq(dsq)
-    xs = xsc(len(d), multihit)
+    xs = xsc(len(d), multihit) if xs is None else np.ascontiguousarray(xs, np.float32)
     sp = np.empty((5, len(d) + 1), np.float32) if specials else None
     v = lib().sxo_forward(p.m, fp(p.mo), fp(p.tp), fp(xs), bp(d), len(d),
                           fp(sp) if specials else ctypes.POINTER(ctypes.c_float)())
     return (v, sp) if specials else v
 
 
-def backward(p: Profile, dsq, multihit: bool = True):
+def backward(p: Profile, dsq, multihit: bool = True, xs=None):
     d = _dsq(dsq)
-    xs = xsc(len(d), multihit)
+    xs = xsc(len(d), multihit) if xs is None else np.ascontiguousarray(xs, np.float32)
     sp = np.empty((5, len(d) + 1), np.float32)
     v = lib().sxo_backward(p.m, fp(p.mo), fp(p.tp), fp(xs), bp(d), len(d), fp(sp))
     return np.float32(v), sp
 
 
-def forward64(p: Profile, dsq) -> float:
+def forward64(p: Profile, dsq, xs=None) -> float:
+    """FP64 Forward of dsq; xs: the 16 length-model values (default: multihit, len(dsq))."""
     d = _dsq(dsq)
-    xs = xsc(len(d))
+    xs = xsc(len(d)) if xs is None else np.ascontiguousarray(xs, np.float32)
     return lib().sxo_forward64(p.m, fp(p.mo), fp(p.tp), fp(xs), bp(d), len(d))
 
 
-def backward64(p: Profile, dsq) -> float:
+def backward64(p: Profile, dsq, xs=None) -> float:
     d = _dsq(dsq)
-    xs = xsc(len(d))
+    xs = xsc(len(d)) if xs is None else np.ascontiguousarray(xs, np.float32)
     return lib().sxo_backward64(p.m, fp(p.mo), fp(p.tp), fp(xs), bp(d), len(d))
 
 

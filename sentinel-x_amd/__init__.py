@@ -39,7 +39,8 @@ EXPORTS = [
     "sx_run_cascade", "sx_cascade_records", "sx_cascade_launches", "sx_autotune", "sx_shape_for", "sx_set_shapes", "sx_get_shapes",
     "sx_autotune_log", "sx_search", "sx_search_host", "sx_results_size", "sx_results_stats", "sx_results_copy",
     "sx_results_free", "sx_encode", "sx_profile_info", "sx_score_pairs_noalloc", "sx_search_hmm_file",
-    "sx_results_models", "sx_results_model", "sx_write_profile_cache", "sx_convert_hmm_file",
+    "sx_results_models", "sx_results_model", "sx_write_profile_cache", "sx_convert_hmm_file", "sx_rescore",
+    "sx_rescore_host",
 ]
 NOALLOC_MAX_PAIRS = 1 << 18  # SX_NOALLOC_MAX_PAIRS
 
@@ -95,8 +96,14 @@ class sx_search_cfg(ctypes.Structure):
     _fields_ = [("cutoff", ctypes.c_int32), ("filters", ctypes.c_int32), ("null2", ctypes.c_int32),
                 ("threads", ctypes.c_int32), ("seq_e", ctypes.c_double), ("dom_e", ctypes.c_double),
                 ("seq_t", ctypes.c_float), ("dom_t", ctypes.c_float), ("f1", ctypes.c_double),
-                ("f2", ctypes.c_double), ("f3", ctypes.c_double), ("z", ctypes.c_double)]
+                ("f2", ctypes.c_double), ("f3", ctypes.c_double), ("z", ctypes.c_double),
+                ("host_rescore", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
+
+ENVELOPE_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("i", "<i4"), ("j", "<i4"), ("null2", "<i4")])
+ENVELOPE_SCORE_DTYPE = np.dtype([("envsc", "<f4"), ("bcksc", "<f4"), ("status", "<i4"), ("oa", "<i4"),
+                                 ("iali", "<i4"), ("jali", "<i4"), ("kali", "<i4"), ("mali", "<i4"),
+                                 ("null2", "<f4", (20,))])
 
 HIT_DTYPE = np.dtype([("profile", "<u4"), ("target", "<u4"), ("qlen", "<u4"), ("tlen", "<u4"), ("score", "<f4"),
                       ("bias", "<f4"), ("evalue", "<f8"), ("nexpected", "<f4"), ("ndom", "<u4"),
@@ -143,6 +150,8 @@ def lib():
         L.sx_score_pairs_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, u64, vp,
                                              ctypes.POINTER(ctypes.c_double)]
         L.sx_specials.argtypes = [vp, ctypes.c_int, vp, vp, u64, vp, vp]
+        L.sx_rescore.argtypes = [vp, vp, u64, vp]
+        L.sx_rescore_host.argtypes = [vp, vp, u64, vp]
         L.sx_run_cascade.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sx_cascade_stats), vp, u64,
                                      ctypes.POINTER(u64)]
         L.sx_cascade_records.argtypes = [vp, vp, u64, ctypes.POINTER(u64)]
@@ -217,9 +226,11 @@ CUTOFFS = {"evalue": 0, "bits": 1, "ga": 2, "nc": 3, "tc": 4}
 
 
 def search_config(cutoff="evalue", seq_e=10.0, dom_e=10.0, seq_t=0.0, dom_t=0.0, filters=True, null2=True,
-                  f1=0.02, f2=1e-3, f3=1e-5, z=None, threads=0):
-    """SearchConfig (search.rs:42-47) with PipelineConfig (pipeline.rs:27-42) as an sx_search_cfg."""
+                  f1=0.02, f2=1e-3, f3=1e-5, z=None, threads=0, host_rescore=False):
+    """SearchConfig (search.rs:42-47) with PipelineConfig (pipeline.rs:27-42) as an sx_search_cfg.
+    host_rescore: envelope rescoring on host threads instead of the device."""
     c = sx_search_cfg()
+    c.host_rescore = int(bool(host_rescore))
     c.cutoff = CUTOFFS[cutoff]
     c.filters, c.null2, c.threads = int(bool(filters)), int(bool(null2)), int(threads)
     c.seq_e, c.dom_e, c.seq_t, c.dom_t = seq_e, dom_e, seq_t, dom_t
@@ -233,7 +244,8 @@ class SearchStats(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_uint64), ("n_rows", ctypes.c_uint64), ("ms_filters", ctypes.c_double),
                 ("ms_specials", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_report", ctypes.c_double),
                 ("n_regions", ctypes.c_uint64), ("bytes_to_host", ctypes.c_uint64), ("n_models", ctypes.c_uint64),
-                ("ms_ingest", ctypes.c_double), ("ms_ingest_wait", ctypes.c_double)]
+                ("ms_ingest", ctypes.c_double), ("ms_ingest_wait", ctypes.c_double),
+                ("ms_rescore", ctypes.c_double), ("n_envelopes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -477,6 +489,17 @@ class Context:
             out.append(sp[o * 5:(o + l + 1) * 5].reshape(5, l + 1))
             o += l + 1
         return out, sc
+
+    def rescore(self, envelopes, on_host: bool = False):
+        """Envelope rescoring (sx_rescore, or sx_rescore_host with on_host) of an
+        ENVELOPE_DTYPE array (or (profile, target, i, j, null2) tuples); returns an
+        ENVELOPE_SCORE_DTYPE array."""
+        env = np.ascontiguousarray(np.asarray(envelopes, ENVELOPE_DTYPE) if not isinstance(envelopes, np.ndarray)
+                                   else envelopes.astype(ENVELOPE_DTYPE))
+        out = np.zeros(len(env), ENVELOPE_SCORE_DTYPE)
+        fn = self._L.sx_rescore_host if on_host else self._L.sx_rescore
+        self._chk(fn(self._h, _ptr(env), len(env), _ptr(out)))
+        return out
 
     def run_cascade(self, form: int = FORM_BATCHED, f1=0.02, f2=1e-3, f3=1e-5, use_graph: bool = True,
                     cap: Optional[int] = None, msv_bound: bool = True, vit_bound: bool = False):

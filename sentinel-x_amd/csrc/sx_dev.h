@@ -159,3 +159,28 @@ struct SxRegionRes {
 struct SxRegion {
     int32_t pair, i, j, multi;
 };
+
+// One envelope for the device rescoring (rescore_isolated_domain, pipeline.rs:545-591):
+// rows i..j (1-based, inclusive) of target `target` against profile `prof`, scored with
+// the unihit length model of the whole target (Xsc::new(L, false), model.rs:284-312):
+// log-space loop/move scores and their probabilities (E->C = 1, E->J = 0).  null2 = 1
+// asks for the null2 sums.  The job's matrices start at ws[wsoff] (4 x (j-i+2) x (M+1)
+// floats, then 14 x (j-i+2) floats of per-row values) and its row log scales at
+// dws[dwsoff] (2 x (j-i+2) doubles).
+struct SxEnvJob {
+    int32_t prof, target, i, j;
+    int32_t null2;
+    float lloop, lmove, ploop, pmove;
+    int32_t pad;
+    int64_t wsoff, dwsoff;
+};
+// Result of one envelope: the unihit Forward and Backward scores, status 0 (rescored),
+// 1 (Forward not finite) or 2 (Backward not finite), the optimal-accuracy coordinates
+// (oa = 1; rows relative to the envelope, 1-based) and the null2 odds of the 20 residues
+// before the degenerate codes and the logarithm (dp.rs:1310-1350).
+struct SxEnvRes {
+    float envsc, bcksc;
+    int32_t status, oa;
+    int32_t iali, jali, kali, mali;
+    float null2[20];
+};

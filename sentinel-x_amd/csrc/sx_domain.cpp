@@ -233,7 +233,7 @@ void posterior_decode(const Profile& p, const Xsc& xs, const DpMatrix& fx, const
     }
 }
 
-void null2_from_posterior(const Profile& p, const Posterior& pp, float out[NCODE]) {
+void null2_raw(const Profile& p, const Posterior& pp, float raw[K]) {
     const int m = p.m, l = pp.l, w = m + 1;
     std::vector<float> sm((size_t)w, 0.0f);
     float si = 0.0f, sn = 0.0f, sj = 0.0f, sc = 0.0f;
@@ -250,15 +250,25 @@ void null2_from_posterior(const Profile& p, const Posterior& pp, float out[NCODE
     const float norm = 1.0f / (float)std::max(l, 1);
     const float xfactor = sn * norm + sc * norm + sj * norm;
     const float isum = si * norm;
-    float null2[NCODE];
     for (int x = 0; x < K; x++) {
         const float* row = p.mo_row(x);
         float v = 0.0f;
         for (int k = 1; k <= m; k++) v += sm[k] * norm * row[k];
-        null2[x] = v + isum + xfactor;
+        raw[x] = v + isum + xfactor;
     }
+}
+
+void null2_log(const float raw[K], float out[NCODE]) {
+    float null2[NCODE];
+    for (int x = 0; x < K; x++) null2[x] = raw[x];
     fill_degenerate(null2);
     for (int x = 0; x < NCODE; x++) out[x] = null2[x] > 0.0f ? std::log(null2[x]) : NEG_INF;
+}
+
+void null2_from_posterior(const Profile& p, const Posterior& pp, float out[NCODE]) {
+    float raw[K];
+    null2_raw(p, pp, raw);
+    null2_log(raw, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -843,6 +853,8 @@ bool is_multidomain(const std::vector<float>& btot, const std::vector<float>& et
     return mx >= rt3;
 }
 
+}  // namespace
+
 std::vector<std::pair<int, int>> region_ensemble(const Profile& p, const uint8_t* dsq, int ireg, int jreg, int l,
                                                  TailWork& ws, std::vector<float>& n2sc) {
     constexpr int NS = 200;
@@ -923,11 +935,7 @@ std::vector<std::pair<int, int>> region_ensemble(const Profile& p, const uint8_t
     return out;
 }
 
-struct Envelope {
-    int i, j;
-    float envsc, domcorr;
-    AliCoords ali;
-};
+namespace {
 
 // rescore_isolated_domain: envelope i..j (1-based, inclusive) in unihit mode
 bool rescore(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool null2, TailWork& ws,
@@ -971,6 +979,63 @@ bool rescore(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool nul
 }
 
 }  // namespace
+
+DeviceEnvelope rescore_raw(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool null2, TailWork& ws,
+                           float* bcksc) {
+    DeviceEnvelope r{};
+    r.status = 1;
+    r.envsc = NEG_INF;
+    *bcksc = NEG_INF;
+    const uint8_t* sub = dsq + (i - 1);
+    const int ls = j - i + 1;
+    const Xsc xs = Xsc::make((size_t)l, false);
+    r.envsc = forward_matrix(p, sub, ls, xs, ws.fx);
+    if (!std::isfinite(r.envsc)) return r;
+    *bcksc = backward_matrix(p, sub, ls, xs, ws.bx);
+    if (!std::isfinite(*bcksc)) {
+        r.status = 2;
+        return r;
+    }
+    r.status = 0;
+    posterior_decode(p, xs, ws.fx, ws.bx, ws.pp);
+    if (null2) null2_raw(p, ws.pp, r.null2raw);
+    r.oa = optimal_accuracy_coords(p, xs, ws.pp, r.ali) ? 1 : 0;
+    return r;
+}
+
+bool envelope_from_device(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool null2, bool null2_done,
+                          const DeviceEnvelope& r, std::vector<float>& n2sc, Envelope& env) {
+    if (r.status != 0) return false;  // Forward or Backward not finite (pipeline.rs:561-567)
+    float domcorr = 0.0f;
+    if (null2) {
+        if (!null2_done) {
+            float n2[NCODE];
+            null2_log(r.null2raw, n2);
+            for (int pos = i; pos <= j; pos++) {
+                const float v = n2[dsq[pos - 1]];
+                n2sc[pos] = std::isfinite(v) ? v : 0.0f;
+            }
+        }
+        for (int pos = i; pos <= j; pos++) domcorr += n2sc[pos];
+    }
+    if (r.oa) {
+        env.ali = {i - 1 + r.ali.iali, i - 1 + r.ali.jali, r.ali.kali, r.ali.mali};
+    } else {
+        // no M state on the optimal-accuracy trace: the reference falls back to a Viterbi
+        // trace of the envelope (pipeline.rs:580-588)
+        const Xsc xs = Xsc::make((size_t)l, false);
+        const std::vector<AliCoords> vt = viterbi_trace(p, dsq + (i - 1), j - i + 1, xs);
+        if (!vt.empty())
+            env.ali = {i - 1 + vt[0].iali, i - 1 + vt[0].jali, vt[0].kali, vt[0].mali};
+        else
+            env.ali = {i, j, 1, p.m};
+    }
+    env.i = i;
+    env.j = j;
+    env.envsc = r.envsc;
+    env.domcorr = domcorr;
+    return true;
+}
 
 // ---------------------------------------------------------------------------
 // pipeline.rs:186-294
@@ -1019,7 +1084,6 @@ bool score_tail(const Profile& p, const uint8_t* dsq, int l, bool null2, float f
 bool score_regions(const Profile& p, const uint8_t* dsq, int l, bool null2, float fwdsc, const TailRegions& r,
                    TailWork& ws, RawHit& out) {
     if (l == 0 || p.m == 0 || !std::isfinite(fwdsc)) return false;
-    const float nullsc = null_score((size_t)l);
     const float nexpected = r.nexpected;
     std::vector<float> n2sc((size_t)l + 1, 0.0f);
     std::vector<Envelope> envs;
@@ -1035,6 +1099,13 @@ bool score_regions(const Profile& p, const uint8_t* dsq, int l, bool null2, floa
         }
     }
     if (nregions == 0 || envs.empty()) return false;
+    return score_envelopes(p, l, null2, fwdsc, nexpected, nregions, n2sc, envs, out);
+}
+
+bool score_envelopes(const Profile& p, int l, bool null2, float fwdsc, float nexpected, int nregions,
+                     const std::vector<float>& n2sc, const std::vector<Envelope>& envs, RawHit& out) {
+    if (nregions == 0 || envs.empty()) return false;
+    const float nullsc = null_score((size_t)l);
 
     // sequence and domain scores
     const float omega_ln = std::log(1.0f / 256.0f);

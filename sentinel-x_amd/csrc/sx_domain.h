@@ -45,6 +45,10 @@ struct Posterior {  // dp.rs:1262-1272
 };
 void posterior_decode(const Profile& p, const Xsc& xs, const DpMatrix& fx, const DpMatrix& bx, Posterior& pp);
 void null2_from_posterior(const Profile& p, const Posterior& pp, float out[NCODE]);
+// null2_from_posterior in two steps: the expected emission odds of the 20 residues
+// (dp.rs:1310-1345), then the degenerate codes and the logarithm (dp.rs:1346-1350).
+void null2_raw(const Profile& p, const Posterior& pp, float raw[K]);
+void null2_log(const float raw[K], float out[NCODE]);
 
 struct AliCoords {
     int iali, jali, kali, mali;
@@ -88,6 +92,48 @@ struct TailWork {
     DpMatrix fx, bx;
     Posterior pp;
 };
+
+// A rescored envelope (pipeline.rs:538-539): rows i..j, its unihit Forward score, the
+// summed null2 correction over its rows and the alignment coordinates.
+struct Envelope {
+    int i, j;
+    float envsc, domcorr;
+    AliCoords ali;
+};
+
+// What the device rescoring of one envelope returns: status 0 = rescored, else Forward
+// (1) or Backward (2) not finite; the optimal-accuracy coordinates relative to the
+// envelope when oa is set; the raw null2 odds (null2_raw) when they were asked for.
+struct DeviceEnvelope {
+    float envsc;
+    int status, oa;
+    AliCoords ali;
+    float null2raw[K];
+};
+
+// The multi-domain resolution of region ireg..jreg (pipeline.rs:298-536): 200 stochastic
+// traces of a multihit Forward matrix of the region, clustered; returns the envelopes and
+// sets n2sc[ireg..jreg] to the traces' null2 correction.
+std::vector<std::pair<int, int>> region_ensemble(const Profile& p, const uint8_t* dsq, int ireg, int jreg, int l,
+                                                 TailWork& ws, std::vector<float>& n2sc);
+
+// The steps of rescore (pipeline.rs:545-591) that the device rescoring replaces, on the
+// host (its parity reference): unihit full-matrix Forward and Backward of envelope i..j,
+// posterior decoding, the raw null2 odds (when null2) and the optimal-accuracy
+// coordinates; *bcksc = the Backward score.
+DeviceEnvelope rescore_raw(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool null2, TailWork& ws,
+                           float* bcksc);
+
+// The rest of rescore (pipeline.rs:568-590) given the device's result for envelope i..j:
+// the null2 correction (written to n2sc[i..j] unless null2_done, then summed), and the
+// coordinates (the Viterbi-trace fallback on the host when the optimal-accuracy trace
+// has no M state).  Returns false where rescore returns None.
+bool envelope_from_device(const Profile& p, const uint8_t* dsq, int i, int j, int l, bool null2, bool null2_done,
+                          const DeviceEnvelope& r, std::vector<float>& n2sc, Envelope& env);
+
+// The sequence and domain scores of score_target (pipeline.rs:243-293) from its envelopes.
+bool score_envelopes(const Profile& p, int l, bool null2, float fwdsc, float nexpected, int nregions,
+                     const std::vector<float>& n2sc, const std::vector<Envelope>& envs, RawHit& out);
 
 // The regions that posterior decoding finds (pipeline.rs:194-242): rows i..j (1-based,
 // inclusive) in order, each with the outcome of the multi-domain test, and the expected

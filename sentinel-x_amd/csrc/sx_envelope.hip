@@ -1,0 +1,668 @@
+// sx_envelope.hip -- envelope rescoring on the device (sentinel-x src/hmmer
+// pipeline.rs:545-591, rescore_isolated_domain): for each envelope, full-matrix Forward
+// and Backward over the envelope's rows with the unihit length model (dp.rs:442-601),
+// posterior decoding (dp.rs:1274-1306), the null2 sums (dp.rs:1310-1350) and the
+// optimal-accuracy alignment with its traceback (dp.rs:1355-1564).
+//
+// One wave64 per envelope.  A DP row is processed in chunks of 64 model positions, one
+// position per lane (k = 64c + lane + 1), so every global access is coalesced.  The
+// previous and current rows sit in LDS.  The serial chains of a row are resolved across
+// the lanes of a chunk by scans: the D chain of Forward and Backward is affine in the
+// value entering the chunk (D_k = A_k + B_k * D_k-1), and the D chain of the
+// optimal-accuracy DP is a max over a gated run (od_k = max(c_k, p_k ? od_k-1 : -inf)).
+// Row sums and maxima are wave reductions.  Forward / Backward sums therefore run in
+// another order than the reference's serial loops (their contract is an error bound, as
+// for the filter kernels); the optimal-accuracy recurrence uses only maxima and single
+// additions, and its traceback follows the reference's tie rules step for step.
+//
+// The matrices live in a device workspace: Forward M and I rows, Backward M and I rows
+// (four floats a cell).  The decoding pass reads them row by row and overwrites them
+// with the optimal-accuracy M, I and D rows, which the traceback then reads.
+#include <hip/hip_runtime.h>
+
+#include "sx_dev.h"
+
+#define SXE_RESCALE 1.0e4f
+
+namespace sxe {
+
+__device__ __forceinline__ float ninf() { return -__builtin_inff(); }
+
+__device__ __forceinline__ float wsum(float v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+    return v;  // every lane holds the same value (each stage pairs lanes symmetrically)
+}
+__device__ __forceinline__ float wmax(float v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
+    return v;
+}
+__device__ __forceinline__ int wmaxi(int v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v = max(v, __shfl_xor(v, s, 64));
+    return v;
+}
+__device__ __forceinline__ int wmini(int v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v = min(v, __shfl_xor(v, s, 64));
+    return v;
+}
+
+// Inclusive scan of affine maps D -> A + B*D over the lanes, left to right: lane l ends
+// with lanes 0..l composed (lane 0 applied first).
+__device__ __forceinline__ void scan_affine_up(float& A, float& B, int lane) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const float a = __shfl_up(A, s, 64), b = __shfl_up(B, s, 64);
+        if (lane >= s) {
+            A = A + B * a;
+            B = B * b;
+        }
+    }
+}
+// The same, right to left: lane l ends with lanes l..63 composed (lane 63 applied first).
+__device__ __forceinline__ void scan_affine_down(float& A, float& B, int lane) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const float a = __shfl_down(A, s, 64), b = __shfl_down(B, s, 64);
+        if (lane + s < 64) {
+            A = A + B * a;
+            B = B * b;
+        }
+    }
+}
+// Inclusive scan of the gated maxima od -> max(C, P ? od : -inf), left to right.
+__device__ __forceinline__ void scan_gmax_up(float& C, int& P, int lane) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const float c = __shfl_up(C, s, 64);
+        const int p = __shfl_up(P, s, 64);
+        if (lane >= s) {
+            C = fmaxf(C, P ? c : ninf());
+            P = P & p;
+        }
+    }
+}
+
+__device__ __forceinline__ float lnf(float x) { return x > 0.0f ? logf(x) : ninf(); }
+
+// dp.rs:32-40
+__device__ __forceinline__ float logsum(float a, float b) {
+    const float hi = a > b ? a : b, lo = a > b ? b : a;
+    const float d = hi - lo;
+    if (!(d < 30.0f)) return hi;
+    return hi + log1pf(expf(-d));
+}
+
+// an edge with zero probability contributes 0 to the max (dp.rs:1380-1420)
+__device__ __forceinline__ float gate(float t, float v) { return t > 0.0f ? v : 0.0f; }
+// an edge with zero probability is not a traceback candidate
+__device__ __forceinline__ float dis(float t, float v) { return t == 0.0f ? ninf() : v; }
+
+}  // namespace sxe
+
+using namespace sxe;
+
+// Dynamic LDS: 7 x (M+1) floats (previous and current M, I, D rows; null2 column sums).
+extern "C" __global__ void __launch_bounds__(64)
+sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
+            double* __restrict__ dws, SxEnvRes* __restrict__ res) {
+    extern __shared__ float lds[];
+    const SxEnvJob job = jobs[blockIdx.x];
+    const int lane = (int)threadIdx.x;
+    const int M = P.m[job.prof];
+    const int W = M + 1;
+    const int Lr = job.j - job.i + 1;
+    const int R = Lr + 1;
+    const int nch = (M + 63) >> 6;
+    const float* __restrict__ mo = P.mo + P.off[job.prof];
+    const float* __restrict__ tp = P.tp + P.toff[job.prof];
+    const float* __restrict__ pmm = tp + SX_TMM * W;
+    const float* __restrict__ pmi = tp + SX_TMI * W;
+    const float* __restrict__ pmd = tp + SX_TMD * W;
+    const float* __restrict__ pim = tp + SX_TIM * W;
+    const float* __restrict__ pii = tp + SX_TII * W;
+    const float* __restrict__ pdm = tp + SX_TDM * W;
+    const float* __restrict__ pdd = tp + SX_TDD * W;
+    const float* __restrict__ pbm = tp + SX_TBM * W;
+    const uint8_t* __restrict__ sub = T.codes + T.off[job.target] + (job.i - 1);
+    // unihit length model: N, J, C loops and moves share one probability; E -> C = 1,
+    // E -> J = 0
+    const float ploop = job.ploop, pmove = job.pmove;
+    const float pe_move = 1.0f, pe_loop = 0.0f;
+    const float lloop = job.lloop, lmove = job.lmove;
+
+    const size_t RW = (size_t)R * W;
+    float* FM = ws + job.wsoff;
+    float* FI = FM + RW;
+    float* BM = FI + RW;
+    float* BI = BM + RW;
+    float* rowf = BI + RW;
+    float* fsn = rowf;
+    float* fsj = fsn + R;
+    float* fsc = fsj + R;
+    float* bsn = fsc + R;
+    float* bsj = bsn + R;
+    float* bsc = bsj + R;
+    float* ppn = bsc + R;
+    float* ppj = ppn + R;
+    float* ppc = ppj + R;
+    float* oxe = ppc + R;
+    float* oxn = oxe + R;
+    float* oxj = oxn + R;
+    float* oxb = oxj + R;
+    float* oxc = oxb + R;
+    double* fls = dws + job.dwsoff;
+    double* bls = fls + R;
+
+    float* aM = lds;
+    float* aI = lds + W;
+    float* aD = lds + 2 * W;
+    float* cM = lds + 3 * W;
+    float* cI = lds + 4 * W;
+    float* cD = lds + 5 * W;
+    float* sm = lds + 6 * W;
+
+    SxEnvRes out;
+    out.envsc = ninf();
+    out.bcksc = ninf();
+    out.status = 1;
+    out.oa = 0;
+    out.iali = out.jali = out.kali = out.mali = 0;
+    for (int x = 0; x < 20; x++) out.null2[x] = 0.0f;
+
+    // ---------------------------------------------------------------- Forward (dp.rs:443-522)
+    for (int k = lane; k < W; k += 64) {
+        aM[k] = aI[k] = aD[k] = 0.0f;
+        cM[k] = cI[k] = cD[k] = 0.0f;
+        sm[k] = 0.0f;
+    }
+    __syncthreads();
+    float xn = 1.0f, xb = pmove, xj = 0.0f, xc = 0.0f;
+    double lnscale = 0.0;
+    if (lane == 0) {
+        fsn[0] = 0.0f;
+        fsj[0] = ninf();
+        fsc[0] = ninf();
+        fls[0] = 0.0;
+    }
+    for (int r = 1; r <= Lr; r++) {
+        const int x = sub[r - 1];
+        const float* __restrict__ eo = mo + (size_t)x * W;
+        const float ins = x != SX_CODE_STAR ? 1.0f : 0.0f;
+        float carry_m = 0.0f, carry_d = 0.0f, part = 0.0f;
+        for (int c = 0; c < nch; c++) {
+            const int k = (c << 6) + lane + 1;
+            const bool valid = k <= M;
+            float mv = 0.0f, iv = 0.0f, A = 0.0f, B = 1.0f;
+            if (valid) {
+                // summation order B, M, I, D
+                mv = (xb * pbm[k] + aM[k - 1] * pmm[k - 1] + aI[k - 1] * pim[k - 1] + aD[k - 1] * pdm[k - 1]) * eo[k];
+                iv = k < M ? (aM[k] * pmi[k] + aI[k] * pii[k]) * ins : 0.0f;
+                B = pdd[k - 1];
+            }
+            float mprev = __shfl_up(mv, 1, 64);
+            if (lane == 0) mprev = carry_m;
+            if (valid) A = mprev * pmd[k - 1];
+            scan_affine_up(A, B, lane);
+            const float dv = A + B * carry_d;
+            if (valid) {
+                cM[k] = mv;
+                cI[k] = iv;
+                cD[k] = dv;
+                part += mv + dv;
+            }
+            carry_m = __shfl(mv, 63, 64);
+            carry_d = __shfl(dv, 63, 64);
+        }
+        float xe = wsum(part);
+        xj = xj * ploop + xe * pe_loop;
+        xc = xc * ploop + xe * pe_move;
+        xn *= ploop;
+        xb = xn * pmove + xj * pmove;
+        float sc = 1.0f;
+        if (xe > SXE_RESCALE) {
+            sc = 1.0f / xe;
+            xn *= sc;
+            xb *= sc;
+            xj *= sc;
+            xc *= sc;
+            lnscale += log((double)xe);
+            xe = 1.0f;
+        }
+        float* gM = FM + (size_t)r * W;
+        float* gI = FI + (size_t)r * W;
+        for (int c = 0; c < nch; c++) {
+            const int k = (c << 6) + lane + 1;
+            if (k <= M) {
+                float vm = cM[k], vi = cI[k];
+                if (sc != 1.0f) {
+                    vm *= sc;
+                    vi *= sc;
+                    cM[k] = vm;
+                    cI[k] = vi;
+                    cD[k] *= sc;
+                }
+                gM[k] = vm;
+                gI[k] = vi;
+            }
+        }
+        if (lane == 0) {
+            const float ls = (float)lnscale;
+            fls[r] = lnscale;
+            fsn[r] = lnf(xn) + ls;
+            fsj[r] = lnf(xj) + ls;
+            fsc[r] = lnf(xc) + ls;
+        }
+        __syncthreads();
+        float* t;
+        t = aM; aM = cM; cM = t;
+        t = aI; aI = cI; cI = t;
+        t = aD; aD = cD; cD = t;
+    }
+    const float envsc = (float)((double)lnf(xc * pmove) + lnscale);
+    out.envsc = envsc;
+    if (!isfinite(envsc)) {
+        if (lane == 0) res[blockIdx.x] = out;
+        return;
+    }
+
+    // ---------------------------------------------------------------- Backward (dp.rs:525-601)
+    {
+        xc = pmove;
+        float xe = xc * pe_move;
+        xj = 0.0f;
+        xn = 0.0f;
+        // row Lr: M and D from E; I = 0
+        float carry_d = 0.0f;
+        float* gM = BM + (size_t)Lr * W;
+        float* gI = BI + (size_t)Lr * W;
+        for (int c = nch - 1; c >= 0; c--) {
+            const int k = (c << 6) + lane + 1;
+            const bool valid = k <= M;
+            float A = 0.0f, B = 1.0f;
+            if (valid) {
+                A = xe;
+                B = k < M ? pdd[k] : 0.0f;
+            }
+            scan_affine_down(A, B, lane);
+            const float dv = A + B * carry_d;
+            float dnext = __shfl_down(dv, 1, 64);
+            if (lane == 63) dnext = carry_d;
+            if (valid) {
+                const float mv = k == M ? xe : xe + dnext * pmd[k];
+                cM[k] = mv;
+                cI[k] = 0.0f;
+                cD[k] = dv;
+                gM[k] = mv;
+                gI[k] = 0.0f;
+            }
+            carry_d = __shfl(dv, 0, 64);
+        }
+        lnscale = 0.0;
+        if (lane == 0) {
+            bls[Lr] = 0.0;
+            bsn[Lr] = ninf();
+            bsj[Lr] = ninf();
+            bsc[Lr] = lnf(xc);
+        }
+        __syncthreads();
+        float* t;
+        t = aM; aM = cM; cM = t;
+        t = aI; aI = cI; cI = t;
+        t = aD; aD = cD; cD = t;
+        float sn1 = ninf();  // N of row 1 (row Lr's N is -inf)
+        for (int r = Lr - 1; r >= 1; r--) {
+            const int xnext = sub[r];
+            const float* __restrict__ eo = mo + (size_t)xnext * W;
+            const float ins = xnext != SX_CODE_STAR ? 1.0f : 0.0f;
+            float part = 0.0f;
+            for (int c = 0; c < nch; c++) {
+                const int k = (c << 6) + lane + 1;
+                if (k <= M) part += aM[k] * pbm[k] * eo[k];
+            }
+            float xbv = wsum(part);
+            xj = xj * ploop + xbv * pmove;
+            xc *= ploop;
+            xe = xj * pe_loop + xc * pe_move;
+            xn = xn * ploop + xbv * pmove;
+            float pmx = ninf();
+            carry_d = 0.0f;
+            for (int c = nch - 1; c >= 0; c--) {
+                const int k = (c << 6) + lane + 1;
+                const bool valid = k <= M;
+                float A = 0.0f, B = 1.0f, mnext = 0.0f, ivv = 0.0f;
+                if (valid) {
+                    if (k < M) {
+                        mnext = aM[k + 1] * eo[k + 1];
+                        ivv = aI[k] * ins;
+                        A = mnext * pdm[k] + xe;
+                        B = pdd[k];
+                    } else {
+                        A = xe;
+                        B = 0.0f;
+                    }
+                }
+                scan_affine_down(A, B, lane);
+                const float dv = A + B * carry_d;
+                float dnext = __shfl_down(dv, 1, 64);
+                if (lane == 63) dnext = carry_d;
+                if (valid) {
+                    float mv, iv;
+                    if (k < M) {
+                        mv = mnext * pmm[k] + ivv * pmi[k] + xe + dnext * pmd[k];
+                        iv = mnext * pim[k] + ivv * pii[k];
+                    } else {
+                        mv = xe;
+                        iv = 0.0f;
+                    }
+                    cM[k] = mv;
+                    cI[k] = iv;
+                    cD[k] = dv;
+                    pmx = fmaxf(pmx, mv);
+                }
+                carry_d = __shfl(dv, 0, 64);
+            }
+            const float rowmax = fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(xbv, xe), xn), xj), xc), wmax(pmx));
+            float sc = 1.0f;
+            if (rowmax > SXE_RESCALE) {
+                sc = 1.0f / rowmax;
+                xbv *= sc;
+                xe *= sc;
+                xn *= sc;
+                xj *= sc;
+                xc *= sc;
+                lnscale += log((double)rowmax);
+            }
+            gM = BM + (size_t)r * W;
+            gI = BI + (size_t)r * W;
+            for (int c = 0; c < nch; c++) {
+                const int k = (c << 6) + lane + 1;
+                if (k <= M) {
+                    float vm = cM[k], vi = cI[k];
+                    if (sc != 1.0f) {
+                        vm *= sc;
+                        vi *= sc;
+                        cM[k] = vm;
+                        cI[k] = vi;
+                        cD[k] *= sc;
+                    }
+                    gM[k] = vm;
+                    gI[k] = vi;
+                }
+            }
+            const float ls = (float)lnscale;
+            if (r == 1) sn1 = lnf(xn) + ls;
+            if (lane == 0) {
+                bls[r] = lnscale;
+                bsn[r] = lnf(xn) + ls;
+                bsj[r] = lnf(xj) + ls;
+                bsc[r] = lnf(xc) + ls;
+            }
+            __syncthreads();
+            t = aM; aM = cM; cM = t;
+            t = aI; aI = cI; cI = t;
+            t = aD; aD = cD; cD = t;
+        }
+        // row 0: B from row 1 and residue 1; the score is N(0)
+        const float* __restrict__ eo = mo + (size_t)sub[0] * W;
+        float part = 0.0f;
+        for (int c = 0; c < nch; c++) {
+            const int k = (c << 6) + lane + 1;
+            if (k <= M) part += aM[k] * pbm[k] * eo[k];
+        }
+        const float b0 = wsum(part);
+        const float sb0 = lnf(b0) + (float)lnscale;
+        out.bcksc = logsum(sn1 + lloop, sb0 + lmove);
+    }
+    if (!isfinite(out.bcksc)) {
+        out.status = 2;
+        if (lane == 0) res[blockIdx.x] = out;
+        return;
+    }
+    out.status = 0;
+    __syncthreads();
+
+    // ---------------------------------------------------------------- posterior decoding,
+    // null2 sums and the optimal-accuracy DP (dp.rs:1274-1306, 1310-1350, 1355-1460)
+    const double overall = (double)out.bcksc;
+    for (int k = lane; k < W; k += 64) {
+        aM[k] = aI[k] = aD[k] = ninf();
+        sm[k] = 0.0f;
+    }
+    __syncthreads();
+    float si = 0.0f, sn = 0.0f, sj = 0.0f, scs = 0.0f;
+    float oxn_p = 0.0f, oxb_p = 0.0f, oxj_p = ninf(), oxc_p = ninf();
+    if (lane == 0) {
+        oxe[0] = ninf();
+        oxn[0] = 0.0f;
+        oxj[0] = ninf();
+        oxb[0] = 0.0f;
+        oxc[0] = ninf();
+        ppn[0] = ppj[0] = ppc[0] = 0.0f;
+    }
+    for (int c = 0; c < nch; c++) {
+        const int k = (c << 6) + lane + 1;
+        if (k <= M) FM[k] = FI[k] = BM[k] = ninf();  // row 0 of the optimal-accuracy matrices
+    }
+    for (int r = 1; r <= Lr; r++) {
+        const double fac = exp(fls[r] + bls[r] - overall);
+        const float pn = (float)exp((double)(fsn[r - 1] + bsn[r] + lloop) - overall);
+        const float pj = (float)exp((double)(fsj[r - 1] + bsj[r] + lloop) - overall);
+        const float pc = (float)exp((double)(fsc[r - 1] + bsc[r] + lloop) - overall);
+        float* gM = FM + (size_t)r * W;
+        float* gI = FI + (size_t)r * W;
+        float* gD = BM + (size_t)r * W;
+        const float* hM = BM + (size_t)r * W;
+        const float* hI = BI + (size_t)r * W;
+        float carry_m = ninf(), carry_d = ninf(), emax = ninf();
+        for (int c = 0; c < nch; c++) {
+            const int k = (c << 6) + lane + 1;
+            const bool valid = k <= M;
+            float mv = ninf(), ov = ninf(), C = ninf();
+            int Pg = 1;
+            if (valid) {
+                const float ppm = (float)((double)gM[k] * (double)hM[k] * fac);
+                const float ppi = k < M ? (float)((double)gI[k] * (double)hI[k] * fac) : 0.0f;
+                sm[k] += ppm;
+                si += ppi;
+                float sv = gate(pbm[k], oxb_p);
+                if (k >= 2) {
+                    sv = fmaxf(sv, gate(pmm[k - 1], aM[k - 1]));
+                    sv = fmaxf(sv, gate(pim[k - 1], aI[k - 1]));
+                    sv = fmaxf(sv, gate(pdm[k - 1], aD[k - 1]));
+                } else {
+                    sv = fmaxf(sv, 0.0f);
+                }
+                mv = sv + ppm;
+                ov = fmaxf(gate(pmi[k], aM[k]), gate(pii[k], aI[k])) + ppi;
+            }
+            // od_1 = -inf; od_k = max(gate(MD, om_k-1), gate(DD, od_k-1)) for k >= 2
+            float mprev = __shfl_up(mv, 1, 64);
+            if (lane == 0) mprev = carry_m;
+            if (valid && k >= 2) {
+                const bool pass = pdd[k - 1] > 0.0f;
+                const float g = gate(pmd[k - 1], mprev);
+                C = pass ? g : fmaxf(g, 0.0f);
+                Pg = pass ? 1 : 0;
+            } else if (valid) {
+                C = ninf();
+                Pg = 0;
+            }
+            scan_gmax_up(C, Pg, lane);
+            const float dv = fmaxf(C, Pg ? carry_d : ninf());
+            if (valid) {
+                cM[k] = mv;
+                cI[k] = ov;
+                cD[k] = dv;
+                gM[k] = mv;
+                gI[k] = ov;
+                gD[k] = dv;
+                emax = fmaxf(emax, fmaxf(mv, dv));
+            }
+            carry_m = __shfl(mv, 63, 64);
+            carry_d = __shfl(dv, 63, 64);
+        }
+        const float xe = wmax(emax);
+        const float xjv = fmaxf(ploop == 0.0f ? 0.0f : oxj_p + pj, pe_loop == 0.0f ? 0.0f : xe);
+        const float xcv = fmaxf(ploop == 0.0f ? 0.0f : oxc_p + pc, pe_move == 0.0f ? 0.0f : xe);
+        const float xnv = ploop == 0.0f ? 0.0f : oxn_p + pn;
+        const float xbv = fmaxf(pmove == 0.0f ? 0.0f : xnv, pmove == 0.0f ? 0.0f : xjv);
+        if (lane == 0) {
+            oxe[r] = xe;
+            oxj[r] = xjv;
+            oxc[r] = xcv;
+            oxn[r] = xnv;
+            oxb[r] = xbv;
+            ppc[r] = pc;
+        }
+        oxj_p = xjv;
+        oxc_p = xcv;
+        oxn_p = xnv;
+        oxb_p = xbv;
+        sn += pn;
+        sj += pj;
+        scs += pc;
+        __syncthreads();
+        float* t;
+        t = aM; aM = cM; cM = t;
+        t = aI; aI = cI; cI = t;
+        t = aD; aD = cD; cD = t;
+    }
+
+    // null2 odds (dp.rs:1310-1350): expected emissions of the envelope's posterior
+    if (job.null2) {
+        const float norm = 1.0f / (float)(Lr > 1 ? Lr : 1);
+        const float xfactor = sn * norm + scs * norm + sj * norm;
+        const float isum = wsum(si) * norm;
+        for (int x = 0; x < 20; x++) {
+            const float* __restrict__ row = mo + (size_t)x * W;
+            float v = 0.0f;
+            for (int c = 0; c < nch; c++) {
+                const int k = (c << 6) + lane + 1;
+                if (k <= M) v += sm[k] * norm * row[k];
+            }
+            out.null2[x] = wsum(v) + isum + xfactor;
+        }
+    }
+
+    // ---------------------------------------------------------------- traceback
+    // (dp.rs:1462-1564).  Every lane follows the same path (the reads are broadcasts);
+    // the E state's choice of the row's best M or D cell is a wave reduction with the
+    // reference's visiting order and tie rules.
+    __threadfence();
+    __syncthreads();
+    enum { ST_C, ST_E, ST_M, ST_I, ST_D, ST_B, ST_J, ST_N };
+    const int qs = max(2, (M - 1) / 4 + 1);
+    int s0 = ST_C, i = Lr, k = 0;
+    bool have = false;
+    int fi = 0, fk = 0, li = 0, lk = 0;
+    const long limit = 8L * (Lr + 2) * (M + 2) + 64;
+    for (long guard = 1; guard <= limit; guard++) {
+        int s1;
+        if (s0 == ST_C) {
+            if (i <= 0) break;
+            const float a = ploop == 0.0f ? ninf() : oxc[i - 1] + ppc[i];
+            const float b = pe_move == 0.0f ? ninf() : oxe[i];
+            s1 = a > b ? ST_C : ST_E;
+        } else if (s0 == ST_E) {
+            // visiting order: stripe q = 0..qs-1, then M cells of the four segments
+            // (kk = r*qs + q + 1), then their D cells; an M cell wins ties against what
+            // came before (>=), a D cell does not (>)
+            const float* rm = FM + (size_t)i * W;
+            const float* rd = BM + (size_t)i * W;
+            float mx = ninf();
+            for (int c = 0; c < nch; c++) {
+                const int kk = (c << 6) + lane + 1;
+                if (kk <= M) mx = fmaxf(mx, fmaxf(rm[kk], rd[kk]));
+            }
+            mx = wmax(mx);
+            int bm = -1, bd = 0x7fffffff;
+            for (int c = 0; c < nch; c++) {
+                const int kk = (c << 6) + lane + 1;
+                if (kk <= M) {
+                    const int q = (kk - 1) % qs, rr = (kk - 1) / qs;
+                    if (rm[kk] == mx) bm = max(bm, q * 8 + rr);
+                    if (rd[kk] == mx) bd = min(bd, q * 8 + 4 + rr);
+                }
+            }
+            bm = wmaxi(bm);
+            bd = wmini(bd);
+            if (bm >= 0) {
+                k = (bm & 7) * qs + (bm >> 3) + 1;
+                s1 = ST_M;
+            } else if (bd != 0x7fffffff) {
+                k = ((bd & 7) - 4) * qs + (bd >> 3) + 1;
+                s1 = ST_D;
+            } else {
+                break;
+            }
+        } else if (s0 == ST_M) {
+            // a consistent trace reaches M only at i >= 1, k >= 1 (row 0 and position 0
+            // hold -inf); the guard keeps every read inside the matrices
+            if (i < 1 || k < 1) break;
+            if (!have) {
+                li = i;
+                lk = k;
+                have = true;
+            }
+            fi = i;
+            fk = k;
+            const size_t prow = (size_t)(i - 1) * W;
+            float mv = ninf(), iv = ninf(), dv = ninf();
+            if (k >= 2) {
+                mv = dis(pmm[k - 1], FM[prow + k - 1]);
+                iv = dis(pim[k - 1], FI[prow + k - 1]);
+                dv = dis(pdm[k - 1], BM[prow + k - 1]);
+            }
+            const float bv = dis(pbm[k], oxb[i - 1]);
+            int best = 0;
+            float bvv = mv;
+            if (iv > bvv) { best = 1; bvv = iv; }
+            if (dv > bvv) { best = 2; bvv = dv; }
+            if (bv > bvv) { best = 3; bvv = bv; }
+            k--;
+            i--;
+            s1 = best == 0 ? ST_M : best == 1 ? ST_I : best == 2 ? ST_D : ST_B;
+        } else if (s0 == ST_D) {
+            if (i < 1 || k < 1) break;
+            const size_t row = (size_t)i * W;
+            float a = ninf(), b = ninf();
+            if (k >= 2) {
+                a = dis(pmd[k - 1], FM[row + k - 1]);
+                b = dis(pdd[k - 1], BM[row + k - 1]);
+            }
+            k--;
+            s1 = a >= b ? ST_M : ST_D;
+        } else if (s0 == ST_I) {
+            if (i < 1 || k < 1 || k > M) break;
+            const size_t prow = (size_t)(i - 1) * W;
+            const float a = dis(pmi[k], FM[prow + k]);
+            const float b = dis(pii[k], FI[prow + k]);
+            i--;
+            s1 = a >= b ? ST_M : ST_I;
+        } else if (s0 == ST_B) {
+            if (i < 0) break;
+            const float a = pmove == 0.0f ? ninf() : oxn[i];
+            const float b = pmove == 0.0f ? ninf() : oxj[i];
+            if (a > b)
+                s1 = ST_N;
+            else
+                break;  // a domain begins here; a unihit trace has one
+        } else {
+            break;  // J or N
+        }
+        if ((s1 == ST_N || s1 == ST_J || s1 == ST_C) && s1 == s0) i--;
+        if (s1 == ST_N) break;
+        s0 = s1;
+    }
+    if (have) {
+        out.oa = 1;
+        out.iali = fi;
+        out.jali = li;
+        out.kali = fk;
+        out.mali = lk;
+    }
+    if (lane == 0) res[blockIdx.x] = out;
+}

@@ -1437,6 +1437,16 @@ extern "C" int sx_search(sx_ctx* ctx, const sx_search_cfg* cfg, sx_results** out
     });
 }
 
+extern "C" int sx_rescore(sx_ctx* ctx, const sx_envelope* env, uint64_t n, sx_envelope_score* out) {
+    if (!ctx || (n && (!env || !out))) return SX_ERR_ARG;
+    return guard(ctx, [&] { sx::rescore_envelopes(*ctx->c, env, n, false, out); });
+}
+
+extern "C" int sx_rescore_host(sx_ctx* ctx, const sx_envelope* env, uint64_t n, sx_envelope_score* out) {
+    if (!ctx || (n && (!env || !out))) return SX_ERR_ARG;
+    return guard(ctx, [&] { sx::rescore_envelopes(*ctx->c, env, n, true, out); });
+}
+
 extern "C" int sx_search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const uint64_t* offsets,
                               uint32_t nt, const sx_search_cfg* cfg, uint64_t npairs, const uint32_t* prof,
                               const uint32_t* tgt, const double* fwd, const double* bwd, const float* fsp,

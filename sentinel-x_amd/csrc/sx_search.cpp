@@ -24,6 +24,7 @@ extern "C" {
 __global__ void sxr_regions(int64_t, const SxRegionIn*, const float*, const double*, const float*, const double*,
                             float*, float*, SxRegionRes*, SxRegion*, int32_t*, int32_t);
 __global__ void sxc_zero_i32(int32_t*, int64_t);
+__global__ void sxe_rescore(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
 }
 
 namespace sx {
@@ -123,6 +124,167 @@ void run_region_tails(const std::vector<Profile>& profs, const std::vector<const
                       std::vector<RawHit>& hit, std::vector<char>& ok) {
     parallel_pairs(n, threads, hit, ok, [&](uint64_t i, TailWork& ws, RawHit& out) {
         return score_regions(profs[prof[i]], seq[tgt[i]], len[tgt[i]], null2, fwd[i], regs[i], ws, out);
+    });
+}
+
+// Device buffers of the envelope rescoring, kept across the chunks of one search.
+struct EnvBufs {
+    DeviceBuf jobs, res, ws, dws;
+};
+
+// Bytes of device workspace per launch of the envelope rescoring (matrices of 16 bytes a
+// cell); a single larger envelope gets a launch of its own, up to SXE_MAX_JOB_BYTES.
+constexpr size_t SXE_BATCH_BYTES = (size_t)4 << 30;
+constexpr size_t SXE_MAX_JOB_BYTES = (size_t)64 << 30;
+
+// Envelope rescoring on the device (sxe_rescore, one wave per envelope): jobs carry
+// profile, target, rows and the null2 flag; the length model and workspace offsets are
+// filled in here.  Results in job order.
+void device_rescore(Ctx& c, EnvBufs& B, std::vector<SxEnvJob>& jobs, std::vector<SxEnvRes>& out) {
+    out.assign(jobs.size(), SxEnvRes{});
+    if (jobs.empty()) return;
+    size_t a = 0;
+    while (a < jobs.size()) {
+        size_t nf = 0, nd = 0;
+        int maxw = 1;
+        size_t b = a;
+        for (; b < jobs.size(); b++) {
+            SxEnvJob& J = jobs[b];
+            if (J.prof < 0 || (size_t)J.prof >= c.profs.size() || J.target < 0 || (uint32_t)J.target >= c.nt)
+                throw SxError(SX_ERR_ARG, "envelope names an unknown profile or target");
+            const int L = c.tlen[(size_t)J.target];
+            if (J.i < 1 || J.j < J.i || J.j > L) throw SxError(SX_ERR_ARG, "envelope outside its target");
+            const size_t W = (size_t)c.profs[(size_t)J.prof].m + 1, R = (size_t)(J.j - J.i) + 2;
+            const size_t f = 4 * R * W + 14 * R, d = 2 * R;
+            if (f * 4 + d * 8 > SXE_MAX_JOB_BYTES) throw SxError(SX_ERR_ARG, "envelope too large for device rescoring");
+            if (b > a && (nf + f) * 4 + (nd + d) * 8 > SXE_BATCH_BYTES) break;
+            const Xsc xs = Xsc::make((size_t)L, false);
+            J.lloop = xs.n_loop;
+            J.lmove = xs.n_move;
+            J.ploop = xs.pn_loop;
+            J.pmove = xs.pn_move;
+            J.pad = 0;
+            J.wsoff = (int64_t)nf;
+            J.dwsoff = (int64_t)nd;
+            nf += f;
+            nd += d;
+            maxw = std::max(maxw, (int)W);
+        }
+        const size_t n = b - a;
+        auto* djobs = (SxEnvJob*)B.jobs.ensure(n * sizeof(SxEnvJob));
+        auto* dres = (SxEnvRes*)B.res.ensure(n * sizeof(SxEnvRes));
+        float* dws = (float*)B.ws.ensure(nf * sizeof(float));
+        double* ddws = (double*)B.dws.ensure(nd * sizeof(double));
+        HIPCHK(hipMemcpyAsync(djobs, jobs.data() + a, n * sizeof(SxEnvJob), hipMemcpyHostToDevice, c.stream));
+        const size_t lds = (size_t)7 * maxw * sizeof(float);
+        if (lds > 64 * 1024)
+            HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&sxe_rescore),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(sxe_rescore, dim3((unsigned)n), dim3(64), lds, c.stream, (const SxEnvJob*)djobs, c.dprof(),
+                           c.dtarg(), dws, ddws, dres);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out.data() + a, dres, n * sizeof(SxEnvRes), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        a = b;
+    }
+}
+
+DeviceEnvelope to_host(const SxEnvRes& r) {
+    DeviceEnvelope d{};
+    d.envsc = r.envsc;
+    d.status = r.status;
+    d.oa = r.oa;
+    d.ali = {r.iali, r.jali, r.kali, r.mali};
+    for (int x = 0; x < K; x++) d.null2raw[x] = r.null2[x];
+    return d;
+}
+
+// The host part of score_target from the regions on, with envelope rescoring on the
+// device: multi-domain regions are resolved on host threads (stochastic traces and
+// clustering), then every envelope (the single-domain regions and the ensembles'
+// envelopes) is rescored on the device in one batch, and the host finishes each pair
+// (null2 correction, Viterbi-trace fallback, scores).  *ms_dev = the device part.
+void run_region_tails_device(Ctx& c, EnvBufs& eb, const std::vector<Profile>& profs,
+                             const std::vector<const uint8_t*>& seq, const std::vector<int>& len, const uint32_t* prof,
+                             const uint32_t* tgt, uint64_t n, const float* fwd, const std::vector<TailRegions>& regs,
+                             bool null2, int threads, std::vector<RawHit>& hit, std::vector<char>& ok, double* ms_dev,
+                             uint64_t* n_env) {
+    // per pair: null2 corrections and the ensembles' envelopes of its multi-domain regions
+    std::vector<std::vector<float>> n2sc(n);
+    std::vector<std::vector<std::vector<std::pair<int, int>>>> ens(n);
+    std::vector<uint64_t> multi;
+    for (uint64_t i = 0; i < n; i++) {
+        for (const RegionSpan& g : regs[i].spans) {
+            const int L = len[tgt[i]];
+            if (g.i < 1 || g.j < g.i || g.j > L) throw SxError(SX_ERR_STATE, "region outside the target");
+            if (g.multi) {
+                multi.push_back(i);
+                break;
+            }
+        }
+    }
+    {
+        std::vector<RawHit> unused_h;
+        std::vector<char> unused_ok;
+        parallel_pairs(multi.size(), threads, unused_h, unused_ok, [&](uint64_t m, TailWork& ws, RawHit&) {
+            const uint64_t i = multi[m];
+            const int L = len[tgt[i]];
+            n2sc[i].assign((size_t)L + 1, 0.0f);
+            for (const RegionSpan& g : regs[i].spans)
+                ens[i].push_back(g.multi ? region_ensemble(profs[prof[i]], seq[tgt[i]], g.i, g.j, L, ws, n2sc[i])
+                                         : std::vector<std::pair<int, int>>{});
+            return true;
+        });
+    }
+    // the envelopes in (pair, region, envelope) order
+    std::vector<SxEnvJob> jobs;
+    std::vector<uint64_t> first(n + 1, 0);
+    for (uint64_t i = 0; i < n; i++) {
+        first[i] = jobs.size();
+        const int L = len[tgt[i]];
+        for (size_t r = 0; r < regs[i].spans.size(); r++) {
+            const RegionSpan& g = regs[i].spans[r];
+            if (!g.multi) {
+                jobs.push_back({(int32_t)prof[i], (int32_t)tgt[i], g.i, g.j, null2 ? 1 : 0});
+            } else {
+                for (const auto& ij : ens[i][r])
+                    if (ij.first >= 1 && ij.second >= ij.first && ij.second <= L)
+                        jobs.push_back({(int32_t)prof[i], (int32_t)tgt[i], ij.first, ij.second, 0});
+            }
+        }
+    }
+    first[n] = jobs.size();
+    std::vector<SxEnvRes> er;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    device_rescore(c, eb, jobs, er);
+    *ms_dev += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    *n_env += jobs.size();
+    parallel_pairs(n, threads, hit, ok, [&](uint64_t i, TailWork&, RawHit& out) {
+        const Profile& p = profs[prof[i]];
+        const uint8_t* dsq = seq[tgt[i]];
+        const int L = len[tgt[i]];
+        if (L == 0 || p.m == 0 || !std::isfinite(fwd[i])) return false;
+        std::vector<float>& n2 = n2sc[i];
+        if (n2.empty()) n2.assign((size_t)L + 1, 0.0f);
+        std::vector<Envelope> envs;
+        uint64_t q = first[i];
+        for (size_t r = 0; r < regs[i].spans.size(); r++) {
+            const RegionSpan& g = regs[i].spans[r];
+            Envelope env;
+            if (!g.multi) {
+                if (envelope_from_device(p, dsq, g.i, g.j, L, null2, false, to_host(er[q++]), n2, env))
+                    envs.push_back(env);
+            } else {
+                for (const auto& ij : ens[i][r]) {
+                    if (!(ij.first >= 1 && ij.second >= ij.first && ij.second <= L)) continue;
+                    if (envelope_from_device(p, dsq, ij.first, ij.second, L, null2, true, to_host(er[q++]), n2, env))
+                        envs.push_back(env);
+                }
+            }
+        }
+        if (q != first[i + 1]) throw std::runtime_error("envelope bookkeeping mismatch");
+        return score_envelopes(p, L, null2, fwd[i], regs[i].nexpected, (int)regs[i].spans.size(), n2, envs, out);
     });
 }
 
@@ -309,6 +471,7 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     std::vector<RawHit> all_hit;
     std::vector<char> all_ok;
     RegionBufs bufs;
+    EnvBufs ebufs;
     while (a < pairs.size()) {
         cp.clear();
         ct.clear();
@@ -331,8 +494,17 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
                                                   cfg.filters ? fin.data() : nullptr, regs, fsc);
         res.stats.ms_specials += ms_since(t0);
         t0 = clk::now();
-        run_region_tails(c.profs, seq, len, cp.data(), ct.data(), n, cfg.filters ? fin.data() : fsc.data(), regs,
-                         cfg.null2 != 0, cfg.threads, hit, ok);
+        const float* fw = cfg.filters ? fin.data() : fsc.data();
+        if (cfg.host_rescore) {
+            run_region_tails(c.profs, seq, len, cp.data(), ct.data(), n, fw, regs, cfg.null2 != 0, cfg.threads, hit,
+                             ok);
+        } else {
+            double ms_dev = 0.0;
+            run_region_tails_device(c, ebufs, c.profs, seq, len, cp.data(), ct.data(), n, fw, regs, cfg.null2 != 0,
+                                    cfg.threads, hit, ok, &ms_dev, &res.stats.n_envelopes);
+            res.stats.ms_rescore += ms_dev;
+            res.stats.ms_tail -= ms_dev;
+        }
         res.stats.ms_tail += ms_since(t0);
         res.stats.n_pairs += n;
         res.stats.n_rows += (uint64_t)rows;
@@ -346,6 +518,51 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     t0 = clk::now();
     tails_and_report(c.profs, len, ths, z, all_p, all_t, all_hit, all_ok, res);
     res.stats.ms_report = ms_since(t0);
+}
+
+void rescore_envelopes(Ctx& c, const sx_envelope* env, uint64_t n, bool on_host, sx_envelope_score* out) {
+    for (uint64_t e = 0; e < n; e++) {
+        const sx_envelope& v = env[e];
+        if (v.profile >= c.profs.size() || v.target >= c.nt)
+            throw SxError(SX_ERR_ARG, "envelope names an unknown profile or target");
+        if (v.i < 1 || v.j < v.i || v.j > c.tlen[v.target]) throw SxError(SX_ERR_ARG, "envelope outside its target");
+    }
+    auto put = [&](uint64_t e, const DeviceEnvelope& d, float bcksc) {
+        sx_envelope_score& o = out[e];
+        o = sx_envelope_score{};
+        o.envsc = d.envsc;
+        o.bcksc = bcksc;
+        o.status = d.status;
+        o.oa = d.status == 0 ? d.oa : 0;
+        if (o.oa) {
+            o.iali = env[e].i - 1 + d.ali.iali;
+            o.jali = env[e].i - 1 + d.ali.jali;
+            o.kali = d.ali.kali;
+            o.mali = d.ali.mali;
+        }
+        if (d.status == 0 && env[e].null2)
+            for (int x = 0; x < K; x++) o.null2[x] = d.null2raw[x];
+    };
+    if (on_host) {
+        std::vector<RawHit> h;
+        std::vector<char> ok;
+        parallel_pairs(n, 0, h, ok, [&](uint64_t e, TailWork& ws, RawHit&) {
+            const sx_envelope& v = env[e];
+            float b = NEG_INF;
+            const DeviceEnvelope d = rescore_raw(c.profs[v.profile], c.h_codes.data() + c.h_toff[v.target], v.i, v.j,
+                                                 c.tlen[v.target], v.null2 != 0, ws, &b);
+            put(e, d, b);
+            return true;
+        });
+        return;
+    }
+    std::vector<SxEnvJob> jobs(n);
+    for (uint64_t e = 0; e < n; e++)
+        jobs[e] = {(int32_t)env[e].profile, (int32_t)env[e].target, env[e].i, env[e].j, env[e].null2 ? 1 : 0};
+    EnvBufs eb;
+    std::vector<SxEnvRes> er;
+    device_rescore(c, eb, jobs, er);
+    for (uint64_t e = 0; e < n; e++) put(e, to_host(er[e]), er[e].bcksc);
 }
 
 void search_hmm_file(Ctx& c, const std::string& path, const sx_search_cfg& cfg, uint32_t batch, sx_results& res) {
@@ -386,6 +603,8 @@ void search_hmm_file(Ctx& c, const std::string& path, const sx_search_cfg& cfg, 
         s.ms_tail += r.stats.ms_tail;
         s.ms_report += r.stats.ms_report;
         s.n_regions += r.stats.n_regions;
+        s.ms_rescore += r.stats.ms_rescore;
+        s.n_envelopes += r.stats.n_envelopes;
         s.bytes_to_host += r.stats.bytes_to_host;
         base += nb;
     }

@@ -37,4 +37,8 @@ void search_host(const sx_hmm* hmms, uint32_t np, const uint8_t* codes, const ui
                  const sx_search_cfg& cfg, uint64_t npairs, const uint32_t* prof, const uint32_t* tgt,
                  const double* fwd, const double* bwd, const float* fsp, const float* bsp, sx_results& res);
 
+// Envelope rescoring of n envelopes on the device (sxe_rescore) or on the host
+// (rescore_raw on host threads); results as sx_envelope_score.
+void rescore_envelopes(Ctx& c, const sx_envelope* env, uint64_t n, bool on_host, sx_envelope_score* out);
+
 }  // namespace sx
