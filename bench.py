@@ -219,8 +219,20 @@ def kernel_mode(sxa, synth, tune_ms, reps=5, shapes=None):
         # special rows (28 bytes a row a pass: five f32 values and the f64 scale)
         out["search"] = {"hits": int(len(hits)), "domains": int(len(doms)), "pairs_in_tail": int(st["n_pairs"]),
                          "rows_in_tail": int(st["n_rows"]), "regions": int(st["n_regions"]),
+                         "envelopes": int(st["n_envelopes"]),
                          "bytes_to_host": int(st["bytes_to_host"]), "special_row_bytes": int(st["n_rows"]) * 56,
-                         **{k: round(st[k], 2) for k in ("ms_filters", "ms_specials", "ms_tail", "ms_report")}}
+                         **{k: round(st[k], 2) for k in ("ms_filters", "ms_specials", "ms_rescore", "ms_tail",
+                                                         "ms_report")}}
+        # the same search with the envelopes rescored on host threads (the reference's place)
+        for _ in range(2):
+            hh, dh = kc.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True, host_rescore=True)
+        sh = kc.last_search_stats
+        out["search"]["host_rescoring"] = {
+            "ms_tail": round(sh["ms_tail"], 2), "threads": os.cpu_count(),
+            "same_hits_and_coordinates": bool(
+                np.array_equal(hits[["profile", "target", "ndom"]], hh[["profile", "target", "ndom"]]) and
+                all(np.array_equal(doms[k], dh[k]) for k in ("hmm_from", "hmm_to", "ali_from", "ali_to",
+                                                             "env_from", "env_to")))}
     # the int16 MSV bound covers models up to 1024 positions: timed on those profiles
     short = [h for h in wl.hmms if h.m <= 1024]
     cells16 = sum(h.m for h in short) * sum(len(s) for s in wl.seqs)
@@ -312,6 +324,23 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
             t.append(st.ms_total)
         pk = float(min(t))
         out["packed"] = {"ms": round(pk, 3), "gcups": round(cells / pk / 1e6, 1), "hits": int(st.n_fwd_pass)}
+        # the whole search over the packed genomes (filters on, E <= 10), envelopes rescored
+        # on the device and, for comparison, on host threads
+        srch = {}
+        for host in (False, True):
+            for _ in range(2):
+                h, d = cm.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True, host_rescore=host, **f)
+            ss = cm.last_search_stats
+            srch["host_rescoring" if host else "device_rescoring"] = {
+                "hits": int(len(h)), "domains": int(len(d)), "envelopes": int(ss["n_envelopes"]),
+                **{k: round(ss[k], 2) for k in ("ms_filters", "ms_specials", "ms_rescore", "ms_tail", "ms_report")}}
+            if not host:
+                h_dev, d_dev = h, d
+        srch["same_hits_and_coordinates"] = bool(
+            np.array_equal(h_dev[["profile", "target", "ndom"]], h[["profile", "target", "ndom"]]) and
+            all(np.array_equal(d_dev[k], d[k]) for k in ("hmm_from", "hmm_to", "ali_from", "ali_to", "env_from",
+                                                         "env_to")))
+        out["search"] = srch
         lat = []
         gcells = 0
         for g in range(n_genomes):

@@ -28,15 +28,35 @@ namespace sxe {
 
 __device__ __forceinline__ float ninf() { return -__builtin_inff(); }
 
+// Cross-lane steps by DPP (gfx9): row shifts inside 16-lane rows, whole-wave shifts by
+// one lane, and v_readlane for the row totals.  A lane with no source keeps `old`.
+template <int CTRL>
+__device__ __forceinline__ float dppf(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float dppm(float src) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float rdl(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Wave sum / maximum; every lane gets the same value (butterflies inside a row: quad
+// xor 1, quad xor 2, half-row mirror, row mirror; then the four row totals).
 __device__ __forceinline__ float wsum(float v) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
-    return v;  // every lane holds the same value (each stage pairs lanes symmetrically)
+    v += dppm<0xB1>(v);
+    v += dppm<0x4E>(v);
+    v += dppm<0x141>(v);
+    v += dppm<0x140>(v);
+    return (rdl(v, 0) + rdl(v, 16)) + (rdl(v, 32) + rdl(v, 48));
 }
 __device__ __forceinline__ float wmax(float v) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
-    return v;
+    v = fmaxf(v, dppm<0xB1>(v));
+    v = fmaxf(v, dppm<0x4E>(v));
+    v = fmaxf(v, dppm<0x141>(v));
+    v = fmaxf(v, dppm<0x140>(v));
+    return fmaxf(fmaxf(rdl(v, 0), rdl(v, 16)), fmaxf(rdl(v, 32), rdl(v, 48)));
 }
 __device__ __forceinline__ int wmaxi(int v) {
 #pragma unroll
@@ -49,40 +69,90 @@ __device__ __forceinline__ int wmini(int v) {
     return v;
 }
 
-// Inclusive scan of affine maps D -> A + B*D over the lanes, left to right: lane l ends
-// with lanes 0..l composed (lane 0 applied first).
+// lane l receives lane l-1's v, lane 0 receives b (wave_shr:1)
+__device__ __forceinline__ float from_prev(float v, float b) { return dppf<0x138>(b, v); }
+// lane l receives lane l+1's v, lane 63 receives b (wave_shl:1)
+__device__ __forceinline__ float from_next(float v, float b) { return dppf<0x130>(b, v); }
+
+template <int CTRL>
+__device__ __forceinline__ void affine_step(float& A, float& B) {
+    const float a = dppf<CTRL>(0.0f, A), b = dppf<CTRL>(1.0f, B);  // identity (0, 1) past the row edge
+    A = A + B * a;
+    B = B * b;
+}
+
+// Inclusive scan of affine maps D -> A + B*D over the wave, left to right: lane l ends
+// with the maps of lanes 0..l composed (lane 0 applied first).
 __device__ __forceinline__ void scan_affine_up(float& A, float& B, int lane) {
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const float a = __shfl_up(A, s, 64), b = __shfl_up(B, s, 64);
-        if (lane >= s) {
-            A = A + B * a;
-            B = B * b;
-        }
-    }
+    affine_step<0x111>(A, B);  // row_shr:1
+    affine_step<0x112>(A, B);
+    affine_step<0x114>(A, B);
+    affine_step<0x118>(A, B);
+    // rows 1..3 take the composition of the rows before them (row totals in lanes 15, 31, 47)
+    const float a0 = rdl(A, 15), b0 = rdl(B, 15), a1 = rdl(A, 31), b1 = rdl(B, 31), a2 = rdl(A, 47), b2 = rdl(B, 47);
+    const float e1a = a0, e1b = b0;
+    const float e2a = a1 + b1 * e1a, e2b = b1 * e1b;
+    const float e3a = a2 + b2 * e2a, e3b = b2 * e2b;
+    const int row = lane >> 4;
+    const float ea = row == 0 ? 0.0f : row == 1 ? e1a : row == 2 ? e2a : e3a;
+    const float eb = row == 0 ? 1.0f : row == 1 ? e1b : row == 2 ? e2b : e3b;
+    A = A + B * ea;
+    B = B * eb;
 }
 // The same, right to left: lane l ends with lanes l..63 composed (lane 63 applied first).
 __device__ __forceinline__ void scan_affine_down(float& A, float& B, int lane) {
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const float a = __shfl_down(A, s, 64), b = __shfl_down(B, s, 64);
-        if (lane + s < 64) {
-            A = A + B * a;
-            B = B * b;
-        }
-    }
+    affine_step<0x101>(A, B);  // row_shl:1
+    affine_step<0x102>(A, B);
+    affine_step<0x104>(A, B);
+    affine_step<0x108>(A, B);
+    // rows 0..2 take the composition of the rows after them (row totals in lanes 16, 32, 48)
+    const float a1 = rdl(A, 16), b1 = rdl(B, 16), a2 = rdl(A, 32), b2 = rdl(B, 32), a3 = rdl(A, 48), b3 = rdl(B, 48);
+    const float f3a = a3, f3b = b3;
+    const float f2a = a2 + b2 * f3a, f2b = b2 * f3b;
+    const float f1a = a1 + b1 * f2a, f1b = b1 * f2b;
+    const int row = lane >> 4;
+    const float ea = row == 3 ? 0.0f : row == 2 ? f3a : row == 1 ? f2a : f1a;
+    const float eb = row == 3 ? 1.0f : row == 2 ? f3b : row == 1 ? f2b : f1b;
+    A = A + B * ea;
+    B = B * eb;
 }
-// Inclusive scan of the gated maxima od -> max(C, P ? od : -inf), left to right.
-__device__ __forceinline__ void scan_gmax_up(float& C, int& P, int lane) {
+
+template <int CTRL>
+__device__ __forceinline__ void gmax_step(float& C, float& P) {
+    const float c = dppf<CTRL>(ninf(), C), p = dppf<CTRL>(1.0f, P);  // identity (-inf, pass)
+    C = fmaxf(C, P != 0.0f ? c : ninf());
+    P = P * p;
+}
+// Inclusive scan of the gated maxima od -> max(C, P ? od : -inf) (P = 1 or 0), left to
+// right.
+__device__ __forceinline__ void scan_gmax_up(float& C, float& P, int lane) {
+    gmax_step<0x111>(C, P);
+    gmax_step<0x112>(C, P);
+    gmax_step<0x114>(C, P);
+    gmax_step<0x118>(C, P);
+    const float c0 = rdl(C, 15), p0 = rdl(P, 15), c1 = rdl(C, 31), p1 = rdl(P, 31), c2 = rdl(C, 47), p2 = rdl(P, 47);
+    const float e1c = c0, e1p = p0;
+    const float e2c = fmaxf(c1, p1 != 0.0f ? e1c : ninf()), e2p = p1 * e1p;
+    const float e3c = fmaxf(c2, p2 != 0.0f ? e2c : ninf()), e3p = p2 * e2p;
+    const int row = lane >> 4;
+    const float ec = row == 0 ? ninf() : row == 1 ? e1c : row == 2 ? e2c : e3c;
+    const float ep = row == 0 ? 1.0f : row == 1 ? e1p : row == 2 ? e2p : e3p;
+    C = fmaxf(C, P != 0.0f ? ec : ninf());
+    P = P * ep;
+}
+
+// dst[k] = src[k] for k = 1..M, lane-strided (lane l copies k = 1 + l + 64u, the positions
+// the chunk loops give it), eight loads in flight at a time
+__device__ __forceinline__ void stage_row(float* dst, const float* __restrict__ src, int M, int lane) {
+    int k = lane + 1;
+    for (; k + 7 * 64 <= M; k += 8 * 64) {
+        float v[8];
 #pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const float c = __shfl_up(C, s, 64);
-        const int p = __shfl_up(P, s, 64);
-        if (lane >= s) {
-            C = fmaxf(C, P ? c : ninf());
-            P = P & p;
-        }
+        for (int u = 0; u < 8; u++) v[u] = src[k + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) dst[k + 64 * u] = v[u];
     }
+    for (; k <= M; k += 64) dst[k] = src[k];
 }
 
 __device__ __forceinline__ float lnf(float x) { return x > 0.0f ? logf(x) : ninf(); }
@@ -104,11 +174,15 @@ __device__ __forceinline__ float dis(float t, float v) { return t == 0.0f ? ninf
 
 using namespace sxe;
 
-// Dynamic LDS: 7 x (M+1) floats (previous and current M, I, D rows; null2 column sums).
-extern "C" __global__ void __launch_bounds__(64)
-sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
-            double* __restrict__ dws, SxEnvRes* __restrict__ res) {
-    extern __shared__ float lds[];
+// LDS, in rows of M+1 floats: previous and current M, I, D rows, the null2 column sums and
+// the staged emission row (8 rows); with TL also the profile's eight transition rows and
+// the four Forward / Backward matrix rows the decoding pass reads (20 rows).  Each row's
+// global operands are staged by a copy loop with several loads in flight, so the chunk
+// loops (serial through their lane scans) wait on LDS, not on memory.
+template <bool TL>
+__device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, const SxProfiles& P,
+                                             const SxTargets& T, float* __restrict__ ws, double* __restrict__ dws,
+                                             SxEnvRes* __restrict__ res, float* lds) {
     const SxEnvJob job = jobs[blockIdx.x];
     const int lane = (int)threadIdx.x;
     const int M = P.m[job.prof];
@@ -117,7 +191,13 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
     const int R = Lr + 1;
     const int nch = (M + 63) >> 6;
     const float* __restrict__ mo = P.mo + P.off[job.prof];
-    const float* __restrict__ tp = P.tp + P.toff[job.prof];
+    const float* __restrict__ tg = P.tp + P.toff[job.prof];
+    if constexpr (TL) {
+        float* tl = lds + 8 * W;
+        for (int k = lane; k < 8 * W; k += 64) tl[k] = tg[k];
+        __syncthreads();
+    }
+    const float* __restrict__ tp = TL ? lds + 8 * W : tg;
     const float* __restrict__ pmm = tp + SX_TMM * W;
     const float* __restrict__ pmi = tp + SX_TMI * W;
     const float* __restrict__ pmd = tp + SX_TMD * W;
@@ -163,6 +243,11 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
     float* cI = lds + 4 * W;
     float* cD = lds + 5 * W;
     float* sm = lds + 6 * W;
+    float* eL = lds + 7 * W;  // staged emission row
+    float* sFM = lds + 16 * W;  // with TL: staged Forward / Backward rows of the decoding pass
+    float* sFI = lds + 17 * W;
+    float* sBM = lds + 18 * W;
+    float* sBI = lds + 19 * W;
 
     SxEnvRes out;
     out.envsc = ninf();
@@ -189,7 +274,9 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
     }
     for (int r = 1; r <= Lr; r++) {
         const int x = sub[r - 1];
-        const float* __restrict__ eo = mo + (size_t)x * W;
+        stage_row(eL, mo + (size_t)x * W, M, lane);
+        __syncthreads();
+        const float* eo = eL;
         const float ins = x != SX_CODE_STAR ? 1.0f : 0.0f;
         float carry_m = 0.0f, carry_d = 0.0f, part = 0.0f;
         for (int c = 0; c < nch; c++) {
@@ -202,8 +289,7 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                 iv = k < M ? (aM[k] * pmi[k] + aI[k] * pii[k]) * ins : 0.0f;
                 B = pdd[k - 1];
             }
-            float mprev = __shfl_up(mv, 1, 64);
-            if (lane == 0) mprev = carry_m;
+            const float mprev = from_prev(mv, carry_m);
             if (valid) A = mprev * pmd[k - 1];
             scan_affine_up(A, B, lane);
             const float dv = A + B * carry_d;
@@ -213,8 +299,8 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                 cD[k] = dv;
                 part += mv + dv;
             }
-            carry_m = __shfl(mv, 63, 64);
-            carry_d = __shfl(dv, 63, 64);
+            carry_m = rdl(mv, 63);
+            carry_d = rdl(dv, 63);
         }
         float xe = wsum(part);
         xj = xj * ploop + xe * pe_loop;
@@ -288,8 +374,7 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
             }
             scan_affine_down(A, B, lane);
             const float dv = A + B * carry_d;
-            float dnext = __shfl_down(dv, 1, 64);
-            if (lane == 63) dnext = carry_d;
+            const float dnext = from_next(dv, carry_d);
             if (valid) {
                 const float mv = k == M ? xe : xe + dnext * pmd[k];
                 cM[k] = mv;
@@ -298,7 +383,7 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                 gM[k] = mv;
                 gI[k] = 0.0f;
             }
-            carry_d = __shfl(dv, 0, 64);
+            carry_d = rdl(dv, 0);
         }
         lnscale = 0.0;
         if (lane == 0) {
@@ -315,10 +400,12 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
         float sn1 = ninf();  // N of row 1 (row Lr's N is -inf)
         for (int r = Lr - 1; r >= 1; r--) {
             const int xnext = sub[r];
-            const float* __restrict__ eo = mo + (size_t)xnext * W;
+            stage_row(eL, mo + (size_t)xnext * W, M, lane);
+            __syncthreads();
+            const float* eo = eL;
             const float ins = xnext != SX_CODE_STAR ? 1.0f : 0.0f;
             float part = 0.0f;
-            for (int c = 0; c < nch; c++) {
+                for (int c = 0; c < nch; c++) {
                 const int k = (c << 6) + lane + 1;
                 if (k <= M) part += aM[k] * pbm[k] * eo[k];
             }
@@ -329,7 +416,7 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
             xn = xn * ploop + xbv * pmove;
             float pmx = ninf();
             carry_d = 0.0f;
-            for (int c = nch - 1; c >= 0; c--) {
+                for (int c = nch - 1; c >= 0; c--) {
                 const int k = (c << 6) + lane + 1;
                 const bool valid = k <= M;
                 float A = 0.0f, B = 1.0f, mnext = 0.0f, ivv = 0.0f;
@@ -346,8 +433,7 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                 }
                 scan_affine_down(A, B, lane);
                 const float dv = A + B * carry_d;
-                float dnext = __shfl_down(dv, 1, 64);
-                if (lane == 63) dnext = carry_d;
+                const float dnext = from_next(dv, carry_d);
                 if (valid) {
                     float mv, iv;
                     if (k < M) {
@@ -362,7 +448,7 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                     cD[k] = dv;
                     pmx = fmaxf(pmx, mv);
                 }
-                carry_d = __shfl(dv, 0, 64);
+                carry_d = rdl(dv, 0);
             }
             const float rowmax = fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(xbv, xe), xn), xj), xc), wmax(pmx));
             float sc = 1.0f;
@@ -406,7 +492,9 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
             t = aD; aD = cD; cD = t;
         }
         // row 0: B from row 1 and residue 1; the score is N(0)
-        const float* __restrict__ eo = mo + (size_t)sub[0] * W;
+        stage_row(eL, mo + (size_t)sub[0] * W, M, lane);
+        __syncthreads();
+        const float* eo = eL;
         float part = 0.0f;
         for (int c = 0; c < nch; c++) {
             const int k = (c << 6) + lane + 1;
@@ -454,17 +542,30 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
         float* gM = FM + (size_t)r * W;
         float* gI = FI + (size_t)r * W;
         float* gD = BM + (size_t)r * W;
+        const float* fM = gM;
+        const float* fI = gI;
         const float* hM = BM + (size_t)r * W;
         const float* hI = BI + (size_t)r * W;
+        if constexpr (TL) {
+            stage_row(sFM, gM, M, lane);
+            stage_row(sFI, gI, M, lane);
+            stage_row(sBM, hM, M, lane);
+            stage_row(sBI, hI, M, lane);
+            __syncthreads();
+            fM = sFM;
+            fI = sFI;
+            hM = sBM;
+            hI = sBI;
+        }
         float carry_m = ninf(), carry_d = ninf(), emax = ninf();
         for (int c = 0; c < nch; c++) {
             const int k = (c << 6) + lane + 1;
             const bool valid = k <= M;
             float mv = ninf(), ov = ninf(), C = ninf();
-            int Pg = 1;
+            float Pg = 1.0f;
             if (valid) {
-                const float ppm = (float)((double)gM[k] * (double)hM[k] * fac);
-                const float ppi = k < M ? (float)((double)gI[k] * (double)hI[k] * fac) : 0.0f;
+                const float ppm = (float)((double)fM[k] * (double)hM[k] * fac);
+                const float ppi = k < M ? (float)((double)fI[k] * (double)hI[k] * fac) : 0.0f;
                 sm[k] += ppm;
                 si += ppi;
                 float sv = gate(pbm[k], oxb_p);
@@ -479,19 +580,18 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                 ov = fmaxf(gate(pmi[k], aM[k]), gate(pii[k], aI[k])) + ppi;
             }
             // od_1 = -inf; od_k = max(gate(MD, om_k-1), gate(DD, od_k-1)) for k >= 2
-            float mprev = __shfl_up(mv, 1, 64);
-            if (lane == 0) mprev = carry_m;
+            const float mprev = from_prev(mv, carry_m);
             if (valid && k >= 2) {
                 const bool pass = pdd[k - 1] > 0.0f;
                 const float g = gate(pmd[k - 1], mprev);
                 C = pass ? g : fmaxf(g, 0.0f);
-                Pg = pass ? 1 : 0;
+                Pg = pass ? 1.0f : 0.0f;
             } else if (valid) {
                 C = ninf();
-                Pg = 0;
+                Pg = 0.0f;
             }
             scan_gmax_up(C, Pg, lane);
-            const float dv = fmaxf(C, Pg ? carry_d : ninf());
+            const float dv = fmaxf(C, Pg != 0.0f ? carry_d : ninf());
             if (valid) {
                 cM[k] = mv;
                 cI[k] = ov;
@@ -501,8 +601,8 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
                 gD[k] = dv;
                 emax = fmaxf(emax, fmaxf(mv, dv));
             }
-            carry_m = __shfl(mv, 63, 64);
-            carry_d = __shfl(dv, 63, 64);
+            carry_m = rdl(mv, 63);
+            carry_d = rdl(dv, 63);
         }
         const float xe = wmax(emax);
         const float xjv = fmaxf(ploop == 0.0f ? 0.0f : oxj_p + pj, pe_loop == 0.0f ? 0.0f : xe);
@@ -665,4 +765,19 @@ sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float*
         out.mali = lk;
     }
     if (lane == 0) res[blockIdx.x] = out;
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
+            double* __restrict__ dws, SxEnvRes* __restrict__ res) {
+    extern __shared__ float lds[];
+    rescore_body<false>(jobs, P, T, ws, dws, res, lds);
+}
+
+// transitions and the decoding pass's rows staged in LDS (20 x (M+1) floats of LDS)
+extern "C" __global__ void __launch_bounds__(64)
+sxe_rescore_tl(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
+               double* __restrict__ dws, SxEnvRes* __restrict__ res) {
+    extern __shared__ float lds[];
+    rescore_body<true>(jobs, P, T, ws, dws, res, lds);
 }

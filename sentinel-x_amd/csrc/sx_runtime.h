@@ -167,6 +167,8 @@ struct Ctx {
     // launches
     unsigned long long* d_heads = nullptr;
     DeviceBuf scratch_out, scratch_tids, scratch_tiles, scratch_roff, scratch_x5, scratch_ls;
+    // envelope rescoring: jobs, results and the matrices' workspace, kept across searches
+    DeviceBuf env_jobs, env_res, env_ws, env_dws;
     std::deque<DeviceBuf> pgroups_keep;
     std::vector<uint64_t> last_order;
     int persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const;

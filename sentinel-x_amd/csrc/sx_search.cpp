@@ -25,6 +25,7 @@ __global__ void sxr_regions(int64_t, const SxRegionIn*, const float*, const doub
                             float*, float*, SxRegionRes*, SxRegion*, int32_t*, int32_t);
 __global__ void sxc_zero_i32(int32_t*, int64_t);
 __global__ void sxe_rescore(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
+__global__ void sxe_rescore_tl(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
 }
 
 namespace sx {
@@ -127,64 +128,104 @@ void run_region_tails(const std::vector<Profile>& profs, const std::vector<const
     });
 }
 
-// Device buffers of the envelope rescoring, kept across the chunks of one search.
-struct EnvBufs {
-    DeviceBuf jobs, res, ws, dws;
-};
-
 // Bytes of device workspace per launch of the envelope rescoring (matrices of 16 bytes a
 // cell); a single larger envelope gets a launch of its own, up to SXE_MAX_JOB_BYTES.
 constexpr size_t SXE_BATCH_BYTES = (size_t)4 << 30;
 constexpr size_t SXE_MAX_JOB_BYTES = (size_t)64 << 30;
+// LDS a launch may take for rows and transition rows (of the 160 KB of a CU)
+constexpr size_t SXE_TL_LDS = (size_t)156 * 1024;
 
 // Envelope rescoring on the device (sxe_rescore, one wave per envelope): jobs carry
 // profile, target, rows and the null2 flag; the length model and workspace offsets are
 // filled in here.  Results in job order.
-void device_rescore(Ctx& c, EnvBufs& B, std::vector<SxEnvJob>& jobs, std::vector<SxEnvRes>& out) {
+//
+// A launch's LDS is sized by its widest model (8 or 20 floats a model position), and an
+// envelope's time by its cells, so the jobs of a workspace batch are grouped by model
+// width (powers of two) into launches of their own, which run side by side on the side
+// streams, and each launch takes its largest envelopes first.
+void device_rescore(Ctx& c, std::vector<SxEnvJob>& jobs, std::vector<SxEnvRes>& out) {
     out.assign(jobs.size(), SxEnvRes{});
     if (jobs.empty()) return;
+    const size_t nj = jobs.size();
+    std::vector<size_t> W(nj), R(nj);
+    for (size_t b = 0; b < nj; b++) {
+        SxEnvJob& J = jobs[b];
+        if (J.prof < 0 || (size_t)J.prof >= c.profs.size() || J.target < 0 || (uint32_t)J.target >= c.nt)
+            throw SxError(SX_ERR_ARG, "envelope names an unknown profile or target");
+        const int L = c.tlen[(size_t)J.target];
+        if (J.i < 1 || J.j < J.i || J.j > L) throw SxError(SX_ERR_ARG, "envelope outside its target");
+        W[b] = (size_t)c.profs[(size_t)J.prof].m + 1;
+        R[b] = (size_t)(J.j - J.i) + 2;
+        if ((4 * R[b] * W[b] + 14 * R[b]) * 4 + 2 * R[b] * 8 > SXE_MAX_JOB_BYTES)
+            throw SxError(SX_ERR_ARG, "envelope too large for device rescoring");
+        const Xsc xs = Xsc::make((size_t)L, false);
+        J.lloop = xs.n_loop;
+        J.lmove = xs.n_move;
+        J.ploop = xs.pn_loop;
+        J.pmove = xs.pn_move;
+        J.pad = 0;
+    }
+    auto bucket = [&](size_t b) {
+        int e = 6;
+        while (((size_t)1 << e) < W[b]) e++;
+        return e;
+    };
+    // launch order: by width bucket, then most cells first
+    std::vector<size_t> order(nj);
+    for (size_t b = 0; b < nj; b++) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        const int bx = bucket(x), by = bucket(y);
+        if (bx != by) return bx < by;
+        return R[x] * W[x] > R[y] * W[y];
+    });
+    std::vector<SxEnvJob> batch;
+    std::vector<SxEnvRes> bres;
     size_t a = 0;
-    while (a < jobs.size()) {
-        size_t nf = 0, nd = 0;
-        int maxw = 1;
-        size_t b = a;
-        for (; b < jobs.size(); b++) {
-            SxEnvJob& J = jobs[b];
-            if (J.prof < 0 || (size_t)J.prof >= c.profs.size() || J.target < 0 || (uint32_t)J.target >= c.nt)
-                throw SxError(SX_ERR_ARG, "envelope names an unknown profile or target");
-            const int L = c.tlen[(size_t)J.target];
-            if (J.i < 1 || J.j < J.i || J.j > L) throw SxError(SX_ERR_ARG, "envelope outside its target");
-            const size_t W = (size_t)c.profs[(size_t)J.prof].m + 1, R = (size_t)(J.j - J.i) + 2;
-            const size_t f = 4 * R * W + 14 * R, d = 2 * R;
-            if (f * 4 + d * 8 > SXE_MAX_JOB_BYTES) throw SxError(SX_ERR_ARG, "envelope too large for device rescoring");
+    while (a < nj) {
+        size_t nf = 0, nd = 0, b = a;
+        batch.clear();
+        for (; b < nj; b++) {
+            const size_t o = order[b];
+            const size_t f = 4 * R[o] * W[o] + 14 * R[o], d = 2 * R[o];
             if (b > a && (nf + f) * 4 + (nd + d) * 8 > SXE_BATCH_BYTES) break;
-            const Xsc xs = Xsc::make((size_t)L, false);
-            J.lloop = xs.n_loop;
-            J.lmove = xs.n_move;
-            J.ploop = xs.pn_loop;
-            J.pmove = xs.pn_move;
-            J.pad = 0;
+            SxEnvJob J = jobs[o];
             J.wsoff = (int64_t)nf;
             J.dwsoff = (int64_t)nd;
             nf += f;
             nd += d;
-            maxw = std::max(maxw, (int)W);
+            batch.push_back(J);
         }
         const size_t n = b - a;
-        auto* djobs = (SxEnvJob*)B.jobs.ensure(n * sizeof(SxEnvJob));
-        auto* dres = (SxEnvRes*)B.res.ensure(n * sizeof(SxEnvRes));
-        float* dws = (float*)B.ws.ensure(nf * sizeof(float));
-        double* ddws = (double*)B.dws.ensure(nd * sizeof(double));
-        HIPCHK(hipMemcpyAsync(djobs, jobs.data() + a, n * sizeof(SxEnvJob), hipMemcpyHostToDevice, c.stream));
-        const size_t lds = (size_t)7 * maxw * sizeof(float);
-        if (lds > 64 * 1024)
-            HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&sxe_rescore),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(sxe_rescore, dim3((unsigned)n), dim3(64), lds, c.stream, (const SxEnvJob*)djobs, c.dprof(),
-                           c.dtarg(), dws, ddws, dres);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out.data() + a, dres, n * sizeof(SxEnvRes), hipMemcpyDeviceToHost, c.stream));
+        auto* djobs = (SxEnvJob*)c.env_jobs.ensure(n * sizeof(SxEnvJob));
+        auto* dres = (SxEnvRes*)c.env_res.ensure(n * sizeof(SxEnvRes));
+        float* dws = (float*)c.env_ws.ensure(nf * sizeof(float));
+        double* ddws = (double*)c.env_dws.ensure(nd * sizeof(double));
+        HIPCHK(hipMemcpyAsync(djobs, batch.data(), n * sizeof(SxEnvJob), hipMemcpyHostToDevice, c.stream));
+        SxProfiles P = c.dprof();
+        SxTargets T = c.dtarg();
+        Fanout fo(c, c.stream);
+        for (size_t g0 = 0; g0 < n;) {
+            const int bk = bucket(order[a + g0]);
+            size_t g1 = g0;
+            size_t maxw = 1;
+            while (g1 < n && bucket(order[a + g1]) == bk) maxw = std::max(maxw, W[order[a + g1++]]);
+            // the transition rows go to LDS too when the widest model of the launch lets them
+            const bool tl = 20 * maxw * sizeof(float) <= SXE_TL_LDS;
+            const size_t lds = (tl ? 20 : 8) * maxw * sizeof(float);
+            const void* fn = tl ? reinterpret_cast<const void*>(&sxe_rescore_tl)
+                                : reinterpret_cast<const void*>(&sxe_rescore);
+            if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const SxEnvJob* jp = djobs + g0;
+            SxEnvRes* rp = dres + g0;
+            void* args[] = {&jp, &P, &T, &dws, &ddws, &rp};
+            HIPCHK(hipLaunchKernel(fn, dim3((unsigned)(g1 - g0)), dim3(64), args, lds, fo.next()));
+            g0 = g1;
+        }
+        fo.join();
+        bres.resize(n);
+        HIPCHK(hipMemcpyAsync(bres.data(), dres, n * sizeof(SxEnvRes), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
+        for (size_t q = 0; q < n; q++) out[order[a + q]] = bres[q];
         a = b;
     }
 }
@@ -204,7 +245,7 @@ DeviceEnvelope to_host(const SxEnvRes& r) {
 // clustering), then every envelope (the single-domain regions and the ensembles'
 // envelopes) is rescored on the device in one batch, and the host finishes each pair
 // (null2 correction, Viterbi-trace fallback, scores).  *ms_dev = the device part.
-void run_region_tails_device(Ctx& c, EnvBufs& eb, const std::vector<Profile>& profs,
+void run_region_tails_device(Ctx& c, const std::vector<Profile>& profs,
                              const std::vector<const uint8_t*>& seq, const std::vector<int>& len, const uint32_t* prof,
                              const uint32_t* tgt, uint64_t n, const float* fwd, const std::vector<TailRegions>& regs,
                              bool null2, int threads, std::vector<RawHit>& hit, std::vector<char>& ok, double* ms_dev,
@@ -257,7 +298,7 @@ void run_region_tails_device(Ctx& c, EnvBufs& eb, const std::vector<Profile>& pr
     std::vector<SxEnvRes> er;
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
-    device_rescore(c, eb, jobs, er);
+    device_rescore(c, jobs, er);
     *ms_dev += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     *n_env += jobs.size();
     parallel_pairs(n, threads, hit, ok, [&](uint64_t i, TailWork&, RawHit& out) {
@@ -471,7 +512,6 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
     std::vector<RawHit> all_hit;
     std::vector<char> all_ok;
     RegionBufs bufs;
-    EnvBufs ebufs;
     while (a < pairs.size()) {
         cp.clear();
         ct.clear();
@@ -500,7 +540,7 @@ void search(Ctx& c, const sx_search_cfg& cfg, sx_results& res) {
                              ok);
         } else {
             double ms_dev = 0.0;
-            run_region_tails_device(c, ebufs, c.profs, seq, len, cp.data(), ct.data(), n, fw, regs, cfg.null2 != 0,
+            run_region_tails_device(c, c.profs, seq, len, cp.data(), ct.data(), n, fw, regs, cfg.null2 != 0,
                                     cfg.threads, hit, ok, &ms_dev, &res.stats.n_envelopes);
             res.stats.ms_rescore += ms_dev;
             res.stats.ms_tail -= ms_dev;
@@ -559,9 +599,8 @@ void rescore_envelopes(Ctx& c, const sx_envelope* env, uint64_t n, bool on_host,
     std::vector<SxEnvJob> jobs(n);
     for (uint64_t e = 0; e < n; e++)
         jobs[e] = {(int32_t)env[e].profile, (int32_t)env[e].target, env[e].i, env[e].j, env[e].null2 ? 1 : 0};
-    EnvBufs eb;
     std::vector<SxEnvRes> er;
-    device_rescore(c, eb, jobs, er);
+    device_rescore(c, jobs, er);
     for (uint64_t e = 0; e < n; e++) put(e, to_host(er[e]), er[e].bcksc);
 }
 

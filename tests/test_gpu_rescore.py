@@ -41,13 +41,15 @@ def fixture34(orc):
 
 @pytest.fixture(scope="module")
 def marker(orc):
-    """12 synthetic models spread over 80-1,500 positions and 300 proteins, 20% of them
-    emitted from the models (with flanks), two with '*' residues planted in the homologue."""
+    """13 synthetic models spread over 80-2,200 positions and 300 proteins, 20% of them
+    emitted from the models (with flanks), two with '*' residues planted in the homologue.
+    Models up to 1,995 positions run the kernel form with the transition rows in LDS, the
+    2,200-position model the form that reads them through the caches."""
     from conftest import load_package
     load_package()
     from sentinel_x_amd import synth
     rng = np.random.default_rng(2024)
-    ms = [80, 120, 180, 250, 330, 420, 520, 700, 900, 1100, 1300, 1500]
+    ms = [80, 120, 180, 250, 330, 420, 520, 700, 900, 1100, 1300, 1500, 2200]
     hmms = [synth.make_profile(rng, m, f"env{m}") for m in ms]
     lens = synth._lognormal_int(rng, 280, 0.5, 50, 2000, 300)
     # homologues are emitted from a local path with flanks, so lengthen those of the
@@ -133,7 +135,7 @@ def test_rescore_marker_envelopes_long_models_and_stars(orc, sxa, gctx, marker):
     gctx.set_targets(seqs)
     env = envelope_set(hits, doms, seqs, np.random.default_rng(11), n_random=60)
     ms = {int(hits["profile"][i]): hmms[int(hits["profile"][i])].m for i in range(len(hits))}
-    assert max(ms.values()) > 1024, "the slice has no long model among its hits"
+    assert max(ms.values()) > 1995, "the slice has no hit of the longest model"
     check(orc, sxa, gctx, [orc.configure(h) for h in hmms], seqs, env)
 
 
