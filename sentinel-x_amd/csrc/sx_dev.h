@@ -122,6 +122,9 @@ struct SxWork {
     int32_t ilv;              // implicit form over the length-sorted order with chunks a
                               // multiple of 64: one-lane groups may read SxTargets::ilv
     int64_t out_stride;
+    int32_t chunk_major;      // implicit form: tile t -> chunk t / nprofiles, profile t % nprofiles
+                              // (every profile's longest targets first), else profile-major
+    int32_t pad0;
     unsigned long long* head; // persistent-queue head; nullptr -> tile = blockIdx.x (+ base)
     int64_t tile_base;
     // Data-dependent tile range (device cascade): when set, the launch covers explicit

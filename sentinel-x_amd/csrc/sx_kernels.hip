@@ -1042,9 +1042,11 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
                                            const uint8_t* seq, int L, float ploop, float pmove, float* x5,
                                            double* lsrow, bool writer) {
     constexpr bool HOIST = TM == TM_REG;
-    // long models: the next emission row is prefetched into registers up to Q = 32 (beyond,
-    // the second copy of the row would leave one wave per SIMD)
-    constexpr bool GEM = TM == TM_GEM && Q <= 32;
+    // long models read their emission rows from global memory, one register a position: a
+    // cell's register is refilled with the next row's value as soon as this row has used
+    // it, so the load has a whole row to arrive (up to 40 positions a lane, 32 for the
+    // special-row writers; wider lanes read the row inline)
+    constexpr bool GEM = TM == TM_GEM && Q <= (SPEC ? 32 : 40);
     using FR = FwdRec<Q, G>;
     const int j = (int)(threadIdx.x & (G - 1));
     const EmRows<Q, G, TM> er(eot, ew, j);
@@ -1070,13 +1072,12 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
     FwdStream rs;
     rs.init(seq);
     int x = rs.next();
-    // long models read their emission rows from global memory: the next row's values are
-    // loaded while this row is computed (the residue stream is known ahead)
-    float eon[GEM ? Q : 1];
+    // long models: this row's emission values, refilled with the next row's as they are used
+    float eb[GEM ? Q : 1];
     if constexpr (GEM) {
         const float* __restrict__ e0 = er.row(x);
 #pragma unroll
-        for (int q = 0; q < Q; q++) eon[q] = e0[q];
+        for (int q = 0; q < Q; q++) eb[q] = e0[q];
     }
     bool star_prev = false;
     for (int i = 0; i < L; i++) {
@@ -1096,19 +1097,12 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
         }
         star_prev = x == SX_CODE_STAR;
         const float* __restrict__ eo = er.row(x);
-        float eoc[GEM ? Q : 1];
         // the next residue, read without a branch: past the last row the stream reads the
         // target's padding, and row 0 is then loaded instead (its values go unused)
         const int xs = rs.next();
         x = i + 1 < L ? xs : 0;
-        if constexpr (GEM) {
-            const float* __restrict__ en = er.row(x);
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                eoc[q] = eon[q];
-                eon[q] = en[q];
-            }
-        }
+        const float* __restrict__ en = er.row(x);
+        if constexpr (!GEM) (void)en;
         const int ro = HOIST ? 0 : opaque(0);
         const float4* __restrict__ rec = reinterpret_cast<const float4*>(trec + j * FR::RSTRIDE + ro);
         // the products PC[q]: read from LDS every row where the tables are (held in
@@ -1135,7 +1129,13 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
             s = fmaf(mpo, ta.x, s);
             s = fmaf(ipo, ta.y, s);
             s = fmaf(dpo, ta.z, s);
-            const float mv = s * (GEM ? eoc[q] : eo[q]);
+            float mv;
+            if constexpr (GEM) {
+                mv = s * eb[q];
+                eb[q] = en[q];
+            } else {
+                mv = s * eo[q];
+            }
             ir[q] = fmaf(mo, tb.x, io * tb.y);
             mr[q] = mv;
             dl[q] = dnext;
@@ -1414,8 +1414,16 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
         ti.out = tl.out;
         ti.by_tid = false;
     } else {
-        const int64_t pi = t / w.nchunks;
-        const int c = (int)(t - pi * w.nchunks);
+        int64_t pi;
+        int c;
+        if (w.chunk_major) {
+            const int64_t np = w.ntiles / w.nchunks;
+            c = (int)(t / np);
+            pi = t - (int64_t)c * np;
+        } else {
+            pi = t / w.nchunks;
+            c = (int)(t - pi * w.nchunks);
+        }
         ti.prof = w.profs[pi];
         ti.first = (int64_t)c * w.chunk;
         const int rem = w.ntids - (int)ti.first;
@@ -1428,13 +1436,26 @@ __device__ __forceinline__ TileInfo tile_of(const SxWork& w, int64_t t) {
 
 // Waves per SIMD the compiler must leave room for: the Forward kernels with transitions in
 // registers need two (at one they leave the SIMD idle through every dependent step).
-template <int STAGE, int TM>
+// So do the long-model Forward kernels up to 40 positions a lane (at most 254 VGPRs, no
+// spills); the special-row writers and the wider ones would spill at two.
+template <int Q, int STAGE, bool SPEC, int TM>
 constexpr int k_min_waves() {
-    return STAGE == ST_FWD && TM == TM_REG ? 2 : 1;
+    return STAGE == ST_FWD && (TM == TM_REG || (TM == TM_GEM && !SPEC && Q <= 40)) ? 2 : 1;
+}
+
+// Threads per workgroup: the table mode's, except the long-model Forward and Backward
+// scorers of 40 and 48 positions a lane.  Their staged transition records (92-111 KB)
+// leave room for one workgroup per CU, so a 256-thread workgroup would run one wave per
+// SIMD; 512 threads (eight pairs of one profile) share the table and fill two.
+template <int Q, int STAGE, bool SPEC, int TM>
+constexpr int k_block() {
+    return TM == TM_GEM && !SPEC && ((STAGE == ST_FWD && (Q == 40 || Q == 48)) || (STAGE == ST_BWD && Q == 40))
+               ? 512
+               : tm_block<TM>();
 }
 
 template <int Q, int G, int STAGE, bool SPEC, int TM>
-__global__ void __launch_bounds__(tm_block<TM>(), (k_min_waves<STAGE, TM>()))
+__global__ void __launch_bounds__((k_block<Q, STAGE, SPEC, TM>()), (k_min_waves<Q, STAGE, SPEC, TM>()))
     k_tiles(SxProfiles P, SxTargets T, SxWork W, void* out, SxSpecialsOut SP) {
     static_assert(STAGE == ST_MSV || STAGE == ST_VIT || STAGE == ST_MSVQ || STAGE == ST_VITQ || !tm_aligned<TM>(),
                   "aligned lane blocks are an MSV and Viterbi layout");
@@ -1706,7 +1727,7 @@ struct SxKernelEntry {
 };
 
 #define SXK_E(ST, Q, G, SP, H) \
-    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::k_wid<Q, G, ST, (int)(H)>(), sxk::tm_block<(int)(H)>(), \
+    {ST, Q, G, (SP) ? 1 : 0, (int)(H), sxk::k_wid<Q, G, ST, (int)(H)>(), sxk::k_block<Q, ST, SP, (int)(H)>(), \
      4 * sxk::lds_floats<Q, G, ST, (int)(H)>(), \
      reinterpret_cast<const void*>(&sxk::k_tiles<Q, G, ST, SP, (int)(H)>)}
 

@@ -32,13 +32,14 @@ def main():
     ap.add_argument("--profiles", type=int, default=100)
     ap.add_argument("--targets", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--len", type=int, default=300, help="median target length (log-normal, sigma 0.5)")
     ap.add_argument("--shapes", nargs="+", required=True, help="q,g,mode triples")
     a = ap.parse_args()
     sxa = load_package()
     from sentinel_x_amd import synth
     rng = np.random.default_rng(a.seed)
     hmms = [synth.make_profile(rng, a.m, f"p{i}") for i in range(a.profiles)]
-    lens = np.clip(np.exp(rng.normal(np.log(300), 0.5, a.targets)), 50, 2000).astype(int)
+    lens = np.clip(np.exp(rng.normal(np.log(a.len), 0.5, a.targets)), 50, max(2000, 8 * a.len)).astype(int)
     seqs = [synth.background_seq(rng, int(n)) for n in lens]
     cells = a.m * a.profiles * int(lens.sum())
     print(f"# stage {a.stage} M {a.m}: {a.profiles} profiles x {a.targets} targets, {cells:.4g} cells", flush=True)
