@@ -578,12 +578,14 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         w.nchunks = (ntids + w.chunk - 1) / w.chunk;
         w.ntiles = (int64_t)w.nchunks * (int64_t)kv.second.size();
         w.out_stride = out_stride;
-        // Viterbi, Forward and Backward take the targets longest first across all profiles,
-        // so the launch does not end on one profile's longest targets; the MSV stages keep
-        // a profile's tiles together (their launches span thousands of profiles)
-        w.chunk_major = stage == SX_STAGE_MSV || stage == SX_STAGE_MSV16 ? 0 : 1;
         if (w.ntiles == 0) continue;
         const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
+        // A workgroup restages the profile tables when its next tile is another profile's.
+        // With fewer tiles per profile than workgroups that happens on nearly every tile
+        // anyway, so Viterbi, Forward and Backward then take the targets longest first
+        // across all profiles (the launch does not end on one profile's longest targets);
+        // otherwise, and for the MSV stages, a profile's tiles stay together.
+        w.chunk_major = stage != SX_STAGE_MSV && stage != SX_STAGE_MSV16 && w.nchunks < grid ? 1 : 0;
         launch(k, stage, w, dout, sp, grid, true, slot++ % 64, fo.next());
     }
     fo.join();
