@@ -413,6 +413,48 @@ def ingest_leg(sxa, synth, f, n_profiles=2000, batches=(256, 512, 0)):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def inclusion_parity(sxa, wl, f, threads):
+    """The whole search (filters on) on the device against the reference's path on the
+    CPU, on every pair of the workload: the oracle's cascade over all pairs, its Forward and
+    Backward special rows for the pairs it passes, and the host tail (sx_search_host) from
+    those rows.  Compared per cutoff (E <= 10, the models' GA, NC and TC): the hits, their
+    domains' coordinates and reporting flags, and the scores (search.rs:131-210 decides
+    inclusion from them)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker
+    hm = [orc.Hmm(name=h.name, m=h.m, mat=h.mat, t=h.t, compo=h.compo, stats=h.stats, ga=h.ga, tc=h.tc, nc=h.nc)
+          for h in wl.hmms]
+    profs = [orc.configure(h) for h in hm]
+    t0 = time.perf_counter()
+    stage_o, _ = orc.Batch(profs, wl.seqs).cascade(**f, nthreads=threads)
+    idx = np.flatnonzero(stage_o == 4)
+    pp, tt = (idx // len(wl.seqs)).astype(np.uint32), (idx % len(wl.seqs)).astype(np.uint32)
+    fwd, bwd, fsp, bsp = [], [], [], []
+    for p, t in zip(pp, tt):
+        v, sp = orc.forward(profs[p], wl.seqs[t], specials=True)
+        b, spb = orc.backward(profs[p], wl.seqs[t])
+        fwd.append(v)
+        fsp.append(sp)
+        bwd.append(float(b))
+        bsp.append(spb)
+    out = {"pairs": len(profs) * len(wl.seqs), "pairs_in_tail_cpu": int(len(idx)),
+           "oracle_s": round(time.perf_counter() - t0, 2)}
+    coords = ("hmm_from", "hmm_to", "ali_from", "ali_to", "env_from", "env_to", "reported")
+    with sxa.Context(0) as c:
+        c.set_profiles(wl.hmms)
+        c.set_targets(wl.seqs)
+        for cut in ("evalue", "ga", "nc", "tc"):
+            kw = dict(cutoff=cut, seq_e=10.0, dom_e=10.0, filters=True, **f)
+            h, d = c.search(**kw)
+            hh, dh = sxa.search_host(hm, wl.seqs, pp, tt, fwd, bwd, fsp, bsp, **kw)
+            same = bool(np.array_equal(h[["profile", "target", "ndom"]], hh[["profile", "target", "ndom"]]) and
+                        all(np.array_equal(d[k], dh[k]) for k in coords))
+            out[cut] = {"hits": int(len(h)), "domains": int(len(d)), "identical_hits_domains_and_flags": same,
+                        "max_score_diff_bits": float(np.abs(h["score"] - hh["score"]).max()) if same and len(h)
+                        else None}
+    return out
+
+
 def forward_error(wl, fwd_gpu, k, threads, z=None):
     """Device Forward (f32 score, as the reference returns it) against the oracle's FP64
     Forward on the first k profiles x all proteins: nats error, and the inclusion decisions
@@ -653,6 +695,9 @@ def main():
             fwd_err["marker_set"]["sample"] = f"first {k1} profiles x all {len(wl1.seqs)} proteins"
             fwd_err["precision"]["sample"] = f"first {k3} profiles x all {len(wl3.seqs)} proteins of the slice"
             log(f"[bench] forward error vs FP64: {fwd_err}")
+            t0 = time.time()
+            fwd_err["inclusion_parity_marker_set"] = inclusion_parity(sxa, wl1, f, threads)
+            log(f"[bench] inclusion parity ({time.time() - t0:.1f} s): {fwd_err['inclusion_parity_marker_set']}")
 
     out = {
         "metric": "GCUPS of the MSV->bias->Viterbi->Forward cascade over all pairs, batched, 1 MI355X",
