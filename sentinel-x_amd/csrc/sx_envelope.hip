@@ -77,7 +77,7 @@ __device__ __forceinline__ float from_next(float v, float b) { return dppf<0x130
 template <int CTRL>
 __device__ __forceinline__ void affine_step(float& A, float& B) {
     const float a = dppf<CTRL>(0.0f, A), b = dppf<CTRL>(1.0f, B);  // identity (0, 1) past the row edge
-    A = A + B * a;
+    A = fmaf(B, a, A);
     B = B * b;
 }
 
@@ -91,12 +91,12 @@ __device__ __forceinline__ void scan_affine_up(float& A, float& B, int lane) {
     // rows 1..3 take the composition of the rows before them (row totals in lanes 15, 31, 47)
     const float a0 = rdl(A, 15), b0 = rdl(B, 15), a1 = rdl(A, 31), b1 = rdl(B, 31), a2 = rdl(A, 47), b2 = rdl(B, 47);
     const float e1a = a0, e1b = b0;
-    const float e2a = a1 + b1 * e1a, e2b = b1 * e1b;
-    const float e3a = a2 + b2 * e2a, e3b = b2 * e2b;
+    const float e2a = fmaf(b1, e1a, a1), e2b = b1 * e1b;
+    const float e3a = fmaf(b2, e2a, a2), e3b = b2 * e2b;
     const int row = lane >> 4;
     const float ea = row == 0 ? 0.0f : row == 1 ? e1a : row == 2 ? e2a : e3a;
     const float eb = row == 0 ? 1.0f : row == 1 ? e1b : row == 2 ? e2b : e3b;
-    A = A + B * ea;
+    A = fmaf(B, ea, A);
     B = B * eb;
 }
 // The same, right to left: lane l ends with lanes l..63 composed (lane 63 applied first).
@@ -108,12 +108,12 @@ __device__ __forceinline__ void scan_affine_down(float& A, float& B, int lane) {
     // rows 0..2 take the composition of the rows after them (row totals in lanes 16, 32, 48)
     const float a1 = rdl(A, 16), b1 = rdl(B, 16), a2 = rdl(A, 32), b2 = rdl(B, 32), a3 = rdl(A, 48), b3 = rdl(B, 48);
     const float f3a = a3, f3b = b3;
-    const float f2a = a2 + b2 * f3a, f2b = b2 * f3b;
-    const float f1a = a1 + b1 * f2a, f1b = b1 * f2b;
+    const float f2a = fmaf(b2, f3a, a2), f2b = b2 * f3b;
+    const float f1a = fmaf(b1, f2a, a1), f1b = b1 * f2b;
     const int row = lane >> 4;
     const float ea = row == 3 ? 0.0f : row == 2 ? f3a : row == 1 ? f2a : f1a;
     const float eb = row == 3 ? 1.0f : row == 2 ? f3b : row == 1 ? f2b : f1b;
-    A = A + B * ea;
+    A = fmaf(B, ea, A);
     B = B * eb;
 }
 
@@ -155,6 +155,11 @@ __device__ __forceinline__ void stage_row(float* dst, const float* __restrict__ 
     for (; k <= M; k += 64) dst[k] = src[k];
 }
 
+// Orders one wave's LDS accesses across its lanes (a wave's LDS operations complete in
+// order; this keeps the compiler from moving them across the point either).  The passes
+// that one wave runs alone use it instead of a workgroup barrier.
+__device__ __forceinline__ void wsync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ float lnf(float x) { return x > 0.0f ? logf(x) : ninf(); }
 
 // dp.rs:32-40
@@ -179,12 +184,19 @@ using namespace sxe;
 // the four Forward / Backward matrix rows the decoding pass reads (20 rows).  Each row's
 // global operands are staged by a copy loop with several loads in flight, so the chunk
 // loops (serial through their lane scans) wait on LDS, not on memory.
-template <bool TL>
+//
+// PAR: a workgroup of two waves, the Forward pass on wave 0 and the Backward pass on wave 1
+// at the same time (they are independent), each with its own rows in LDS; after one
+// workgroup barrier wave 0 runs the decoding pass and the traceback alone.  LDS then holds
+// Forward rows (7 x (M+1) floats), Backward rows (7), with TL the transition rows (8), and
+// one exchange slot; the decoding pass reuses the first 11 rows.
+template <bool TL, bool PAR>
 __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, const SxProfiles& P,
                                              const SxTargets& T, float* __restrict__ ws, double* __restrict__ dws,
                                              SxEnvRes* __restrict__ res, float* lds) {
     const SxEnvJob job = jobs[blockIdx.x];
-    const int lane = (int)threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)(threadIdx.x >> 6);
     const int M = P.m[job.prof];
     const int W = M + 1;
     const int Lr = job.j - job.i + 1;
@@ -192,12 +204,14 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
     const int nch = (M + 63) >> 6;
     const float* __restrict__ mo = P.mo + P.off[job.prof];
     const float* __restrict__ tg = P.tp + P.toff[job.prof];
+    const int tof = PAR ? 14 * W : 8 * W;  // the staged transition rows (TL)
     if constexpr (TL) {
-        float* tl = lds + 8 * W;
-        for (int k = lane; k < 8 * W; k += 64) tl[k] = tg[k];
+        float* tl = lds + tof;
+        for (int k = (int)threadIdx.x; k < 8 * W; k += (int)blockDim.x) tl[k] = tg[k];
         __syncthreads();
     }
-    const float* __restrict__ tp = TL ? lds + 8 * W : tg;
+    float* xslot = lds + (TL ? tof + 8 * W : tof);  // PAR: the Backward score, wave 1 -> wave 0
+    const float* __restrict__ tp = TL ? lds + tof : tg;
     const float* __restrict__ pmm = tp + SX_TMM * W;
     const float* __restrict__ pmi = tp + SX_TMI * W;
     const float* __restrict__ pmd = tp + SX_TMD * W;
@@ -236,18 +250,20 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
     double* fls = dws + job.dwsoff;
     double* bls = fls + R;
 
-    float* aM = lds;
-    float* aI = lds + W;
-    float* aD = lds + 2 * W;
-    float* cM = lds + 3 * W;
-    float* cI = lds + 4 * W;
-    float* cD = lds + 5 * W;
-    float* sm = lds + 6 * W;
-    float* eL = lds + 7 * W;  // staged emission row
-    float* sFM = lds + 16 * W;  // with TL: staged Forward / Backward rows of the decoding pass
-    float* sFI = lds + 17 * W;
-    float* sBM = lds + 18 * W;
-    float* sBI = lds + 19 * W;
+    // the pass's rows: previous and current M, I, D, and its staged emission row
+    float* rb = PAR && wave == 1 ? lds + 7 * W : lds;
+    float* aM = rb;
+    float* aI = rb + W;
+    float* aD = rb + 2 * W;
+    float* cM = rb + 3 * W;
+    float* cI = rb + 4 * W;
+    float* cD = rb + 5 * W;
+    float* eL = rb + (PAR ? 6 * W : 7 * W);
+    float* sm = lds + 6 * W;  // decoding pass: null2 column sums
+    float* sFM = lds + (PAR ? 7 * W : 16 * W);  // with TL: staged Forward / Backward rows of the decoding pass
+    float* sFI = sFM + W;
+    float* sBM = sFM + 2 * W;
+    float* sBI = sFM + 3 * W;
 
     SxEnvRes out;
     out.envsc = ninf();
@@ -257,14 +273,14 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
     out.iali = out.jali = out.kali = out.mali = 0;
     for (int x = 0; x < 20; x++) out.null2[x] = 0.0f;
 
+    float xn = 1.0f, xb = pmove, xj = 0.0f, xc = 0.0f;
     // ---------------------------------------------------------------- Forward (dp.rs:443-522)
+    if (!PAR || wave == 0) {
     for (int k = lane; k < W; k += 64) {
         aM[k] = aI[k] = aD[k] = 0.0f;
         cM[k] = cI[k] = cD[k] = 0.0f;
-        sm[k] = 0.0f;
     }
-    __syncthreads();
-    float xn = 1.0f, xb = pmove, xj = 0.0f, xc = 0.0f;
+    wsync();
     double lnscale = 0.0;
     if (lane == 0) {
         fsn[0] = 0.0f;
@@ -275,7 +291,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
     for (int r = 1; r <= Lr; r++) {
         const int x = sub[r - 1];
         stage_row(eL, mo + (size_t)x * W, M, lane);
-        __syncthreads();
+        wsync();
         const float* eo = eL;
         const float ins = x != SX_CODE_STAR ? 1.0f : 0.0f;
         float carry_m = 0.0f, carry_d = 0.0f, part = 0.0f;
@@ -292,7 +308,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
             const float mprev = from_prev(mv, carry_m);
             if (valid) A = mprev * pmd[k - 1];
             scan_affine_up(A, B, lane);
-            const float dv = A + B * carry_d;
+            const float dv = fmaf(B, carry_d, A);
             if (valid) {
                 cM[k] = mv;
                 cI[k] = iv;
@@ -341,21 +357,24 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
             fsj[r] = lnf(xj) + ls;
             fsc[r] = lnf(xc) + ls;
         }
-        __syncthreads();
+        wsync();
         float* t;
         t = aM; aM = cM; cM = t;
         t = aI; aI = cI; cI = t;
         t = aD; aD = cD; cD = t;
     }
-    const float envsc = (float)((double)lnf(xc * pmove) + lnscale);
-    out.envsc = envsc;
-    if (!isfinite(envsc)) {
-        if (lane == 0) res[blockIdx.x] = out;
-        return;
+    out.envsc = (float)((double)lnf(xc * pmove) + lnscale);
+    }
+    if constexpr (!PAR) {
+        if (!isfinite(out.envsc)) {
+            if (lane == 0) res[blockIdx.x] = out;
+            return;
+        }
     }
 
     // ---------------------------------------------------------------- Backward (dp.rs:525-601)
-    {
+    if (!PAR || wave == 1) {
+        double lnscale = 0.0;
         xc = pmove;
         float xe = xc * pe_move;
         xj = 0.0f;
@@ -373,7 +392,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
                 B = k < M ? pdd[k] : 0.0f;
             }
             scan_affine_down(A, B, lane);
-            const float dv = A + B * carry_d;
+            const float dv = fmaf(B, carry_d, A);
             const float dnext = from_next(dv, carry_d);
             if (valid) {
                 const float mv = k == M ? xe : xe + dnext * pmd[k];
@@ -392,7 +411,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
             bsj[Lr] = ninf();
             bsc[Lr] = lnf(xc);
         }
-        __syncthreads();
+        wsync();
         float* t;
         t = aM; aM = cM; cM = t;
         t = aI; aI = cI; cI = t;
@@ -401,7 +420,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
         for (int r = Lr - 1; r >= 1; r--) {
             const int xnext = sub[r];
             stage_row(eL, mo + (size_t)xnext * W, M, lane);
-            __syncthreads();
+            wsync();
             const float* eo = eL;
             const float ins = xnext != SX_CODE_STAR ? 1.0f : 0.0f;
             float part = 0.0f;
@@ -432,7 +451,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
                     }
                 }
                 scan_affine_down(A, B, lane);
-                const float dv = A + B * carry_d;
+                const float dv = fmaf(B, carry_d, A);
                 const float dnext = from_next(dv, carry_d);
                 if (valid) {
                     float mv, iv;
@@ -486,14 +505,14 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
                 bsj[r] = lnf(xj) + ls;
                 bsc[r] = lnf(xc) + ls;
             }
-            __syncthreads();
+            wsync();
             t = aM; aM = cM; cM = t;
             t = aI; aI = cI; cI = t;
             t = aD; aD = cD; cD = t;
         }
         // row 0: B from row 1 and residue 1; the score is N(0)
         stage_row(eL, mo + (size_t)sub[0] * W, M, lane);
-        __syncthreads();
+        wsync();
         const float* eo = eL;
         float part = 0.0f;
         for (int c = 0; c < nch; c++) {
@@ -503,6 +522,20 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
         const float b0 = wsum(part);
         const float sb0 = lnf(b0) + (float)lnscale;
         out.bcksc = logsum(sn1 + lloop, sb0 + lmove);
+        if (PAR && lane == 0) xslot[0] = out.bcksc;
+    }
+    if constexpr (PAR) {
+        // the two passes' rows and scores in global memory and the exchange slot become
+        // visible to wave 0, which goes on alone
+        __threadfence();
+        __syncthreads();
+        if (wave == 1) return;
+        __threadfence();
+        out.bcksc = xslot[0];
+        if (!isfinite(out.envsc)) {
+            if (lane == 0) res[blockIdx.x] = out;
+            return;
+        }
     }
     if (!isfinite(out.bcksc)) {
         out.status = 2;
@@ -510,7 +543,14 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
         return;
     }
     out.status = 0;
-    __syncthreads();
+    // the decoding pass's rows: the first ones of LDS
+    aM = lds;
+    aI = lds + W;
+    aD = lds + 2 * W;
+    cM = lds + 3 * W;
+    cI = lds + 4 * W;
+    cD = lds + 5 * W;
+    wsync();
 
     // ---------------------------------------------------------------- posterior decoding,
     // null2 sums and the optimal-accuracy DP (dp.rs:1274-1306, 1310-1350, 1355-1460)
@@ -519,7 +559,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
         aM[k] = aI[k] = aD[k] = ninf();
         sm[k] = 0.0f;
     }
-    __syncthreads();
+    wsync();
     float si = 0.0f, sn = 0.0f, sj = 0.0f, scs = 0.0f;
     float oxn_p = 0.0f, oxb_p = 0.0f, oxj_p = ninf(), oxc_p = ninf();
     if (lane == 0) {
@@ -551,7 +591,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
             stage_row(sFI, gI, M, lane);
             stage_row(sBM, hM, M, lane);
             stage_row(sBI, hI, M, lane);
-            __syncthreads();
+            wsync();
             fM = sFM;
             fI = sFI;
             hM = sBM;
@@ -624,7 +664,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
         sn += pn;
         sj += pj;
         scs += pc;
-        __syncthreads();
+        wsync();
         float* t;
         t = aM; aM = cM; cM = t;
         t = aI; aI = cI; cI = t;
@@ -652,7 +692,7 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
     // the E state's choice of the row's best M or D cell is a wave reduction with the
     // reference's visiting order and tie rules.
     __threadfence();
-    __syncthreads();
+    wsync();
     enum { ST_C, ST_E, ST_M, ST_I, ST_D, ST_B, ST_J, ST_N };
     const int qs = max(2, (M - 1) / 4 + 1);
     int s0 = ST_C, i = Lr, k = 0;
@@ -767,17 +807,28 @@ __device__ __forceinline__ void rescore_body(const SxEnvJob* __restrict__ jobs, 
     if (lane == 0) res[blockIdx.x] = out;
 }
 
+// one wave per envelope, transitions through the caches (8 x (M+1) floats of LDS)
 extern "C" __global__ void __launch_bounds__(64)
 sxe_rescore(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
             double* __restrict__ dws, SxEnvRes* __restrict__ res) {
     extern __shared__ float lds[];
-    rescore_body<false>(jobs, P, T, ws, dws, res, lds);
+    rescore_body<false, false>(jobs, P, T, ws, dws, res, lds);
 }
 
-// transitions and the decoding pass's rows staged in LDS (20 x (M+1) floats of LDS)
-extern "C" __global__ void __launch_bounds__(64)
-sxe_rescore_tl(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
+// two waves per envelope (Forward and Backward side by side), transitions through the
+// caches (14 x (M+1) + 1 floats of LDS)
+extern "C" __global__ void __launch_bounds__(128)
+sxe_rescore_p2(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
                double* __restrict__ dws, SxEnvRes* __restrict__ res) {
     extern __shared__ float lds[];
-    rescore_body<true>(jobs, P, T, ws, dws, res, lds);
+    rescore_body<false, true>(jobs, P, T, ws, dws, res, lds);
+}
+
+// two waves per envelope, transitions and the decoding pass's rows staged in LDS
+// (22 x (M+1) + 1 floats of LDS)
+extern "C" __global__ void __launch_bounds__(128)
+sxe_rescore_p2t(const SxEnvJob* __restrict__ jobs, SxProfiles P, SxTargets T, float* __restrict__ ws,
+                double* __restrict__ dws, SxEnvRes* __restrict__ res) {
+    extern __shared__ float lds[];
+    rescore_body<true, true>(jobs, P, T, ws, dws, res, lds);
 }

@@ -25,7 +25,8 @@ __global__ void sxr_regions(int64_t, const SxRegionIn*, const float*, const doub
                             float*, float*, SxRegionRes*, SxRegion*, int32_t*, int32_t);
 __global__ void sxc_zero_i32(int32_t*, int64_t);
 __global__ void sxe_rescore(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
-__global__ void sxe_rescore_tl(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
+__global__ void sxe_rescore_p2(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
+__global__ void sxe_rescore_p2t(const SxEnvJob*, SxProfiles, SxTargets, float*, double*, SxEnvRes*);
 }
 
 namespace sx {
@@ -133,13 +134,13 @@ void run_region_tails(const std::vector<Profile>& profs, const std::vector<const
 constexpr size_t SXE_BATCH_BYTES = (size_t)4 << 30;
 constexpr size_t SXE_MAX_JOB_BYTES = (size_t)64 << 30;
 // LDS a launch may take for rows and transition rows (of the 160 KB of a CU)
-constexpr size_t SXE_TL_LDS = (size_t)156 * 1024;
+constexpr size_t SXE_LDS = (size_t)156 * 1024;
 
 // Envelope rescoring on the device (sxe_rescore, one wave per envelope): jobs carry
 // profile, target, rows and the null2 flag; the length model and workspace offsets are
 // filled in here.  Results in job order.
 //
-// A launch's LDS is sized by its widest model (8 or 20 floats a model position), and an
+// A launch's LDS is sized by its widest model (8, 14 or 22 floats a model position), and an
 // envelope's time by its cells, so the jobs of a workspace batch are grouped by model
 // width (powers of two) into launches of their own, which run side by side on the side
 // streams, and each launch takes its largest envelopes first.
@@ -209,16 +210,29 @@ void device_rescore(Ctx& c, std::vector<SxEnvJob>& jobs, std::vector<SxEnvRes>& 
             size_t g1 = g0;
             size_t maxw = 1;
             while (g1 < n && bucket(order[a + g1]) == bk) maxw = std::max(maxw, W[order[a + g1++]]);
-            // the transition rows go to LDS too when the widest model of the launch lets them
-            const bool tl = 20 * maxw * sizeof(float) <= SXE_TL_LDS;
-            const size_t lds = (tl ? 20 : 8) * maxw * sizeof(float);
-            const void* fn = tl ? reinterpret_cast<const void*>(&sxe_rescore_tl)
-                                : reinterpret_cast<const void*>(&sxe_rescore);
+            // the widest form the launch's widest model lets fit: two waves with the
+            // transition rows in LDS, two waves, or one wave
+            const size_t p2t = (22 * maxw + 4) * sizeof(float), p2 = (14 * maxw + 4) * sizeof(float);
+            const size_t one = 8 * maxw * sizeof(float);
+            const void* fn;
+            size_t lds;
+            int threads = 128;
+            if (p2t <= SXE_LDS) {
+                fn = reinterpret_cast<const void*>(&sxe_rescore_p2t);
+                lds = p2t;
+            } else if (p2 <= SXE_LDS) {
+                fn = reinterpret_cast<const void*>(&sxe_rescore_p2);
+                lds = p2;
+            } else {
+                fn = reinterpret_cast<const void*>(&sxe_rescore);
+                lds = one;
+                threads = 64;
+            }
             if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             const SxEnvJob* jp = djobs + g0;
             SxEnvRes* rp = dres + g0;
             void* args[] = {&jp, &P, &T, &dws, &ddws, &rp};
-            HIPCHK(hipLaunchKernel(fn, dim3((unsigned)(g1 - g0)), dim3(64), args, lds, fo.next()));
+            HIPCHK(hipLaunchKernel(fn, dim3((unsigned)(g1 - g0)), dim3(threads), args, lds, fo.next()));
             g0 = g1;
         }
         fo.join();
