@@ -101,7 +101,8 @@ def host_cpus():
 
 
 def cpu_baseline(wl, recs, f, budget_s, threads=None):
-    """The oracle's threaded cascade on a bounded sample (first k profiles x all proteins)."""
+    """The oracle's threaded cascade on a bounded sample (first k profiles x all proteins);
+    with the device's records (recs), the decisions on that sample compared with them."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the checker, timed here as the CPU baseline
     hc = host_cpus()
@@ -120,19 +121,44 @@ def cpu_baseline(wl, recs, f, budget_s, threads=None):
     stage, sc = batch.cascade(**f, nthreads=threads)
     dt = time.perf_counter() - t0
     cells = int(ms[:k].sum()) * tot_l
-    # decision agreement with the device cascade on the same pairs
     nt = len(wl.seqs)
+    out = {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "port",
+           "sample": f"cascade on {k} of {len(ms)} profiles x all {nt} proteins ({cells:.3g} MSV cells, "
+                     f"{dt:.1f} s)",
+           "host": hc, "per_core_gcups": round(cells / dt / 1e9 / threads, 3)}
+    if recs is None:
+        return out
+    # decision agreement with the device cascade on the same pairs
     sel = recs[recs["profile"] < k]
     gkey = sel["profile"].astype(np.int64) * nt + sel["target"].astype(np.int64)
     okey = np.flatnonzero(stage >= 1)
     same_set = bool(np.array_equal(np.sort(gkey), okey))
     order = np.argsort(gkey)
     agree = int(np.sum(sel["stage"][order] == stage[np.sort(gkey)])) if same_set else 0
+    out["check"] = {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}
+    return out
+
+
+def cpu_forward(wl, budget_s, threads):
+    """The oracle's f32 Forward (the reference's dp.rs:229-298) on every protein of the first
+    k profiles, within about budget_s: the CPU side of the precision-stress workload, which
+    runs Forward on every pair."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker, timed here as the CPU baseline
+    ms = np.array([h.m for h in wl.hmms], np.int64)
+    tot_l = sum(len(x) for x in wl.seqs)
+    k = 1
+    while k < len(ms) and ms[:k + 1].sum() * tot_l / (0.5e9 * threads) < budget_s:
+        k += 1
+    hm = [orc.Hmm(name=h.name, m=h.m, mat=h.mat, t=h.t, compo=h.compo, stats=h.stats) for h in wl.hmms[:k]]
+    batch = orc.Batch([orc.configure(h) for h in hm], wl.seqs)
+    t0 = time.perf_counter()
+    batch.forward(nthreads=threads)
+    dt = time.perf_counter() - t0
+    cells = int(ms[:k].sum()) * tot_l
     return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "port",
-            "sample": f"cascade on {k} of {len(ms)} profiles x all {nt} proteins ({cells:.3g} MSV cells, "
-                      f"{dt:.1f} s)",
-            "host": hc, "per_core_gcups": round(cells / dt / 1e9 / threads, 3),
-            "check": {"msv_survivor_set_identical": same_set, "stage_agreement": f"{agree}/{len(okey)}"}}
+            "sample": f"f32 Forward on {k} of {len(ms)} profiles x all {len(wl.seqs)} proteins ({cells:.3g} cells, "
+                      f"{dt:.1f} s)"}
 
 
 def plain_leg(sxa, ctx, wl, n_plain, f, graph, bound, vbound):
@@ -295,7 +321,7 @@ def single_m200_leg(sxa, synth, f, reps=5):
             c0.score_all(stage, sxa.FORM_BATCHED, want=False)
             ms = float(np.median([c0.score_all(stage, sxa.FORM_BATCHED, want=False)[1] for _ in range(reps)]))
             out[name] = {"ms": round(ms, 3), "gcups": round(cells / ms / 1e6, 1)}
-    return out
+    return wl, out
 
 
 def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=1000.0):
@@ -356,7 +382,7 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
                              "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
                              "latency_ms_max": round(float(lat.max()), 2),
                              "gcups": round(gcells / lat.sum() / 1e6, 1)}
-    return out
+    return wl, out
 
 
 def ingest_leg(sxa, synth, f, n_profiles=2000, batches=(256, 512, 0)):
@@ -656,7 +682,7 @@ def main():
                 for r in ls]
     log(f"[bench] launches: {launches}")
 
-    kmode = prec = None
+    kmode = prec = wl1 = wl3 = None
     if not args.no_kernel_mode:
         t0 = time.time()
         wl1, kmode, fwd1 = kernel_mode(sxa, synth, min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0,
@@ -665,15 +691,15 @@ def main():
         wl3, fwd3, prec = precision_gpu(sxa, synth)
         log(f"[bench] precision slice: {prec}")
 
-    single = mag = None
+    single = mag = wl0 = wl4 = None
     if not args.no_single:
         t0 = time.time()
-        single = single_m200_leg(sxa, synth, f)
+        wl0, single = single_m200_leg(sxa, synth, f)
         log(f"[bench] single-M200 ({time.time() - t0:.1f} s): {single}")
     if args.mag_sweep != "none":
         t0 = time.time()
         ps, ng = (1.0, 200) if args.mag_sweep == "full" else (0.2, 40)
-        mag = mag_sweep_leg(sxa, synth, f, ps, ng, shapes="tune" if args.shapes == "tune" else None,
+        wl4, mag = mag_sweep_leg(sxa, synth, f, ps, ng, shapes="tune" if args.shapes == "tune" else None,
                             tune_ms=min(args.tune_ms, 1000.0) if args.tune_ms > 0 else 0)
         log(f"[bench] mag-sweep ({time.time() - t0:.1f} s): {mag}")
 
@@ -698,6 +724,18 @@ def main():
             t0 = time.time()
             fwd_err["inclusion_parity_marker_set"] = inclusion_parity(sxa, wl1, f, threads)
             log(f"[bench] inclusion parity ({time.time() - t0:.1f} s): {fwd_err['inclusion_parity_marker_set']}")
+        # the CPU side of the other workloads, each on a bounded sample
+        threads = host_cpus()["threads"]
+        cpu_cfg = {}
+        if wl0 is not None:
+            cpu_cfg["single_m200"] = cpu_baseline(wl0, None, f, 2.0)
+        if wl1 is not None:
+            cpu_cfg["marker_set"] = cpu_baseline(wl1, None, f, 3.0)
+            cpu_cfg["precision_forward"] = cpu_forward(wl3, 4.0, threads)
+        if wl4 is not None:
+            cpu_cfg["mag_sweep"] = cpu_baseline(wl4, None, f, 3.0)
+        cpu["per_workload"] = cpu_cfg
+        log(f"[bench] cpu per workload: {cpu_cfg}")
 
     out = {
         "metric": "GCUPS of the MSV->bias->Viterbi->Forward cascade over all pairs, batched, 1 MI355X",
