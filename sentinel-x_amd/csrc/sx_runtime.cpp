@@ -315,12 +315,13 @@ void Ctx::set_profiles(std::vector<Profile>&& ps) {
     d_qsc = dupload(qsc);
     d_qtbm = dupload(qtbm);
     d_qtsc = dupload(qtsc);
-    // padded emission rows of the long models (read from global memory by their kernels)
+    // padded emission rows of the models over 512 positions (read from global memory by
+    // the global-emission kernels)
     std::vector<float> gmsc, gmo;
     std::vector<int64_t> goff(profs.size(), -1);
     for (size_t i = 0; i < profs.size(); i++) {
         const Profile& p = profs[i];
-        if (p.m <= SX_GEM_MIN_M) continue;
+        if (p.m <= SX_GEM_COPY_MIN_M) continue;
         goff[i] = (int64_t)gmsc.size();
         const size_t base = gmsc.size(), w = (size_t)p.m + 1;
         gmsc.resize(base + (size_t)SX_NCODE * SX_GEM_WIDTH, -std::numeric_limits<float>::infinity());
@@ -436,11 +437,18 @@ Shape Ctx::shape_for(int stage, int form, int m) const {
     if (quant && m > SX_Q_MAX_M) return default_shape(stage, form, m);  // throws
     if (!quant && m > SX_GEM_MIN_M) return gem_shape(m);  // long models: one fixed shape
     if (form == SX_FORM_BATCHED) {
-        if (forced.count(stage)) return forced.at(stage);
+        auto gem_ok = [&](const Shape& s) {
+            // a global-emission shape reads the padded copy, which only longer models have
+            if (s.hoist == 2 && m <= SX_GEM_COPY_MIN_M)
+                throw SxError(SX_ERR_ARG, "a global-emission shape (table mode 2) covers models of over " +
+                                              std::to_string(SX_GEM_COPY_MIN_M) + " positions only");
+            return s;
+        };
+        if (forced.count(stage)) return gem_ok(forced.at(stage));
         auto it = tuned.find(stage);
         if (it != tuned.end()) {
             for (const auto& e : it->second)
-                if (m <= e.first) return e.second;
+                if (m <= e.first) return gem_ok(e.second);
         }
     }
     return default_shape(stage, form, m);
@@ -832,6 +840,7 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
     const double per_bucket = budget_ms / std::max<size_t>(1, present.size());
     int lo = 0;
     for (int hi : edges) {
+        const int blo = lo;  // the bucket holds models of blo+1 .. hi positions
         std::vector<int32_t> inb;
         int mmax = 0;
         for (size_t p = 0; p < profs.size(); p++)
@@ -863,7 +872,9 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         std::vector<Shape> cands;
         for (const auto& kv : kernels) {
             const KernelInfo& k = kv.second;
-            if (k.stage != stage || k.spec || k.hoist == 2) continue;
+            if (k.stage != stage || k.spec) continue;
+            // global-emission shapes: Forward buckets whose models all have the padded copy
+            if (k.hoist == 2 && !(stage == SX_STAGE_FORWARD && blo >= SX_GEM_COPY_MIN_M && k.q <= 16)) continue;
             if ((long)k.q * k.g < mmax) continue;
             // an MSV shape with an aligned-block twin: the twin reads LDS without bank
             // conflicts and is always the faster of the two
@@ -1343,8 +1354,13 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
         std::vector<std::pair<int, sx::Shape>> tab;
         for (uint32_t i = 0; i < n; i++) {
             const int h = hoist ? hoist[i] : 0;
-            if (h < 0 || h > 4 || h == 2)
-                throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (long models have a fixed shape)");
+            if (h < 0 || h > 4 || (h == 2 && stage != SX_STAGE_FORWARD))
+                throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (2 for Forward only; long models have "
+                                          "a fixed shape)");
+            // a mode-2 row may only serve models over 512 positions (the lookup checks it)
+            if (h == 2 && max_m[i] <= SX_GEM_COPY_MIN_M)
+                throw SxError(SX_ERR_ARG, "a global-emission row must cover models over " +
+                                              std::to_string(SX_GEM_COPY_MIN_M) + " positions");
             const bool quant = stage == SX_STAGE_MSV16 || stage == SX_STAGE_VIT16;
             const int top = quant ? SX_Q_MAX_M : SX_GEM_MIN_M;
             if (max_m[i] > top)

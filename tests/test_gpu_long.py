@@ -143,3 +143,52 @@ def test_long_and_short_models_in_one_cascade(orc, gctx, long_case):
 def test_model_longer_than_supported_is_rejected(orc, gctx, sxa):
     with pytest.raises(sxa.SxError):
         gctx.set_profiles([random_hmm(orc, 4097, 1)])
+
+
+MID_MS = [513, 600, 768, 769, 1000, 1024]
+
+
+@pytest.mark.parametrize("q", [12, 16])
+def test_forward_global_emission_shapes_for_models_over_512(orc, gctx, q):
+    """Forward of models of 513-1024 positions may run the global-emission form (64 lanes,
+    12 or 16 positions a lane, emission rows from the padded copy that models over 512
+    positions get): within FWD_TOL nats of the FP64 oracle on every pair, including
+    targets of 1-3 residues, '*' and degenerate residues and a 4,000-residue homologue."""
+    ms = [m for m in MID_MS if m <= 64 * q]
+    hmms = [random_hmm(orc, m, 7100 + m) for m in ms]
+    rng = np.random.default_rng(72 + q)
+    seqs = [random_seq(rng, L, star_frac=0.01, degen_frac=0.02) for L in (1, 2, 3, 40)]
+    seqs += [emit_homolog(orc, hmms[i % len(hmms)], rng, int(rng.integers(300, 900))) for i in range(6)]
+    seqs.append(emit_homolog(orc, hmms[-1], rng, 4000))
+    profs = [orc.configure(h) for h in hmms]
+    want = orc.Batch(profs, seqs).forward64(nthreads=8).reshape(len(profs), len(seqs))
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(2, [(512, 8, 64, 1), (64 * q, q, 64, 2)])
+    try:
+        assert gctx.shape_for(2, ms[0]) == (q, 64, 2)
+        got, _ = gctx.score_all(2, 1)
+    finally:
+        gctx.set_shapes(2, [])
+    fin = np.isfinite(want)
+    assert np.array_equal(np.isfinite(got), fin)
+    assert np.abs(got[fin] - want[fin]).max() <= FWD_TOL
+
+
+def test_global_emission_rows_serve_only_models_over_512(orc, gctx):
+    """The padded emission copy exists only for models over 512 positions: a mode-2 row that
+    would serve shorter models is refused when set, and a lookup of a shorter model in a
+    table whose only row is mode 2 fails with an error (never a launch)."""
+    with pytest.raises(Exception):
+        gctx.set_shapes(2, [(512, 8, 64, 2)])
+    with pytest.raises(Exception):
+        gctx.set_shapes(1, [(1024, 16, 64, 2)])  # Forward only
+    hmms = [random_hmm(orc, 300, 7300), random_hmm(orc, 700, 7301)]
+    gctx.set_profiles(hmms)
+    gctx.set_targets([random_seq(np.random.default_rng(73), 50)])
+    gctx.set_shapes(2, [(1024, 16, 64, 2)])
+    try:
+        with pytest.raises(Exception):
+            gctx.score_all(2, 1)
+    finally:
+        gctx.set_shapes(2, [])
