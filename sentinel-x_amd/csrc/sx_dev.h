@@ -78,8 +78,9 @@ struct SxProfiles {
 // longest supported model is 64 x 64 = 4096 positions.
 #define SX_GEM_MIN_M 1024
 // Models longer than this get the padded global copy of their emission rows: the long
-// models, and those of 513-1024 positions, whose Forward may also run the global-emission
-// kernels (64 lanes, 12 or 16 positions a lane) when the autotuner finds them faster.
+// models, and those of 513-1024 positions, whose Viterbi and Forward may also run the
+// global-emission kernels (64 lanes, 12 or 16 positions a lane) when the autotuner finds
+// them faster.
 #define SX_GEM_COPY_MIN_M 512
 #define SX_MAX_M 4096
 #define SX_GEM_WIDTH 4104  // >= SX_MAX_M + 2 + 3, a multiple of 8

@@ -1793,8 +1793,8 @@ static const SxKernelEntry k_table[] = {
     SXK_ALLG(3, 32, true, false), SXK_ALLG(3, 48, true, false),
     // long models (M > 1024): 64 lanes per pair, emission rows from global memory
     SXK_GEM(0), SXK_GEM(1), SXK_GEM(2), SXK_GEM(3),
-    // Forward of models of 513-1024 positions through the global-emission form
-    SXK_E(2, 12, 64, false, 2), SXK_E(2, 16, 64, false, 2),
+    // Viterbi and Forward of models of 513-1024 positions through the global-emission form
+    SXK_E(1, 12, 64, false, 2), SXK_E(1, 16, 64, false, 2), SXK_E(2, 12, 64, false, 2), SXK_E(2, 16, 64, false, 2),
     SXK_GEMS(2), SXK_GEMS(3),
     // quantised MSV bound (stage 4): one-lane groups, and aligned lane blocks of 8, 16, 32
     SXK_E(4, 16, 1, false, 0), SXK_E(4, 24, 1, false, 0), SXK_E(4, 32, 1, false, 0), SXK_E(4, 48, 1, false, 0),

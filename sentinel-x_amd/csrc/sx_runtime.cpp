@@ -873,8 +873,11 @@ std::vector<TuneRecord> Ctx::autotune(int stage, double budget_ms) {
         for (const auto& kv : kernels) {
             const KernelInfo& k = kv.second;
             if (k.stage != stage || k.spec) continue;
-            // global-emission shapes: Forward buckets whose models all have the padded copy
-            if (k.hoist == 2 && !(stage == SX_STAGE_FORWARD && blo >= SX_GEM_COPY_MIN_M && k.q <= 16)) continue;
+            // global-emission shapes: Viterbi and Forward buckets whose models all have the
+            // padded copy
+            if (k.hoist == 2 && !((stage == SX_STAGE_VITERBI || stage == SX_STAGE_FORWARD) &&
+                                  blo >= SX_GEM_COPY_MIN_M && k.q <= 16))
+                continue;
             if ((long)k.q * k.g < mmax) continue;
             // an MSV shape with an aligned-block twin: the twin reads LDS without bank
             // conflicts and is always the faster of the two
@@ -1354,9 +1357,9 @@ extern "C" int sx_set_shapes(sx_ctx* ctx, int stage, uint32_t n, const int32_t* 
         std::vector<std::pair<int, sx::Shape>> tab;
         for (uint32_t i = 0; i < n; i++) {
             const int h = hoist ? hoist[i] : 0;
-            if (h < 0 || h > 4 || (h == 2 && stage != SX_STAGE_FORWARD))
-                throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (2 for Forward only; long models have "
-                                          "a fixed shape)");
+            if (h < 0 || h > 4 || (h == 2 && stage != SX_STAGE_VITERBI && stage != SX_STAGE_FORWARD))
+                throw SxError(SX_ERR_ARG, "table mode must be 0, 1, 3 or 4 (2 for Viterbi and Forward only; long "
+                                          "models have a fixed shape)");
             // a mode-2 row may only serve models over 512 positions (the lookup checks it)
             if (h == 2 && max_m[i] <= SX_GEM_COPY_MIN_M)
                 throw SxError(SX_ERR_ARG, "a global-emission row must cover models over " +

@@ -182,7 +182,7 @@ def test_global_emission_rows_serve_only_models_over_512(orc, gctx):
     with pytest.raises(Exception):
         gctx.set_shapes(2, [(512, 8, 64, 2)])
     with pytest.raises(Exception):
-        gctx.set_shapes(1, [(1024, 16, 64, 2)])  # Forward only
+        gctx.set_shapes(0, [(1024, 16, 64, 2)])  # Viterbi and Forward only
     hmms = [random_hmm(orc, 300, 7300), random_hmm(orc, 700, 7301)]
     gctx.set_profiles(hmms)
     gctx.set_targets([random_seq(np.random.default_rng(73), 50)])
@@ -192,3 +192,26 @@ def test_global_emission_rows_serve_only_models_over_512(orc, gctx):
             gctx.score_all(2, 1)
     finally:
         gctx.set_shapes(2, [])
+
+
+@pytest.mark.parametrize("q", [12, 16])
+def test_viterbi_global_emission_shapes_for_models_over_512(orc, gctx, q):
+    """Viterbi of models of 513-1024 positions through the global-emission form: bit-exact
+    with the oracle on every pair (targets of 1-3 residues, '*' and degenerate residues,
+    homologues)."""
+    ms = [m for m in MID_MS if m <= 64 * q]
+    hmms = [random_hmm(orc, m, 7200 + m) for m in ms]
+    rng = np.random.default_rng(82 + q)
+    seqs = [random_seq(rng, L, star_frac=0.01, degen_frac=0.02) for L in (1, 2, 3, 40)]
+    seqs += [emit_homolog(orc, hmms[i % len(hmms)], rng, int(rng.integers(300, 900))) for i in range(6)]
+    profs = [orc.configure(h) for h in hmms]
+    want = orc.Batch(profs, seqs).viterbi(nthreads=8).reshape(len(profs), len(seqs))
+    gctx.set_profiles(hmms)
+    gctx.set_targets(seqs)
+    gctx.set_shapes(1, [(512, 8, 64, 1), (64 * q, q, 64, 2)])
+    try:
+        assert gctx.shape_for(1, ms[0]) == (q, 64, 2)
+        got, _ = gctx.score_all(1, 1)
+    finally:
+        gctx.set_shapes(1, [])
+    assert np.array_equal(got, want)
