@@ -31,10 +31,13 @@ def workload_key(wl):
     return f"{wl.name}: {len(wl.hmms)} profiles x {len(wl.seqs)} proteins"
 
 
-def tune(sxa, ctx, tune_ms, log=print):
-    """Autotune every stage on ctx's profiles and targets; returns {stage: {table, log}}."""
+def tune(sxa, ctx, tune_ms, log=print, only=None):
+    """Autotune every stage (or the stages in `only`) on ctx's profiles and targets; returns
+    {stage: {table, log}}."""
     out = {}
     for st, f in STAGES:
+        if only is not None and st not in only:
+            continue
         t0 = time.time()
         ctx.autotune(st, tune_ms * f)
         out[str(st)] = {"table": [list(r) for r in ctx.get_shapes(st)],
@@ -59,7 +62,13 @@ def main():
     ap.add_argument("--target-scale", type=float, default=1.0)
     ap.add_argument("--tune-ms", type=float, default=3000.0, help="per-stage budget (x3 for MSV and the bound)")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--stages", type=int, nargs="+", default=None,
+                    help="tune only these stages and keep the others' tables from --out (which must exist)")
     a = ap.parse_args()
+    prev = None
+    if a.stages:
+        with open(a.out) as fh:
+            prev = json.load(fh)
     sxa = load_package()
     from sentinel_x_amd import synth
     import bench
@@ -68,10 +77,19 @@ def main():
     with sxa.Context(0) as ctx:
         ctx.set_profiles(wl.hmms)
         ctx.set_targets(wl.seqs)
-        stages = tune(sxa, ctx, a.tune_ms, log=lambda *m: print(*m, file=sys.stderr, flush=True))
+        stages = tune(sxa, ctx, a.tune_ms, log=lambda *m: print(*m, file=sys.stderr, flush=True), only=a.stages)
+    out = {"workload": workload_key(wl), "tune_ms": a.tune_ms, "stages": stages}
+    if prev is not None:
+        if prev["workload"] != out["workload"]:
+            raise SystemExit(f"{a.out} holds tables of another workload ({prev['workload']})")
+        merged = dict(prev["stages"])
+        merged.update(stages)
+        out["stages"] = {k: merged[k] for k in sorted(merged, key=lambda k: [s for s, _ in STAGES].index(int(k)))}
+        out["tune_ms"] = prev["tune_ms"]
+        out["tune_ms_by_stage"] = {**prev.get("tune_ms_by_stage", {}), **{str(st): a.tune_ms for st in a.stages}}
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     with open(a.out, "w") as fh:
-        json.dump({"workload": workload_key(wl), "tune_ms": a.tune_ms, "stages": stages}, fh, indent=1)
+        json.dump(out, fh, indent=1)
     print(f"wrote {a.out}", flush=True)
 
 
