@@ -105,6 +105,23 @@ __device__ __forceinline__ float grp_from_prev(float v, float b) {
     }
 }
 
+// grp_from_prev(v, 0): the shifts read with bound_ctrl, so an invalid source lane reads
+// 0 and no old value has to be set first.
+template <int G>
+__device__ __forceinline__ float grp_from_prev0(float v) {
+    if constexpr (G == 1) {
+        return 0.0f;
+    } else if constexpr (G <= 16) {
+        float s = dppf<0x111, 0xF, 0xF, true>(0.0f, v);  // row_shr:1
+        if constexpr (G < 16) s = ((lane64() & (G - 1)) == 0) ? 0.0f : s;
+        return s;
+    } else {
+        float s = dppf<0x138, 0xF, 0xF, true>(0.0f, v);  // wave_shr:1
+        if constexpr (G == 32) s = ((lane64() & 31) == 0) ? 0.0f : s;
+        return s;
+    }
+}
+
 // Lane j of the group receives lane j+1's v; lane G-1 receives b.
 template <int G>
 __device__ __forceinline__ float grp_from_next(float v, float b) {
@@ -144,22 +161,23 @@ struct RowShl<8> { static constexpr int c = 0x108; };
 
 // One step of an inclusive (Hillis-Steele) scan of affine maps D -> A + B*D over the
 // lanes of a group; lanes with no partner S lanes away combine with the identity (0, 1).
+// A's partner value is read with bound_ctrl (an invalid source lane reads 0, the identity's
+// A), so no old value has to be set first.  Within a 16-lane row (G <= 16) a lane's B only
+// matters while its composite has not yet reached the group's first lane: once it has, its
+// partner A is 0 from then on and its B multiplies only the B of later lanes' composites,
+// which have reached it too.  So B's partner value is read with bound_ctrl as well, and
+// lanes of the row's other groups are not masked out of it.  Wider groups combine the
+// rows' total B values, so there B keeps the identity 1.
 template <int G, int S, bool FROM_LEFT>
 __device__ __forceinline__ void scan_step(float& A, float& B) {
     if constexpr (G > S && S < 16) {
-        float a2, b2;
-        if constexpr (FROM_LEFT) {
-            a2 = dppf<RowShr<S>::c>(0.0f, A);
-            b2 = dppf<RowShr<S>::c>(1.0f, B);
-        } else {
-            a2 = dppf<RowShl<S>::c>(0.0f, A);
-            b2 = dppf<RowShl<S>::c>(1.0f, B);
-        }
+        constexpr int C = FROM_LEFT ? RowShr<S>::c : RowShl<S>::c;
+        float a2 = dppf<C, 0xF, 0xF, true>(0.0f, A);
+        const float b2 = G <= 16 ? dppf<C, 0xF, 0xF, true>(0.0f, B) : dppf<C>(1.0f, B);
         if constexpr (G < 16) {
             const int jg = lane64() & (G - 1);
             const bool ok = FROM_LEFT ? (jg >= S) : (jg + S < G);
             a2 = ok ? a2 : 0.0f;
-            b2 = ok ? b2 : 1.0f;
         }
         A = fmaf(B, a2, A);
         B = B * b2;
@@ -1029,14 +1047,20 @@ __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, cons
 // ---------------------------------------------------------------------------
 // Forward (dp.rs:229-298), multihit probabilities e_move = e_loop = 1/2
 // ---------------------------------------------------------------------------
-// One forward pass over a lane's cells per row.  The D chain is affine across lanes:
-// with c the D value at the lane's first position k0 (set by the lanes before it), the
-// lane's values are D[k0+q] = Dl[q] + c * PC[q], where Dl is the chain started from
-// Dl[0] = 0, and the value it hands on is out(c) = Dl_out + c * PC[Q].  The pass computes
-// M, I and Dl cell by cell (each cell's record read once), an affine scan of the lanes'
-// maps gives every lane its c, and the row sum adds c * sum(PC) to the sum of M and Dl.
-// The stored D values stay in the (Dl, c) form: the next row corrects each one as it
-// reads it (one FMA per cell), so no extra pass is needed.
+// The D chain is affine across lanes: with c the D value at the lane's first position k0
+// (set by the lanes before it), the lane's values are D[k0+q] = Dl[q] + c * PC[q], where
+// Dl is the chain started from Dl[0] = 0, and the value it hands on is out(c) = Dl_out +
+// c * PC[Q].  Each row computes M, I and Dl over the lane's cells, an affine scan of the
+// lanes' maps gives every lane its c, and the row sum adds c * sum(PC) to the sum of M and
+// Dl.  The stored D values stay in the (Dl, c) form: the next row corrects each one as it
+// reads it (one FMA per cell), so no extra pass is needed.  With the records in registers
+// (TM_REG) a row is two passes: M and I in descending position order, so that every cell
+// overwrites its own registers and the loop carries no register rotation (in one
+// ascending pass the compiler copies the cells' registers once a row), then the Dl chain
+// ascending (8-9% faster).  With the records in LDS it is one ascending pass, each record
+// read once: there the two passes measured up to 17% slower (the second pass's serial
+// chain of Q FMAs, with {MD, DD} held in registers or read again), though they issue
+// fewer instructions.
 template <int Q, int G, bool SPEC, int TM>
 __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew, const float* __restrict__ trec,
                                            const uint8_t* seq, int L, float ploop, float pmove, float* x5,
@@ -1053,6 +1077,19 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
     const float* __restrict__ pcb = trec + FR::REC + j * FR::PSTRIDE;
     const float blk = pcb[Q];       // the block's D-chain slope
     const float spc = pcb[Q + 1];   // sum of PC[0..Q-1]
+    // TM_REG: the lane's records and D-chain products loaded once, held in registers
+    // TM_REG: the lane's records and D-chain products loaded once, held in registers
+    float4 hra[HOIST ? Q : 1], hrb[HOIST ? Q : 1];
+    float hpc[HOIST ? Q : 1];
+    if constexpr (HOIST) {
+        const float4* __restrict__ r0 = reinterpret_cast<const float4*>(trec + j * FR::RSTRIDE);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            hra[q] = r0[2 * q];
+            hrb[q] = r0[2 * q + 1];
+            hpc[q] = pcb[q];
+        }
+    }
     float mr[Q], ir[Q], dl[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -1084,11 +1121,7 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
         // a '*' residue emits no insert state (dp.rs:256,260): the inserts of a '*' row are
         // cleared at the top of the next row, under a wave vote, instead of a multiply in
         // every cell (the row itself reads no insert state of its own)
-        if constexpr (HOIST) {
-            // (a branch here would keep the transitions from being hoisted into registers)
-#pragma unroll
-            for (int q = 0; q < Q; q++) ir[q] = star_prev ? 0.0f : ir[q];
-        } else if (__builtin_amdgcn_ballot_w64(star_prev) != 0ull) [[unlikely]] {
+        if (__builtin_amdgcn_ballot_w64(star_prev) != 0ull) [[unlikely]] {
             __asm__ volatile("s_nop 0");  // a side effect: this rare path stays a branch, not selects
             if (star_prev) {
 #pragma unroll
@@ -1105,51 +1138,74 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
         if constexpr (!GEM) (void)en;
         const int ro = HOIST ? 0 : opaque(0);
         const float4* __restrict__ rec = reinterpret_cast<const float4*>(trec + j * FR::RSTRIDE + ro);
-        // the products PC[q]: read from LDS every row where the tables are (held in
-        // registers they would cost Q more VGPRs), and with the transitions in registers
-        // formed as a running product of their DD values (an LDS read through an opaque
-        // address would keep the transitions from being hoisted)
         const float* __restrict__ pc = pcb + ro;
-        // the previous lane's last M, I and (corrected) D of the stored row
-        const float dlast = fmaf(c, HOIST ? pcb[Q - 1] : pc[Q - 1], dl[Q - 1]);
-        float mpo = grp_from_prev<G>(mr[Q - 1], 0.0f);
-        float ipo = grp_from_prev<G>(ir[Q - 1], 0.0f);
-        float dpo = grp_from_prev<G>(dlast, 0.0f);
-        float dnext = 0.0f;  // Dl of the next cell
         float em = 0.0f, ed = 0.0f;
-        float pcq = 1.0f;  // PC[q], with the transitions in registers
+        float dnext = 0.0f;  // Dl of the next cell
+        if constexpr (HOIST) {
+            // M and I in descending position order, each cell overwriting its own registers
+            // (cell q reads the stored row at q-1 and q only, so the loop carries no
+            // register rotation), then the Dl chain ascending
+            const float mpi = grp_from_prev0<G>(mr[Q - 1]);
+            const float ipi = grp_from_prev0<G>(ir[Q - 1]);
+            const float dpi = grp_from_prev0<G>(fmaf(c, hpc[Q - 1], dl[Q - 1]));
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const float4 ta = rec[2 * q], tb = rec[2 * q + 1];  // {MM IM DM BM}, {MI II MD DD}
-            const float mo = mr[q], io = ir[q];
-            const float pq = HOIST ? pcq : pc[q];
-            if constexpr (HOIST) pcq *= tb.w;
-            const float dold = q + 1 < Q ? fmaf(c, pq, dl[q]) : dlast;  // stored D at this cell
-            float s = xb * ta.w;
-            s = fmaf(mpo, ta.x, s);
-            s = fmaf(ipo, ta.y, s);
-            s = fmaf(dpo, ta.z, s);
-            float mv;
-            if constexpr (GEM) {
-                mv = s * eb[q];
-                eb[q] = en[q];
-            } else {
-                mv = s * eo[q];
+            for (int q = Q - 1; q >= 0; q--) {
+                const float4 ta = hra[q], tb = hrb[q];  // {MM IM DM BM}, {MI II MD DD}
+                const float mp = q ? mr[q - 1] : mpi;
+                const float ip = q ? ir[q - 1] : ipi;
+                const float dp = q ? fmaf(c, hpc[q - 1], dl[q - 1]) : dpi;  // stored D at q-1
+                float s = xb * ta.w;
+                s = fmaf(mp, ta.x, s);
+                s = fmaf(ip, ta.y, s);
+                s = fmaf(dp, ta.z, s);
+                const float mv = s * eo[q];
+                ir[q] = fmaf(mr[q], tb.x, ir[q] * tb.y);
+                mr[q] = mv;
+                em = q == Q - 1 ? mv : em + mv;
             }
-            ir[q] = fmaf(mo, tb.x, io * tb.y);
-            mr[q] = mv;
-            dl[q] = dnext;
-            em += mv;
-            ed += dnext;
-            dnext = fmaf(dnext, tb.w, mv * tb.z);
-            mpo = mo;
-            ipo = io;
-            dpo = dold;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                dl[q] = dnext;
+                if (q) ed = q == 1 ? dnext : ed + dnext;
+                dnext = q ? fmaf(dnext, hrb[q].w, mr[q] * hrb[q].z) : mr[q] * hrb[q].z;
+            }
+        } else {
+            // records in LDS: one ascending pass, each record read once
+            const float dlast = fmaf(c, pc[Q - 1], dl[Q - 1]);
+            float mpo = grp_from_prev0<G>(mr[Q - 1]);
+            float ipo = grp_from_prev0<G>(ir[Q - 1]);
+            float dpo = grp_from_prev0<G>(dlast);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const float4 ta = rec[2 * q], tb = rec[2 * q + 1];  // {MM IM DM BM}, {MI II MD DD}
+                const float mo = mr[q], io = ir[q];
+                const float dold = q + 1 < Q ? fmaf(c, pc[q], dl[q]) : dlast;  // stored D at this cell
+                float s = xb * ta.w;
+                s = fmaf(mpo, ta.x, s);
+                s = fmaf(ipo, ta.y, s);
+                s = fmaf(dpo, ta.z, s);
+                float mv;
+                if constexpr (GEM) {
+                    mv = s * eb[q];
+                    eb[q] = en[q];
+                } else {
+                    mv = s * eo[q];
+                }
+                ir[q] = fmaf(mo, tb.x, io * tb.y);
+                mr[q] = mv;
+                dl[q] = dnext;
+                em += mv;
+                ed += dnext;
+                dnext = fmaf(dnext, tb.w, mv * tb.z);
+                mpo = mo;
+                ipo = io;
+                dpo = dold;
+            }
         }
         float cn = dnext;  // the lane's map out(c) = dnext + c * blk, scanned over the group
         if constexpr (G > 1) {
             grp_affine_scan<G, true>(cn, blk);
-            cn = grp_from_prev<G>(cn, 0.0f);
+            cn = grp_from_prev0<G>(cn);
         } else {
             cn = 0.0f;
         }
