@@ -211,6 +211,7 @@ private:
         size_t idx;
         std::string text;
         size_t line0;
+        bool failed;  // the file could not be read past the previous record
     };
     struct Result {
         bool ok = false, have = false;
@@ -225,11 +226,11 @@ private:
         std::string rec, line;
         size_t lineno = 0, line0 = 0, idx = 0;
         char buf[1 << 16];
-        auto push = [&](std::string&& text, size_t first) {
+        auto push = [&](std::string&& text, size_t first, bool failed = false) {
             std::unique_lock<std::mutex> lk(mu_);
             cv_space_.wait(lk, [&] { return stop_ || idx - taken_ < ahead_; });
             if (stop_) return false;
-            work_.push_back({idx++, std::move(text), first});
+            work_.push_back({idx++, std::move(text), first, failed});
             cv_work_.notify_one();
             return true;
         };
@@ -251,7 +252,15 @@ private:
                 line0 = lineno;
             }
         }
-        if (ok && !only_space(rec)) push(std::move(rec), line0);
+        int err = Z_OK;
+        gzerror(gz_, &err);
+        if (ok && err != Z_OK) {
+            // a read or decompression error is not the end of the file: the record that
+            // follows fails, so the one-pass reader takes over there and reports it
+            push(std::string(), line0, true);
+        } else if (ok && !only_space(rec)) {
+            push(std::move(rec), line0);
+        }
         std::lock_guard<std::mutex> lk(mu_);
         reader_done_ = true;
         n_records_ = idx;
@@ -270,6 +279,12 @@ private:
                 work_.pop_front();
             }
             Result r;
+            if (w.failed) {
+                std::lock_guard<std::mutex> lk(mu_);
+                results_[w.idx] = std::move(r);  // not ok
+                cv_res_.notify_all();
+                continue;
+            }
             try {
                 auto rd = HmmReader::from_string(std::move(w.text), path_, w.line0);
                 Hmm h;

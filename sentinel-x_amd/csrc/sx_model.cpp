@@ -284,7 +284,14 @@ struct HmmReader::Impl {
             char buf[65536];
             bool any = false;
             for (;;) {
-                if (!gzgets(gz, buf, sizeof buf)) break;
+                if (!gzgets(gz, buf, sizeof buf)) {
+                    // end of input, or a read / decompression error (a damaged or cut
+                    // gzip stream), which must not pass for the end of the file
+                    int err = Z_OK;
+                    const char* msg = gzerror(gz, &err);
+                    if (err != Z_OK) throw std::runtime_error(source + ": reading: " + (msg ? msg : "error"));
+                    break;
+                }
                 any = true;
                 line += buf;
                 if (!line.empty() && line.back() == '\n') break;

@@ -157,6 +157,26 @@ def test_malformed_files_fail_with_the_line_of_the_one_pass_reader(sxa, synth_fi
     assert f"{cut}: unexpected end of file (" in str(e.value)
 
 
+def test_damaged_gzip_is_an_error_not_an_end_of_file(sxa, synth_file, tmp_path):
+    """A gzip stream that stops decompressing (here: every model decompresses, then the
+    trailer's CRC does not match) is a read error, as flate2's is to the reference
+    (parse.rs:48-49), not a shorter file."""
+    path, hmms = synth_file
+    data = gzip.compress(open(path, "rb").read())
+    bad = bytearray(data)
+    bad[-8] ^= 0xFF  # the CRC-32 of the trailer
+    gz = str(tmp_path / "crc.hmm.gz")
+    open(gz, "wb").write(bytes(bad))
+    for th in (1, 8):
+        with pytest.raises(sxa.SxError, match="reading"):
+            sxa.convert_hmm_file(gz, str(tmp_path / f"crc{th}.sxpc"), threads=th)
+        assert not os.path.exists(str(tmp_path / f"crc{th}.sxpc"))
+    # the same stream intact reads in full
+    good = str(tmp_path / "ok.hmm.gz")
+    open(good, "wb").write(data)
+    assert sxa.convert_hmm_file(good, str(tmp_path / "ok.sxpc"), threads=8) == len(hmms)
+
+
 def test_empty_file_and_damaged_cache(sxa, synth_file, tmp_path):
     empty = str(tmp_path / "empty.hmm")
     open(empty, "w").write("\n\n")
