@@ -171,6 +171,9 @@ const int CB = 256;  // cascade kernels' block size
 // capacity of the device list of bias scores handed to the host; past it the host scans
 // every record's flag
 const int32_t BIAS_LIST_CAP = 1 << 16;
+// an MSV launch with fewer tiles than this many per workgroup of a full grid spends a good
+// part of its time on its last tiles: the cascade's MSV launches then run side by side
+const int MSV_SIDE_TILES = 16;
 
 int64_t nblk(int64_t n) { return (n + CB - 1) / CB; }
 
@@ -614,8 +617,16 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         }
         fo.join();
     };
-    // 1. MSV over all pairs (the f32 scores, or the int16 bound of each pair)
+    // 1. MSV over all pairs (the f32 scores, or the int16 bound of each pair).  When every
+    // launch has many tiles for each workgroup of a full grid, the launches run one after
+    // the other on the chain's stream, each filling the device on its own.  Otherwise
+    // (fewer profiles or targets) a launch's last tiles leave the device idle for a good
+    // part of its time, so all of them go out side by side on the side streams: launch i
+    // on side stream i mod 4 with head slot i mod 32, so launches that share a slot share
+    // a stream.
     const int msv_stage = S.bound ? SX_STAGE_MSV16 : 0;
+    std::vector<SxWork> msv_work;
+    bool msv_side = false;
     for (auto& gr : S.msv_groups) {
         const KernelInfo& k = c.kernel(msv_stage, gr.s.q, gr.s.g, false, gr.s.hoist);
         SxWork w{};
@@ -628,13 +639,25 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
         w.ntiles = (int64_t)w.nchunks * gr.n;
         w.out_stride = T;
         w.out_by_pos = 1;  // usc[profile][length-sorted position]: contiguous per tile
-        const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), w.ntiles);
-        // MSV launches run one after the other: each fills the device on its own
-        if (with_events) HIPCHK(hipEventRecord(S.lev0[lslot], st));
-        c.launch(k, msv_stage, w, S.usc.p, nosp, grid, true, slot % 32);
-        if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], st));
-        slot++;
-        lslot++;
+        const int full = c.persistent_grid(k, Ctx::lds_bytes(k), (int64_t)1 << 40);
+        if (w.ntiles < (int64_t)MSV_SIDE_TILES * full) msv_side = true;
+        msv_work.push_back(w);
+    }
+    {
+        Fanout fo(c, st);
+        for (size_t i = 0; i < S.msv_groups.size(); i++) {
+            const auto& gr = S.msv_groups[i];
+            const KernelInfo& k = c.kernel(msv_stage, gr.s.q, gr.s.g, false, gr.s.hoist);
+            const SxWork& w = msv_work[i];
+            const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), w.ntiles);
+            hipStream_t ls = msv_side ? fo.side(slot) : st;
+            if (with_events) HIPCHK(hipEventRecord(S.lev0[lslot], ls));
+            c.launch(k, msv_stage, w, S.usc.p, nosp, grid, true, slot % 32, ls);
+            if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], ls));
+            slot++;
+            lslot++;
+        }
+        fo.join();
     }
     int64_t* nclamp = (int64_t*)S.nclamp.p;
     const float* usc = (const float*)S.usc.p;

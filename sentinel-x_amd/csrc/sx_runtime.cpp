@@ -483,9 +483,11 @@ int Ctx::persistent_grid(const KernelInfo& k, size_t lds, int64_t ntiles) const 
 
 Fanout::Fanout(Ctx& c_, hipStream_t o) : c(c_), origin(o) { HIPCHK(hipEventRecord(c.fork_ev, origin)); }
 
-hipStream_t Fanout::next() {
+hipStream_t Fanout::next() { return side(n++); }
+
+hipStream_t Fanout::side(int k) {
     static_assert(Ctx::NSIDE == sizeof(used) / sizeof(used[0]), "one flag per side stream");
-    const int i = n++ % Ctx::NSIDE;
+    const int i = k % Ctx::NSIDE;
     if (!used[i]) {
         HIPCHK(hipStreamWaitEvent(c.side[i], c.fork_ev, 0));
         used[i] = true;

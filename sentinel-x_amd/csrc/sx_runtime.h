@@ -81,6 +81,7 @@ struct Fanout {
     Fanout(const Fanout&) = delete;
     Fanout& operator=(const Fanout&) = delete;
     hipStream_t next();
+    hipStream_t side(int i);  // side stream i mod NSIDE (ordered after the fork)
     void join();
     Ctx& c;
     hipStream_t origin;
