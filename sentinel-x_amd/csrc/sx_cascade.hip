@@ -513,7 +513,7 @@ sxc_records(const int64_t* __restrict__ n_dev, int64_t cap, const int32_t* __res
 
 // ---------------------------------------------------------------------------
 // Posterior decoding and region finding next to Forward / Backward
-// (pipeline.rs:186-242), one thread per pair.
+// (pipeline.rs:186-242), one wave per pair.
 // ---------------------------------------------------------------------------
 
 // dp.rs:32-40 in f32
@@ -532,28 +532,39 @@ __device__ __forceinline__ float sxr_lnf(float x) { return x > 0.0f ? logf(x) : 
 // (Forward row 0: N = 0, E = J = C = -inf; Backward row 0: E = J = C = -inf; row L:
 // N = B = J = -inf).  With overall = the Backward score (the given Forward score if that
 // is not finite, pipeline.rs:190), the posterior sums of pipeline.rs:194-206 and the
-// region scan of :208-242 run in one pass over the rows; btot / etot are kept in bt / et
-// (rows 0..L at roff) for the multi-domain test (is_multidomain, :527-536) of each region
-// as it closes.  res[p] gets the Forward score of the rows, the Backward score, nexpected
-// = btot[L] and the region count; each region goes to the list regs as (pair, i, j,
-// multi) at a slot taken from *nreg_all (a pair's regions keep their order; slots past
-// cap are counted, not written).
-extern "C" __global__ void __launch_bounds__(SXC_BLOCK)
+// region scan of :208-242 run over the rows; btot / etot are kept in bt / et (rows 0..L
+// at roff) for the multi-domain test (is_multidomain, :527-536) of each region as it
+// closes.  res[p] gets the Forward score of the rows, the Backward score, nexpected =
+// btot[L] and the region count; each region goes to the list regs as (pair, i, j, multi)
+// at a slot taken from *nreg_all (a pair's regions keep their order; slots past cap are
+// counted, not written).
+//
+// One wave per pair.  A row's posterior terms (its B and E posteriors and the N/J/C
+// occupancy, five logs and five exponentials) depend only on rows r-1 and r of the two
+// passes, so the 64 lanes compute 64 rows' terms at once; the running sums, their stores
+// and the region scan then take the rows in order, uniformly across the wave, with lane
+// 0 storing btot / etot and reading them back for the multi-domain test (one thread's
+// own stores, in program order).  Every value is the one-thread scan's: the same
+// operations on the same operands, the sums in row order.
+extern "C" __global__ void __launch_bounds__(64)
 sxr_regions(int64_t n, const SxRegionIn* __restrict__ in, const float* __restrict__ fx5,
             const double* __restrict__ fls, const float* __restrict__ bx5, const double* __restrict__ bls,
             float* bt, float* et, SxRegionRes* __restrict__ res, SxRegion* __restrict__ regs, int32_t* nreg_all,
             int32_t cap) {
-    const int64_t p = (int64_t)blockIdx.x * SXC_BLOCK + threadIdx.x;
+    const int64_t p = blockIdx.x;
+    const int lane = (int)threadIdx.x;
     if (p >= n) return;
     const SxRegionIn a = in[p];
     const int L = a.len;
     const float NI = -__builtin_inff();
     SxRegionRes o;
     if (L == 0) {
-        o.fsc = o.bsc = NI;
-        o.nexp = 0.0f;
-        o.nreg = 0;
-        res[p] = o;
+        if (lane == 0) {
+            o.fsc = o.bsc = NI;
+            o.nexp = 0.0f;
+            o.nreg = 0;
+            res[p] = o;
+        }
         return;
     }
     const float* F = fx5 + a.roff * 5;
@@ -566,67 +577,97 @@ sxr_regions(int64_t n, const SxRegionIn* __restrict__ in, const float* __restric
     const float bb0 = sxr_lnf(B[1]) + (float)BL[0];
     const float bs = sxr_logsum(bn1 + a.lloop, bb0 + a.lmove);
     const float overall = isfinite(bs) ? bs : (a.has_fwd ? a.fwd : fs);
-    // row 0 of each pass: Forward N B J C, Backward B (the rest is not read)
-    float fn = 0.0f, fb = sxr_lnf(F[1]) + (float)FL[0], fj = NI, fc = NI;
-    float bb = bb0;
     float btot = 0.0f, etot = 0.0f;
     float* BT = bt + a.roff;
     float* ET = et + a.roff;
-    BT[0] = 0.0f;
-    ET[0] = 0.0f;
+    if (lane == 0) {
+        BT[0] = 0.0f;
+        ET[0] = 0.0f;
+    }
     const float rt1 = 0.25f, rt2 = 0.10f, rt3 = 0.20f;
     int i = -1, nr = 0;
     bool triggered = false;
-    for (int r = 1; r <= L; r++) {
-        const float fl = (float)FL[r], bl = (float)BL[r];
-        const float* fr = F + (int64_t)r * 5;
-        const float* br = B + (int64_t)r * 5;
-        const float bnr = r < L ? sxr_lnf(br[0]) + bl : NI;
-        const float ber = sxr_lnf(br[2]) + bl;
-        const float bjr = r < L ? sxr_lnf(br[3]) + bl : NI;
-        const float bcr = sxr_lnf(br[4]) + bl;
-        const float fer = sxr_lnf(fr[2]) + fl;
-        const float bt_prev = btot, et_prev = etot;
-        btot = btot + expf(fb + bb - overall);
-        etot = etot + expf(fer + ber - overall);
-        const float njc = expf(fn + bnr + a.lloop - overall) + expf(fj + bjr + a.lloop - overall) +
-                          expf(fc + bcr + a.lloop - overall);
-        const float mocc = 1.0f - njc;
-        BT[r] = btot;
-        ET[r] = etot;
-        if (!triggered) {
-            if (mocc - (btot - bt_prev) < rt2 || i == -1) i = r;
-            if (mocc >= rt1) triggered = true;
-        } else if (mocc - (etot - et_prev) < rt2) {
-            const int ri = i > 1 ? i : 1;
-            float mx = -1.0f;
-            for (int jj = ri; jj <= r; jj++) {
-                const float v = fminf(ET[jj] - ET[ri - 1], btot - BT[jj - 1]);
-                if (v > mx) mx = v;
+    for (int base = 1; base <= L; base += 64) {
+        // this lane's row r: its terms from rows r-1 and r
+        const int r = base + lane;
+        float tb = 0.0f, te = 0.0f, mocc = 0.0f;
+        if (r <= L) {
+            const float fl = (float)FL[r], bl = (float)BL[r];
+            const float* fr = F + (int64_t)r * 5;
+            const float* br = B + (int64_t)r * 5;
+            const float bnr = r < L ? sxr_lnf(br[0]) + bl : NI;
+            const float ber = sxr_lnf(br[2]) + bl;
+            const float bjr = r < L ? sxr_lnf(br[3]) + bl : NI;
+            const float bcr = sxr_lnf(br[4]) + bl;
+            const float fer = sxr_lnf(fr[2]) + fl;
+            // row r-1's Forward N B J C and Backward B (row 0: N = 0, J = C = -inf)
+            float fn, fb, fj, fc, bb;
+            if (r == 1) {
+                fn = 0.0f;
+                fb = sxr_lnf(F[1]) + (float)FL[0];
+                fj = NI;
+                fc = NI;
+                bb = bb0;
+            } else {
+                const float fl1 = (float)FL[r - 1];
+                const float* fp = fr - 5;
+                fn = sxr_lnf(fp[0]) + fl1;
+                fb = sxr_lnf(fp[1]) + fl1;
+                fj = sxr_lnf(fp[3]) + fl1;
+                fc = sxr_lnf(fp[4]) + fl1;
+                bb = sxr_lnf(br[1 - 5]) + (float)BL[r - 1];
             }
-            const int32_t slot = atomicAdd(nreg_all, 1);
-            if (slot < cap) {
-                SxRegion g;
-                g.pair = (int32_t)p;
-                g.i = ri;
-                g.j = r;
-                g.multi = mx >= rt3 ? 1 : 0;
-                regs[slot] = g;
-            }
-            nr++;
-            i = -1;
-            triggered = false;
+            tb = expf(fb + bb - overall);
+            te = expf(fer + ber - overall);
+            const float njc = expf(fn + bnr + a.lloop - overall) + expf(fj + bjr + a.lloop - overall) +
+                              expf(fc + bcr + a.lloop - overall);
+            mocc = 1.0f - njc;
         }
-        // this row's Forward N B J C and Backward B are the next row's i-1 values
-        fn = sxr_lnf(fr[0]) + fl;
-        fb = sxr_lnf(fr[1]) + fl;
-        fj = sxr_lnf(fr[3]) + fl;
-        fc = sxr_lnf(fr[4]) + fl;
-        bb = r < L ? sxr_lnf(br[1]) + bl : NI;
+        const int rows = L - base + 1 < 64 ? L - base + 1 : 64;
+        for (int k = 0; k < rows; k++) {
+            const int rr = base + k;
+            const float tbk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tb), k));
+            const float tek = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(te), k));
+            const float mk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mocc), k));
+            const float bt_prev = btot, et_prev = etot;
+            btot = btot + tbk;
+            etot = etot + tek;
+            if (lane == 0) {
+                BT[rr] = btot;
+                ET[rr] = etot;
+            }
+            if (!triggered) {
+                if (mk - (btot - bt_prev) < rt2 || i == -1) i = rr;
+                if (mk >= rt1) triggered = true;
+            } else if (mk - (etot - et_prev) < rt2) {
+                const int ri = i > 1 ? i : 1;
+                if (lane == 0) {
+                    float mx = -1.0f;
+                    for (int jj = ri; jj <= rr; jj++) {
+                        const float v = fminf(ET[jj] - ET[ri - 1], btot - BT[jj - 1]);
+                        if (v > mx) mx = v;
+                    }
+                    const int32_t slot = atomicAdd(nreg_all, 1);
+                    if (slot < cap) {
+                        SxRegion g;
+                        g.pair = (int32_t)p;
+                        g.i = ri;
+                        g.j = rr;
+                        g.multi = mx >= rt3 ? 1 : 0;
+                        regs[slot] = g;
+                    }
+                }
+                nr++;
+                i = -1;
+                triggered = false;
+            }
+        }
     }
-    o.fsc = fs;
-    o.bsc = bs;
-    o.nexp = btot;
-    o.nreg = nr;
-    res[p] = o;
+    if (lane == 0) {
+        o.fsc = fs;
+        o.bsc = bs;
+        o.nexp = btot;
+        o.nreg = nr;
+        res[p] = o;
+    }
 }

@@ -456,9 +456,12 @@ Shape Ctx::shape_for(int stage, int form, int m) const {
 
 Shape Ctx::shape_for_spec(int stage, int m) const {
     if (m > SX_GEM_MIN_M) return gem_shape(m);
-    // special-row writers: Q in {4, 8} with register-resident transitions, {16, 32, 48} in LDS
-    for (int g : {1, 2, 4, 8, 16, 32, 64})
-        for (int q : {4, 8, 16, 32, 48}) {
+    // special-row writers: Q in {4, 8} with register-resident transitions, {16, 32, 48} in LDS.
+    // They run on the few pairs that pass the filters, whose longest pairs set the time, so
+    // the shape takes the fewest positions a lane (the shortest row per lane) that 64 lanes
+    // cover, then the fewest lanes of that Q.
+    for (int q : {4, 8, 16, 32, 48})
+        for (int g : {1, 2, 4, 8, 16, 32, 64}) {
             if ((long)q * g < m) continue;
             if (g == 1 && q > 16) continue;
             auto it = kernels.find(key(stage, q, g, 1, q <= 8 ? 1 : 0));

@@ -374,7 +374,7 @@ uint64_t device_regions(Ctx& c, RegionBufs& B, const uint32_t* prof, const uint3
     // a region takes at least two rows (the row that triggers it and the row that closes
     // it), so a pair of length L has at most L/2
     const int64_t cap = rows / 2 + (int64_t)n;
-    if (cap > INT32_MAX) throw SxError(SX_ERR_ARG, "too many special rows in one region scan");
+    if (cap > INT32_MAX || n > INT32_MAX) throw SxError(SX_ERR_ARG, "too many special rows in one region scan");
     auto* din = (SxRegionIn*)B.in.ensure(n * sizeof(SxRegionIn));
     auto* dres = (SxRegionRes*)B.res.ensure(n * sizeof(SxRegionRes));
     auto* dreg = (SxRegion*)B.regs.ensure((size_t)cap * sizeof(SxRegion));
@@ -383,8 +383,8 @@ uint64_t device_regions(Ctx& c, RegionBufs& B, const uint32_t* prof, const uint3
     float* det = (float*)B.et.ensure((size_t)rows * sizeof(float));
     HIPCHK(hipMemcpyAsync(din, in.data(), n * sizeof(SxRegionIn), hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(sxc_zero_i32, dim3(1), dim3(256), 0, c.stream, dcnt, (int64_t)1);
-    const int64_t nb = ((int64_t)n + 255) / 256;
-    hipLaunchKernelGGL(sxr_regions, dim3((unsigned)nb), dim3(256), 0, c.stream, (int64_t)n, (const SxRegionIn*)din,
+    // one wave per pair
+    hipLaunchKernelGGL(sxr_regions, dim3((unsigned)n), dim3(64), 0, c.stream, (int64_t)n, (const SxRegionIn*)din,
                        (const float*)fx5, (const double*)fls, (const float*)bx5, (const double*)bls, dbt, det, dres,
                        dreg, dcnt, (int32_t)cap);
     HIPCHK(hipGetLastError());
