@@ -596,11 +596,17 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
     };
     // A stage's runs cover disjoint tiles of a list: they go out side by side on the side
     // streams (the small runs of a stage no longer wait for the large ones), each with its
-    // own head slot (slots repeat only every 32 launches, on the same side stream).
+    // own head slot (slots repeat only every 32 launches, on the same side stream).  The
+    // runs of the longest models go first: on a small cascade their longest pairs set the
+    // stage's time, and a side stream would otherwise start them last.  Launch events keep
+    // the runs' order.
     auto stage_runs = [&](int stage, std::vector<CascadeState::Run>& runs, DeviceBuf& tl, DeviceBuf& tids,
                           DeviceBuf& toff, void* out) {
         Fanout fo(c, st);
-        for (auto& r : runs) {
+        const int lslot0 = lslot;
+        for (size_t ri = runs.size(); ri-- > 0;) {
+            const auto& r = runs[ri];
+            const int ls_ev = lslot0 + (int)ri;
             const KernelInfo& k = c.kernel(stage, r.s.q, r.s.g, false, r.s.hoist);
             SxWork w{};
             w.tiles = (const SxTile*)tl.p;
@@ -610,11 +616,11 @@ static void enqueue_chain(Ctx& c, CascadeState& S, bool with_events) {
             w.ntiles = 0;
             const int grid = c.persistent_grid(k, Ctx::lds_bytes(k), 1 << 30);
             hipStream_t ls = fo.next();
-            if (with_events) HIPCHK(hipEventRecord(S.lev0[lslot], ls));
+            if (with_events) HIPCHK(hipEventRecord(S.lev0[ls_ev], ls));
             c.launch(k, stage, w, out, nosp, grid, true, slot++ % 32 + 32, ls);
-            if (with_events) HIPCHK(hipEventRecord(S.lev[lslot], ls));
-            lslot++;
+            if (with_events) HIPCHK(hipEventRecord(S.lev[ls_ev], ls));
         }
+        lslot = lslot0 + (int)runs.size();
         fo.join();
     };
     // 1. MSV over all pairs (the f32 scores, or the int16 bound of each pair).  When every
