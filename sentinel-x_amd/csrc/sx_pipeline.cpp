@@ -796,7 +796,8 @@ static uint64_t run_batched(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, u
     CascadeState& S = *c.cascade;
     const int bound = (st->bounds & SX_BOUND_MSV) ? 1 : 0, vbound = (st->bounds & SX_BOUND_VITERBI) ? 1 : 0;
     if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
-        S.bound != bound || S.vbound != vbound || c.cascade_dirty) {
+        S.bound != bound || S.vbound != vbound || S.thr_uncal != c.cascade_pass_uncalibrated ||
+        c.cascade_dirty) {
         build_plan(c, S, st->f1, st->f2, st->f3, st->bounds);
         const int64_t P = (int64_t)c.profs.size(), T = c.nt;
         // profiles the bound skips make every pair a candidate
@@ -963,7 +964,8 @@ static uint64_t run_plain(Ctx& c, sx_cascade_stats* st, sx_pair_scores* out, uin
     CascadeState& S = *c.cascade;
     const int bound = (st->bounds & SX_BOUND_MSV) ? 1 : 0, vbound = (st->bounds & SX_BOUND_VITERBI) ? 1 : 0;
     if (S.nprof != c.profs.size() || S.ntarg != c.nt || S.f1 != st->f1 || S.f2 != st->f2 || S.f3 != st->f3 ||
-        S.bound != bound || S.vbound != vbound || c.cascade_dirty) {
+        S.bound != bound || S.vbound != vbound || S.thr_uncal != c.cascade_pass_uncalibrated ||
+        c.cascade_dirty) {
         build_plan(c, S, st->f1, st->f2, st->f3, st->bounds);
         alloc_buffers(c, S, 1 << 16);
         c.cascade_dirty = false;
@@ -1211,17 +1213,16 @@ std::vector<PassPair> cascade_pass_pairs(Ctx& c, double f1, double f2, double f3
     st.use_graph = 1;
     st.bounds = SX_BOUND_MSV;  // the int16 Viterbi bound is no faster than the f32 Viterbi
     c.ensure_pidx();
+    // the plan is rebuilt whenever this flag differs from the one it was built with, so
+    // repeated searches reuse it (and its captured graph) and the next cascade rebuilds it
     c.cascade_pass_uncalibrated = true;
-    c.cascade_dirty = true;
     try {
         (void)run_batched(c, &st, nullptr, 0);
     } catch (...) {
         c.cascade_pass_uncalibrated = false;
-        c.cascade_dirty = true;
         throw;
     }
     c.cascade_pass_uncalibrated = false;
-    c.cascade_dirty = true;
     std::vector<PassPair> out;
     for (const auto& r : c.cascade->last)
         if (r.stage == 4) out.push_back({r.profile, r.target, r.fwd});
