@@ -27,6 +27,8 @@ def main():
     ap.add_argument("--workload", choices=("marker", "mag"), default="mag")
     ap.add_argument("--profile-scale", type=float, default=0.2)
     ap.add_argument("--genomes", type=int, default=40)
+    ap.add_argument("--shapes", default=None,
+                    help="shape table file to install (stages it lists); default: the built-in shapes")
     a = ap.parse_args()
     sxa = load_package()
     from sentinel_x_amd import synth
@@ -41,6 +43,10 @@ def main():
     with sxa.Context(0) as c:
         c.set_profiles(wl.hmms)
         c.set_targets(wl.seqs)
+        if a.shapes:
+            with open(a.shapes) as fh:
+                for st, v in json.load(fh)["stages"].items():
+                    c.set_shapes(int(st), [tuple(r) for r in v["table"]])
         for host in (False, True):
             for _ in range(2):
                 h, d = c.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True, host_rescore=host)
