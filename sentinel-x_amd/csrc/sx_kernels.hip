@@ -935,8 +935,17 @@ __device__ __forceinline__ float vit_pair(const float* __restrict__ em, int ew, 
 #pragma unroll
         for (int q = 1; q < Q; q++) dr[q] = fmaxf(mr[q - 1] + tmd[q - 1], dr[q - 1] + tdd[q - 1]);
         if constexpr (G > 1) {
-            // lazy-F: re-run the chain from each lane's true incoming D until nothing changes
-            for (int it = 1; it < G; it++) {
+            // lazy-F: re-run the chain from each lane's true incoming D until nothing
+            // changes.  The first pass runs without a test (a lane's carry raises its first
+            // D on nearly every row, and a pass that raises nothing changes nothing); the
+            // loop then tests before each further pass
+            {
+                const float c = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF) + tdd[-1];
+                dr[0] = fmaxf(dr[0], c);
+#pragma unroll
+                for (int q = 1; q < Q; q++) dr[q] = fmaxf(dr[q], dr[q - 1] + tdd[q - 1]);
+            }
+            for (int it = 2; it < G; it++) {
                 const float din = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
                 const float c = din + tdd[-1];
                 if (!__any(c > dr[0])) break;
@@ -1027,8 +1036,17 @@ __device__ __forceinline__ float vit_pair_aln(const float* __restrict__ em, cons
 #pragma unroll
         for (int q = 1; q < Q; q++) dr[q] = fmaxf(mr[q - 1] + tmd[q], dr[q - 1] + tdd[q]);
         if constexpr (G > 1) {
-            // lazy-F: re-run the chain from each lane's true incoming D until nothing changes
-            for (int it = 1; it < G; it++) {
+            // lazy-F: re-run the chain from each lane's true incoming D until nothing
+            // changes.  The first pass runs without a test (a lane's carry raises its first
+            // D on nearly every row, and a pass that raises nothing changes nothing); the
+            // loop then tests before each further pass
+            {
+                const float c = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF) + tdd[0];
+                dr[0] = fmaxf(dr[0], c);
+#pragma unroll
+                for (int q = 1; q < Q; q++) dr[q] = fmaxf(dr[q], dr[q - 1] + tdd[q]);
+            }
+            for (int it = 2; it < G; it++) {
                 const float din = grp_from_prev<G>(dr[Q - 1], SX_NEG_INF);
                 const float c = din + tdd[0];
                 if (!__any(c > dr[0])) break;
