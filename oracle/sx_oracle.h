@@ -19,8 +19,8 @@
  *   sp      : 5 x (L+1) floats: N, B, E, J, C rows.
  *
  * Pinning (what checks each function independently of this restatement) is
- * listed per function below; "parity unpinned" marks a value no golden vector
- * of the reference fixes.
+ * listed per function below.  Where the reference holds no golden vector for a
+ * value, the function is pinned by the independent checks named there.
  */
 #ifndef SX_ORACLE_H
 #define SX_ORACLE_H
@@ -54,8 +54,9 @@ void sxo_xsc(int64_t l, int multihit, float* xsc16);
 /* model.rs:334-337 */
 float sxo_null_score(int64_t l);
 
-/* dp.rs:92-145 (msv_kernel).  Parity unpinned by the reference; pinned by
- * a generic max-product over the explicit MSV state graph (tests). */
+/* dp.rs:92-145 (msv_kernel).  No golden vector in the reference; pinned by a
+ * generic max-product over the explicit MSV state graph, the flat-model closed
+ * form and '*'-only targets (tests). */
 float sxo_msv(int m, const float* msc, float msv_tbm, const uint8_t* dsq, int64_t l);
 
 /* Quantised MSV upper bound: the product's int16 MSV prefilter, which has no
@@ -89,8 +90,9 @@ int32_t sxo_q_const(float v);
 int32_t sxo_vit16(int m, const float* msc, const float* tsc, const float* xsc16, const uint8_t* dsq, int64_t l);
 int16_t sxo_q_trans(float t);
 
-/* dp.rs:152-189.  Parity unpinned by the reference; pinned by a generic
- * max-product over the explicit Plan7 state graph (tests). */
+/* dp.rs:152-189.  No golden vector in the reference; pinned by a generic
+ * max-product over the explicit Plan7 state graph and brute-force path
+ * enumeration on tiny cases (tests). */
 float sxo_viterbi(int m, const float* msc, const float* tsc, const float* xsc16,
                   const uint8_t* dsq, int64_t l);
 
@@ -111,7 +113,7 @@ double sxo_forward64(int m, const float* mo, const float* tp, const float* xsc16
 double sxo_backward64(int m, const float* mo, const float* tp, const float* xsc16,
                       const uint8_t* dsq, int64_t l);
 
-/* pipeline.rs:107-150 (bias_filter_score).  Parity unpinned by the
+/* pipeline.rs:107-150 (bias_filter_score).  No golden vector in the
  * reference; pinned by an independent log-space two-state forward (tests). */
 float sxo_bias_filter(int m, const float* compo, const uint8_t* dsq, int64_t l);
 
