@@ -1109,6 +1109,15 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
     // TM_REG: the lane's records and D-chain products loaded once, held in registers
     float4 hra[HOIST ? Q : 1], hrb[HOIST ? Q : 1];
     float hpc[HOIST ? Q : 1];
+    // XSUM: the lane's part of the row sum, sum(M) + sum(Dl), as one weighted sum of M.
+    // Dl[t] = sum_{r<t} M[r] MD[r] prod_{s=r+1}^{t-1} DD[s], so sum_t Dl[t] = sum_r M[r]
+    // MD[r] W[r] with W[Q-1] = 0, W[r] = 1 + DD[r+1] W[r+1]: the weights X[r] = 1 + MD[r]
+    // W[r] depend on the profile only, and the row sum costs one FMA a cell instead of two
+    // adds (with the records in registers at 12 positions a lane: 3-5% faster at 8-16
+    // lanes; the two-row lanes of 4 and 8 positions would lose a wave per SIMD to the Q
+    // more registers, and 16 positions would spill)
+    constexpr bool XSUM = HOIST && Q == 12 && !fwd_rows2<Q, TM>();
+    float hx[XSUM ? Q : 1];
     if constexpr (HOIST) {
         const float4* __restrict__ r0 = reinterpret_cast<const float4*>(trec + j * FR::RSTRIDE);
 #pragma unroll
@@ -1116,6 +1125,14 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
             hra[q] = r0[2 * q];
             hrb[q] = r0[2 * q + 1];
             hpc[q] = pcb[q];
+        }
+        if constexpr (XSUM) {
+            float w = 0.0f;
+#pragma unroll
+            for (int q = Q - 1; q >= 0; q--) {
+                if (q < Q - 1) w = fmaf(hrb[q + 1].w, w, 1.0f);
+                hx[q] = fmaf(hrb[q].z, w, 1.0f);
+            }
         }
     }
     float mr[Q], ir[Q], dl[Q];
@@ -1190,12 +1207,16 @@ __device__ __forceinline__ double fwd_pair(const float* __restrict__ eot, int ew
                 const float mv = s * eo[q];
                 ir[q] = fmaf(mr[q], tb.x, ir[q] * tb.y);
                 mr[q] = mv;
-                em = q == Q - 1 ? mv : em + mv;
+                if constexpr (XSUM)
+                    em = q == Q - 1 ? mv * hx[q] : fmaf(mv, hx[q], em);
+                else
+                    em = q == Q - 1 ? mv : em + mv;
             }
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 dl[q] = dnext;
-                if (q) ed = q == 1 ? dnext : ed + dnext;
+                if constexpr (!XSUM)
+                    if (q) ed = q == 1 ? dnext : ed + dnext;
                 dnext = q ? fmaf(dnext, hrb[q].w, mr[q] * hrb[q].z) : mr[q] * hrb[q].z;
             }
         } else {
