@@ -344,7 +344,9 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
     latency of a genome is the host wall time from setting its proteins to holding its
     records (p50 / p99 over genomes), and the throughput is the sweep's total cells over
     the summed latencies."""
+    t0 = time.time()
     wl = synth.make_workload(4, profile_scale=profile_scale, n_genomes=n_genomes)
+    log(f"[bench] mag-sweep workload: {len(wl.hmms)} profiles x {len(wl.seqs)} proteins ({time.time() - t0:.1f} s)")
     calibrate(sxa, synth, wl.hmms, 200)
     ms = np.array([h.m for h in wl.hmms], np.int64)
     lens = np.array([len(x) for x in wl.seqs], np.int64)
@@ -362,6 +364,7 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
             t.append(st.ms_total)
         pk = float(min(t))
         out["packed"] = {"ms": round(pk, 3), "gcups": round(cells / pk / 1e6, 1), "hits": int(st.n_fwd_pass)}
+        log(f"[bench] mag-sweep packed: {out['packed']}")
         # the whole search over the packed genomes (filters on, E <= 10), envelopes rescored
         # on the device and, for comparison, on host threads
         srch = {}
@@ -369,6 +372,7 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
             for _ in range(2):
                 h, d = cm.search(cutoff="evalue", seq_e=10.0, dom_e=10.0, filters=True, host_rescore=host, **f)
             ss = cm.last_search_stats
+            log(f"[bench] mag-sweep search ({'host' if host else 'device'} rescoring): {len(h)} hits")
             srch["host_rescoring" if host else "device_rescoring"] = {
                 "hits": int(len(h)), "domains": int(len(d)), "envelopes": int(ss["n_envelopes"]),
                 **{k: round(ss[k], 2) for k in ("ms_filters", "ms_specials", "ms_rescore", "ms_tail", "ms_report")}}
@@ -388,6 +392,8 @@ def mag_sweep_leg(sxa, synth, f, profile_scale, n_genomes, shapes=None, tune_ms=
             cm.set_targets(seqs)
             st, rec = cm.run_cascade(sxa.FORM_BATCHED, use_graph=False, **f)
             lat.append((time.perf_counter() - t0) * 1e3)
+            if g % 25 == 24:
+                log(f"[bench] mag-sweep per genome: {g + 1} of {n_genomes}, last {lat[-1]:.1f} ms")
             gcells += int(ms.sum()) * int(sum(len(x) for x in seqs))
         lat = np.array(lat)
         out["per_genome"] = {"latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
