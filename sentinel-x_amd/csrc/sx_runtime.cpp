@@ -488,6 +488,19 @@ Fanout::Fanout(Ctx& c_, hipStream_t o) : c(c_), origin(o) { HIPCHK(hipEventRecor
 
 hipStream_t Fanout::next() { return side(n++); }
 
+hipStream_t Fanout::pick(bool exclusive, int* slot) {
+    constexpr int R = 64 / Ctx::NSIDE;  // slots per stream
+    int i;
+    if (exclusive) {
+        i = 0;
+        *slot = Ctx::NSIDE * (n_excl++ % R);
+    } else {
+        i = 1 + n_other % (Ctx::NSIDE - 1);
+        *slot = i + Ctx::NSIDE * ((n_other++ / (Ctx::NSIDE - 1)) % R);
+    }
+    return side(i);
+}
+
 hipStream_t Fanout::side(int k) {
     static_assert(Ctx::NSIDE == sizeof(used) / sizeof(used[0]), "one flag per side stream");
     const int i = k % Ctx::NSIDE;
@@ -576,9 +589,13 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
     pb.ensure(flat.size() * sizeof(int32_t));
     HIPCHK(hipMemcpyAsync(pb.p, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    int slot = 0;
     size_t gi = 0;
-    Fanout fo(*this, stream);  // the shape groups side by side (head slots differ within 64)
+    // The shape groups side by side, except the long-model groups (tables from global
+    // memory, transition records filling 48-111 KB of LDS): those go one after another
+    // (measured on 200 models of 1,500-2,500 positions x 400 proteins of 8,000-35,000
+    // residues: the 32- and 40-position lane groups 2.79 s side by side, 2.21 s one after
+    // the other).
+    Fanout fo(*this, stream);
     for (auto& kv : groups) {
         const int q = std::get<0>(kv.first), g = std::get<1>(kv.first), h = std::get<2>(kv.first);
         const KernelInfo& k = kernel(stage, q, g, false, h);
@@ -599,7 +616,9 @@ void Ctx::run_implicit(int stage, int form, const std::vector<int32_t>& plist, c
         // across all profiles (the launch does not end on one profile's longest targets);
         // otherwise, and for the MSV stages, a profile's tiles stay together.
         w.chunk_major = stage != SX_STAGE_MSV && stage != SX_STAGE_MSV16 && w.nchunks < grid ? 1 : 0;
-        launch(k, stage, w, dout, sp, grid, true, slot++ % 64, fo.next());
+        int slot;
+        hipStream_t ls = fo.pick(h == 2, &slot);
+        launch(k, stage, w, dout, sp, grid, true, slot, ls);
     }
     fo.join();
 }
@@ -671,8 +690,7 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
     // the host arrays above die with this call: the copies complete before it returns
     HIPCHK(hipStreamSynchronize(stream));
     size_t toff = 0;
-    int slot = 0;
-    Fanout fo(*this, stream);  // batched: the shape groups side by side
+    Fanout fo(*this, stream);  // batched: the shape groups side by side, long-model groups in turn
     for (auto& kv : tiles) {
         const int q = std::get<0>(kv.first), g = std::get<1>(kv.first);
         const int h = std::get<2>(kv.first);
@@ -685,7 +703,9 @@ void Ctx::run_pairs(int stage, int form, const uint32_t* prof, const uint32_t* t
         w.ntiles = (int64_t)kv.second.size();
         if (form == SX_FORM_BATCHED) {
             const int grid = persistent_grid(k, lds_bytes(k), w.ntiles);
-            launch(k, stage, w, dout, sp, grid, true, slot++ % 64, fo.next());
+            int slot;
+            hipStream_t ls = fo.pick(h == 2, &slot);
+            launch(k, stage, w, dout, sp, grid, true, slot, ls);
         } else {
             launch(k, stage, w, dout, sp, (int)w.ntiles, false, 0);
         }

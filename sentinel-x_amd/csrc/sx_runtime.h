@@ -82,10 +82,18 @@ struct Fanout {
     Fanout& operator=(const Fanout&) = delete;
     hipStream_t next();
     hipStream_t side(int i);  // side stream i mod NSIDE (ordered after the fork)
+    // The stream and head slot (0-63) of a launch that does not use next(): exclusive
+    // launches one after another on side stream 0, the others in turn on the other side
+    // streams.  A slot is congruent to its stream's index modulo NSIDE, so launches that
+    // share a slot share a stream.  For persistent kernels whose tables take most of a
+    // CU's LDS: side by side, each one's resident workgroups keep the other's off their
+    // CUs until it ends.
+    hipStream_t pick(bool exclusive, int* slot);
     void join();
     Ctx& c;
     hipStream_t origin;
     int n = 0;
+    int n_excl = 0, n_other = 0;
     bool used[4] = {};
 };
 
